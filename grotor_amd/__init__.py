@@ -1,0 +1,285 @@
+"""grotor_amd: the MI355X (gfx950) routing hot path of GRotor.
+
+Python mirror of the C ABI in include/grotor_amd.h, used by the tests and the
+benchmark.  All compute runs in libgrotor_amd.so (hand-written gfx950
+kernels); there is no CPU fallback: without the built library every entry
+point raises, and without a HIP device Context() raises.
+
+Reference entry points mirrored here (reference tree file:line):
+  Context.refresh          graph::refresh_cost         src/cost/incremental_update.cuh:98
+  Context.batch_plan       generate_batches_cpu_timed  src/batching/generate_batches.hpp:14
+  Context.lshape_batch     Lshape_route_warp launch    src/routing/lshape.cuh:168
+  Context.commit_batch     graph::commit_batch_demand  src/routing/demand.cuh:313
+  Context.lshape_run       L-shape batch loop          src/routing/lshape.cuh:148-180
+  Context.ripup            graph::ripup                src/routing/congestion.cuh:114
+  Design.prepare           prepare_routing_nets etc.   src/preprocess.hpp:1448
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgrotor_amd.so")
+ROUTE_PER_PIN = 12
+
+GR_OK, GR_E_ARG, GR_E_HIP, GR_E_CAPACITY, GR_E_STATE, GR_E_DEVICE, GR_E_IO = range(7)
+
+
+class GrError(RuntimeError):
+    """A C ABI call returned a non-zero status."""
+
+    def __init__(self, code, message):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int), ("X", ctypes.c_int), ("Y", ctypes.c_int), ("DIR", ctypes.c_int),
+                ("unit_length_wire_cost", ctypes.c_double), ("unit_via_cost", ctypes.c_double),
+                ("unit_length_short_costs", _f64p), ("x_edge_len", _i32p), ("y_edge_len", _i32p),
+                ("capacity", _f32p)]
+
+
+class TopologyDesc(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int), ("X", ctypes.c_int), ("Y", ctypes.c_int), ("DIR", ctypes.c_int),
+                ("nets", ctypes.c_int), ("subnets", ctypes.c_int),
+                ("order", _i32p), ("node_off", _i32p), ("nodes", _i32p), ("parents", _i32p),
+                ("pt_off", _i32p), ("pts", _i32p), ("hs_off", _i32p), ("hsegs", _i32p),
+                ("vs_off", _i32p), ("vsegs", _i32p), ("pin_acc_num", _i32p), ("capacity", _f32p),
+                ("unit_length_short_costs", _f64p), ("x_edge_len", _i32p), ("y_edge_len", _i32p),
+                ("unit_length_wire_cost", ctypes.c_double), ("unit_via_cost", ctypes.c_double)]
+
+
+def lib():
+    """Load libgrotor_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GrError(GR_E_STATE, f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.gr_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _call(name, *args):
+    status = getattr(lib(), name)(*args)
+    if status != GR_OK:
+        raise GrError(status, f"{name}: {lib().gr_last_error().decode()}")
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    _call("gr_device_name", buf, 256)
+    return buf.value.decode()
+
+
+class Design:
+    """Host design database: CAP/NET input, synthetic instances, front end."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self._topo = None
+
+    @classmethod
+    def read(cls, cap, net):
+        h = ctypes.c_void_p()
+        _call("gr_design_read", cap.encode(), net.encode(), ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def generate(cls, config, seed=0, net_scale=1.0, grid_side=0):
+        h = ctypes.c_void_p()
+        _call("gr_design_generate", ctypes.c_int(config), ctypes.c_ulonglong(seed), ctypes.c_double(net_scale),
+              ctypes.c_int(grid_side), ctypes.byref(h))
+        return cls(h)
+
+    def write(self, cap=None, net=None):
+        _call("gr_design_write", self._h, cap.encode() if cap else None, net.encode() if net else None)
+
+    def prepare(self, threads=8):
+        _call("gr_design_prepare", self._h, ctypes.c_int(threads))
+        self._topo = None
+        return self
+
+    def topology(self):
+        """Stage topology as numpy arrays (copies)."""
+        if self._topo is None:
+            t = TopologyDesc()
+            _call("gr_design_topology", self._h, ctypes.byref(t))
+            n = t.nets
+
+            def arr(p, count, dtype=np.int32):
+                if count == 0:
+                    return np.zeros(0, dtype)
+                return np.ctypeslib.as_array(p, shape=(count,)).astype(dtype, copy=True)
+
+            node_off = arr(t.node_off, n + 1)
+            pt_off = arr(t.pt_off, n + 1)
+            hs_off = arr(t.hs_off, n + 1)
+            vs_off = arr(t.vs_off, n + 1)
+            cells = t.L * t.X * t.Y
+            self._topo = dict(
+                L=t.L, X=t.X, Y=t.Y, DIR=t.DIR, nets=n, subnets=t.subnets,
+                order=arr(t.order, n), node_off=node_off, nodes=arr(t.nodes, int(node_off[-1])),
+                parents=arr(t.parents, int(node_off[-1])), pt_off=pt_off, pts=arr(t.pts, int(pt_off[-1])),
+                hs_off=hs_off, hsegs=arr(t.hsegs, 2 * int(hs_off[-1])), vs_off=vs_off,
+                vsegs=arr(t.vsegs, 2 * int(vs_off[-1])), pin_acc_num=arr(t.pin_acc_num, t.subnets + 1),
+                capacity=arr(t.capacity, cells, np.float32),
+                unit_length_short_costs=arr(t.unit_length_short_costs, t.L, np.float64),
+                x_edge_len=arr(t.x_edge_len, t.X - 1), y_edge_len=arr(t.y_edge_len, t.Y - 1),
+                unit_length_wire_cost=t.unit_length_wire_cost, unit_via_cost=t.unit_via_cost)
+        return self._topo
+
+    def write_output(self, routes, path):
+        routes = _i32(routes)
+        _call("gr_design_write_output", self._h, _ptr(routes, ctypes.c_int), path.encode())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.gr_design_destroy(self._h)
+            self._h = None
+
+
+class Context:
+    """Device state of one routing run (the device half of RouterState)."""
+
+    def __init__(self, grid):
+        g = grid
+        self.L, self.X, self.Y, self.DIR = int(g["L"]), int(g["X"]), int(g["Y"]), int(g["DIR"])
+        self._keep = [np.ascontiguousarray(g["unit_length_short_costs"], np.float64), _i32(g["x_edge_len"]),
+                      _i32(g["y_edge_len"]), np.ascontiguousarray(g["capacity"], np.float32)]
+        desc = GridDesc(self.L, self.X, self.Y, self.DIR, float(g["unit_length_wire_cost"]),
+                        float(g["unit_via_cost"]), _ptr(self._keep[0], ctypes.c_double),
+                        _ptr(self._keep[1], ctypes.c_int), _ptr(self._keep[2], ctypes.c_int),
+                        _ptr(self._keep[3], ctypes.c_float))
+        h = ctypes.c_void_p()
+        _call("gr_ctx_create", ctypes.byref(desc), ctypes.byref(h))
+        self._h = h
+        self.cells = self.L * self.X * self.Y
+        self.route_words = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gr_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+    def set_routes(self, pin_acc_num):
+        pin_acc_num = _i32(pin_acc_num)
+        _call("gr_ctx_set_routes", self._h, ctypes.c_int(len(pin_acc_num) - 1), _ptr(pin_acc_num, ctypes.c_int))
+        self.route_words = ROUTE_PER_PIN * int(pin_acc_num[-1]) + 1
+
+    def refresh(self, full=False):
+        _call("gr_cost_refresh_full" if full else "gr_cost_refresh", self._h)
+
+    def costs(self, wcost=True, vcost=True, presum=True):
+        w = np.empty(self.cells, np.float32) if wcost else None
+        v = np.empty(self.cells, np.float32) if vcost else None
+        p = np.empty(self.cells, np.float64) if presum else None
+        _call("gr_cost_download", self._h, _ptr(w, ctypes.c_float), _ptr(v, ctypes.c_float), _ptr(p, ctypes.c_double))
+        return w, v, p
+
+    def upload_demand(self, demand):
+        d = np.ascontiguousarray(demand, np.float32)
+        _call("gr_demand_upload", self._h, _ptr(d, ctypes.c_float))
+
+    def demand(self):
+        d = np.empty(self.cells, np.float32)
+        _call("gr_demand_download", self._h, _ptr(d, ctypes.c_float))
+        return d
+
+    def timestamps(self):
+        t = np.empty(self.cells, np.int32)
+        _call("gr_timestamp_download", self._h, _ptr(t, ctypes.c_int))
+        return t
+
+    def stats(self):
+        wl = ctypes.c_double()
+        vias = ctypes.c_longlong()
+        _call("gr_stats_download", self._h, ctypes.byref(wl), ctypes.byref(vias))
+        return wl.value, vias.value
+
+    def batch_plan(self, pt_off, pts, hs_off, hsegs, vs_off, vsegs, limit=1000000, sparsity=1):
+        """Batch index of every net (priority order) and the batch count."""
+        arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
+        n = len(arrs[0]) - 1
+        color = np.empty(max(n, 1), np.int32)
+        nb = ctypes.c_int()
+        _call("gr_batch_plan", self._h, ctypes.c_int(n), *[_ptr(a, ctypes.c_int) for a in arrs],
+              ctypes.c_int(limit), ctypes.c_int(sparsity), _ptr(color, ctypes.c_int), ctypes.byref(nb))
+        return color[:n], nb.value
+
+    def lshape_upload(self, net_ids, node_off, nodes, parents):
+        arrs = [_i32(a) for a in (net_ids, node_off, nodes, parents)]
+        _call("gr_lshape_upload", self._h, ctypes.c_int(len(arrs[0])), *[_ptr(a, ctypes.c_int) for a in arrs])
+
+    def lshape_batch(self, begin, count, stamp):
+        _call("gr_lshape_batch", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_int(stamp))
+
+    def commit_batch(self, begin, count, stamp, K=1):
+        _call("gr_commit_batch", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_int(stamp),
+              ctypes.c_int(K))
+
+    def lshape_run(self, batch_off, first_stamp=1, mode=0):
+        """Run the batch loop; returns the synchronized interval in ms."""
+        batch_off = _i32(batch_off)
+        ms = ctypes.c_double()
+        _call("gr_lshape_run", self._h, ctypes.c_int(len(batch_off) - 1), _ptr(batch_off, ctypes.c_int),
+              ctypes.c_int(first_stamp), ctypes.c_int(mode), ctypes.byref(ms))
+        return ms.value
+
+    def routes(self):
+        r = np.empty(self.route_words, np.int32)
+        _call("gr_routes_download", self._h, _ptr(r, ctypes.c_int), ctypes.c_longlong(self.route_words))
+        return r
+
+    def ripup(self, net_count, threshold=0):
+        flags = np.empty(max(net_count, 1), np.uint8)
+        _call("gr_ripup", self._h, ctypes.c_int(threshold), _ptr(flags, ctypes.c_ubyte))
+        return flags[:net_count]
+
+
+def batches_from_colors(color, nb):
+    """Reference batch lists (vector<vector<int>>) from per-slot batch indices:
+    batch b holds the priority slots with color b in priority order."""
+    color = np.asarray(color)
+    order = np.argsort(color, kind="stable")
+    counts = np.bincount(color, minlength=nb)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return [order[off[b]:off[b + 1]] for b in range(nb)]
+
+
+def execution_schedule(topo, color, nb):
+    """Stage arrays in execution order: batches reversed (last color first,
+    src/routing/lshape.cuh:70) and nets grouped by batch (:72-77)."""
+    lists = batches_from_colors(color, nb)[::-1]
+    slots = np.concatenate(lists) if lists else np.zeros(0, np.int64)
+    batch_off = np.concatenate([[0], np.cumsum([len(b) for b in lists])]).astype(np.int32)
+    node_off = topo["node_off"]
+    sizes = node_off[slots + 1] - node_off[slots]
+    new_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    idx = np.concatenate([np.arange(node_off[s], node_off[s + 1]) for s in slots]) if len(slots) else np.zeros(0, np.int64)
+    return dict(net_ids=topo["order"][slots].astype(np.int32), node_off=new_off,
+                nodes=topo["nodes"][idx].astype(np.int32), parents=topo["parents"][idx].astype(np.int32),
+                batch_off=batch_off, slots=slots)

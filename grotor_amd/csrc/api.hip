@@ -1,0 +1,622 @@
+// C ABI of the gfx950 hot path: device context, layout conversion and the
+// per-batch launch sequences.  See include/grotor_amd.h for the reference
+// entry point each function replaces.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <memory>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/grotor_amd.h"
+#include "gr_kernels.hpp"
+#include "gr_status.hpp"
+
+using namespace grotor;
+
+namespace {
+
+void hip_check(hipError_t status, const char* op) {
+    if (status != hipSuccess) throw GrError(GR_E_HIP, std::string(op) + ": " + hipGetErrorString(status));
+}
+
+struct DeviceArena {
+    std::vector<void*> blocks;
+    template <class T>
+    T* alloc(size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "hipMalloc");
+        blocks.push_back(p);
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    void release(T*& p) {
+        if (!p) return;
+        auto it = std::find(blocks.begin(), blocks.end(), (void*)p);
+        if (it != blocks.end()) blocks.erase(it);
+        (void)hipFree(p);
+        p = nullptr;
+    }
+    ~DeviceArena() {
+        for (void* p : blocks) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+struct gr_ctx {
+    DeviceArena mem;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    DeviceGrid g{};
+    int tracks = 0, maxlen = 0;
+    size_t cells = 0;
+    float *capacity = nullptr, *demand = nullptr, *vcost = nullptr, *wcost = nullptr;
+    double* presum = nullptr;
+    int* timestamp = nullptr;
+    unsigned* track_dirty = nullptr;
+    Stats* stats = nullptr;
+    int* error = nullptr;
+    CostParams cp;
+    // route storage
+    int net_count = 0;
+    std::vector<int> pin_acc_num_h;
+    int* pin_acc_num = nullptr;
+    int* routes = nullptr;
+    long long route_words = 0;
+    // L-shape stage topology
+    int stage_count = 0;
+    std::vector<int> node_off_h;
+    int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr, *slots = nullptr;
+    double* gscratch = nullptr;
+    size_t gscratch_bytes = 0;
+    // rip-up
+    int* of_sum = nullptr;
+    unsigned char* of_flags = nullptr;
+
+    ~gr_ctx() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+constexpr int kSmallTree = 16;
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+void download(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// IDX order <-> track-major order (host side, one pass)
+template <class T>
+void to_track_major(const DeviceGrid& g, const T* src, std::vector<T>& dst) {
+    dst.resize(size_t(g.L) * g.XY);
+    for (int l = 0; l < g.L; ++l)
+        for (int x = 0; x < g.X; ++x)
+            for (int y = 0; y < g.Y; ++y) dst[cell_of(g, l, x, y)] = src[(size_t(l) * g.X + x) * g.Y + y];
+}
+template <class T>
+void from_track_major(const DeviceGrid& g, const std::vector<T>& src, T* dst) {
+    for (int l = 0; l < g.L; ++l)
+        for (int x = 0; x < g.X; ++x)
+            for (int y = 0; y < g.Y; ++y) dst[(size_t(l) * g.X + x) * g.Y + y] = src[cell_of(g, l, x, y)];
+}
+
+void check_device_error(gr_ctx* c, const char* op) {
+    int err = 0;
+    download(&err, c->error, sizeof(int), c->stream);
+    if (!err) return;
+    const int zero = 0;
+    upload(c->error, &zero, sizeof(int), c->stream);
+    switch (err) {
+        case kDevRouteOverflow: throw GrError(GR_E_CAPACITY, std::string(op) + ": route storage exceeded");
+        case kDevNoSolution: throw GrError(GR_E_DEVICE, std::string(op) + ": no finite layer assignment");
+        case kDevSpinLimit: throw GrError(GR_E_DEVICE, std::string(op) + ": batching wait exceeded its spin limit");
+        case kDevColorOverflow: throw GrError(GR_E_CAPACITY, std::string(op) + ": more than 4096 batches");
+        default: throw GrError(GR_E_DEVICE, std::string(op) + ": device error " + std::to_string(err));
+    }
+}
+
+void launch_refresh(gr_ctx* c, int force) {
+    hipLaunchKernelGGL(refresh_tracks_kernel, dim3(c->tracks), dim3(kRefreshThreads), size_t(c->maxlen) * sizeof(double),
+                       c->stream, c->g, c->capacity, c->demand, c->vcost, c->presum, (float*)nullptr, c->track_dirty,
+                       force, c->cp);
+    hip_check(hipGetLastError(), "refresh_tracks_kernel");
+}
+
+CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
+    CommitArgs a;
+    a.g = c->g;
+    a.demand = c->demand;
+    a.timestamp = c->timestamp;
+    a.track_dirty = c->track_dirty;
+    a.stats = c->stats;
+    a.rs = {c->routes, c->pin_acc_num};
+    a.net_ids = c->net_ids;
+    a.slots = nullptr;
+    a.begin = begin;
+    a.count = count;
+    a.stamp = stamp;
+    a.K = K;
+    return a;
+}
+
+void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K) {
+    if (!count) return;
+    const CommitArgs a = commit_args(c, begin, count, stamp, K);
+    const int blocks = std::min((count + 3) / 4, 4096);
+    hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    hipLaunchKernelGGL(commit_vias_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    hip_check(hipGetLastError(), "commit kernels");
+}
+
+// Per batch: stage slots reordered so that small trees come first, and the
+// node capacity needed by each of the two groups.
+struct BatchLaunch {
+    int begin, small, large, nmax_small, nmax_large;
+};
+
+BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots) {
+    BatchLaunch b{begin, 0, 0, 1, 1};
+    std::vector<int> big;
+    for (int s = begin; s < begin + count; ++s) {
+        const int n = c->node_off_h[s + 1] - c->node_off_h[s];
+        if (n <= kSmallTree) {
+            slots[begin + b.small++] = s;
+            b.nmax_small = std::max(b.nmax_small, n);
+        } else {
+            big.push_back(s);
+            b.nmax_large = std::max(b.nmax_large, n);
+        }
+    }
+    b.large = int(big.size());
+    for (int i = 0; i < b.large; ++i) slots[begin + b.small + i] = big[i];
+    return b;
+}
+
+constexpr size_t kLdsLimit = 64 * 1024;
+
+void ensure_gscratch(gr_ctx* c, size_t bytes) {
+    if (bytes <= c->gscratch_bytes) return;
+    c->mem.release(c->gscratch);
+    c->gscratch = c->mem.alloc<double>((bytes + 7) / 8);
+    c->gscratch_bytes = bytes;
+}
+
+void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
+    LshapeArgs a;
+    a.g = c->g;
+    a.vcost = c->vcost;
+    a.presum = c->presum;
+    a.rs = {c->routes, c->pin_acc_num};
+    a.net_ids = c->net_ids;
+    a.node_off = c->node_off;
+    a.nodes = c->nodes;
+    a.parents = c->parents;
+    a.slots = c->slots;
+    a.gscratch = c->gscratch;
+    a.error = c->error;
+    if (b.small) {
+        a.begin = b.begin;
+        a.count = b.small;
+        a.nmax = b.nmax_small;
+        hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.small), dim3(kWave), lshape_scratch_bytes(c->g.L, a.nmax),
+                           c->stream, a);
+    }
+    if (b.large) {
+        a.begin = b.begin + b.small;
+        a.count = b.large;
+        a.nmax = b.nmax_large;
+        const size_t bytes = lshape_scratch_bytes(c->g.L, a.nmax);
+        if (bytes <= kLdsLimit) {
+            hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.large), dim3(kWave), bytes, c->stream, a);
+        } else {
+            ensure_gscratch(c, ((bytes + 7) / 8) * 8 * size_t(b.large));
+            a.gscratch = c->gscratch;
+            hipLaunchKernelGGL(lshape_dp_global_kernel, dim3(b.large), dim3(kWave), 0, c->stream, a);
+        }
+    }
+    hip_check(hipGetLastError(), "lshape_dp kernels");
+}
+
+void require_stage(gr_ctx* c, int begin, int count) {
+    if (!c->net_ids) throw GrError(GR_E_STATE, "no L-shape stage topology uploaded");
+    if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
+    if (begin < 0 || count < 0 || begin + count > c->stage_count) throw GrError(GR_E_ARG, "slot range outside the stage");
+}
+
+}  // namespace
+
+extern "C" {
+
+int gr_device_name(char* buffer, int size) {
+    return guarded([&] {
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (!count) throw GrError(GR_E_HIP, "no HIP device");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+        snprintf(buffer, size_t(size), "%s %s", prop.name, prop.gcnArchName);
+    });
+}
+
+int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
+    return guarded([&] {
+        if (!grid || !out) throw GrError(GR_E_ARG, "null argument");
+        if (grid->L < 1 || grid->L > kMaxLayers) throw GrError(GR_E_ARG, "routing layers must be 1..16");
+        if (grid->X < 2 || grid->Y < 2) throw GrError(GR_E_ARG, "grid must be at least 2 x 2");
+        if (size_t(grid->L) * grid->X * grid->Y > size_t(INT_MAX)) throw GrError(GR_E_ARG, "grid exceeds 2^31 cells");
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (!count) throw GrError(GR_E_HIP, "no HIP device");
+        auto c = std::make_unique<gr_ctx>();
+        hip_check(hipSetDevice(0), "hipSetDevice");
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
+        c->cus = prop.multiProcessorCount;
+        hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        DeviceGrid& g = c->g;
+        g.L = grid->L, g.X = grid->X, g.Y = grid->Y, g.DIR = grid->DIR & 1, g.XY = grid->X * grid->Y;
+        g.unit_wire = grid->unit_length_wire_cost;
+        g.unit_via = grid->unit_via_cost;
+        for (int l = 0; l < kMaxLayers; ++l) g.short_cost[l] = l < g.L ? grid->unit_length_short_costs[l] : 0.0;
+        int t = 0;
+        for (int l = 0; l < g.L; ++l) {
+            g.track_begin[l] = t;
+            t += layer_dir(g, l) ? g.X : g.Y;
+        }
+        for (int l = g.L; l <= kMaxLayers; ++l) g.track_begin[l] = t;
+        c->tracks = t;
+        c->maxlen = std::max(g.X, g.Y);
+        c->cells = size_t(g.L) * g.XY;
+        int* xl = c->mem.alloc<int>(g.X - 1);
+        int* yl = c->mem.alloc<int>(g.Y - 1);
+        upload(xl, grid->x_edge_len, sizeof(int) * (g.X - 1), c->stream);
+        upload(yl, grid->y_edge_len, sizeof(int) * (g.Y - 1), c->stream);
+        g.x_edge_len = xl, g.y_edge_len = yl;
+        c->capacity = c->mem.alloc<float>(c->cells);
+        c->demand = c->mem.alloc<float>(c->cells);
+        c->vcost = c->mem.alloc<float>(c->cells);
+        c->presum = c->mem.alloc<double>(c->cells);
+        c->timestamp = c->mem.alloc<int>(c->cells);
+        c->track_dirty = c->mem.alloc<unsigned>(c->tracks);
+        c->stats = c->mem.alloc<Stats>(1);
+        c->error = c->mem.alloc<int>(1);
+        std::vector<float> cap;
+        to_track_major(g, grid->capacity, cap);
+        upload(c->capacity, cap.data(), sizeof(float) * c->cells, c->stream);
+        hip_check(hipMemsetAsync(c->demand, 0, sizeof(float) * c->cells, c->stream), "clear demand");
+        hip_check(hipMemsetAsync(c->timestamp, 0, sizeof(int) * c->cells, c->stream), "clear timestamps");
+        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream), "clear statistics");
+        hip_check(hipMemsetAsync(c->error, 0, sizeof(int), c->stream), "clear error");
+        std::vector<unsigned> ones(c->tracks, 1u);
+        upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "context setup");
+        *out = c.release();
+    });
+}
+
+int gr_ctx_destroy(gr_ctx* ctx) {
+    return guarded([&] {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int gr_ctx_set_routes(gr_ctx* c, int net_count, const int* pin_acc_num) {
+    return guarded([&] {
+        if (!c || net_count < 0 || !pin_acc_num) throw GrError(GR_E_ARG, "invalid route storage arguments");
+        c->mem.release(c->routes);
+        c->mem.release(c->pin_acc_num);
+        c->mem.release(c->of_flags);
+        c->net_count = net_count;
+        c->pin_acc_num_h.assign(pin_acc_num, pin_acc_num + net_count + 1);
+        c->route_words = (long long)kRoutePerPin * pin_acc_num[net_count] + 1;
+        c->pin_acc_num = c->mem.alloc<int>(net_count + 1);
+        upload(c->pin_acc_num, pin_acc_num, sizeof(int) * (net_count + 1), c->stream);
+        c->routes = c->mem.alloc<int>(size_t(c->route_words));
+        hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
+        hip_check(hipStreamSynchronize(c->stream), "route storage");
+    });
+}
+
+int gr_cost_refresh(gr_ctx* c) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        launch_refresh(c, 0);
+        hip_check(hipStreamSynchronize(c->stream), "gr_cost_refresh");
+    });
+}
+
+int gr_cost_refresh_full(gr_ctx* c) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        launch_refresh(c, 1);
+        hip_check(hipStreamSynchronize(c->stream), "gr_cost_refresh_full");
+    });
+}
+
+int gr_cost_download(gr_ctx* c, float* wcost, float* vcost, double* presum) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (wcost) {
+            // wcost is not kept resident; recompute it into a temporary with
+            // the refresh kernel's own arithmetic (vcost/presum rewritten with
+            // identical values)
+            float* tmp = c->mem.alloc<float>(c->cells);
+            hipLaunchKernelGGL(refresh_tracks_kernel, dim3(c->tracks), dim3(kRefreshThreads),
+                               size_t(c->maxlen) * sizeof(double), c->stream, c->g, c->capacity, c->demand, c->vcost,
+                               c->presum, tmp, c->track_dirty, 1, c->cp);
+            hip_check(hipGetLastError(), "refresh for wcost");
+            std::vector<float> h(c->cells);
+            download(h.data(), tmp, sizeof(float) * c->cells, c->stream);
+            c->mem.release(tmp);
+            from_track_major(c->g, h, wcost);
+        }
+        if (vcost) {
+            std::vector<float> h(c->cells);
+            download(h.data(), c->vcost, sizeof(float) * c->cells, c->stream);
+            from_track_major(c->g, h, vcost);
+        }
+        if (presum) {
+            std::vector<double> h(c->cells);
+            download(h.data(), c->presum, sizeof(double) * c->cells, c->stream);
+            from_track_major(c->g, h, presum);
+        }
+    });
+}
+
+int gr_demand_upload(gr_ctx* c, const float* demand) {
+    return guarded([&] {
+        if (!c || !demand) throw GrError(GR_E_ARG, "null argument");
+        std::vector<float> h;
+        to_track_major(c->g, demand, h);
+        upload(c->demand, h.data(), sizeof(float) * c->cells, c->stream);
+        std::vector<unsigned> ones(c->tracks, 1u);
+        upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "demand upload");
+    });
+}
+
+int gr_demand_download(gr_ctx* c, float* demand) {
+    return guarded([&] {
+        if (!c || !demand) throw GrError(GR_E_ARG, "null argument");
+        std::vector<float> h(c->cells);
+        download(h.data(), c->demand, sizeof(float) * c->cells, c->stream);
+        from_track_major(c->g, h, demand);
+    });
+}
+
+int gr_timestamp_download(gr_ctx* c, int* ts) {
+    return guarded([&] {
+        if (!c || !ts) throw GrError(GR_E_ARG, "null argument");
+        std::vector<int> h(c->cells);
+        download(h.data(), c->timestamp, sizeof(int) * c->cells, c->stream);
+        from_track_major(c->g, h, ts);
+    });
+}
+
+int gr_stats_download(gr_ctx* c, double* wl, long long* vias) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        Stats s;
+        download(&s, c->stats, sizeof(Stats), c->stream);
+        if (wl) *wl = s.total_wirelength;
+        if (vias) *vias = (long long)s.total_vias;
+    });
+}
+
+int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_off, const int* nodes,
+                     const int* parents) {
+    return guarded([&] {
+        if (!c || count < 0 || !net_ids || !node_off || !nodes || !parents) throw GrError(GR_E_ARG, "null argument");
+        for (int k = 0; k < count; ++k) {
+            if (net_ids[k] < 0 || net_ids[k] >= c->net_count) throw GrError(GR_E_ARG, "net id outside route storage");
+            const int n = node_off[k + 1] - node_off[k];
+            if (n < 1) throw GrError(GR_E_ARG, "empty tree");
+            if (parents[node_off[k]] != -1) throw GrError(GR_E_ARG, "tree root must have parent -1");
+            for (int i = 1; i < n; ++i)
+                if (parents[node_off[k] + i] < 0 || parents[node_off[k] + i] >= i)
+                    throw GrError(GR_E_ARG, "parents must precede children (DFS preorder)");
+            for (int i = 0; i < n; ++i) {
+                const int v = nodes[node_off[k] + i];
+                if (v < 0 || v >= (c->g.L + 1) * c->g.XY) throw GrError(GR_E_ARG, "node index outside the grid");
+            }
+        }
+        for (int* p : {c->net_ids, c->node_off, c->nodes, c->parents, c->slots}) {
+            int* q = p;
+            c->mem.release(q);
+        }
+        c->net_ids = c->node_off = c->nodes = c->parents = c->slots = nullptr;
+        const int total = node_off[count];
+        c->stage_count = count;
+        c->node_off_h.assign(node_off, node_off + count + 1);
+        c->net_ids = c->mem.alloc<int>(count);
+        c->node_off = c->mem.alloc<int>(count + 1);
+        c->nodes = c->mem.alloc<int>(total);
+        c->parents = c->mem.alloc<int>(total);
+        c->slots = c->mem.alloc<int>(count);
+        upload(c->net_ids, net_ids, sizeof(int) * count, c->stream);
+        upload(c->node_off, node_off, sizeof(int) * (count + 1), c->stream);
+        upload(c->nodes, nodes, sizeof(int) * total, c->stream);
+        upload(c->parents, parents, sizeof(int) * total, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "stage upload");
+    });
+}
+
+int gr_lshape_batch(gr_ctx* c, int begin, int count, int stamp) {
+    (void)stamp;
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        require_stage(c, begin, count);
+        std::vector<int> slots(c->stage_count);
+        const BatchLaunch b = plan_batch(c, begin, count, slots);
+        upload(c->slots + begin, slots.data() + begin, sizeof(int) * count, c->stream);
+        launch_lshape(c, b);
+        hip_check(hipStreamSynchronize(c->stream), "gr_lshape_batch");
+        check_device_error(c, "gr_lshape_batch");
+    });
+}
+
+int gr_commit_batch(gr_ctx* c, int begin, int count, int stamp, int K) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        require_stage(c, begin, count);
+        launch_commit(c, begin, count, stamp, K);
+        hip_check(hipStreamSynchronize(c->stream), "gr_commit_batch");
+    });
+}
+
+int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, int mode, double* elapsed_ms) {
+    return guarded([&] {
+        if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
+        if (mode != 0) throw GrError(GR_E_ARG, "execution mode not available in this build");
+        require_stage(c, 0, batch_off[nbatch]);
+        std::vector<int> slots(c->stage_count);
+        std::vector<BatchLaunch> plan(nbatch);
+        for (int b = 0; b < nbatch; ++b) {
+            if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
+            plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+        }
+        upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "stage plan");
+        hip_check(hipDeviceSynchronize(), "before DAG interval");
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int b = 0; b < nbatch; ++b) {
+            launch_refresh(c, 0);
+            launch_lshape(c, plan[b]);
+            launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
+        }
+        hip_check(hipStreamSynchronize(c->stream), "DAG interval");
+        const auto t1 = std::chrono::steady_clock::now();
+        if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        check_device_error(c, "gr_lshape_run");
+    });
+}
+
+int gr_routes_download(gr_ctx* c, int* routes, long long words) {
+    return guarded([&] {
+        if (!c || !routes || words < 0) throw GrError(GR_E_ARG, "invalid arguments");
+        if (words > c->route_words) throw GrError(GR_E_ARG, "more words than the route storage holds");
+        download(routes, c->routes, sizeof(int) * size_t(words), c->stream);
+    });
+}
+
+int gr_ripup(gr_ctx* c, int threshold, unsigned char* is_of_net) {
+    return guarded([&] {
+        if (!c || !is_of_net) throw GrError(GR_E_ARG, "null argument");
+        if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
+        if (!c->of_sum) c->of_sum = c->mem.alloc<int>(c->cells);
+        if (!c->of_flags) c->of_flags = c->mem.alloc<unsigned char>(c->net_count);
+        RipupArgs a{c->g, c->capacity, c->demand, c->of_sum, threshold};
+        hipLaunchKernelGGL(overflow_prefix_kernel, dim3(c->tracks), dim3(256), 0, c->stream, a);
+        if (c->net_count)
+            hipLaunchKernelGGL(overflow_nets_kernel, dim3((c->net_count + 255) / 256), dim3(256), 0, c->stream, c->g,
+                               c->of_sum, RouteStore{c->routes, c->pin_acc_num}, c->net_count, c->of_flags);
+        hip_check(hipGetLastError(), "ripup kernels");
+        download(is_of_net, c->of_flags, size_t(c->net_count), c->stream);
+    });
+}
+
+int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int* hs_off, const int* hsegs,
+                  const int* vs_off, const int* vsegs, int limit, int sparsity, int* color, int* num_batches) {
+    return guarded([&] {
+        if (!c || n < 0 || !pt_off || !hs_off || !vs_off || !color || !num_batches)
+            throw GrError(GR_E_ARG, "null argument");
+        if (sparsity < 1) throw GrError(GR_E_ARG, "sparsity must be positive");
+        if (limit < 1) throw GrError(GR_E_ARG, "batch limit must be positive");
+        *num_batches = 0;
+        if (!n) return;
+        const int X = c->g.X, Y = c->g.Y, XY = c->g.XY;
+        for (int q = 0; q < pt_off[n]; ++q)
+            if (pts[q] < 0 || pts[q] >= XY) throw GrError(GR_E_ARG, "batching point outside the grid");
+        for (int s = 0; s < hs_off[n]; ++s)
+            if (hsegs[2 * s] < 0 || hsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
+        for (int s = 0; s < vs_off[n]; ++s)
+            if (vsegs[2 * s] < 0 || vsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
+        DeviceArena tmp;
+        hipStream_t st = c->stream;
+        auto up = [&](const int* h, size_t count) {
+            int* d = tmp.alloc<int>(count);
+            if (count) upload(d, h, sizeof(int) * count, st);
+            return d;
+        };
+        BatchGeom b;
+        b.n = n, b.X = X, b.Y = Y, b.sparsity = sparsity;
+        b.pt_off = up(pt_off, n + 1);
+        b.pts = up(pts, pt_off[n]);
+        b.hs_off = up(hs_off, n + 1);
+        b.hsegs = up(hsegs, 2 * size_t(hs_off[n]));
+        b.vs_off = up(vs_off, n + 1);
+        b.vsegs = up(vsegs, 2 * size_t(vs_off[n]));
+        int* mark_count = tmp.alloc<int>(n + 1);
+        int* mark_off = tmp.alloc<int>(n + 1);
+        hip_check(hipMemsetAsync(mark_count, 0, sizeof(int) * (n + 1), st), "clear mark counts");
+        const int grid_n = (n + 255) / 256;
+        hipLaunchKernelGGL(count_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_count);
+        size_t temp_bytes = 0, t2 = 0;
+        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, mark_count, mark_off, n + 1, st), "scan size");
+        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, mark_count, mark_off, XY + 1, st), "scan size");
+        temp_bytes = std::max(temp_bytes, t2);
+        void* temp = tmp.alloc<char>(temp_bytes);
+        hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, mark_count, mark_off, n + 1, st), "scan marks");
+        int total_marks = 0;
+        download(&total_marks, mark_off + n, sizeof(int), st);
+        if (total_marks < 0) throw GrError(GR_E_CAPACITY, "more than 2^31 batching marks");
+        int* mark_cell = tmp.alloc<int>(total_marks);
+        int* cell_count = tmp.alloc<int>(XY + 1);
+        int* cell_off = tmp.alloc<int>(XY + 1);
+        int* cell_cursor = tmp.alloc<int>(XY);
+        int* cell_list = tmp.alloc<int>(total_marks);
+        hip_check(hipMemsetAsync(cell_count, 0, sizeof(int) * (XY + 1), st), "clear cell counts");
+        hip_check(hipMemsetAsync(cell_cursor, 0, sizeof(int) * XY, st), "clear cell cursors");
+        hipLaunchKernelGGL(emit_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, cell_count, mark_cell);
+        hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, cell_count, cell_off, XY + 1, st), "scan cells");
+        hipLaunchKernelGGL(fill_lists_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, mark_cell, cell_off,
+                           cell_cursor, cell_list);
+        constexpr int kMaxColors = 4096;
+        int* d_color = tmp.alloc<int>(n);
+        int* d_count = tmp.alloc<int>(kMaxColors);
+        int* d_misc = tmp.alloc<int>(2);  // next, max color
+        hipLaunchKernelGGL(fill_int_kernel, dim3(std::min((n + 255) / 256, 4096)), dim3(256), 0, st, d_color, -1,
+                           (long long)n);
+        hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
+        const int misc[2] = {0, -1};
+        upload(d_misc, misc, sizeof(misc), st);
+        ColorArgs ca;
+        ca.b = b;
+        ca.cell_off = cell_off;
+        ca.cell_list = cell_list;
+        ca.color = d_color;
+        ca.color_count = d_count;
+        ca.next = d_misc;
+        ca.max_color = d_misc + 1;
+        ca.error = c->error;
+        ca.spin_limit = 1ll << 22;
+        const int waves = std::max(1, std::min(c->cus * 8, n));
+        hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), 0, st, ca);
+        hip_check(hipGetLastError(), "batching kernels");
+        download(color, d_color, sizeof(int) * n, st);
+        int maxc[2];
+        download(maxc, d_misc, sizeof(maxc), st);
+        check_device_error(c, "gr_batch_plan");
+        const int nb = maxc[1] + 1;
+        std::vector<int> counts(nb);
+        download(counts.data(), d_count, sizeof(int) * nb, st);
+        for (int v : counts)
+            if (v > limit)
+                throw GrError(GR_E_CAPACITY, "a batch reaches the batch size limit; exact limited batching is not supported");
+        *num_batches = nb;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,152 @@
+// C ABI of the host front end: design I/O, synthetic instances, preparation
+// of the stage topology and .out output.
+#include <cstdio>
+#include <memory>
+#include <string>
+
+#include "../csrc/gr_status.hpp"
+#include "design.hpp"
+
+namespace grotor {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& message) { g_last_error = message; }
+}  // namespace grotor
+
+using namespace grotor;
+
+struct gr_design {
+    Design d;
+    bool prepared = false;
+    std::vector<int> order, node_off, nodes, parents, pt_off, pts, hs_off, hsegs, vs_off, vsegs;
+    std::vector<float> capacity;  // routing layers, IDX order
+    std::vector<double> short_costs;
+};
+
+extern "C" {
+
+const char* gr_last_error(void) { return g_last_error.c_str(); }
+
+int gr_design_read(const char* cap_path, const char* net_path, gr_design** out) {
+    return guarded([&] {
+        if (!cap_path || !net_path || !out) throw GrError(GR_E_ARG, "null argument");
+        auto d = std::make_unique<gr_design>();
+        try {
+            read_design(cap_path, net_path, d->d);
+        } catch (const std::invalid_argument& e) {
+            throw GrError(GR_E_IO, e.what());
+        }
+        *out = d.release();
+    });
+}
+
+int gr_design_generate(int config, unsigned long long seed, double net_scale, int grid_side, gr_design** out) {
+    return guarded([&] {
+        if (!out || net_scale <= 0) throw GrError(GR_E_ARG, "invalid arguments");
+        auto d = std::make_unique<gr_design>();
+        GenSpec spec;
+        spec.config = config;
+        spec.seed = seed;
+        spec.net_scale = net_scale;
+        spec.grid_override = grid_side;
+        generate_design(spec, d->d);
+        *out = d.release();
+    });
+}
+
+int gr_design_write(const gr_design* design, const char* cap_path, const char* net_path) {
+    return guarded([&] {
+        if (!design) throw GrError(GR_E_ARG, "null design");
+        if (cap_path) write_cap(cap_path, design->d.grid);
+        if (net_path) write_net(net_path, design->d);
+    });
+}
+
+int gr_design_destroy(gr_design* design) {
+    delete design;
+    return GR_OK;
+}
+
+int gr_design_prepare(gr_design* g, int threads) {
+    return guarded([&] {
+        if (!g) throw GrError(GR_E_ARG, "null design");
+        Design& d = g->d;
+        select_pins(d, threads);
+        build_subnets(d, threads);
+        g->order = hpwl_order(d);
+        const int n = int(g->order.size());
+        g->node_off.assign(1, 0);
+        g->pt_off.assign(1, 0);
+        g->hs_off.assign(1, 0);
+        g->vs_off.assign(1, 0);
+        g->nodes.clear(), g->parents.clear(), g->pts.clear(), g->hsegs.clear(), g->vsegs.clear();
+        std::vector<int> nodes, parents;
+        for (int k = 0; k < n; ++k) {
+            const Subnet& s = d.subnets[g->order[k]];
+            preorder(s, nodes, parents);
+            g->nodes.insert(g->nodes.end(), nodes.begin(), nodes.end());
+            g->parents.insert(g->parents.end(), parents.begin(), parents.end());
+            g->node_off.push_back(int(g->nodes.size()));
+            g->pts.insert(g->pts.end(), s.points.begin(), s.points.end());
+            g->pt_off.push_back(int(g->pts.size()));
+            for (auto& e : s.hsegs) g->hsegs.push_back(e.first), g->hsegs.push_back(e.second);
+            g->hs_off.push_back(int(g->hsegs.size() / 2));
+            for (auto& e : s.vsegs) g->vsegs.push_back(e.first), g->vsegs.push_back(e.second);
+            g->vs_off.push_back(int(g->vsegs.size() / 2));
+        }
+        const Grid& gr = d.grid;
+        const int L = gr.routing_layers();
+        g->capacity.assign(gr.capacity.begin() + size_t(gr.X) * gr.Y, gr.capacity.end());
+        g->short_costs.assign(gr.unit_length_short_costs.begin() + 1, gr.unit_length_short_costs.end());
+        (void)L;
+        g->prepared = true;
+    });
+}
+
+int gr_design_topology(gr_design* g, gr_topology* t) {
+    return guarded([&] {
+        if (!g || !t) throw GrError(GR_E_ARG, "null argument");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        const Grid& gr = g->d.grid;
+        t->L = gr.routing_layers();
+        t->X = gr.X;
+        t->Y = gr.Y;
+        t->DIR = gr.DIR();
+        t->nets = int(g->order.size());
+        t->subnets = int(g->d.subnets.size());
+        t->order = g->order.data();
+        t->node_off = g->node_off.data();
+        t->nodes = g->nodes.data();
+        t->parents = g->parents.data();
+        t->pt_off = g->pt_off.data();
+        t->pts = g->pts.data();
+        t->hs_off = g->hs_off.data();
+        t->hsegs = g->hsegs.data();
+        t->vs_off = g->vs_off.data();
+        t->vsegs = g->vsegs.data();
+        t->pin_acc_num = g->d.pin_acc_num.data();
+        t->capacity = g->capacity.data();
+        t->unit_length_short_costs = g->short_costs.data();
+        t->x_edge_len = gr.x_edge_len.data();
+        t->y_edge_len = gr.y_edge_len.data();
+        t->unit_length_wire_cost = gr.unit_length_wire_cost;
+        t->unit_via_cost = gr.unit_via_cost;
+    });
+}
+
+int gr_design_write_output(const gr_design* g, const int* routes, const char* out_path) {
+    return guarded([&] {
+        if (!g || !routes || !out_path) throw GrError(GR_E_ARG, "null argument");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        const auto& acc = g->d.pin_acc_num;
+        const std::string text =
+            format_output(g->d, [&](int id) { return routes + (size_t)acc[id] * 12; });
+        FILE* f = fopen(out_path, "w");
+        if (!f) throw GrError(GR_E_IO, std::string("cannot open output: ") + out_path);
+        const size_t written = fwrite(text.data(), 1, text.size(), f);
+        if (fclose(f) || written != text.size()) throw GrError(GR_E_IO, "failed to write routing output");
+    });
+}
+
+}  // extern "C"

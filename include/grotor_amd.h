@@ -1,0 +1,185 @@
+/*
+ * grotor_amd: C ABI of the MI355X (gfx950) routing hot path.
+ *
+ * The reference routes through header-included C++/CUDA with no FFI; its
+ * de-facto operator boundary is the set of host entry points its stage loops
+ * call.  Each function here replaces one of them (cited file:line in the
+ * reference tree) and keeps its argument meaning and error behaviour, with
+ * plain pointers and sizes in place of RouterState / RoutingDeviceView.
+ *
+ * Conventions
+ *  - Every function returns 0 (GR_OK) or a GR_E_* status; gr_last_error()
+ *    returns the message of the last failure on the calling thread (the
+ *    reference throws std::runtime_error("<op>: <hip error>"),
+ *    include/cuda_memory.hpp:12-16).
+ *  - Grid arrays exchanged through this ABI use the reference index
+ *    IDX(l, x, y) = (l * X + x) * Y + y over the L routing layers
+ *    (include/router_state.hpp:267).  Route words use the same encoding.
+ *  - Host pointers are read or written synchronously; device state lives in
+ *    an opaque context that owns all device memory
+ *    (include/cuda_memory.hpp:17-53) and one HIP stream.
+ *  - A context needs a HIP device; without one gr_ctx_create fails with
+ *    GR_E_HIP.  There is no CPU fallback.
+ */
+#ifndef GROTOR_AMD_H
+#define GROTOR_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GR_OK = 0,
+    GR_E_ARG = 1,      /* invalid argument (std::invalid_argument in the reference) */
+    GR_E_HIP = 2,      /* HIP runtime failure, or no device */
+    GR_E_CAPACITY = 3, /* route storage or batch limit exceeded (device assert) */
+    GR_E_STATE = 4,    /* call out of order (e.g. DP before topology upload) */
+    GR_E_DEVICE = 5,   /* device-side check failed (no finite DP solution, spin limit) */
+    GR_E_IO = 6        /* file input/output */
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char *gr_last_error(void);
+
+/* Name and architecture of device 0, e.g. "AMD Instinct MI355X gfx950". */
+int gr_device_name(char *buffer, int size);
+
+typedef struct gr_ctx gr_ctx;
+
+/* Routing grid of L routing layers (metal layer l + 1 of the CAP file).
+ * Replaces the grid half of cudb::build_cuda_database
+ * (src/preprocess.hpp:1625-1785). */
+typedef struct gr_grid_desc {
+    int L, X, Y;
+    int DIR;                                /* direction of routing layer 0: 0 = horizontal */
+    double unit_length_wire_cost;           /* db::unit_length_wire_cost */
+    double unit_via_cost;                   /* db::unit_via_cost */
+    const double *unit_length_short_costs;  /* [L], of metal layers 1..L */
+    const int *x_edge_len;                  /* [X-1] */
+    const int *y_edge_len;                  /* [Y-1] */
+    const float *capacity;                  /* [L*X*Y], IDX order */
+} gr_grid_desc;
+
+/* Create / destroy a device context.  Demand and timestamps start at zero,
+ * the first gr_cost_refresh computes every track
+ * (graph::initialize_incremental_cost, src/cost/incremental_update.cuh:7-42). */
+int gr_ctx_create(const gr_grid_desc *grid, gr_ctx **out);
+int gr_ctx_destroy(gr_ctx *ctx);
+
+/* Route storage: ROUTE_PER_PIN = 12 words per pin, net i at word
+ * 12 * pin_acc_num[i] (include/router_state.hpp:31, src/preprocess.hpp:1699-1704). */
+int gr_ctx_set_routes(gr_ctx *ctx, int net_count, const int *pin_acc_num);
+
+/* Cost refresh: wcost, vcost and the Fenwick-ordered prefix presum of every
+ * track whose demand changed since the last refresh
+ * (graph::refresh_cost, src/cost/incremental_update.cuh:98-132). */
+int gr_cost_refresh(gr_ctx *ctx);
+/* Recompute every track regardless of dirty state (graph::update_cost plus a
+ * full prefix rebuild, src/cost/full_update.cuh:93-104). */
+int gr_cost_refresh_full(gr_ctx *ctx);
+/* Copy the current costs to the host in IDX order; any pointer may be NULL.
+ * wcost is recomputed from the current demand on request. */
+int gr_cost_download(gr_ctx *ctx, float *wcost, float *vcost, double *presum);
+
+int gr_demand_upload(gr_ctx *ctx, const float *demand);   /* IDX order; marks every track dirty */
+int gr_demand_download(gr_ctx *ctx, float *demand);
+int gr_timestamp_download(gr_ctx *ctx, int *timestamp);
+/* Statistics accumulated by commits (RoutingStatistics, include/router_state.hpp:17-20). */
+int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vias);
+
+/* Exact first-fit greedy batching of n nets given in priority order
+ * (cudb::generate_batches_rsmt, src/batching/generate_batches_cpu.hpp:7-73;
+ * timed wrapper generate_batches_cpu_timed, src/batching/generate_batches.hpp:14-25).
+ * points: 2D cells x*Y + y; hsegs / vsegs: (first, second) pairs of 2D cells.
+ * Writes the batch index of net k to color[k] and the batch count.  Batches
+ * are exactly the reference's lists: batch b holds the nets with color b in
+ * priority order.  Runs on the GPU (bin-grid conflict lists + in-order
+ * colouring); `limit` is honoured exactly. */
+int gr_batch_plan(gr_ctx *ctx, int n, const int *pt_off, const int *pts,
+                  const int *hs_off, const int *hsegs, const int *vs_off, const int *vsegs,
+                  int limit, int sparsity, int *color, int *num_batches);
+
+/* L-shape stage topology, uploaded once per stage (src/routing/lshape.cuh:126-138):
+ * net_ids[k] is the subnet solved at stage slot k; its DFS-preorder tree is
+ * nodes[node_off[k] .. node_off[k+1]) with parents (parents[first] = -1).
+ * A node is a 3D index whose layer is the pin's fixed routing layer or L for
+ * a Steiner node (src/preprocess.hpp:127-129). */
+int gr_lshape_upload(gr_ctx *ctx, int count, const int *net_ids, const int *node_off,
+                     const int *nodes, const int *parents);
+
+/* DP of stage slots [begin, begin + count) against the current costs; writes
+ * routes, does not commit (Lshape_route_warp launch, src/routing/lshape.cuh:168). */
+int gr_lshape_batch(gr_ctx *ctx, int begin, int count, int stamp);
+
+/* Commit the routes of stage slots [begin, begin + count): wires, then vias
+ * (graph::commit_batch_demand, src/routing/demand.cuh:313-319). */
+int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
+
+/* The whole L-shape batch loop (src/routing/lshape.cuh:148-180): for each
+ * batch b, stamp = first_stamp + b, refresh costs, solve, commit.
+ * batch_off has nbatch + 1 entries over the uploaded stage slots.
+ * mode: 0 = one launch per step (the reference's loop),
+ *       1 = the per-batch launches replayed from a captured HIP graph,
+ *       2 = one persistent kernel for the whole stage.
+ * *elapsed_ms receives the synchronized interval (RoutingDagTimer,
+ * include/router_state.hpp:238-265). */
+int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,
+                  int mode, double *elapsed_ms);
+
+/* Copy `words` route words (from word 0) to the host (RouterState::download_routes,
+ * include/router_state.hpp:215-222). */
+int gr_routes_download(gr_ctx *ctx, int *routes, long long words);
+
+/* Rip-up classification over all nets of the route storage
+ * (graph::ripup, src/routing/congestion.cuh:114-136): is_of_net[i] = 1 if net
+ * i's route crosses more than 10 edges with capacity + threshold <= demand. */
+int gr_ripup(gr_ctx *ctx, int threshold, unsigned char *is_of_net);
+
+/* ---- host front end and pipeline (src/main.cu, src/pipeline.hpp) ---- */
+
+typedef struct gr_design gr_design;
+
+/* Read reference-format CAP/NET files (db::read, src/input.hpp:247-419). */
+int gr_design_read(const char *cap_path, const char *net_path, gr_design **out);
+/* Synthetic instance of workload configuration 1..6; net_scale multiplies
+ * the configuration's net count, grid_side (0 = default) overrides the side. */
+int gr_design_generate(int config, unsigned long long seed, double net_scale, int grid_side,
+                       gr_design **out);
+int gr_design_write(const gr_design *design, const char *cap_path, const char *net_path);
+int gr_design_destroy(gr_design *design);
+
+/* Access-point selection, trees, subnet split and batching geometry
+ * (net::init src/input.hpp:38-243; prepare_routing_nets
+ * src/preprocess.hpp:1448-1623; net::construct_rsmt src/preprocess.hpp:72-150). */
+int gr_design_prepare(gr_design *design, int threads);
+
+/* Stage topology of the prepared design in priority order (subnets sorted by
+ * hpwl descending, src/routing/lshape.cuh:53-55).  Arrays are owned by the
+ * design and stay valid until it is destroyed or prepared again. */
+typedef struct gr_topology {
+    int L, X, Y, DIR;
+    int nets;               /* stage slots */
+    int subnets;            /* routed subnets (route storage units) */
+    const int *order;       /* [nets] subnet id per priority slot */
+    const int *node_off;    /* [nets+1] */
+    const int *nodes;       /* DFS preorder 3D nodes */
+    const int *parents;
+    const int *pt_off, *pts;    /* batching points (2D) */
+    const int *hs_off, *hsegs;  /* pairs */
+    const int *vs_off, *vsegs;
+    const int *pin_acc_num;     /* [subnets+1] */
+    const float *capacity;      /* [L*X*Y] routing layers, IDX order */
+    const double *unit_length_short_costs; /* [L] */
+    const int *x_edge_len, *y_edge_len;
+    double unit_length_wire_cost, unit_via_cost;
+} gr_topology;
+int gr_design_topology(gr_design *design, gr_topology *out);
+
+/* Write the .out solution for the prepared design given its route words
+ * (graph::output_nets, src/output.hpp:79-112). */
+int gr_design_write_output(const gr_design *design, const int *routes, const char *out_path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,109 @@
+/*
+ * CPU oracle for the GRotor routing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path
+ * is compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library never links or calls it.
+ *
+ * Every function restates the reference algorithm it names (file:line in the
+ * reference tree) in plain C, single threaded, with -ffp-contract=off so that
+ * every floating-point operation is one IEEE-754 rounding, as the reference's
+ * HIP build compiles it (CMakeLists.txt:37).
+ *
+ * Grid arrays use the reference layout IDX(l, x, y) = (l * X + x) * Y + y
+ * (include/router_state.hpp:267) over the L routing layers (metal layer l + 1).
+ * Layer direction: dir(l) = (l & 1) ^ DIR, 0 = horizontal (edges along x),
+ * 1 = vertical (edges along y) (src/preprocess.hpp:1621).
+ */
+#ifndef GROTOR_ORACLE_H
+#define GROTOR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_grid {
+    int L, X, Y, DIR;
+    double unit_length_wire_cost;
+    double unit_via_cost;
+    const double *unit_length_short_costs; /* [L] */
+    const int *x_edge_len;                 /* [X-1] */
+    const int *y_edge_len;                 /* [Y-1] */
+} ora_grid;
+
+/* Project exponential: binary32 in, binary32 out.  Stands in for the CUDA
+ * intrinsic __expf used by src/cost/full_update.cuh:11-20, whose numerics are
+ * not reproducible here.  Restated identically in the HIP kernels. */
+float ora_expf(float x);
+
+/* src/cost/full_update.cuh:9-32 with of_cost_scale = 1.  `slope` and
+ * `zero_slope` are the 0.5 and 1.5 constants of of_cost_scaled. */
+double ora_of_cost(float capacity, float demand, double slope, double zero_slope);
+double ora_incremental_of_cost(float capacity, float demand, double incre_demand,
+                               double short_cost, double slope, double zero_slope);
+
+/* Full cost refresh: wcost (src/cost/full_update.cuh:34-51), vcost
+ * (:58-83) and the Fenwick-ordered exclusive prefix presum
+ * (src/cost/fenwick.cuh:129-168).  Any of the outputs may be NULL. */
+void ora_cost_refresh(const ora_grid *g, const float *capacity, const float *demand,
+                      float *wcost, float *vcost, double *presum);
+void ora_cost_refresh_slopes(const ora_grid *g, const float *capacity, const float *demand,
+                             double slope, double zero_slope, double via_weight,
+                             float *wcost, float *vcost, double *presum);
+
+/* Fenwick prefix of one track: leaves w[0..len-1] (float), output
+ * presum[0..len-1] (double), restating src/cost/fenwick.cuh:129-168. */
+void ora_fenwick_track(const float *w, int len, double *presum);
+/* Dense Hillis-Steele prefix of one track (src/cost/prefix_scan.cuh:74-114). */
+void ora_dense_prefix_track(const float *w, int len, double *presum);
+
+/* First-fit greedy batching (src/batching/generate_batches_cpu.hpp:7-73).
+ * Nets are given in priority order k = 0..n-1.  points: 2D cells x*Y+y;
+ * hsegs/vsegs: pairs (first, second) of 2D cells.  Writes the batch index of
+ * every net to color[k] and returns the number of batches. */
+int ora_batches(int X, int Y, int n,
+                const int *pt_off, const int *pts,
+                const int *hs_off, const int *hsegs,
+                const int *vs_off, const int *vsegs,
+                int limit, int sparsity, int *color);
+
+/* L-shape tree DP of one net (src/routing/lshape_thread.cuh:18-171 with the
+ * accumulation order of src/routing/lshape_warp.cuh:150-192).
+ * nodes: 3D indices (fixed layer, or L for Steiner); parents: preorder,
+ * parents[0] = -1.  Writes routes[0] = count and the segment words, returns
+ * 0, or -1 if more than cap words would be written, or -2 if no finite
+ * assignment exists. */
+int ora_lshape_net(const ora_grid *g, const float *vcost, const double *presum,
+                   int n, const int *nodes, const int *parents, int *routes, int cap);
+
+/* Demand commit of one batch, wires then vias
+ * (src/routing/demand.cuh:236-319).  routes_of[k] points at net k's words. */
+void ora_commit_batch(const ora_grid *g, float *demand, int *timestamp,
+                      int count, int *const *routes_of, int stamp, int K,
+                      double *total_wirelength, long long *total_vias);
+
+/* Rip-up classification (src/routing/congestion.cuh:8-58, 114-136). */
+void ora_ripup(const ora_grid *g, const float *capacity, const float *demand,
+               int threshold, int net_count, int *const *routes_of,
+               unsigned char *is_of_net);
+
+/* One L-shape stage over a batch schedule (src/routing/lshape.cuh:153-179).
+ * net_ids/node_off/nodes/parents are in execution order; batch_off has
+ * nbatch+1 entries.  route_off[net] is the word offset of that net's route
+ * area and route_cap[net] its size.  The stage refreshes costs before every
+ * batch.  `*stamp` is incremented once per batch.  Returns 0 or a negative
+ * error from ora_lshape_net. */
+int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
+                     int *timestamp, float *wcost, float *vcost, double *presum,
+                     int nbatch, const int *batch_off, const int *net_ids,
+                     const int *node_off, const int *nodes, const int *parents,
+                     const long long *route_off, const int *route_cap, int *routes,
+                     int *stamp, double *total_wirelength, long long *total_vias,
+                     int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+as the checker; the product never imports it.  Grid arrays use the
+reference IDX(l, x, y) order.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+ROUTE_PER_PIN = 12
+
+_lib = None
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class OraGrid(ctypes.Structure):
+    _fields_ = [("L", ctypes.c_int), ("X", ctypes.c_int), ("Y", ctypes.c_int), ("DIR", ctypes.c_int),
+                ("unit_length_wire_cost", ctypes.c_double), ("unit_via_cost", ctypes.c_double),
+                ("unit_length_short_costs", _f64p), ("x_edge_len", _i32p), ("y_edge_len", _i32p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.ora_expf.restype = ctypes.c_float
+        _lib.ora_expf.argtypes = [ctypes.c_float]
+        _lib.ora_of_cost.restype = ctypes.c_double
+        _lib.ora_of_cost.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_double, ctypes.c_double]
+        _lib.ora_incremental_of_cost.restype = ctypes.c_double
+        _lib.ora_incremental_of_cost.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_double]
+    return _lib
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Grid:
+    """Grid parameters shared by every oracle call."""
+
+    def __init__(self, grid):
+        self.L, self.X, self.Y, self.DIR = int(grid["L"]), int(grid["X"]), int(grid["Y"]), int(grid["DIR"])
+        self.short = np.ascontiguousarray(grid["unit_length_short_costs"], np.float64)
+        self.xlen = _i32(grid["x_edge_len"])
+        self.ylen = _i32(grid["y_edge_len"])
+        self.capacity = np.ascontiguousarray(grid["capacity"], np.float32)
+        self.wire = float(grid["unit_length_wire_cost"])
+        self.via = float(grid["unit_via_cost"])
+        self.c = OraGrid(self.L, self.X, self.Y, self.DIR, self.wire, self.via, _p(self.short, ctypes.c_double),
+                         _p(self.xlen, ctypes.c_int), _p(self.ylen, ctypes.c_int))
+        self.cells = self.L * self.X * self.Y
+
+    def refresh(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0):
+        demand = np.ascontiguousarray(demand, np.float32)
+        w = np.empty(self.cells, np.float32)
+        v = np.empty(self.cells, np.float32)
+        p = np.empty(self.cells, np.float64)
+        lib().ora_cost_refresh_slopes(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float),
+                                      _p(demand, ctypes.c_float), ctypes.c_double(slope), ctypes.c_double(zero_slope),
+                                      ctypes.c_double(via_weight), _p(w, ctypes.c_float), _p(v, ctypes.c_float),
+                                      _p(p, ctypes.c_double))
+        return w, v, p
+
+    def lshape_net(self, vcost, presum, nodes, parents, cap=10**6):
+        nodes, parents = _i32(nodes), _i32(parents)
+        routes = np.zeros(cap, np.int32)
+        rc = lib().ora_lshape_net(ctypes.byref(self.c), _p(vcost, ctypes.c_float), _p(presum, ctypes.c_double),
+                                  ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
+                                  _p(routes, ctypes.c_int), ctypes.c_int(cap))
+        return rc, routes[:routes[0]] if rc == 0 else routes
+
+    def lshape_stage(self, sched, pin_acc_num, threads=1, first_stamp=0):
+        """Run the L-shape batch loop; returns (routes, demand, timestamps, wl, vias)."""
+        pin_acc_num = _i32(pin_acc_num)
+        nsub = len(pin_acc_num) - 1
+        route_off = (ROUTE_PER_PIN * pin_acc_num[:-1].astype(np.int64)).astype(np.int64)
+        route_cap = (ROUTE_PER_PIN * np.diff(pin_acc_num)).astype(np.int32)
+        routes = np.zeros(ROUTE_PER_PIN * int(pin_acc_num[-1]) + 1, np.int32)
+        demand = np.zeros(self.cells, np.float32)
+        ts = np.zeros(self.cells, np.int32)
+        w = np.empty(self.cells, np.float32)
+        v = np.empty(self.cells, np.float32)
+        p = np.empty(self.cells, np.float64)
+        stamp = ctypes.c_int(first_stamp)
+        wl = ctypes.c_double(0)
+        vias = ctypes.c_longlong(0)
+        bo = _i32(sched["batch_off"])
+        arrs = [_i32(sched[k]) for k in ("net_ids", "node_off", "nodes", "parents")]
+        rc = lib().ora_lshape_stage(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                                    _p(ts, ctypes.c_int), _p(w, ctypes.c_float), _p(v, ctypes.c_float),
+                                    _p(p, ctypes.c_double), ctypes.c_int(len(bo) - 1), _p(bo, ctypes.c_int),
+                                    *[_p(a, ctypes.c_int) for a in arrs], _p(route_off, ctypes.c_longlong),
+                                    _p(route_cap, ctypes.c_int), _p(routes, ctypes.c_int), ctypes.byref(stamp),
+                                    ctypes.byref(wl), ctypes.byref(vias), ctypes.c_int(threads))
+        if rc:
+            raise RuntimeError(f"oracle L-shape stage failed ({rc})")
+        del nsub
+        return routes, demand, ts, wl.value, vias.value
+
+    def ripup(self, demand, routes, pin_acc_num, threshold=0):
+        pin_acc_num = _i32(pin_acc_num)
+        n = len(pin_acc_num) - 1
+        routes = _i32(routes)
+        base = routes.ctypes.data
+        ptrs = (ctypes.POINTER(ctypes.c_int) * max(n, 1))()
+        for i in range(n):
+            ptrs[i] = ctypes.cast(base + 4 * ROUTE_PER_PIN * int(pin_acc_num[i]), _i32p)
+        flags = np.zeros(max(n, 1), np.uint8)
+        demand = np.ascontiguousarray(demand, np.float32)
+        lib().ora_ripup(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                        ctypes.c_int(threshold), ctypes.c_int(n), ptrs, _p(flags, ctypes.c_ubyte))
+        return flags[:n]
+
+
+def batches(X, Y, pt_off, pts, hs_off, hsegs, vs_off, vsegs, limit=1000000, sparsity=1):
+    arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
+    n = len(arrs[0]) - 1
+    color = np.zeros(max(n, 1), np.int32)
+    nb = lib().ora_batches(ctypes.c_int(X), ctypes.c_int(Y), ctypes.c_int(n), *[_p(a, ctypes.c_int) for a in arrs],
+                           ctypes.c_int(limit), ctypes.c_int(sparsity), _p(color, ctypes.c_int))
+    return color[:n], nb
+
+
+def fenwick_track(w):
+    w = np.ascontiguousarray(w, np.float32)
+    out = np.empty(len(w), np.float64)
+    lib().ora_fenwick_track(_p(w, ctypes.c_float), ctypes.c_int(len(w)), _p(out, ctypes.c_double))
+    return out
+
+
+def dense_prefix_track(w):
+    w = np.ascontiguousarray(w, np.float32)
+    out = np.empty(len(w), np.float64)
+    lib().ora_dense_prefix_track(_p(w, ctypes.c_float), ctypes.c_int(len(w)), _p(out, ctypes.c_double))
+    return out
+
+
+def expf(x):
+    return lib().ora_expf(float(x))
+
+
+def evaluator_path():
+    return os.path.join(REF_DIR, "evaluator")
+
+
+def run_evaluator(cap, net, out, timeout=600):
+    """Score a solution with the reference evaluator built into oracle/_ref/."""
+    res = subprocess.run([evaluator_path(), cap, net, out], capture_output=True, text=True, timeout=timeout)
+    scores = {}
+    for line in res.stdout.splitlines():
+        for key in ("wirelength cost", "via cost", "overflow cost", "total cost"):
+            if line.startswith(key):
+                scores[key.split()[0]] = float(line.split()[-1])
+        if line.startswith("Number of open nets"):
+            scores["open"] = int(line.split()[-1])
+        if line.startswith("Number of incompleted nets"):
+            scores["incomplete"] = int(line.split()[-1])
+    return scores, res

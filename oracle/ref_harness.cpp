@@ -1,0 +1,98 @@
+// Golden-vector harness over the reference's own host code -- TEST
+// INFRASTRUCTURE ONLY, built into oracle/_ref/ by oracle/Makefile from the
+// reference sources where they lie; never shipped or linked by the product.
+//
+// Runs, in the reference's order: db::read (src/input.hpp:247-419, with
+// net::init pin selection), flute::readLUT, prepare_routing_nets
+// (src/preprocess.hpp:1448-1623), construct_rsmt for every subnet
+// (src/routing/lshape.cuh:33-44), the hpwl sort (src/routing/lshape.cuh:53-55)
+// and generate_batches_rsmt (src/batching/generate_batches_cpu.hpp:7-73),
+// then prints one JSON object per line:
+//   {"grid": ...}, {"net": ...} per design net, {"sub": ...} per subnet,
+//   {"order": [...]}, {"batches": [[...], ...]}.
+//
+// usage: ref_harness <cap> <net> <flute_lut_dir> [sparsity]
+#include <unistd.h>
+
+#include "batching/generate_batches_cpu.hpp"
+#include "preprocess.hpp"
+
+static void print_list(const std::vector<int>& v) {
+    printf("[");
+    for (size_t i = 0; i < v.size(); ++i) printf(i ? ",%d" : "%d", v[i]);
+    printf("]");
+}
+
+int main(int argc, char** argv) {
+    if (argc < 4) {
+        fprintf(stderr, "usage: %s cap net flute_lut_dir [sparsity]\n", argv[0]);
+        return 2;
+    }
+    const int sparsity = argc > 4 ? atoi(argv[4]) : 1;
+    program_start = std::chrono::high_resolution_clock::now();
+    // keep the reference's progress lines off stdout
+    const int saved = dup(1);
+    if (!freopen("/dev/null", "w", stdout)) return 3;
+    db::read(argv[1], argv[2]);
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd)) return 3;
+    if (chdir(argv[3])) return 4;
+    flute::readLUT();
+    if (chdir(cwd)) return 4;
+    cudb::RouterState state;
+    cudb::prepare_routing_nets(state);
+    auto& nets = state.nets;
+    for (auto& n : nets) n.construct_rsmt(state, false, false);
+    std::vector<int> order(nets.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+    std::sort(order.begin(), order.end(), [&](int l, int r) { return nets[l].hpwl > nets[r].hpwl; });
+    auto work = order;
+    auto batches = cudb::generate_batches_rsmt(state, work, 1000000, sparsity);
+    fflush(stdout);
+    dup2(saved, 1);
+    close(saved);
+    stdout = fdopen(1, "w");
+    const auto& v = state.device;
+    printf("{\"grid\":{\"L\":%d,\"X\":%d,\"Y\":%d,\"DIR\":%d,\"sparsity\":%d}}\n", v.L, v.X, v.Y, v.DIR, sparsity);
+    for (size_t i = 0; i < db::nets.size(); ++i) {
+        const auto& n = db::nets[i];
+        printf("{\"net\":%zu,\"name\":\"%s\",\"pins\":", i, n.name.c_str());
+        print_list(n.pins);
+        printf(",\"extra\":");
+        print_list(n.extra_routes);
+        printf(",\"subnets\":");
+        print_list(n.subnets);
+        printf("}\n");
+    }
+    for (size_t i = 0; i < nets.size(); ++i) {
+        const auto& n = nets[i];
+        printf("{\"sub\":%zu,\"orig\":%d,\"hpwl\":%d,\"pins\":", i, n.original_net_id, n.hpwl);
+        print_list(n.pins);
+        printf(",\"nodes\":");
+        print_list(n.rsmt.back());
+        printf(",\"adj\":[");
+        for (size_t j = 0; j + 1 < n.rsmt.size(); ++j) {
+            if (j) printf(",");
+            print_list(n.rsmt[j]);
+        }
+        printf("],\"points\":");
+        print_list(n.points);
+        std::vector<int> h, w;
+        for (auto& e : n.rsmt_h_segments) h.push_back(e.first), h.push_back(e.second);
+        for (auto& e : n.rsmt_v_segments) w.push_back(e.first), w.push_back(e.second);
+        printf(",\"hsegs\":");
+        print_list(h);
+        printf(",\"vsegs\":");
+        print_list(w);
+        printf("}\n");
+    }
+    printf("{\"order\":");
+    print_list(order);
+    printf("}\n{\"batches\":[");
+    for (size_t b = 0; b < batches.size(); ++b) {
+        if (b) printf(",");
+        print_list(batches[b]);
+    }
+    printf("]}\n");
+    return 0;
+}

@@ -1,0 +1,94 @@
+"""GPU path (libgrotor_amd.so on a HIP device) against the CPU oracle.
+
+Integer and index results must be identical; floating-point results are
+bit-identical by construction (same operations in the same order, no
+contraction), so every comparison here is exact."""
+import numpy as np
+import pytest
+
+from tests._instances import random_demand, topology
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32 if a.dtype == np.float32 else np.uint64)
+
+
+@pytest.mark.parametrize("config,seed", [(1, 0), (6, 3)])
+def test_refresh_matches_oracle(ga, ora, config, seed):
+    _, topo = topology(config, seed)
+    print(f"refresh config={config} seed={seed}", flush=True)
+    ctx = ga.Context(topo)
+    grid = ora.Grid(topo)
+    for s in range(3):
+        demand = random_demand(topo, seed * 10 + s)
+        ctx.upload_demand(demand)
+        ctx.refresh()
+        w, v, p = ctx.costs()
+        wo, vo, po = grid.refresh(demand)
+        assert np.array_equal(_bits(w), _bits(wo))
+        assert np.array_equal(_bits(v), _bits(vo))
+        assert np.array_equal(_bits(p), _bits(po))
+    ctx.close()
+
+
+@pytest.mark.parametrize("config,seed", [(1, 0), (6, 1)])
+def test_batch_plan_matches_oracle(ga, ora, config, seed):
+    _, t = topology(config, seed)
+    print(f"batch_plan config={config} seed={seed} nets={t['nets']}", flush=True)
+    ctx = ga.Context(t)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ctx.batch_plan(*geom)
+    ocolor, onb = ora.batches(t["X"], t["Y"], *geom)
+    assert nb == onb
+    assert np.array_equal(color, ocolor)
+    for sparsity in (2, 5):
+        color, nb = ctx.batch_plan(*geom, sparsity=sparsity)
+        ocolor, onb = ora.batches(t["X"], t["Y"], *geom, sparsity=sparsity)
+        assert nb == onb and np.array_equal(color, ocolor)
+    ctx.close()
+
+
+@pytest.mark.parametrize("config,seed", [(1, 0), (6, 2)])
+def test_lshape_stage_matches_oracle(ga, ora, config, seed):
+    _, t = topology(config, seed)
+    print(f"lshape stage config={config} seed={seed}", flush=True)
+    color, nb = ora.batches(t["X"], t["Y"], t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=8)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=0)
+    assert ms > 0
+    routes = ctx.routes()
+    assert np.array_equal(routes, routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    assert np.array_equal(ctx.timestamps(), ts_o)
+    wl, vias = ctx.stats()
+    assert wl == wl_o and vias == vias_o
+    flags = ctx.ripup(t["subnets"], 0)
+    flags_o = ora.Grid(t).ripup(demand_o, routes_o, t["pin_acc_num"], 0)
+    assert np.array_equal(flags, flags_o)
+    ctx.close()
+
+
+def test_single_batch_entry_points(ga, ora):
+    """gr_lshape_batch + gr_commit_batch over one batch equal the oracle."""
+    _, t = topology(6, 4)
+    color, nb = ora.batches(t["X"], t["Y"], t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    sched = ga.execution_schedule(t, color, nb)
+    bo = sched["batch_off"]
+    first = {k: sched[k] for k in sched}
+    first["batch_off"] = bo[:2]
+    routes_o, demand_o, _, _, _ = ora.Grid(t).lshape_stage(first, t["pin_acc_num"])
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.refresh()
+    ctx.lshape_batch(0, int(bo[1]), 1)
+    ctx.commit_batch(0, int(bo[1]), 1)
+    assert np.array_equal(ctx.routes(), routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    ctx.close()
