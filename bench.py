@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Routing hot-path benchmark: routed nets/s (batching + DAG) on one MI355X.
+
+Workload (default): configuration 2 -- 512x512 grid, 9 metal layers (8 routing
+layers), capacities U{6..16} with 5% of routing cells blocked, 300k nets
+(60% 2-pin, 30% 3-5 pins, 8% clustered 20-120-pin nets), seed 0.  The
+front end (pin selection, trees, >12-pin split) runs once outside the timed
+region; a step is one L-shape stage over all routed subnets: GPU batching
+(exact first-fit greedy) followed by the batch loop (cost refresh, DP,
+commit).  As in the reference's metric (docs/amd.md:64-65,
+script/experiments/run_amd.py:98-108) a step's time is the batching call's
+wall time plus the synchronized DAG interval; host-side regrouping of the
+nets between the two is outside both, as it is in the reference.
+
+Prints one JSON line.  `roofline` reports the kernel class with the largest
+device time in the timed region, timed with HIP events on the context's
+stream; `cpu_baseline` times the CPU oracle (the restated reference path) on
+the same instance and checks that its routes equal the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KIND_NAMES = ["refresh_tracks_kernel", "lshape_dp_kernel", "commit_wires_kernel", "commit_vias_kernel",
+              "batch_marks_kernels", "color_kernel", "ripup_kernels"]
+CONFIG_TEXT = {
+    1: "128x128x5, 20k nets",
+    2: "512x512x9, 300k nets",
+    3: "1024x1024x10, 1M nets",
+    5: "2048x2048x10, 4M nets",
+    6: "64x64x5, 2k nets",
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--net-scale", type=float, default=1.0)
+    ap.add_argument("--mode", type=int, default=0, help="0 plain launches, 1 HIP graph, 2 persistent")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU path is out of scope).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import grotor_amd as ga
+
+    t0 = time.perf_counter()
+    design = ga.Design.generate(args.config, seed=args.seed, net_scale=args.net_scale)
+    design.prepare(args.threads)
+    topo = design.topology()
+    prep_s = time.perf_counter() - t0
+    print(f"[bench] instance config={args.config} subnets={topo['nets']} nodes={int(topo['node_off'][-1])} "
+          f"prepare={prep_s:.2f}s device={ga.device_name()}", file=sys.stderr, flush=True)
+
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"])
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+
+    def step():
+        ctx.reset()
+        b0 = time.perf_counter()
+        color, nb = ctx.batch_plan(*geom)
+        batch_ms = (time.perf_counter() - b0) * 1e3
+        sched = ga.execution_schedule(topo, color, nb)
+        ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+        dag_ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=args.mode)
+        return batch_ms, dag_ms, nb, color, sched
+
+    for i in range(args.warmup):
+        bm, dm, nb, _, _ = step()
+        print(f"[bench] warmup {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms, {nb} batches", file=sys.stderr, flush=True)
+
+    ctx.profile_begin()
+    total_ms = 0.0
+    batch_total = dag_total = 0.0
+    for i in range(args.steps):
+        bm, dm, nb, color, sched = step()
+        batch_total += bm
+        dag_total += dm
+        total_ms += bm + dm
+        print(f"[bench] step {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms", file=sys.stderr, flush=True)
+    prof = ctx.profile_end()
+    routes = ctx.routes()
+    wl, vias = ctx.stats()
+
+    nets = topo["nets"]
+    value = nets * args.steps / (total_ms / 1e3)
+    ms_per_step = total_ms / args.steps
+    dom = max(range(len(prof["ms"])), key=lambda k: prof["ms"][k])
+    dom_ms = prof["ms"][dom]
+    dom_launches = prof["launches"][dom]
+    achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
+                "launches": int(dom_launches),
+                "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}
+    kernels = {KIND_NAMES[k]: {"ms": round(prof["ms"][k], 4), "launches": int(prof["launches"][k]),
+                               "GBs": round(prof["bytes"][k] / (prof["ms"][k] / 1e3) / 1e9, 2) if prof["ms"][k] else 0}
+               for k in range(len(prof["ms"])) if prof["launches"][k]}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        c0 = time.perf_counter()
+        ocolor, onb = ora.batches(topo["X"], topo["Y"], *geom)
+        cpu_batch_s = time.perf_counter() - c0
+        osched = ga.execution_schedule(topo, ocolor, onb)
+        c1 = time.perf_counter()
+        oroutes, _, _, owl, ovias = ora.Grid(topo).lshape_stage(osched, topo["pin_acc_num"], threads=args.threads)
+        cpu_dag_s = time.perf_counter() - c1
+        same = bool(onb == nb and np.array_equal(ocolor, color) and np.array_equal(oroutes, routes)
+                    and owl == wl and ovias == vias)
+        cpu = {"value": round(nets / (cpu_batch_s + cpu_dag_s), 1), "unit": "nets/s", "cores": args.threads,
+               "kind": "port",
+               "sample": f"whole workload: {nets} subnets; CPU oracle batching on 1 thread "
+                         f"({cpu_batch_s * 1e3:.1f} ms) + DAG loop on {args.threads} OpenMP threads "
+                         f"({cpu_dag_s * 1e3:.1f} ms)",
+               "batching_ms": round(cpu_batch_s * 1e3, 3), "dag_ms": round(cpu_dag_s * 1e3, 3),
+               "routes_identical_to_gpu": same}
+        print(f"[bench] cpu: batching {cpu_batch_s * 1e3:.1f} ms, DAG {cpu_dag_s * 1e3:.1f} ms, identical={same}",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "routed nets/s (batching+DAG) at equal evaluator cost",
+        "value": round(value, 1),
+        "unit": "nets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} L-shape stage ({CONFIG_TEXT.get(args.config, '')}, "
+                               f"{nets} routed subnets): GPU batching + L-shape batch loop",
+                   "routed_subnets": nets, "batches": nb, "mode": args.mode,
+                   "batching_ms_per_step": round(batch_total / args.steps, 4),
+                   "dag_ms_per_step": round(dag_total / args.steps, 4),
+                   "total_wirelength": wl, "total_vias": vias, "parallelism": "single GPU",
+                   "kernels": kernels},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -254,6 +254,23 @@ class Context:
         _call("gr_routes_download", self._h, _ptr(r, ctypes.c_int), ctypes.c_longlong(self.route_words))
         return r
 
+    def reset(self):
+        _call("gr_ctx_reset", self._h)
+
+    KINDS = 7
+
+    def profile_begin(self):
+        _call("gr_profile_begin", self._h)
+
+    def profile_end(self):
+        """Per kernel class: summed device ms, launches and algorithmic bytes."""
+        ms = np.zeros(self.KINDS, np.float64)
+        n = np.zeros(self.KINDS, np.int64)
+        by = np.zeros(self.KINDS, np.float64)
+        _call("gr_profile_end", self._h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_longlong),
+              _ptr(by, ctypes.c_double), ctypes.c_int(self.KINDS))
+        return {"ms": ms.tolist(), "launches": n.tolist(), "bytes": by.tolist()}
+
     def ripup(self, net_count, threshold=0):
         flags = np.empty(max(net_count, 1), np.uint8)
         _call("gr_ripup", self._h, ctypes.c_int(threshold), _ptr(flags, ctypes.c_ubyte))

@@ -73,13 +73,31 @@ struct gr_ctx {
     int stage_count = 0;
     std::vector<int> node_off_h;
     int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr, *slots = nullptr;
+    std::vector<int> slots_h;
     double* gscratch = nullptr;
     size_t gscratch_bytes = 0;
     // rip-up
     int* of_sum = nullptr;
     unsigned char* of_flags = nullptr;
+    int batch_mask_words = 2;  // color mask width of the last batching call
+    int color_waves_per_cu = 16;
+    // kernel profiling (HIP events on the context stream)
+    struct ProfRec {
+        int kind;
+        hipEvent_t a, b;
+        double bytes;
+    };
+    bool profiling = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> events;
+    size_t next_event = 0;
+    unsigned long long* refreshed = nullptr;
+    std::vector<double> slot_dp_bytes;  // algorithmic bytes of the DP per stage slot
+    Stats prof_stats0{};
+    unsigned long long refreshed0 = 0;
 
     ~gr_ctx() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -87,6 +105,29 @@ struct gr_ctx {
 namespace {
 
 constexpr int kSmallTree = 16;
+
+hipEvent_t take_event(gr_ctx* c) {
+    if (c->next_event == c->events.size()) {
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        c->events.push_back(e);
+    }
+    return c->events[c->next_event++];
+}
+
+// Run `launch` between two events on the context stream when profiling.
+template <class F>
+void timed(gr_ctx* c, int kind, double bytes, F&& launch) {
+    if (!c->profiling) {
+        launch();
+        return;
+    }
+    hipEvent_t a = take_event(c), b = take_event(c);
+    hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+    launch();
+    hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+    c->prof.push_back({kind, a, b, bytes});
+}
 
 void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
     hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
@@ -127,9 +168,11 @@ void check_device_error(gr_ctx* c, const char* op) {
 }
 
 void launch_refresh(gr_ctx* c, int force) {
-    hipLaunchKernelGGL(refresh_tracks_kernel, dim3(c->tracks), dim3(kRefreshThreads), size_t(c->maxlen) * sizeof(double),
-                       c->stream, c->g, c->capacity, c->demand, c->vcost, c->presum, (float*)nullptr, c->track_dirty,
-                       force, c->cp);
+    timed(c, GR_K_REFRESH, 0.0, [&] {
+        hipLaunchKernelGGL(refresh_tracks_kernel, dim3(c->tracks), dim3(kRefreshThreads),
+                           size_t(c->maxlen) * sizeof(double), c->stream, c->g, c->capacity, c->demand, c->vcost,
+                           c->presum, (float*)nullptr, c->track_dirty, force, c->cp, c->refreshed);
+    });
     hip_check(hipGetLastError(), "refresh_tracks_kernel");
 }
 
@@ -154,8 +197,10 @@ void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K) {
     if (!count) return;
     const CommitArgs a = commit_args(c, begin, count, stamp, K);
     const int blocks = std::min((count + 3) / 4, 4096);
-    hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
-    hipLaunchKernelGGL(commit_vias_kernel, dim3(blocks), dim3(256), 0, c->stream, a);
+    timed(c, GR_K_COMMIT_WIRES, 0.0,
+          [&] { hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
+    timed(c, GR_K_COMMIT_VIAS, 0.0,
+          [&] { hipLaunchKernelGGL(commit_vias_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
     hip_check(hipGetLastError(), "commit kernels");
 }
 
@@ -205,24 +250,37 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
     a.slots = c->slots;
     a.gscratch = c->gscratch;
     a.error = c->error;
+    auto bytes_of = [&](int first, int count) {
+        double sum = 0;
+        if (c->profiling)
+            for (int i = first; i < first + count; ++i) sum += c->slot_dp_bytes[c->slots_h[i]];
+        return sum;
+    };
     if (b.small) {
         a.begin = b.begin;
         a.count = b.small;
         a.nmax = b.nmax_small;
-        hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.small), dim3(kWave), lshape_scratch_bytes(c->g.L, a.nmax),
-                           c->stream, a);
+        timed(c, GR_K_LSHAPE, bytes_of(b.begin, b.small), [&] {
+            hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.small), dim3(kWave), lshape_scratch_bytes(c->g.L, a.nmax),
+                               c->stream, a);
+        });
     }
     if (b.large) {
         a.begin = b.begin + b.small;
         a.count = b.large;
         a.nmax = b.nmax_large;
         const size_t bytes = lshape_scratch_bytes(c->g.L, a.nmax);
+        const double alg = bytes_of(b.begin + b.small, b.large);
         if (bytes <= kLdsLimit) {
-            hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.large), dim3(kWave), bytes, c->stream, a);
+            timed(c, GR_K_LSHAPE, alg, [&] {
+                hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.large), dim3(kWave), bytes, c->stream, a);
+            });
         } else {
             ensure_gscratch(c, ((bytes + 7) / 8) * 8 * size_t(b.large));
             a.gscratch = c->gscratch;
-            hipLaunchKernelGGL(lshape_dp_global_kernel, dim3(b.large), dim3(kWave), 0, c->stream, a);
+            timed(c, GR_K_LSHAPE, alg, [&] {
+                hipLaunchKernelGGL(lshape_dp_global_kernel, dim3(b.large), dim3(kWave), 0, c->stream, a);
+            });
         }
     }
     hip_check(hipGetLastError(), "lshape_dp kernels");
@@ -291,6 +349,8 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         c->track_dirty = c->mem.alloc<unsigned>(c->tracks);
         c->stats = c->mem.alloc<Stats>(1);
         c->error = c->mem.alloc<int>(1);
+        c->refreshed = c->mem.alloc<unsigned long long>(1);
+        hip_check(hipMemsetAsync(c->refreshed, 0, sizeof(unsigned long long), c->stream), "clear counter");
         std::vector<float> cap;
         to_track_major(g, grid->capacity, cap);
         upload(c->capacity, cap.data(), sizeof(float) * c->cells, c->stream);
@@ -349,17 +409,20 @@ int gr_cost_download(gr_ctx* c, float* wcost, float* vcost, double* presum) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (wcost) {
-            // wcost is not kept resident; recompute it into a temporary with
-            // the refresh kernel's own arithmetic (vcost/presum rewritten with
-            // identical values)
-            float* tmp = c->mem.alloc<float>(c->cells);
+            // wcost is not kept resident: recompute it from the current demand
+            // with the refresh kernel into temporaries, leaving the resident
+            // costs and dirty flags untouched
+            DeviceArena scratch;
+            float* tmp = scratch.alloc<float>(c->cells);
+            float* tv = scratch.alloc<float>(c->cells);
+            double* tp = scratch.alloc<double>(c->cells);
+            unsigned* td = scratch.alloc<unsigned>(c->tracks);
             hipLaunchKernelGGL(refresh_tracks_kernel, dim3(c->tracks), dim3(kRefreshThreads),
-                               size_t(c->maxlen) * sizeof(double), c->stream, c->g, c->capacity, c->demand, c->vcost,
-                               c->presum, tmp, c->track_dirty, 1, c->cp);
+                               size_t(c->maxlen) * sizeof(double), c->stream, c->g, c->capacity, c->demand, tv, tp,
+                               tmp, td, 1, c->cp, (unsigned long long*)nullptr);
             hip_check(hipGetLastError(), "refresh for wcost");
             std::vector<float> h(c->cells);
             download(h.data(), tmp, sizeof(float) * c->cells, c->stream);
-            c->mem.release(tmp);
             from_track_major(c->g, h, wcost);
         }
         if (vcost) {
@@ -449,6 +512,20 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
         upload(c->node_off, node_off, sizeof(int) * (count + 1), c->stream);
         upload(c->nodes, nodes, sizeof(int) * total, c->stream);
         upload(c->parents, parents, sizeof(int) * total, c->stream);
+        // algorithmic bytes of the DP per net: tree input 8n, via stacks
+        // 4*L*n, straight edges 16*L, bent edges 40*L, route words 32n
+        const int L = c->g.L, X = c->g.X, Y = c->g.Y;
+        c->slot_dp_bytes.assign(count, 0.0);
+        for (int k = 0; k < count; ++k) {
+            const int b = node_off[k], n = node_off[k + 1] - b;
+            double bytes = 8.0 * n + 4.0 * L * n + 32.0 * n;
+            for (int i = 1; i < n; ++i) {
+                const int v = nodes[b + i], q = nodes[b + parents[b + i]];
+                const bool st = (v / Y % X == q / Y % X) || (v % Y == q % Y);
+                bytes += (st ? 16.0 : 40.0) * L;
+            }
+            c->slot_dp_bytes[k] = bytes;
+        }
         hip_check(hipStreamSynchronize(c->stream), "stage upload");
     });
 }
@@ -460,6 +537,7 @@ int gr_lshape_batch(gr_ctx* c, int begin, int count, int stamp) {
         require_stage(c, begin, count);
         std::vector<int> slots(c->stage_count);
         const BatchLaunch b = plan_batch(c, begin, count, slots);
+        c->slots_h = slots;
         upload(c->slots + begin, slots.data() + begin, sizeof(int) * count, c->stream);
         launch_lshape(c, b);
         hip_check(hipStreamSynchronize(c->stream), "gr_lshape_batch");
@@ -487,6 +565,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
             plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
         }
+        c->slots_h = slots;
         upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "stage plan");
         hip_check(hipDeviceSynchronize(), "before DAG interval");
@@ -500,6 +579,66 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         const auto t1 = std::chrono::steady_clock::now();
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_lshape_run");
+    });
+}
+
+int gr_ctx_reset(gr_ctx* c) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        hip_check(hipMemsetAsync(c->demand, 0, sizeof(float) * c->cells, c->stream), "clear demand");
+        hip_check(hipMemsetAsync(c->timestamp, 0, sizeof(int) * c->cells, c->stream), "clear timestamps");
+        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream), "clear statistics");
+        if (c->routes)
+            hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
+        std::vector<unsigned> ones(c->tracks, 1u);
+        upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "gr_ctx_reset");
+    });
+}
+
+int gr_profile_begin(gr_ctx* c) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        hip_check(hipStreamSynchronize(c->stream), "gr_profile_begin");
+        c->prof.clear();
+        c->next_event = 0;
+        download(&c->prof_stats0, c->stats, sizeof(Stats), c->stream);
+        download(&c->refreshed0, c->refreshed, sizeof(unsigned long long), c->stream);
+        c->profiling = true;
+    });
+}
+
+int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, int kinds) {
+    return guarded([&] {
+        if (!c || kinds < 0) throw GrError(GR_E_ARG, "invalid arguments");
+        hip_check(hipStreamSynchronize(c->stream), "gr_profile_end");
+        c->profiling = false;
+        std::vector<double> t(GR_K_COUNT, 0.0), by(GR_K_COUNT, 0.0);
+        std::vector<long long> n(GR_K_COUNT, 0);
+        for (const auto& r : c->prof) {
+            float e = 0.f;
+            hip_check(hipEventElapsedTime(&e, r.a, r.b), "hipEventElapsedTime");
+            t[r.kind] += e;
+            by[r.kind] += r.bytes;
+            n[r.kind] += 1;
+        }
+        Stats s1;
+        unsigned long long refreshed1 = 0;
+        download(&s1, c->stats, sizeof(Stats), c->stream);
+        download(&refreshed1, c->refreshed, sizeof(unsigned long long), c->stream);
+        // refresh: read capacity + demand, write vcost (4 B each) and presum (8 B)
+        by[GR_K_REFRESH] += 20.0 * double(refreshed1 - c->refreshed0);
+        // wires: 4 B atomic read-modify-write + 4 B timestamp + 4 B dirty mark per unit edge
+        by[GR_K_COMMIT_WIRES] += 12.0 * double(s1.wire_edges - c->prof_stats0.wire_edges);
+        // vias: 4 B timestamp + 4 B demand per via layer step
+        by[GR_K_COMMIT_VIAS] += 8.0 * double((long long)(s1.total_vias - c->prof_stats0.total_vias));
+        for (int k = 0; k < std::min(kinds, (int)GR_K_COUNT); ++k) {
+            if (ms) ms[k] = t[k];
+            if (launches) launches[k] = n[k];
+            if (bytes) bytes[k] = by[k];
+        }
+        c->prof.clear();
+        c->next_event = 0;
     });
 }
 
@@ -562,53 +701,132 @@ int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int
         int* mark_off = tmp.alloc<int>(n + 1);
         hip_check(hipMemsetAsync(mark_count, 0, sizeof(int) * (n + 1), st), "clear mark counts");
         const int grid_n = (n + 255) / 256;
-        hipLaunchKernelGGL(count_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_count);
-        size_t temp_bytes = 0, t2 = 0;
+        // algorithmic bytes of mark expansion: per net 4*|points| + 8*|segments|
+        // + 4, plus 8 per marked cell (one read-modify-write per mark)
+        const double geom_bytes = 4.0 * pt_off[n] + 8.0 * (hs_off[n] + vs_off[n]) + 4.0 * n;
+        timed(c, GR_K_BATCH_MARKS, geom_bytes,
+              [&] { hipLaunchKernelGGL(count_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_count); });
+        size_t temp_bytes = 0;
         hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, mark_count, mark_off, n + 1, st), "scan size");
-        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, mark_count, mark_off, XY + 1, st), "scan size");
-        temp_bytes = std::max(temp_bytes, t2);
         void* temp = tmp.alloc<char>(temp_bytes);
         hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, mark_count, mark_off, n + 1, st), "scan marks");
         int total_marks = 0;
         download(&total_marks, mark_off + n, sizeof(int), st);
         if (total_marks < 0) throw GrError(GR_E_CAPACITY, "more than 2^31 batching marks");
-        int* mark_cell = tmp.alloc<int>(total_marks);
-        int* cell_count = tmp.alloc<int>(XY + 1);
+        // (cell << 32 | net) keys, sorted: per-cell lists in priority order
+        unsigned long long* keys = tmp.alloc<unsigned long long>(total_marks);
+        unsigned long long* sorted = tmp.alloc<unsigned long long>(total_marks);
+        unsigned long long* ukeys = tmp.alloc<unsigned long long>(total_marks);
+        int* d_unique = tmp.alloc<int>(1);
+        timed(c, GR_K_BATCH_MARKS, 8.0 * total_marks, [&] {
+            hipLaunchKernelGGL(emit_keys_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, keys);
+        });
+        int cell_bits = 1;
+        while ((1ll << cell_bits) < XY) ++cell_bits;
+        size_t sort_bytes = 0, uniq_bytes = 0;
+        hip_check(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, sorted, total_marks, 0, 32 + cell_bits, st),
+                  "sort size");
+        hip_check(hipcub::DeviceSelect::Unique(nullptr, uniq_bytes, sorted, ukeys, d_unique, total_marks, st),
+                  "unique size");
+        void* temp2 = tmp.alloc<char>(std::max(sort_bytes, uniq_bytes));
+        timed(c, GR_K_BATCH_MARKS, 16.0 * total_marks, [&] {
+            hip_check(hipcub::DeviceRadixSort::SortKeys(temp2, sort_bytes, keys, sorted, total_marks, 0, 32 + cell_bits, st),
+                      "sort marks");
+        });
+        timed(c, GR_K_BATCH_MARKS, 16.0 * total_marks, [&] {
+            hip_check(hipcub::DeviceSelect::Unique(temp2, uniq_bytes, sorted, ukeys, d_unique, total_marks, st),
+                      "unique marks");
+        });
         int* cell_off = tmp.alloc<int>(XY + 1);
-        int* cell_cursor = tmp.alloc<int>(XY);
-        int* cell_list = tmp.alloc<int>(total_marks);
-        hip_check(hipMemsetAsync(cell_count, 0, sizeof(int) * (XY + 1), st), "clear cell counts");
-        hip_check(hipMemsetAsync(cell_cursor, 0, sizeof(int) * XY, st), "clear cell cursors");
-        hipLaunchKernelGGL(emit_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, cell_count, mark_cell);
-        hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, cell_count, cell_off, XY + 1, st), "scan cells");
-        hipLaunchKernelGGL(fill_lists_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, mark_cell, cell_off,
-                           cell_cursor, cell_list);
-        constexpr int kMaxColors = 4096;
+        int* list_net = tmp.alloc<int>(total_marks);
+        timed(c, GR_K_BATCH_MARKS, 12.0 * total_marks, [&] {
+            hipLaunchKernelGGL(build_lists_kernel, dim3(total_marks / 256 + 1), dim3(256), 0, st, ukeys, d_unique, XY,
+                               cell_off, list_net);
+        });
+        // readers: the list entry of every (point cell, net) pair; reader ranks
+        int* is_reader = tmp.alloc<int>(total_marks + 1);
+        int* reader_scan = tmp.alloc<int>(total_marks + 1);
+        int* entry_info = tmp.alloc<int>(total_marks);
+        int* claimed = tmp.alloc<int>(total_marks);
+        hip_check(hipMemsetAsync(is_reader, 0, sizeof(int) * (total_marks + 1), st), "clear reader flags");
+        size_t scan2 = 0;
+        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, scan2, is_reader, reader_scan, total_marks + 1, st),
+                  "scan size");
+        void* temp3 = tmp.alloc<char>(scan2);
+        timed(c, GR_K_BATCH_MARKS, 12.0 * pt_off[n], [&] {
+            hipLaunchKernelGGL(mark_readers_kernel, dim3(grid_n), dim3(256), 0, st, b, cell_off, list_net, is_reader);
+        });
+        timed(c, GR_K_BATCH_MARKS, 8.0 * total_marks, [&] {
+            hip_check(hipcub::DeviceScan::ExclusiveSum(temp3, scan2, is_reader, reader_scan, total_marks + 1, st),
+                      "scan readers");
+        });
+        timed(c, GR_K_BATCH_MARKS, 20.0 * total_marks, [&] {
+            hipLaunchKernelGGL(entry_info_kernel, dim3(total_marks / 256 + 1), dim3(256), 0, st, ukeys, d_unique,
+                               cell_off, is_reader, reader_scan, entry_info);
+        });
+        constexpr int kMaxColors = 64 * kMaxMaskWords;
         int* d_color = tmp.alloc<int>(n);
         int* d_count = tmp.alloc<int>(kMaxColors);
         int* d_misc = tmp.alloc<int>(2);  // next, max color
-        hipLaunchKernelGGL(fill_int_kernel, dim3(std::min((n + 255) / 256, 4096)), dim3(256), 0, st, d_color, -1,
-                           (long long)n);
-        hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
-        const int misc[2] = {0, -1};
-        upload(d_misc, misc, sizeof(misc), st);
-        ColorArgs ca;
-        ca.b = b;
-        ca.cell_off = cell_off;
-        ca.cell_list = cell_list;
-        ca.color = d_color;
-        ca.color_count = d_count;
-        ca.next = d_misc;
-        ca.max_color = d_misc + 1;
-        ca.error = c->error;
-        ca.spin_limit = 1ll << 22;
-        const int waves = std::max(1, std::min(c->cus * 8, n));
-        hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), 0, st, ca);
-        hip_check(hipGetLastError(), "batching kernels");
+        int* writes = tmp.alloc<int>(XY);
+        int* reads = tmp.alloc<int>(XY);
+        int words = std::max(1, std::min(kMaxMaskWords, c->batch_mask_words));
+        int maxc[2] = {0, -1};
+        while (true) {
+            DeviceArena masks;
+            unsigned long long* mask = masks.alloc<unsigned long long>(size_t(XY) * words);
+            hip_check(hipMemsetAsync(mask, 0, sizeof(unsigned long long) * size_t(XY) * words, st), "clear masks");
+            hip_check(hipMemsetAsync(writes, 0, sizeof(int) * XY, st), "clear write counts");
+            hip_check(hipMemsetAsync(reads, 0, sizeof(int) * XY, st), "clear read counts");
+            hip_check(hipMemsetAsync(claimed, 0, sizeof(int) * total_marks, st), "clear claims");
+            hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
+            const int misc[2] = {0, -1};
+            upload(d_misc, misc, sizeof(misc), st);
+            ColorArgs ca;
+            ca.b = b;
+            ca.mark_off = mark_off;
+            ca.keys = keys;
+            ca.cell_off = cell_off;
+            ca.list_net = list_net;
+            ca.entry_info = entry_info;
+            ca.claimed = claimed;
+            ca.mask = mask;
+            ca.writes = writes;
+            ca.reads = reads;
+            ca.mask_words = words;
+            ca.color = d_color;
+            ca.color_count = d_count;
+            ca.next = d_misc;
+            ca.max_color = d_misc + 1;
+            ca.error = c->error;
+            ca.spin_limit = 1ll << 22;
+            const int waves = std::max(1, std::min(c->cus * c->color_waves_per_cu, n));
+            // colouring: per point a list lookup (8 B bounds, binary search),
+            // the write count and W mask words, the read count (4 B); per mark
+            // the lookup, claim, entry info, read count, one mask word and the
+            // write count; 4 B color per net
+            const double color_bytes = (8.0 + 4.0 + 8.0 * words + 4.0) * pt_off[n] +
+                                       (8.0 + 4.0 + 4.0 + 4.0 + 8.0 + 4.0 + 8.0) * total_marks + 4.0 * n;
+            const size_t color_rec = c->prof.size();
+            timed(c, GR_K_BATCH_COLOR, color_bytes,
+                  [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), 0, st, ca); });
+            hip_check(hipGetLastError(), "batching kernels");
+            download(maxc, d_misc, sizeof(maxc), st);
+            int err = 0;
+            download(&err, c->error, sizeof(int), st);
+            if (err == kDevColorOverflow && words < kMaxMaskWords) {
+                // more colors than the mask holds: clear and rerun with wider masks
+                const int zero = 0;
+                upload(c->error, &zero, sizeof(int), st);
+                if (c->profiling) c->prof.resize(color_rec);
+                words = std::min(kMaxMaskWords, words * 2);
+                continue;
+            }
+            check_device_error(c, "gr_batch_plan");
+            break;
+        }
+        c->batch_mask_words = std::max(1, (maxc[1] + 1 + 63) / 64);
         download(color, d_color, sizeof(int) * n, st);
-        int maxc[2];
-        download(maxc, d_misc, sizeof(maxc), st);
-        check_device_error(c, "gr_batch_plan");
         const int nb = maxc[1] + 1;
         std::vector<int> counts(nb);
         download(counts.data(), d_count, sizeof(int) * nb, st);

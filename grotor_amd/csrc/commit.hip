@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
         const int* r = route_of(a.rs, a.net_ids[slot]);
         const int end = r[0];
         double length = 0.0;
+        long long edges = 0;
         for (int i = 1; i < end; i += 2) {
             const int ra = r[i], rb = r[i + 1], layer = ra / g.XY;
             if (layer != rb / g.XY) continue;
@@ -40,6 +41,7 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
             const bool along_y = x0 == x1;
             const int lo = along_y ? min(y0, y1) : min(x0, x1);
             const int hi = along_y ? max(y0, y1) : max(x0, x1);
+            edges += hi - lo;
             if (lane == 0) {
                 a.timestamp[cell_of(g, layer, x0, y0)] = a.stamp;
                 a.timestamp[cell_of(g, layer, x1, y1)] = a.stamp;
@@ -53,6 +55,7 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
         }
         for (int off = kWave / 2; off; off >>= 1) length += __shfl_xor(length, off, kWave);
         if (lane == 0 && length != 0.0) atomicAdd(&a.stats->total_wirelength, length * a.K);
+        if (lane == 0 && edges) atomicAdd(&a.stats->wire_edges, (unsigned long long)edges);
     }
 }
 

@@ -23,7 +23,7 @@ namespace grotor {
 __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
     DeviceGrid g, const float* __restrict__ capacity, const float* __restrict__ demand,
     float* __restrict__ vcost, double* __restrict__ presum, float* __restrict__ wcost_out,
-    unsigned* __restrict__ track_dirty, int force, CostParams cp) {
+    unsigned* __restrict__ track_dirty, int force, CostParams cp, unsigned long long* __restrict__ refreshed) {
     const int T = blockIdx.x;
     if (!force && !track_dirty[T]) return;
     int l = 0;
@@ -89,7 +89,10 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
         for (int j = p; j >= 0; j = (j & (j + 1)) - 1) sum += fv[j];
         presum[base + p] = sum;
     }
-    if (threadIdx.x == 0) track_dirty[T] = 0;
+    if (threadIdx.x == 0) {
+        track_dirty[T] = 0;
+        if (refreshed) atomicAdd(refreshed, (unsigned long long)len);
+    }
 }
 
 }  // namespace grotor

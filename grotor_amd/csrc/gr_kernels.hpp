@@ -17,6 +17,7 @@ struct CostParams {
 struct Stats {
     double total_wirelength;
     unsigned long long total_vias;
+    unsigned long long wire_edges;  // unit wire edges committed (for byte accounting)
 };
 
 // Error codes written by kernels into the context's device error word.
@@ -30,7 +31,7 @@ enum DeviceError : int {
 
 __global__ void refresh_tracks_kernel(DeviceGrid g, const float* capacity, const float* demand, float* vcost,
                                       double* presum, float* wcost_out, unsigned* track_dirty, int force,
-                                      CostParams cp);
+                                      CostParams cp, unsigned long long* refreshed);
 
 struct RouteStore {
     int* routes;
@@ -92,13 +93,25 @@ struct BatchGeom {
     const int* vsegs;
 };
 __global__ void count_marks_kernel(BatchGeom b, int* mark_count);
-__global__ void emit_marks_kernel(BatchGeom b, const int* mark_off, int* cell_count, int* mark_cell);
-__global__ void fill_lists_kernel(BatchGeom b, const int* mark_off, const int* mark_cell, const int* cell_off,
-                                  int* cell_cursor, int* cell_list);
+__global__ void emit_keys_kernel(BatchGeom b, const int* mark_off, unsigned long long* keys);
+__global__ void build_lists_kernel(const unsigned long long* ukeys, const int* unique_count, int cells, int* cell_off,
+                                   int* list_net);
+__global__ void mark_readers_kernel(BatchGeom b, const int* cell_off, const int* list_net, int* is_reader);
+__global__ void entry_info_kernel(const unsigned long long* ukeys, const int* unique_count, const int* cell_off,
+                                  const int* is_reader, const int* reader_scan, int* entry_info);
+constexpr int kMaxMaskWords = 64;  // colors 0..4095
 struct ColorArgs {
     BatchGeom b;
+    const int* mark_off;
+    const unsigned long long* keys;  // marks in emission order (cell << 32 | net)
     const int* cell_off;
-    const int* cell_list;
+    const int* list_net;
+    const int* entry_info;     // 2 * reader rank + is_reader, per list entry
+    int* claimed;              // per list entry: write done
+    unsigned long long* mask;  // [cells * mask_words]
+    int* writes;               // [cells] completed writes
+    int* reads;                // [cells] completed readers
+    int mask_words;
     int* color;
     int* color_count;
     int* next;

@@ -126,6 +126,30 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
 int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,
                   int mode, double *elapsed_ms);
 
+/* Reset the routing state to that of a fresh context: demand, timestamps,
+ * statistics and route words cleared, every track marked for refresh (the
+ * DAG-throughput benchmark's reset, src/experiments/lshape_benchmark.cuh:84-97). */
+int gr_ctx_reset(gr_ctx *ctx);
+
+/* Kernel classes timed by gr_profile_*. */
+enum {
+    GR_K_REFRESH = 0,      /* cost refresh */
+    GR_K_LSHAPE = 1,       /* L-shape DP */
+    GR_K_COMMIT_WIRES = 2,
+    GR_K_COMMIT_VIAS = 3,
+    GR_K_BATCH_MARKS = 4,  /* batching: mark expansion and per-cell lists */
+    GR_K_BATCH_COLOR = 5,  /* batching: in-order colouring */
+    GR_K_RIPUP = 6,
+    GR_K_COUNT = 7
+};
+
+/* Start timing every kernel launch of this context with HIP events on its
+ * stream; gr_profile_end returns, per kernel class, the summed device time
+ * (ms), the launch count and the algorithmic HBM bytes of those launches.
+ * No reference counterpart (the reference prints stage wall times only). */
+int gr_profile_begin(gr_ctx *ctx);
+int gr_profile_end(gr_ctx *ctx, double *ms, long long *launches, double *bytes, int kinds);
+
 /* Copy `words` route words (from word 0) to the host (RouterState::download_routes,
  * include/router_state.hpp:215-222). */
 int gr_routes_download(gr_ctx *ctx, int *routes, long long words);

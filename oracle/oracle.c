@@ -116,71 +116,85 @@ void ora_dense_prefix_track(const float *w, int len, double *presum) {
     free(s);
 }
 
+/* One routing track: wcost/vcost of every cell (src/cost/full_update.cuh:34-83)
+ * and the Fenwick prefix of the track (src/cost/fenwick.cuh:129-168).  Tracks
+ * are independent: a cell's costs read only itself and its predecessor on
+ * the track, and presum reads only the track's wcost. */
+static void refresh_track(const ora_grid *g, const float *capacity, const float *demand,
+                          double slope, double zero_slope, double via_weight, int l, int t,
+                          float *w, float *vcost, double *presum, float *tw, double *tp) {
+    const int X = g->X, Y = g->Y;
+    const int dir = (l & 1) ^ g->DIR;
+    const int len = dir ? Y : X;
+    const double sc = g->unit_length_short_costs[l];
+    for (int p = 0; p < len; ++p) {
+        const int x = dir ? t : p, y = dir ? p : t;
+        const size_t i = (size_t)IDX3(g, l, x, y);
+        float wc = (float)ora_incremental_of_cost(capacity[i], demand[i], 1.0, sc, slope, zero_slope);
+        if (dir == 0 && x < X - 1)
+            wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->x_edge_len[x]);
+        if (dir == 1 && y < Y - 1)
+            wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->y_edge_len[y]);
+        if (w) w[i] = wc;
+        tw[p] = wc;
+        if (vcost) {
+            const size_t q = (size_t)(p > 0 ? (dir ? IDX3(g, l, x, y - 1) : IDX3(g, l, x - 1, y)) : (int)i);
+            double v;
+            if (p == 0)
+                v = ora_incremental_of_cost(capacity[i], demand[i], 1.0, sc, slope, zero_slope);
+            else if (p == len - 1)
+                v = ora_incremental_of_cost(capacity[q], demand[q], 1.0, sc, slope, zero_slope);
+            else
+                v = ora_incremental_of_cost(capacity[i], demand[i], 0.5, sc, slope, zero_slope) +
+                    ora_incremental_of_cost(capacity[q], demand[q], 0.5, sc, slope, zero_slope);
+            float vc = (float)v;
+            vc = (float)((double)vc + via_weight * g->unit_via_cost);
+            vcost[i] = vc;
+        }
+    }
+    if (presum) {
+        ora_fenwick_track(tw, len, tp);
+        for (int p = 0; p < len; ++p) presum[dir ? IDX3(g, l, t, p) : IDX3(g, l, p, t)] = tp[p];
+    }
+}
+
+static int track_count(const ora_grid *g, int l) { return ((l & 1) ^ g->DIR) ? g->X : g->Y; }
+
+/* Refresh the tracks whose dirty flag is set (all when dirty is NULL) and
+ * clear the flags: graph::refresh_cost (src/cost/incremental_update.cuh:98-132)
+ * at track granularity, which gives the same values as its per-cell form. */
+static void refresh_dirty(const ora_grid *g, const float *capacity, const float *demand,
+                          double slope, double zero_slope, double via_weight, unsigned char *dirty,
+                          float *wcost, float *vcost, double *presum, int threads) {
+    int total = 0;
+    for (int l = 0; l < g->L; ++l) total += track_count(g, l);
+    const int maxlen = g->X > g->Y ? g->X : g->Y;
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(threads > 0 ? threads : 1)
+#endif
+    {
+        float *tw = (float *)malloc(sizeof(float) * (size_t)maxlen);
+        double *tp = (double *)malloc(sizeof(double) * (size_t)maxlen);
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+        for (int T = 0; T < total; ++T) {
+            if (dirty && !dirty[T]) continue;
+            int l = 0, t = T;
+            while (t >= track_count(g, l)) t -= track_count(g, l++);
+            refresh_track(g, capacity, demand, slope, zero_slope, via_weight, l, t, wcost, vcost, presum, tw, tp);
+            if (dirty) dirty[T] = 0;
+        }
+        free(tw);
+        free(tp);
+    }
+}
+
 void ora_cost_refresh_slopes(const ora_grid *g, const float *capacity, const float *demand,
                              double slope, double zero_slope, double via_weight,
                              float *wcost, float *vcost, double *presum) {
-    const int L = g->L, X = g->X, Y = g->Y;
-    const size_t cells = (size_t)L * X * Y;
-    float *w = wcost ? wcost : (float *)malloc(sizeof(float) * cells);
-    for (int l = 0; l < L; ++l) {
-        const int dir = (l & 1) ^ g->DIR;
-        const double sc = g->unit_length_short_costs[l];
-        for (int x = 0; x < X; ++x)
-            for (int y = 0; y < Y; ++y) {
-                const size_t i = (size_t)IDX3(g, l, x, y);
-                /* update_single_wcost_ispd24, src/cost/full_update.cuh:34-51 */
-                float wc = (float)ora_incremental_of_cost(capacity[i], demand[i], 1.0, sc, slope, zero_slope);
-                if (dir == 0 && x < X - 1)
-                    wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->x_edge_len[x]);
-                if (dir == 1 && y < Y - 1)
-                    wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->y_edge_len[y]);
-                w[i] = wc;
-                if (vcost) {
-                    /* update_single_vcost_ispd24, src/cost/full_update.cuh:58-83 */
-                    double v;
-                    if (dir) {
-                        const size_t p = (size_t)IDX3(g, l, x, y > 0 ? y - 1 : 0);
-                        if (y == 0)
-                            v = ora_incremental_of_cost(capacity[i], demand[i], 1.0, sc, slope, zero_slope);
-                        else if (y == Y - 1)
-                            v = ora_incremental_of_cost(capacity[p], demand[p], 1.0, sc, slope, zero_slope);
-                        else
-                            v = ora_incremental_of_cost(capacity[i], demand[i], 0.5, sc, slope, zero_slope) +
-                                ora_incremental_of_cost(capacity[p], demand[p], 0.5, sc, slope, zero_slope);
-                    } else {
-                        const size_t p = (size_t)IDX3(g, l, x > 0 ? x - 1 : 0, y);
-                        if (x == 0)
-                            v = ora_incremental_of_cost(capacity[i], demand[i], 1.0, sc, slope, zero_slope);
-                        else if (x == X - 1)
-                            v = ora_incremental_of_cost(capacity[p], demand[p], 1.0, sc, slope, zero_slope);
-                        else
-                            v = ora_incremental_of_cost(capacity[i], demand[i], 0.5, sc, slope, zero_slope) +
-                                ora_incremental_of_cost(capacity[p], demand[p], 0.5, sc, slope, zero_slope);
-                    }
-                    float vc = (float)v;
-                    vc = (float)((double)vc + via_weight * g->unit_via_cost);
-                    vcost[i] = vc;
-                }
-            }
-    }
-    if (presum) {
-        const int maxlen = X > Y ? X : Y;
-        float *track = (float *)malloc(sizeof(float) * (size_t)maxlen);
-        double *pre = (double *)malloc(sizeof(double) * (size_t)maxlen);
-        for (int l = 0; l < L; ++l) {
-            const int dir = (l & 1) ^ g->DIR;
-            const int len = dir ? Y : X, tracks = dir ? X : Y;
-            for (int t = 0; t < tracks; ++t) {
-                for (int p = 0; p < len; ++p)
-                    track[p] = w[dir ? IDX3(g, l, t, p) : IDX3(g, l, p, t)];
-                ora_fenwick_track(track, len, pre);
-                for (int p = 0; p < len; ++p) presum[dir ? IDX3(g, l, t, p) : IDX3(g, l, p, t)] = pre[p];
-            }
-        }
-        free(track);
-        free(pre);
-    }
-    if (!wcost) free(w);
+    refresh_dirty(g, capacity, demand, slope, zero_slope, via_weight, NULL, wcost, vcost, presum, 1);
 }
 
 void ora_cost_refresh(const ora_grid *g, const float *capacity, const float *demand,
@@ -387,9 +401,15 @@ done:
 /* ------------------------------------------------------------------------ */
 /* Demand commit: src/routing/demand.cuh:236-319                            */
 /* ------------------------------------------------------------------------ */
+static int track_of(const ora_grid *g, int l, int x, int y) {
+    int T = 0;
+    for (int k = 0; k < l; ++k) T += track_count(g, k);
+    return T + (((l & 1) ^ g->DIR) ? x : y);
+}
+
 void ora_commit_batch(const ora_grid *g, float *demand, int *timestamp,
                       int count, int *const *routes_of, int stamp, int K,
-                      double *total_wirelength, long long *total_vias) {
+                      double *total_wirelength, long long *total_vias, unsigned char *dirty) {
     const int X = g->X, Y = g->Y, XY = X * Y;
     for (int k = 0; k < count; ++k) { /* commit_batch_wires */
         const int *r = routes_of[k];
@@ -406,6 +426,7 @@ void ora_commit_batch(const ora_grid *g, float *demand, int *timestamp,
             for (int pos = lo; pos < hi; ++pos) {
                 const int idx = layer * XY + (along_y ? x0 * Y + pos : pos * Y + y0);
                 demand[idx] += (float)K;
+                if (dirty) dirty[track_of(g, layer, idx / Y % X, idx % Y)] = 1;
                 timestamp[idx] = stamp;
                 length += along_y ? (double)g->y_edge_len[pos] : (double)g->x_edge_len[pos];
             }
@@ -428,6 +449,7 @@ void ora_commit_batch(const ora_grid *g, float *demand, int *timestamp,
                 const int along_y = (layer & 1) ^ g->DIR;
                 const int position = along_y ? y : x, length = along_y ? Y : X;
                 const int stride = along_y ? 1 : Y;
+                if (dirty) dirty[track_of(g, layer, x, y)] = 1;
                 if (!position) demand[idx] += (float)K;
                 else if (position == length - 1) demand[idx - stride] += (float)K;
                 else {
@@ -493,11 +515,14 @@ int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
     for (int b = 0; b < nbatch; ++b)
         if (batch_off[b + 1] - batch_off[b] > maxcount) maxcount = batch_off[b + 1] - batch_off[b];
     int **rof = (int **)malloc(sizeof(int *) * (size_t)(maxcount > 0 ? maxcount : 1));
-    (void)threads;
+    int tracks = 0;
+    for (int l = 0; l < g->L; ++l) tracks += track_count(g, l);
+    unsigned char *dirty = (unsigned char *)malloc((size_t)tracks);
+    memset(dirty, 1, (size_t)tracks);
     for (int b = 0; b < nbatch && !status; ++b) {
         const int begin = batch_off[b], count = batch_off[b + 1] - begin;
         const int s = ++*stamp;
-        ora_cost_refresh(g, capacity, demand, wcost, vcost, presum);
+        refresh_dirty(g, capacity, demand, 0.5, 1.5, 1.0, dirty, wcost, vcost, presum, threads);
         int bad = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads > 0 ? threads : 1) reduction(| : bad)
@@ -512,8 +537,9 @@ int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
         }
         if (bad) { status = -1; break; }
         for (int k = 0; k < count; ++k) rof[k] = routes + route_off[net_ids[begin + k]];
-        ora_commit_batch(g, demand, timestamp, count, rof, s, 1, total_wirelength, total_vias);
+        ora_commit_batch(g, demand, timestamp, count, rof, s, 1, total_wirelength, total_vias, dirty);
     }
+    free(dirty);
     free(rof);
     return status;
 }

@@ -82,7 +82,8 @@ int ora_lshape_net(const ora_grid *g, const float *vcost, const double *presum,
  * (src/routing/demand.cuh:236-319).  routes_of[k] points at net k's words. */
 void ora_commit_batch(const ora_grid *g, float *demand, int *timestamp,
                       int count, int *const *routes_of, int stamp, int K,
-                      double *total_wirelength, long long *total_vias);
+                      double *total_wirelength, long long *total_vias,
+                      unsigned char *dirty_tracks);
 
 /* Rip-up classification (src/routing/congestion.cuh:8-58, 114-136). */
 void ora_ripup(const ora_grid *g, const float *capacity, const float *demand,
@@ -92,8 +93,8 @@ void ora_ripup(const ora_grid *g, const float *capacity, const float *demand,
 /* One L-shape stage over a batch schedule (src/routing/lshape.cuh:153-179).
  * net_ids/node_off/nodes/parents are in execution order; batch_off has
  * nbatch+1 entries.  route_off[net] is the word offset of that net's route
- * area and route_cap[net] its size.  The stage refreshes costs before every
- * batch.  `*stamp` is incremented once per batch.  Returns 0 or a negative
+ * area and route_cap[net] its size.  Before every batch the stage refreshes
+ * the tracks whose demand changed (all tracks before the first batch).  `*stamp` is incremented once per batch.  Returns 0 or a negative
  * error from ora_lshape_net. */
 int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
                      int *timestamp, float *wcost, float *vcost, double *presum,
