@@ -22,6 +22,7 @@ struct gr_design {
     std::vector<int> order, node_off, nodes, parents, pt_off, pts, hs_off, hsegs, vs_off, vsegs;
     std::vector<float> capacity;  // routing layers, IDX order
     std::vector<double> short_costs;
+    std::vector<int> net_pin_off, net_pins, net_extra_off, net_extra, net_sub_off, net_subs;
 };
 
 extern "C" {
@@ -132,6 +133,50 @@ int gr_design_topology(gr_design* g, gr_topology* t) {
         t->y_edge_len = gr.y_edge_len.data();
         t->unit_length_wire_cost = gr.unit_length_wire_cost;
         t->unit_via_cost = gr.unit_via_cost;
+    });
+}
+
+int gr_design_nets(gr_design* g, int* count, const int** pin_off, const int** pins, const int** extra_off,
+                   const int** extra, const int** subnet_off, const int** subnets) {
+    return guarded([&] {
+        if (!g || !count) throw GrError(GR_E_ARG, "null argument");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        g->net_pin_off.assign(1, 0), g->net_extra_off.assign(1, 0), g->net_sub_off.assign(1, 0);
+        g->net_pins.clear(), g->net_extra.clear(), g->net_subs.clear();
+        for (const auto& n : g->d.nets) {
+            g->net_pins.insert(g->net_pins.end(), n.pins.begin(), n.pins.end());
+            g->net_extra.insert(g->net_extra.end(), n.extra_routes.begin(), n.extra_routes.end());
+            g->net_subs.insert(g->net_subs.end(), n.subnets.begin(), n.subnets.end());
+            g->net_pin_off.push_back(int(g->net_pins.size()));
+            g->net_extra_off.push_back(int(g->net_extra.size()));
+            g->net_sub_off.push_back(int(g->net_subs.size()));
+        }
+        *count = int(g->d.nets.size());
+        if (pin_off) *pin_off = g->net_pin_off.data();
+        if (pins) *pins = g->net_pins.data();
+        if (extra_off) *extra_off = g->net_extra_off.data();
+        if (extra) *extra = g->net_extra.data();
+        if (subnet_off) *subnet_off = g->net_sub_off.data();
+        if (subnets) *subnets = g->net_subs.data();
+    });
+}
+
+int gr_tree_geometry(int Y, int n, const int* positions, const int* adj_off, const int* adj, int* points,
+                     int* npoints, int* hsegs, int* nhsegs, int* vsegs, int* nvsegs) {
+    return guarded([&] {
+        if (n < 0 || !positions || !adj_off || !adj || !npoints || !nhsegs || !nvsegs)
+            throw GrError(GR_E_ARG, "null argument");
+        std::vector<std::vector<int>> adjacency(n);
+        for (int i = 0; i < n; ++i) adjacency[i].assign(adj + adj_off[i], adj + adj_off[i + 1]);
+        std::vector<int> pos(positions, positions + n), pts;
+        std::vector<std::pair<int, int>> hs, vs;
+        tree_geometry(adjacency, pos, Y, pts, hs, vs);
+        if (int(pts.size()) > *npoints || int(hs.size()) > *nhsegs || int(vs.size()) > *nvsegs)
+            throw GrError(GR_E_CAPACITY, "geometry output buffers too small");
+        std::copy(pts.begin(), pts.end(), points);
+        for (size_t i = 0; i < hs.size(); ++i) hsegs[2 * i] = hs[i].first, hsegs[2 * i + 1] = hs[i].second;
+        for (size_t i = 0; i < vs.size(); ++i) vsegs[2 * i] = vs[i].first, vsegs[2 * i + 1] = vs[i].second;
+        *npoints = int(pts.size()), *nhsegs = int(hs.size()), *nvsegs = int(vs.size());
     });
 }
 

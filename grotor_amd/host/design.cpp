@@ -443,6 +443,38 @@ static std::vector<std::vector<int>> build_tree(const std::vector<int>& position
     return graph;
 }
 
+void tree_geometry(const std::vector<std::vector<int>>& adjacency, const std::vector<int>& pos, int Y,
+                   std::vector<int>& points, std::vector<std::pair<int, int>>& hsegs,
+                   std::vector<std::pair<int, int>>& vsegs) {
+    // src/preprocess.hpp:100-125: every node, then per tree edge (j < i) its
+    // segments (both L orientations of a bent edge) and corner / end points
+    points.insert(points.end(), pos.begin(), pos.end());
+    for (int i = 0; i < int(pos.size()); i++) {
+        const int xi = pos[i] / Y, yi = pos[i] % Y;
+        for (int j : adjacency[i])
+            if (j < i) {
+                const int xj = pos[j] / Y, yj = pos[j] % Y;
+                const int minx = std::min(xi, xj), maxx = std::max(xi, xj), miny = std::min(yi, yj), maxy = std::max(yi, yj);
+                if (xi != xj && yi != yj) {
+                    hsegs.emplace_back(minx * Y + miny, maxx * Y + miny);
+                    hsegs.emplace_back(minx * Y + maxy, maxx * Y + maxy);
+                    vsegs.emplace_back(minx * Y + miny, minx * Y + maxy);
+                    vsegs.emplace_back(maxx * Y + miny, maxx * Y + maxy);
+                    points.push_back(xi * Y + yj);
+                    points.push_back(xj * Y + yi);
+                } else if (xi != xj) {
+                    hsegs.emplace_back(minx * Y + miny, maxx * Y + miny);
+                    points.push_back(minx * Y + miny);
+                    points.push_back(maxx * Y + miny);
+                } else if (yi != yj) {
+                    vsegs.emplace_back(minx * Y + miny, minx * Y + maxy);
+                    points.push_back(minx * Y + miny);
+                    points.push_back(minx * Y + maxy);
+                }
+            }
+    }
+}
+
 void construct_geometry(Subnet& net, int X, int Y, int L) {
     std::map<int, int> layer;
     std::vector<int> positions;
@@ -456,35 +488,11 @@ void construct_geometry(Subnet& net, int X, int Y, int L) {
     if (net.rsmt.empty()) net.rsmt = build_tree(positions, Y);
     else
         for (auto& e : net.rsmt.back()) e %= X * Y;
-    // src/preprocess.hpp:94-126
     net.hsegs.clear();
     net.vsegs.clear();
-    net.points = net.rsmt.back();
-    const auto& pos = net.rsmt.back();
-    for (int i = 0; i < int(pos.size()); i++) {
-        const int xi = pos[i] / Y, yi = pos[i] % Y;
-        for (int j : net.rsmt[i])
-            if (j < i) {
-                const int xj = pos[j] / Y, yj = pos[j] % Y;
-                const int minx = std::min(xi, xj), maxx = std::max(xi, xj), miny = std::min(yi, yj), maxy = std::max(yi, yj);
-                if (xi != xj && yi != yj) {
-                    net.hsegs.emplace_back(minx * Y + miny, maxx * Y + miny);
-                    net.hsegs.emplace_back(minx * Y + maxy, maxx * Y + maxy);
-                    net.vsegs.emplace_back(minx * Y + miny, minx * Y + maxy);
-                    net.vsegs.emplace_back(maxx * Y + miny, maxx * Y + maxy);
-                    net.points.push_back(xi * Y + yj);
-                    net.points.push_back(xj * Y + yi);
-                } else if (xi != xj) {
-                    net.hsegs.emplace_back(minx * Y + miny, maxx * Y + miny);
-                    net.points.push_back(minx * Y + miny);
-                    net.points.push_back(maxx * Y + miny);
-                } else if (yi != yj) {
-                    net.vsegs.emplace_back(minx * Y + miny, minx * Y + maxy);
-                    net.points.push_back(minx * Y + miny);
-                    net.points.push_back(minx * Y + maxy);
-                }
-            }
-    }
+    net.points.clear();
+    std::vector<std::vector<int>> adjacency(net.rsmt.begin(), net.rsmt.end() - 1);
+    tree_geometry(adjacency, net.rsmt.back(), Y, net.points, net.hsegs, net.vsegs);
     for (auto& e : net.rsmt.back()) e += (layer.count(e) ? layer[e] : L) * X * Y;
 }
 

@@ -79,6 +79,12 @@ void build_subnets(Design& design, int threads);
 // (pin layer, or L for Steiner nodes) as at src/preprocess.hpp:127-129.
 void construct_geometry(Subnet& net, int X, int Y, int L);
 
+// Batching geometry of one tree (src/preprocess.hpp:94-126): positions are
+// 2D cells, adjacency lists as in rsmt; appends to the outputs.
+void tree_geometry(const std::vector<std::vector<int>>& adjacency, const std::vector<int>& positions, int Y,
+                   std::vector<int>& points, std::vector<std::pair<int, int>>& hsegs,
+                   std::vector<std::pair<int, int>>& vsegs);
+
 // DFS preorder of a subnet tree from node 0 (src/routing/lshape.cuh:86-102).
 void preorder(const Subnet& net, std::vector<int>& nodes, std::vector<int>& parents);
 

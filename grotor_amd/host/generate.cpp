@@ -19,6 +19,9 @@
 //      pins 2 (p .55) | U{3..8} (p .35) | Zipf(2.2) on [9, 300],
 //      bbox median 12 (<= 1024).
 //   6: 64x64x5, 2k nets, distributions of 1 (finite-difference instance).
+//   7: 64x64x10, 1500 nets, distributions of 1, and 30% of pins with 2-3
+//      access points on metal 1-3 around the pin (front-end fixture; the
+//      reference reads 10-layer grids only).
 #include <algorithm>
 #include <cmath>
 #include <random>
@@ -42,6 +45,7 @@ struct Config {
     int bbox_cap;
     int pin_model;  // 1: configs 1/2/4, 3: Zipf 2..200, 5: config 5
     double cluster_fraction;
+    double multi_access = 0.0;  // fraction of pins with several access points
 };
 
 Config config_of(int c) {
@@ -52,6 +56,7 @@ Config config_of(int c) {
         case 4: return {512, 9, 300000, 6, 16, 0, 0, 0, 0, 0.05, 0.0, 10, 256, 1, 0.08};
         case 5: return {2048, 10, 4000000, 8, 20, 40, 0, 0, 0, 0.0, 0.03, 12, 1024, 5, 0.0};
         case 6: return {64, 5, 2000, 4, 12, 10, 8, 24, 2, 0.0, 0.0, 6, 64, 1, 0.0};
+        case 7: return {64, 10, 1500, 4, 12, 6, 6, 16, 2, 0.0, 0.0, 6, 64, 1, 0.0, 0.3};
         default: throw std::invalid_argument("unknown workload configuration");
     }
 }
@@ -170,7 +175,24 @@ void generate_design(const GenSpec& spec, Design& design) {
                 y = std::uniform_int_distribution<int>(y0, y0 + h - 1)(rng);
             }
             if (!used.insert(x * Y + y).second) continue;  // one pin per cell
-            net.access_points.push_back({1, x * Y + y});
+            std::vector<int> ap{1, x * Y + y};
+            if (cfg.multi_access > 0 && u01(rng) < cfg.multi_access) {
+                const int extra = std::uniform_int_distribution<int>(1, 2)(rng);
+                for (int e = 0; e < extra; ++e) {
+                    const int l = std::uniform_int_distribution<int>(0, 2)(rng);
+                    const int ex = std::min(X - 1, std::max(0, x + std::uniform_int_distribution<int>(-1, 1)(rng)));
+                    const int ey = std::min(Y - 1, std::max(0, y + std::uniform_int_distribution<int>(-1, 1)(rng)));
+                    // access points of one pin occupy distinct 2D cells (the
+                    // reference's selection asserts this, src/input.hpp:179)
+                    bool seen = false;
+                    for (int j = 1; j <= ap[0]; ++j) seen |= ap[j] % (X * Y) == ex * Y + ey;
+                    if (seen) continue;
+                    ap.push_back(l * X * Y + ex * Y + ey);
+                    ap[0]++;
+                    minx = std::min(minx, ex), maxx = std::max(maxx, ex), miny = std::min(miny, ey), maxy = std::max(maxy, ey);
+                }
+            }
+            net.access_points.push_back(ap);
             minx = std::min(minx, x), maxx = std::max(maxx, x), miny = std::min(miny, y), maxy = std::max(maxy, y);
         }
         net.minx = minx, net.maxx = maxx, net.miny = miny, net.maxy = maxy;

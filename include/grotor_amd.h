@@ -199,6 +199,18 @@ typedef struct gr_topology {
 } gr_topology;
 int gr_design_topology(gr_design *design, gr_topology *out);
 
+/* Per design net of a prepared design: selected pins (3D metal indices),
+ * extra_routes pairs and routed subnet ids, as CSR arrays owned by the design
+ * (db::net::pins / extra_routes / subnets, src/input.hpp:15-24). */
+int gr_design_nets(gr_design *design, int *count, const int **pin_off, const int **pins,
+                   const int **extra_off, const int **extra, const int **subnet_off, const int **subnets);
+
+/* Batching geometry of one tree (net::construct_rsmt, src/preprocess.hpp:94-126):
+ * positions are 2D cells x*Y + y, adjacency in CSR.  Capacities in, counts
+ * out (*npoints points, *nhsegs / *nvsegs segment pairs). */
+int gr_tree_geometry(int Y, int n, const int *positions, const int *adj_off, const int *adj,
+                     int *points, int *npoints, int *hsegs, int *nhsegs, int *vsegs, int *nvsegs);
+
 /* Write the .out solution for the prepared design given its route words
  * (graph::output_nets, src/output.hpp:79-112). */
 int gr_design_write_output(const gr_design *design, const int *routes, const char *out_path);

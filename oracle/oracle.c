@@ -273,8 +273,16 @@ static double wire_cost(const ora_grid *g, const double *presum, int l, int xmin
     return presum[IDX3(g, l, xmax, ymax)] - presum[IDX3(g, l, xmin, ymin)];
 }
 
+int ora_lshape_net_cost(const ora_grid *g, const float *vcost, const double *presum,
+                        int n, const int *nodes, const int *parents, int *routes, int cap, double *best);
+
 int ora_lshape_net(const ora_grid *g, const float *vcost, const double *presum,
                    int n, const int *nodes, const int *parents, int *routes, int cap) {
+    return ora_lshape_net_cost(g, vcost, presum, n, nodes, parents, routes, cap, NULL);
+}
+
+int ora_lshape_net_cost(const ora_grid *g, const float *vcost, const double *presum,
+                        int n, const int *nodes, const int *parents, int *routes, int cap, double *best) {
     const int L = g->L, X = g->X, Y = g->Y, DIR = g->DIR, LL = L * L;
     double *dist = (double *)malloc(sizeof(double) * (size_t)n * LL);
     int *from = (int *)calloc((size_t)n * LL, sizeof(int));
@@ -358,6 +366,7 @@ int ora_lshape_net(const ora_grid *g, const float *vcost, const double *presum,
         for (int hi = lo; hi < L; ++hi)
             if (dist[lo * L + hi] < dist[choice]) choice = lo * L + hi;
     range[0] = choice;
+    if (best) *best = dist[choice];
     int status = 0;
     for (int i = 1; i < n; ++i) range[i] = from[(size_t)i * LL + range[parents[i]]] / LL;
     for (int i = 0; i < n; ++i)

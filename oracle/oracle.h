@@ -77,6 +77,9 @@ int ora_batches(int X, int Y, int n,
  * assignment exists. */
 int ora_lshape_net(const ora_grid *g, const float *vcost, const double *presum,
                    int n, const int *nodes, const int *parents, int *routes, int cap);
+/* As ora_lshape_net, also returning the optimal tree cost (root table minimum). */
+int ora_lshape_net_cost(const ora_grid *g, const float *vcost, const double *presum,
+                        int n, const int *nodes, const int *parents, int *routes, int cap, double *best);
 
 /* Demand commit of one batch, wires then vias
  * (src/routing/demand.cuh:236-319).  routes_of[k] points at net k's words. */

@@ -81,13 +81,17 @@ class Grid:
                                       _p(p, ctypes.c_double))
         return w, v, p
 
-    def lshape_net(self, vcost, presum, nodes, parents, cap=10**6):
+    def lshape_net(self, vcost, presum, nodes, parents, cap=10**6, with_cost=False):
         nodes, parents = _i32(nodes), _i32(parents)
+        vcost = np.ascontiguousarray(vcost, np.float32)
+        presum = np.ascontiguousarray(presum, np.float64)
         routes = np.zeros(cap, np.int32)
-        rc = lib().ora_lshape_net(ctypes.byref(self.c), _p(vcost, ctypes.c_float), _p(presum, ctypes.c_double),
-                                  ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
-                                  _p(routes, ctypes.c_int), ctypes.c_int(cap))
-        return rc, routes[:routes[0]] if rc == 0 else routes
+        best = ctypes.c_double()
+        rc = lib().ora_lshape_net_cost(ctypes.byref(self.c), _p(vcost, ctypes.c_float), _p(presum, ctypes.c_double),
+                                       ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
+                                       _p(routes, ctypes.c_int), ctypes.c_int(cap), ctypes.byref(best))
+        r = routes[:routes[0]] if rc == 0 else routes
+        return (rc, r, best.value) if with_cost else (rc, r)
 
     def lshape_stage(self, sched, pin_acc_num, threads=1, first_stamp=0):
         """Run the L-shape batch loop; returns (routes, demand, timestamps, wl, vias)."""

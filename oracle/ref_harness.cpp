@@ -17,6 +17,15 @@
 #include "batching/generate_batches_cpu.hpp"
 #include "preprocess.hpp"
 
+static void print_json_string(const std::string& s) {
+    putchar('"');
+    for (char ch : s) {
+        if (ch == '"' || ch == '\\') putchar('\\');
+        putchar(ch);
+    }
+    putchar('"');
+}
+
 static void print_list(const std::vector<int>& v) {
     printf("[");
     for (size_t i = 0; i < v.size(); ++i) printf(i ? ",%d" : "%d", v[i]);
@@ -56,7 +65,9 @@ int main(int argc, char** argv) {
     printf("{\"grid\":{\"L\":%d,\"X\":%d,\"Y\":%d,\"DIR\":%d,\"sparsity\":%d}}\n", v.L, v.X, v.Y, v.DIR, sparsity);
     for (size_t i = 0; i < db::nets.size(); ++i) {
         const auto& n = db::nets[i];
-        printf("{\"net\":%zu,\"name\":\"%s\",\"pins\":", i, n.name.c_str());
+        printf("{\"net\":%zu,\"name\":", i);
+        print_json_string(n.name);
+        printf(",\"pins\":");
         print_list(n.pins);
         printf(",\"extra\":");
         print_list(n.extra_routes);

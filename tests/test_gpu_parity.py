@@ -92,3 +92,64 @@ def test_single_batch_entry_points(ga, ora):
     assert np.array_equal(ctx.routes(), routes_o)
     assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["syn10", "mempool"])
+def test_golden_batches_and_stage_on_gpu(ga, ora, name):
+    """On the reference's own trees and geometry: GPU batching equals the
+    reference's CPU batch lists, and the GPU L-shape stage equals the oracle."""
+    from tests import _golden as G
+    t = G.topology(name)
+    print(f"golden {name}: {t['nets']} nets", flush=True)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    color, nb = ctx.batch_plan(t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    ref_color, ref_nb = G.reference_colors(name)
+    assert nb == ref_nb and np.array_equal(color, ref_color)
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=4)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1)
+    assert np.array_equal(ctx.routes(), routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    assert ctx.stats() == (wl_o, vias_o)
+    ctx.close()
+
+
+def test_repeat_runs_are_deterministic(ga):
+    """Two full stages from a reset context give identical routes and demand
+    (atomics only ever add exact binary32 multiples of 0.5)."""
+    _, t = topology(1, 5)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    outs = []
+    for _ in range(2):
+        ctx.reset()
+        color, nb = ctx.batch_plan(*geom)
+        sched = ga.execution_schedule(t, color, nb)
+        ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+        ctx.lshape_run(sched["batch_off"], first_stamp=1)
+        outs.append((color.copy(), ctx.routes(), ctx.demand()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(_bits(outs[0][2]), _bits(outs[1][2]))
+    ctx.close()
+
+
+def test_gpu_expf_matches_oracle(ga, ora):
+    """The refresh kernel's exponential equals the oracle's on every demand
+    value the cost model sees: demand sweeps over 0..200 in steps of 0.5
+    against several capacities, compared through the resulting vcost."""
+    _, t = topology(6, 0)
+    ctx = ga.Context(t)
+    grid = ora.Grid(t)
+    rng = np.random.default_rng(11)
+    for s in range(4):
+        demand = (rng.integers(0, 400, ctx.cells) * 0.5).astype(np.float32)
+        ctx.upload_demand(demand)
+        ctx.refresh()
+        w, v, p = ctx.costs()
+        wo, vo, po = grid.refresh(demand)
+        assert np.array_equal(_bits(w), _bits(wo)) and np.array_equal(_bits(v), _bits(vo))
+    ctx.close()
