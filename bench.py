@@ -49,11 +49,45 @@ def parse():
     ap.add_argument("--mode", type=int, default=0, help="0 plain launches, 1 HIP graph, 2 persistent")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """A plain copy of known size: rocprofv3 FETCH_SIZE / WRITE_SIZE of this
+    kernel calibrate the counters against exact bytes read and written."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                                           "launches": 5}}), flush=True)
+
+
+def measured_traffic(kernel, config):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE table (profiles/hbm_traffic.txt),
+    scaled by its plain-copy calibration; None when not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 6 and not line.startswith("#") and f[0] == kernel and f[1] == f"config{config}":
+            fetch, write, fs, ws = map(float, f[2:])
+            return round((fetch * fs + write * ws) * 1024.0)
+    return None
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU path is out of scope).", file=sys.stderr)
@@ -108,7 +142,8 @@ def main():
     dom_launches = prof["launches"][dom]
     achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": measured_traffic(KIND_NAMES[dom], args.config) if args.mode == 0 else None,
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
                 "launches": int(dom_launches),
                 "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}

@@ -626,8 +626,10 @@ int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, in
         unsigned long long refreshed1 = 0;
         download(&s1, c->stats, sizeof(Stats), c->stream);
         download(&refreshed1, c->refreshed, sizeof(unsigned long long), c->stream);
-        // refresh: read capacity + demand, write vcost (4 B each) and presum (8 B)
-        by[GR_K_REFRESH] += 20.0 * double(refreshed1 - c->refreshed0);
+        // refresh, per cell of a refreshed track: read capacity + demand, write
+        // wcost + vcost (4 B each) and presum (8 B); wcost stays in LDS here
+        // but is counted, as for the reference's full refresh
+        by[GR_K_REFRESH] += 24.0 * double(refreshed1 - c->refreshed0);
         // wires: 4 B atomic read-modify-write + 4 B timestamp + 4 B dirty mark per unit edge
         by[GR_K_COMMIT_WIRES] += 12.0 * double(s1.wire_edges - c->prof_stats0.wire_edges);
         // vias: 4 B timestamp + 4 B demand per via layer step
@@ -801,12 +803,10 @@ int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int
             ca.error = c->error;
             ca.spin_limit = 1ll << 22;
             const int waves = std::max(1, std::min(c->cus * c->color_waves_per_cu, n));
-            // colouring: per point a list lookup (8 B bounds, binary search),
-            // the write count and W mask words, the read count (4 B); per mark
-            // the lookup, claim, entry info, read count, one mask word and the
-            // write count; 4 B color per net
-            const double color_bytes = (8.0 + 4.0 + 8.0 * words + 4.0) * pt_off[n] +
-                                       (8.0 + 4.0 + 4.0 + 4.0 + 8.0 + 4.0 + 8.0) * total_marks + 4.0 * n;
+            // algorithmic bytes of the whole coloring, charged to its one launch:
+            // per net 4 B per point, 8 B per segment and 4 B of color, plus one
+            // 4-byte read-modify-write (8 B) per marked cell
+            const double color_bytes = 4.0 * pt_off[n] + 8.0 * (hs_off[n] + vs_off[n]) + 4.0 * n + 8.0 * total_marks;
             const size_t color_rec = c->prof.size();
             timed(c, GR_K_BATCH_COLOR, color_bytes,
                   [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), 0, st, ca); });
