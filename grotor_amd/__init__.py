@@ -163,6 +163,7 @@ class Context:
 
     def __init__(self, grid):
         g = grid
+        self._geom_n = 0
         self.L, self.X, self.Y, self.DIR = int(g["L"]), int(g["X"]), int(g["Y"]), int(g["DIR"])
         self._keep = [np.ascontiguousarray(g["unit_length_short_costs"], np.float64), _i32(g["x_edge_len"]),
                       _i32(g["y_edge_len"]), np.ascontiguousarray(g["capacity"], np.float32)]
@@ -224,10 +225,26 @@ class Context:
         """Batch index of every net (priority order) and the batch count."""
         arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
         n = len(arrs[0]) - 1
+        self._geom_n = n
         color = np.empty(max(n, 1), np.int32)
         nb = ctypes.c_int()
         _call("gr_batch_plan", self._h, ctypes.c_int(n), *[_ptr(a, ctypes.c_int) for a in arrs],
               ctypes.c_int(limit), ctypes.c_int(sparsity), _ptr(color, ctypes.c_int), ctypes.byref(nb))
+        return color[:n], nb.value
+
+    def batch_geometry(self, pt_off, pts, hs_off, hsegs, vs_off, vsegs):
+        """Upload the batching geometry (kept until the next call)."""
+        arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
+        self._geom_n = len(arrs[0]) - 1
+        _call("gr_batch_geometry", self._h, ctypes.c_int(self._geom_n), *[_ptr(a, ctypes.c_int) for a in arrs])
+
+    def batch_colors(self, limit=1000000, sparsity=1):
+        """Batch the uploaded geometry: (color per net, batch count)."""
+        n = self._geom_n
+        color = np.empty(max(n, 1), np.int32)
+        nb = ctypes.c_int()
+        _call("gr_batch_colors", self._h, ctypes.c_int(limit), ctypes.c_int(sparsity), _ptr(color, ctypes.c_int),
+              ctypes.byref(nb))
         return color[:n], nb.value
 
     def lshape_upload(self, net_ids, node_off, nodes, parents):

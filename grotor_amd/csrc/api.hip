@@ -47,6 +47,47 @@ struct DeviceArena {
     }
 };
 
+// Sequential aligned sub-buffers of a base pointer (sizes only when base is null).
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += std::max<size_t>(count, 1) * sizeof(T);
+        return p;
+    }
+};
+
+// Grow-only device buffer, carved into aligned sub-buffers by a layout pass
+// that runs twice: once without a base to size it, once to assign pointers.
+struct Workspace {
+    char* base = nullptr;
+    size_t cap = 0;
+    Workspace() = default;
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() {
+        if (base) (void)hipFree(base);
+    }
+    template <class Layout>
+    void carve(Layout&& layout) {
+        Carver dry{nullptr};
+        layout(dry);
+        if (dry.off > cap) {
+            if (base) hip_check(hipFree(base), "hipFree workspace");
+            base = nullptr;
+            cap = 0;
+            const size_t want = dry.off + dry.off / 4;
+            hip_check(hipMalloc(reinterpret_cast<void**>(&base), want), "hipMalloc workspace");
+            cap = want;
+        }
+        Carver real{base};
+        layout(real);
+    }
+};
+
 }  // namespace
 
 struct gr_ctx {
@@ -79,7 +120,13 @@ struct gr_ctx {
     // rip-up
     int* of_sum = nullptr;
     unsigned char* of_flags = nullptr;
-    int batch_mask_words = 2;  // color mask width of the last batching call
+    // batching: resident geometry (gr_batch_geometry) and grow-only scratch
+    Workspace geom_ws, batch_ws1, batch_ws2, pm_ws;
+    BatchGeom geom{};
+    bool has_geom = false;
+    long long geom_points = 0, geom_hsegs = 0, geom_vsegs = 0;
+    std::vector<int> geom_pt_off;  // host copy of the point offsets
+    int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 16;
     // kernel profiling (HIP events on the context stream)
     struct ProfRec {
@@ -94,6 +141,7 @@ struct gr_ctx {
     unsigned long long* refreshed = nullptr;
     std::vector<double> slot_dp_bytes;  // algorithmic bytes of the DP per stage slot
     Stats prof_stats0{};
+    unsigned long long prof_wire_edges = 0, prof_vias = 0;  // counts of profiled steps before a reset
     unsigned long long refreshed0 = 0;
 
     ~gr_ctx() {
@@ -585,6 +633,14 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
 int gr_ctx_reset(gr_ctx* c) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
+        if (c->profiling) {
+            // keep the committed edge and via counts of the profiled interval
+            Stats now;
+            download(&now, c->stats, sizeof(Stats), c->stream);
+            c->prof_wire_edges += now.wire_edges - c->prof_stats0.wire_edges;
+            c->prof_vias += now.total_vias - c->prof_stats0.total_vias;
+            c->prof_stats0 = Stats{};
+        }
         hip_check(hipMemsetAsync(c->demand, 0, sizeof(float) * c->cells, c->stream), "clear demand");
         hip_check(hipMemsetAsync(c->timestamp, 0, sizeof(int) * c->cells, c->stream), "clear timestamps");
         hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream), "clear statistics");
@@ -603,6 +659,7 @@ int gr_profile_begin(gr_ctx* c) {
         c->prof.clear();
         c->next_event = 0;
         download(&c->prof_stats0, c->stats, sizeof(Stats), c->stream);
+        c->prof_wire_edges = c->prof_vias = 0;
         download(&c->refreshed0, c->refreshed, sizeof(unsigned long long), c->stream);
         c->profiling = true;
     });
@@ -631,9 +688,9 @@ int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, in
         // but is counted, as for the reference's full refresh
         by[GR_K_REFRESH] += 24.0 * double(refreshed1 - c->refreshed0);
         // wires: 4 B atomic read-modify-write + 4 B timestamp + 4 B dirty mark per unit edge
-        by[GR_K_COMMIT_WIRES] += 12.0 * double(s1.wire_edges - c->prof_stats0.wire_edges);
+        by[GR_K_COMMIT_WIRES] += 12.0 * double(c->prof_wire_edges + (s1.wire_edges - c->prof_stats0.wire_edges));
         // vias: 4 B timestamp + 4 B demand per via layer step
-        by[GR_K_COMMIT_VIAS] += 8.0 * double((long long)(s1.total_vias - c->prof_stats0.total_vias));
+        by[GR_K_COMMIT_VIAS] += 8.0 * double(c->prof_vias + (s1.total_vias - c->prof_stats0.total_vias));
         for (int k = 0; k < std::min(kinds, (int)GR_K_COUNT); ++k) {
             if (ms) ms[k] = t[k];
             if (launches) launches[k] = n[k];
@@ -668,133 +725,192 @@ int gr_ripup(gr_ctx* c, int threshold, unsigned char* is_of_net) {
     });
 }
 
-int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int* hs_off, const int* hsegs,
-                  const int* vs_off, const int* vsegs, int limit, int sparsity, int* color, int* num_batches) {
+int gr_batch_geometry(gr_ctx* c, int n, const int* pt_off, const int* pts, const int* hs_off, const int* hsegs,
+                      const int* vs_off, const int* vsegs) {
     return guarded([&] {
-        if (!c || n < 0 || !pt_off || !hs_off || !vs_off || !color || !num_batches)
-            throw GrError(GR_E_ARG, "null argument");
+        if (!c || n < 0 || !pt_off || !hs_off || !vs_off) throw GrError(GR_E_ARG, "null argument");
+        c->has_geom = false;
+        const int XY = c->g.XY;
+        auto check_off = [&](const int* off, const char* what) {
+            if (off[0] != 0) throw GrError(GR_E_ARG, std::string(what) + " offsets must start at 0");
+            for (int k = 0; k < n; ++k)
+                if (off[k + 1] < off[k]) throw GrError(GR_E_ARG, std::string(what) + " offsets must not decrease");
+        };
+        check_off(pt_off, "point");
+        check_off(hs_off, "horizontal segment");
+        check_off(vs_off, "vertical segment");
+        const long long np = pt_off[n], nh = hs_off[n], nv = vs_off[n];
+        if ((np && !pts) || (nh && !hsegs) || (nv && !vsegs)) throw GrError(GR_E_ARG, "null geometry array");
+        for (long long q = 0; q < np; ++q)
+            if (pts[q] < 0 || pts[q] >= XY) throw GrError(GR_E_ARG, "batching point outside the grid");
+        for (long long s = 0; s < nh; ++s)
+            if (hsegs[2 * s] < 0 || hsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
+        for (long long s = 0; s < nv; ++s)
+            if (vsegs[2 * s] < 0 || vsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
+        BatchGeom& b = c->geom;
+        c->geom_ws.carve([&](auto& cv) {
+            b.pt_off = cv.template take<int>(n + 1);
+            b.pts = cv.template take<int>(np);
+            b.hs_off = cv.template take<int>(n + 1);
+            b.hsegs = cv.template take<int>(2 * nh);
+            b.vs_off = cv.template take<int>(n + 1);
+            b.vsegs = cv.template take<int>(2 * nv);
+        });
+        hipStream_t st = c->stream;
+        upload((void*)b.pt_off, pt_off, sizeof(int) * (n + 1), st);
+        if (np) upload((void*)b.pts, pts, sizeof(int) * np, st);
+        upload((void*)b.hs_off, hs_off, sizeof(int) * (n + 1), st);
+        if (nh) upload((void*)b.hsegs, hsegs, sizeof(int) * 2 * nh, st);
+        upload((void*)b.vs_off, vs_off, sizeof(int) * (n + 1), st);
+        if (nv) upload((void*)b.vsegs, vsegs, sizeof(int) * 2 * nv, st);
+        hip_check(hipStreamSynchronize(st), "batching geometry upload");
+        b.n = n, b.X = c->g.X, b.Y = c->g.Y, b.sparsity = 1;
+        c->geom_points = np, c->geom_hsegs = nh, c->geom_vsegs = nv;
+        c->geom_pt_off.assign(pt_off, pt_off + n + 1);
+        c->has_geom = true;
+    });
+}
+
+int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_batches) {
+    return guarded([&] {
+        if (!c || !color || !num_batches) throw GrError(GR_E_ARG, "null argument");
+        if (!c->has_geom) throw GrError(GR_E_STATE, "no batching geometry (gr_batch_geometry)");
         if (sparsity < 1) throw GrError(GR_E_ARG, "sparsity must be positive");
         if (limit < 1) throw GrError(GR_E_ARG, "batch limit must be positive");
         *num_batches = 0;
+        BatchGeom b = c->geom;
+        b.sparsity = sparsity;
+        const int n = b.n;
         if (!n) return;
-        const int X = c->g.X, Y = c->g.Y, XY = c->g.XY;
-        for (int q = 0; q < pt_off[n]; ++q)
-            if (pts[q] < 0 || pts[q] >= XY) throw GrError(GR_E_ARG, "batching point outside the grid");
-        for (int s = 0; s < hs_off[n]; ++s)
-            if (hsegs[2 * s] < 0 || hsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
-        for (int s = 0; s < vs_off[n]; ++s)
-            if (vsegs[2 * s] < 0 || vsegs[2 * s + 1] >= XY) throw GrError(GR_E_ARG, "segment outside the grid");
-        DeviceArena tmp;
+        const int XY = c->g.XY;
+        const long long npts = c->geom_points;
         hipStream_t st = c->stream;
-        auto up = [&](const int* h, size_t count) {
-            int* d = tmp.alloc<int>(count);
-            if (count) upload(d, h, sizeof(int) * count, st);
-            return d;
-        };
-        BatchGeom b;
-        b.n = n, b.X = X, b.Y = Y, b.sparsity = sparsity;
-        b.pt_off = up(pt_off, n + 1);
-        b.pts = up(pts, pt_off[n]);
-        b.hs_off = up(hs_off, n + 1);
-        b.hsegs = up(hsegs, 2 * size_t(hs_off[n]));
-        b.vs_off = up(vs_off, n + 1);
-        b.vsegs = up(vsegs, 2 * size_t(vs_off[n]));
-        int* mark_count = tmp.alloc<int>(n + 1);
-        int* mark_off = tmp.alloc<int>(n + 1);
-        hip_check(hipMemsetAsync(mark_count, 0, sizeof(int) * (n + 1), st), "clear mark counts");
         const int grid_n = (n + 255) / 256;
-        // algorithmic bytes of mark expansion: per net 4*|points| + 8*|segments|
-        // + 4, plus 8 per marked cell (one read-modify-write per mark)
-        const double geom_bytes = 4.0 * pt_off[n] + 8.0 * (hs_off[n] + vs_off[n]) + 4.0 * n;
+
+        // marks per net and their offsets
+        int *mark_count = nullptr, *mark_off = nullptr;
+        void* scan_tmp = nullptr;
+        size_t scan_bytes = 0;
+        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, mark_count, mark_off, n + 1, st), "scan size");
+        c->batch_ws1.carve([&](auto& cv) {
+            mark_count = cv.template take<int>(n + 1);
+            mark_off = cv.template take<int>(n + 1);
+            scan_tmp = cv.template take<char>(scan_bytes);
+        });
+        hip_check(hipMemsetAsync(mark_count + n, 0, sizeof(int), st), "clear mark count tail");
+        const double geom_bytes = 4.0 * npts + 8.0 * (c->geom_hsegs + c->geom_vsegs) + 4.0 * n;
         timed(c, GR_K_BATCH_MARKS, geom_bytes,
               [&] { hipLaunchKernelGGL(count_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_count); });
-        size_t temp_bytes = 0;
-        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, temp_bytes, mark_count, mark_off, n + 1, st), "scan size");
-        void* temp = tmp.alloc<char>(temp_bytes);
-        hip_check(hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, mark_count, mark_off, n + 1, st), "scan marks");
+        hip_check(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, mark_count, mark_off, n + 1, st), "scan marks");
         int total_marks = 0;
         download(&total_marks, mark_off + n, sizeof(int), st);
         if (total_marks < 0) throw GrError(GR_E_CAPACITY, "more than 2^31 batching marks");
-        // (cell << 32 | net) keys, sorted: per-cell lists in priority order
-        unsigned long long* keys = tmp.alloc<unsigned long long>(total_marks);
-        unsigned long long* sorted = tmp.alloc<unsigned long long>(total_marks);
-        unsigned long long* ukeys = tmp.alloc<unsigned long long>(total_marks);
-        int* d_unique = tmp.alloc<int>(1);
-        timed(c, GR_K_BATCH_MARKS, 8.0 * total_marks, [&] {
-            hipLaunchKernelGGL(emit_keys_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, keys);
-        });
+        const int M = total_marks;
+
+        // per-cell lists, reader records
         int cell_bits = 1;
         while ((1ll << cell_bits) < XY) ++cell_bits;
-        size_t sort_bytes = 0, uniq_bytes = 0;
-        hip_check(hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, keys, sorted, total_marks, 0, 32 + cell_bits, st),
-                  "sort size");
-        hip_check(hipcub::DeviceSelect::Unique(nullptr, uniq_bytes, sorted, ukeys, d_unique, total_marks, st),
-                  "unique size");
-        void* temp2 = tmp.alloc<char>(std::max(sort_bytes, uniq_bytes));
-        timed(c, GR_K_BATCH_MARKS, 16.0 * total_marks, [&] {
-            hip_check(hipcub::DeviceRadixSort::SortKeys(temp2, sort_bytes, keys, sorted, total_marks, 0, 32 + cell_bits, st),
-                      "sort marks");
+        unsigned *cell = nullptr, *tag = nullptr, *cell2 = nullptr, *tag2 = nullptr;
+        int *head = nullptr, *eidx = nullptr, *list_net = nullptr, *einfo = nullptr, *marker = nullptr;
+        int *prev_marker = nullptr, *slot_of = nullptr, *rcount = nullptr, *d_color = nullptr, *d_count = nullptr;
+        int *d_misc = nullptr, *d_closed = nullptr;
+        int4* rec = nullptr;
+        void* tmp = nullptr;
+        size_t sort_bytes = 0, sum_bytes = 0, max_bytes = 0;
+        {
+            hipcub::DoubleBuffer<unsigned> k0(cell, cell2), v0(tag, tag2);
+            hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, v0, std::max(M, 1), 0, cell_bits, st),
+                      "sort size");
+            hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, sum_bytes, head, eidx, M + 1, st), "scan size");
+            hip_check(hipcub::DeviceScan::ExclusiveScan(nullptr, max_bytes, marker, prev_marker, hipcub::Max(), -1,
+                                                        std::max(M, 1), st),
+                      "scan size");
+        }
+        constexpr int kMaxColors = kColorsPerWord * kMaxMaskWords;
+        c->batch_ws2.carve([&](auto& cv) {
+            cell = cv.template take<unsigned>(M);
+            tag = cv.template take<unsigned>(M);
+            cell2 = cv.template take<unsigned>(M);
+            tag2 = cv.template take<unsigned>(M);
+            head = cv.template take<int>(M + 1);
+            eidx = cv.template take<int>(M + 1);
+            list_net = cv.template take<int>(M);
+            einfo = cv.template take<int>(M);
+            marker = cv.template take<int>(M);
+            prev_marker = cv.template take<int>(M);
+            slot_of = cv.template take<int>(M);
+            rec = cv.template take<int4>(npts);
+            rcount = cv.template take<int>(n);
+            d_color = cv.template take<int>(n);
+            d_count = cv.template take<int>(kMaxColors);
+            d_misc = cv.template take<int>(2);  // dequeue counter, max color
+            d_closed = cv.template take<int>(2 * kMaxColors);
+            tmp = cv.template take<char>(std::max(sort_bytes, std::max(sum_bytes, max_bytes)));
         });
-        timed(c, GR_K_BATCH_MARKS, 16.0 * total_marks, [&] {
-            hip_check(hipcub::DeviceSelect::Unique(temp2, uniq_bytes, sorted, ukeys, d_unique, total_marks, st),
-                      "unique marks");
-        });
-        int* cell_off = tmp.alloc<int>(XY + 1);
-        int* list_net = tmp.alloc<int>(total_marks);
-        timed(c, GR_K_BATCH_MARKS, 12.0 * total_marks, [&] {
-            hipLaunchKernelGGL(build_lists_kernel, dim3(total_marks / 256 + 1), dim3(256), 0, st, ukeys, d_unique, XY,
-                               cell_off, list_net);
-        });
-        // readers: the list entry of every (point cell, net) pair; reader ranks
-        int* is_reader = tmp.alloc<int>(total_marks + 1);
-        int* reader_scan = tmp.alloc<int>(total_marks + 1);
-        int* entry_info = tmp.alloc<int>(total_marks);
-        int* claimed = tmp.alloc<int>(total_marks);
-        hip_check(hipMemsetAsync(is_reader, 0, sizeof(int) * (total_marks + 1), st), "clear reader flags");
-        size_t scan2 = 0;
-        hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, scan2, is_reader, reader_scan, total_marks + 1, st),
-                  "scan size");
-        void* temp3 = tmp.alloc<char>(scan2);
-        timed(c, GR_K_BATCH_MARKS, 12.0 * pt_off[n], [&] {
-            hipLaunchKernelGGL(mark_readers_kernel, dim3(grid_n), dim3(256), 0, st, b, cell_off, list_net, is_reader);
-        });
-        timed(c, GR_K_BATCH_MARKS, 8.0 * total_marks, [&] {
-            hip_check(hipcub::DeviceScan::ExclusiveSum(temp3, scan2, is_reader, reader_scan, total_marks + 1, st),
-                      "scan readers");
-        });
-        timed(c, GR_K_BATCH_MARKS, 20.0 * total_marks, [&] {
-            hipLaunchKernelGGL(entry_info_kernel, dim3(total_marks / 256 + 1), dim3(256), 0, st, ukeys, d_unique,
-                               cell_off, is_reader, reader_scan, entry_info);
-        });
-        constexpr int kMaxColors = 64 * kMaxMaskWords;
-        int* d_color = tmp.alloc<int>(n);
-        int* d_count = tmp.alloc<int>(kMaxColors);
-        int* d_misc = tmp.alloc<int>(2);  // next, max color
-        int* writes = tmp.alloc<int>(XY);
-        int* reads = tmp.alloc<int>(XY);
+        const int grid_m = M / 256 + 1;
+        hip_check(hipMemsetAsync(rcount, 0, sizeof(int) * n, st), "clear reader counts");
+        if (M) {
+            timed(c, GR_K_BATCH_MARKS, 8.0 * M, [&] {
+                hipLaunchKernelGGL(emit_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, cell, tag);
+            });
+            hipcub::DoubleBuffer<unsigned> keys(cell, cell2), vals(tag, tag2);
+            timed(c, GR_K_BATCH_MARKS, 32.0 * M, [&] {
+                hip_check(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, vals, M, 0, cell_bits, st),
+                          "sort marks");
+            });
+            const unsigned* sc = keys.Current();
+            const unsigned* sv = vals.Current();
+            timed(c, GR_K_BATCH_MARKS, 12.0 * M, [&] {
+                hipLaunchKernelGGL(entry_heads_kernel, dim3(grid_m), dim3(256), 0, st, sc, sv, M, head);
+                hip_check(hipcub::DeviceScan::ExclusiveSum(tmp, sum_bytes, head, eidx, M + 1, st), "scan entries");
+                hipLaunchKernelGGL(build_entries_kernel, dim3(grid_m), dim3(256), 0, st, sc, sv, eidx, M, list_net,
+                                   einfo, marker);
+            });
+            int U = 0;
+            download(&U, eidx + M, sizeof(int), st);
+            const int grid_u = U / 256 + 1;
+            timed(c, GR_K_BATCH_MARKS, 20.0 * U, [&] {
+                hip_check(hipcub::DeviceScan::ExclusiveScan(tmp, max_bytes, marker, prev_marker, hipcub::Max(), -1, U, st),
+                          "scan markers");
+                hipLaunchKernelGGL(reader_slots_kernel, dim3(grid_u), dim3(256), 0, st, U, list_net, einfo, b.pt_off,
+                                   rcount, slot_of);
+                hipLaunchKernelGGL(reader_records_kernel, dim3(grid_u), dim3(256), 0, st, U, einfo, prev_marker,
+                                   slot_of, rec);
+            });
+            hip_check(hipGetLastError(), "batching list kernels");
+        }
+
+        // coloring, widened on color overflow, re-run from the first net that
+        // would overfill a batch
+        std::vector<int> closed_color, closed_from;
+        int first = 0;
         int words = std::max(1, std::min(kMaxMaskWords, c->batch_mask_words));
-        int maxc[2] = {0, -1};
+        std::vector<int> counts;
+        std::vector<int> host_color;
+        bool have_colors = false;
         while (true) {
-            DeviceArena masks;
-            unsigned long long* mask = masks.alloc<unsigned long long>(size_t(XY) * words);
-            hip_check(hipMemsetAsync(mask, 0, sizeof(unsigned long long) * size_t(XY) * words, st), "clear masks");
-            hip_check(hipMemsetAsync(writes, 0, sizeof(int) * XY, st), "clear write counts");
-            hip_check(hipMemsetAsync(reads, 0, sizeof(int) * XY, st), "clear read counts");
-            hip_check(hipMemsetAsync(claimed, 0, sizeof(int) * total_marks, st), "clear claims");
+            unsigned long long* pm = nullptr;
+            c->pm_ws.carve([&](auto& cv) { pm = cv.template take<unsigned long long>(size_t(npts) * words); });
+            const long long slot0 = c->geom_pt_off[first];
+            hip_check(hipMemsetAsync(pm + slot0 * words, 0, sizeof(unsigned long long) * size_t(npts - slot0) * words, st),
+                      "clear prefix masks");
+            hipLaunchKernelGGL(fill_int_kernel, dim3((n - first + 255) / 256), dim3(256), 0, st, d_color + first, -1,
+                               (long long)(n - first));
             hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
-            const int misc[2] = {0, -1};
+            const int misc[2] = {first, -1};
             upload(d_misc, misc, sizeof(misc), st);
+            const int nclosed = int(closed_color.size());
+            if (nclosed) {
+                upload(d_closed, closed_color.data(), sizeof(int) * nclosed, st);
+                upload(d_closed + kMaxColors, closed_from.data(), sizeof(int) * nclosed, st);
+            }
             ColorArgs ca;
-            ca.b = b;
-            ca.mark_off = mark_off;
-            ca.keys = keys;
-            ca.cell_off = cell_off;
+            ca.n = n;
+            ca.pt_off = b.pt_off;
+            ca.rcount = rcount;
+            ca.rec = rec;
             ca.list_net = list_net;
-            ca.entry_info = entry_info;
-            ca.claimed = claimed;
-            ca.mask = mask;
-            ca.writes = writes;
-            ca.reads = reads;
+            ca.pm = pm;
             ca.mask_words = words;
             ca.color = d_color;
             ca.color_count = d_count;
@@ -802,39 +918,77 @@ int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int
             ca.max_color = d_misc + 1;
             ca.error = c->error;
             ca.spin_limit = 1ll << 22;
-            const int waves = std::max(1, std::min(c->cus * c->color_waves_per_cu, n));
-            // algorithmic bytes of the whole coloring, charged to its one launch:
+            ca.nclosed = nclosed;
+            ca.closed_color = d_closed;
+            ca.closed_from = d_closed + kMaxColors;
+            const int waves = std::max(1, std::min(c->cus * c->color_waves_per_cu, n - first));
+            const size_t lds = sizeof(unsigned long long) * size_t(kWave + 1) * words;
+            // algorithmic bytes of the whole batching, charged to this launch:
             // per net 4 B per point, 8 B per segment and 4 B of color, plus one
             // 4-byte read-modify-write (8 B) per marked cell
-            const double color_bytes = 4.0 * pt_off[n] + 8.0 * (hs_off[n] + vs_off[n]) + 4.0 * n + 8.0 * total_marks;
+            const double color_bytes =
+                4.0 * npts + 8.0 * (c->geom_hsegs + c->geom_vsegs) + 4.0 * n + 8.0 * double(M);
             const size_t color_rec = c->prof.size();
             timed(c, GR_K_BATCH_COLOR, color_bytes,
-                  [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), 0, st, ca); });
-            hip_check(hipGetLastError(), "batching kernels");
+                  [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), lds, st, ca); });
+            hip_check(hipGetLastError(), "color_kernel");
+            int maxc[2] = {0, -1};
             download(maxc, d_misc, sizeof(maxc), st);
             int err = 0;
             download(&err, c->error, sizeof(int), st);
             if (err == kDevColorOverflow && words < kMaxMaskWords) {
-                // more colors than the mask holds: clear and rerun with wider masks
                 const int zero = 0;
                 upload(c->error, &zero, sizeof(int), st);
                 if (c->profiling) c->prof.resize(color_rec);
                 words = std::min(kMaxMaskWords, words * 2);
+                first = 0;
+                closed_color.clear();
+                closed_from.clear();
                 continue;
             }
-            check_device_error(c, "gr_batch_plan");
-            break;
+            check_device_error(c, "gr_batch_colors");
+            const int nb_run = maxc[1] + 1;
+            counts.assign(nb_run, 0);
+            download(counts.data(), d_count, sizeof(int) * nb_run, st);
+            bool over = false;
+            for (int v : counts) over |= v > limit;
+            if (!over && !first) break;
+            // exact limit: find the first net that would overfill a batch
+            host_color.resize(n);
+            download(host_color.data(), d_color, sizeof(int) * n, st);
+            have_colors = true;
+            std::vector<int> fill(kMaxColors, 0);
+            int kstar = -1;
+            for (int k = 0; k < n; ++k) {
+                if (++fill[host_color[k]] > limit) {
+                    kstar = k;
+                    break;
+                }
+            }
+            if (kstar < 0) {
+                counts.assign(kMaxColors, 0);
+                int top = -1;
+                for (int k = 0; k < n; ++k) ++counts[host_color[k]], top = std::max(top, host_color[k]);
+                counts.resize(top + 1);
+                break;
+            }
+            if (int(closed_color.size()) >= kMaxColors) throw GrError(GR_E_CAPACITY, "too many full batches");
+            closed_color.push_back(host_color[kstar]);
+            closed_from.push_back(kstar);
+            first = kstar;
         }
-        c->batch_mask_words = std::max(1, (maxc[1] + 1 + 63) / 64);
-        download(color, d_color, sizeof(int) * n, st);
-        const int nb = maxc[1] + 1;
-        std::vector<int> counts(nb);
-        download(counts.data(), d_count, sizeof(int) * nb, st);
-        for (int v : counts)
-            if (v > limit)
-                throw GrError(GR_E_CAPACITY, "a batch reaches the batch size limit; exact limited batching is not supported");
-        *num_batches = nb;
+        c->batch_mask_words = words;
+        if (have_colors) std::memcpy(color, host_color.data(), sizeof(int) * n);
+        else download(color, d_color, sizeof(int) * n, st);
+        *num_batches = int(counts.size());
     });
+}
+
+int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int* hs_off, const int* hsegs,
+                  const int* vs_off, const int* vsegs, int limit, int sparsity, int* color, int* num_batches) {
+    const int rc = gr_batch_geometry(c, n, pt_off, pts, hs_off, hsegs, vs_off, vsegs);
+    if (rc) return rc;
+    return gr_batch_colors(c, limit, sparsity, color, num_batches);
 }
 
 }  // extern "C"

@@ -1,46 +1,47 @@
 // Exact first-fit greedy batching on gfx950.
 //
 // Reference: cudb::generate_batches_rsmt (src/batching/generate_batches_cpu.hpp:7-73).
-// Nets are taken in priority order k; net k joins the first batch none of
-// whose earlier members marked any of k's points.  Marks are the cells of
-// the net's horizontal / vertical segments (every `sparsity` cells) and the
-// stencil {(x-1,y), (x,y-1), (x,y), (x+1,y)} of every point (:35-40).
+// Nets are taken in priority order k; net k joins the first batch that is not
+// full and none of whose earlier members marked any of k's points.  Marks are
+// the cells of the net's horizontal / vertical segments (every `sparsity`
+// cells) and the stencil {(x-1,y), (x,y-1), (x,y), (x+1,y)} of every point
+// (:35-40, :59-66).
 //
-// Equivalently color(k) = mex{ color(j) : j < k, marks(j) contain a point of
-// k }: net k needs, for each of its points p, the set of colors of the
-// earlier nets that mark p.  The GPU form (after the reference's HIPRT chunk
-// path, src/batching/hiprt/grid_chunk.cuh, with the BVH query replaced by
-// direct indexing of a uniform G-cell bin grid):
-//   1. count_marks / emit_keys: every net's marked cells as 64-bit keys
-//      (cell << 32 | k); a radix sort and a unique pass give, per cell, the
-//      list of nets marking it in priority order (CSR, one entry per net);
-//   2. every cell keeps a color bitmask, a count of completed writes and a
-//      count of completed reads.  A list entry is a reader when the cell is
-//      one of its net's points (every entry writes: the point stencil marks
-//      the cell itself).  A reader at list position i needs exactly the
-//      colors of the i entries before it; writes commute, so the only order
-//      kept is between a reader and the writes around it: the reader waits
-//      until i writes are done, and a writer waits until every reader before
-//      it has read;
-//   3. color_kernel: a persistent kernel dequeues nets in priority order from
-//      a device counter.  The wavefront owning net k reads the masks of its
-//      points (each once the cell's write count reaches k's position), takes
-//      the minimum excluded color, then ORs its color into every cell it
-//      marks (each once the cell's read count covers the readers before k's
-//      entry) and counts the write (release).  Spins use relaxed loads with
-//      one acquire fence once satisfied.
-// Every wait is on strictly earlier nets, which were dequeued earlier, so the
-// earliest unfinished net always progresses and every wave exits once the
-// counter passes n.  Work is linear in points + marks and the wait chains
-// follow the true dependency depth of the greedy coloring.
-// The batch size limit of the reference is checked on the host afterwards:
-// if no batch exceeds it the unlimited coloring equals the limited one.
+// Equivalently color(k) = mex( {color(j) : j < k, marks(j) contain a point of
+// k} + {batches already full when k is placed} ).  The GPU form:
+//   1. count_marks / emit_marks: every mark as (cell, net << 1 | self), where
+//      `self` flags the stencil centre of one of the net's points; a stable
+//      radix sort on the cell gives, per cell, the marks in priority order;
+//   2. entry_heads / build_entries: one list entry per (cell, net) pair.  An
+//      entry is a reader when the cell is one of its net's points.  A reader
+//      at list position e needs the colors of every entry before it; the
+//      previous reader r of the same cell has already folded the colors of
+//      the entries before r into its prefix mask P(r), so
+//          P(e) = P(r) | colors of entries r .. e-1
+//      (from the list start when e is the cell's first reader).  A max-scan
+//      over "reader or first entry of a cell" positions finds r;
+//   3. reader_slots / reader_records: per net, a contiguous record per reader
+//      (start entry, own entry, slot of the previous reader);
+//   4. color_kernel: a persistent kernel dequeues nets in priority order.  The
+//      wavefront owning net k waits for the prefix masks of the previous
+//      readers and for the colors of the entries in its ranges, publishes its
+//      own prefix masks, takes the minimum excluded color and publishes it.
+// Every value another wavefront waits on is a single word written once: a
+// net's color (-1 until set) or one 64-bit prefix-mask word whose top bit
+// says it is ready (63 colors per word).  Waits are therefore relaxed atomic
+// loads of that one word, with no fences: nothing is published through a
+// second location.  Each wait is on a strictly earlier net, which was
+// dequeued earlier, so the earliest unfinished net always progresses and
+// every wave exits once the dequeue counter passes n.  Each list entry is
+// read by at most one reader, so the work is linear in points + marks.
+// The batch size limit is applied by the host: colors stay exact up to the
+// first net that would overfill a batch; from that net on the batch is
+// closed (a forbidden color for later nets) and the suffix is colored again.
 #include "gr_kernels.hpp"
 
 namespace grotor {
 
-
-__device__ __forceinline__ int hseg_marks(int first, int second, int step) {
+__device__ __forceinline__ int seg_marks(int first, int second, int step) {
     return second >= first ? (second - first) / step + 1 : 0;
 }
 
@@ -48,8 +49,8 @@ __global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= b.n) return;
     int m = 0;
-    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s) m += hseg_marks(b.hsegs[2 * s], b.hsegs[2 * s + 1], b.sparsity * b.Y);
-    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s) m += hseg_marks(b.vsegs[2 * s], b.vsegs[2 * s + 1], b.sparsity);
+    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s) m += seg_marks(b.hsegs[2 * s], b.hsegs[2 * s + 1], b.sparsity * b.Y);
+    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s) m += seg_marks(b.vsegs[2 * s], b.vsegs[2 * s + 1], b.sparsity);
     for (int q = b.pt_off[k]; q < b.pt_off[k + 1]; ++q) {
         const int px = b.pts[q] / b.Y, py = b.pts[q] % b.Y;
         m += 1 + (px > 0) + (px + 1 < b.X) + (py > 0);
@@ -57,145 +58,197 @@ __global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark
     mark_count[k] = m;
 }
 
-template <class F>
-__device__ __forceinline__ void for_each_mark(const BatchGeom& b, int k, F&& f) {
-    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s)
-        for (int p = b.hsegs[2 * s]; p <= b.hsegs[2 * s + 1]; p += b.sparsity * b.Y) f(p);
-    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s)
-        for (int p = b.vsegs[2 * s]; p <= b.vsegs[2 * s + 1]; p += b.sparsity) f(p);
-    for (int q = b.pt_off[k]; q < b.pt_off[k + 1]; ++q) {
-        const int px = b.pts[q] / b.Y, py = b.pts[q] % b.Y;
-        if (px > 0) f((px - 1) * b.Y + py);
-        if (py > 0) f(px * b.Y + py - 1);
-        f(px * b.Y + py);
-        if (px + 1 < b.X) f((px + 1) * b.Y + py);
-    }
-}
-
-__global__ __launch_bounds__(256) void emit_keys_kernel(BatchGeom b, const int* mark_off,
-                                                        unsigned long long* keys) {
+__global__ __launch_bounds__(256) void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell,
+                                                         unsigned* tag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= b.n) return;
     int m = mark_off[k];
-    for_each_mark(b, k, [&](int cell) { keys[m++] = ((unsigned long long)cell << 32) | (unsigned)k; });
-}
-
-// cell_off[c] = first list entry of cell c; list_net[e] = net of entry e.
-__global__ __launch_bounds__(256) void build_lists_kernel(const unsigned long long* ukeys, const int* unique_count,
-                                                          int cells, int* cell_off, int* list_net) {
-    const int U = *unique_count;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e > U) return;
-    const int cell = e < U ? int(ukeys[e] >> 32) : cells;
-    const int prev = e > 0 ? int(ukeys[e - 1] >> 32) : -1;
-    for (int c = prev + 1; c <= cell; ++c) cell_off[c] = e;
-    if (e < U) list_net[e] = int(ukeys[e] & 0xffffffffu);
-}
-
-// position of net k in the sorted list [begin, end) (k is present)
-__device__ __forceinline__ int list_position(const int* list, int begin, int end, int k) {
-    int lo = begin, hi = end;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (list[mid] < k) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// entry_info[e] = 1 for the list entry of every (point cell, net) pair
-__global__ __launch_bounds__(256) void mark_readers_kernel(BatchGeom b, const int* cell_off, const int* list_net,
-                                                           int* is_reader) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= b.n) return;
+    const unsigned plain = unsigned(k) << 1;
+    auto put = [&](int c, unsigned t) {
+        cell[m] = unsigned(c);
+        tag[m] = t;
+        ++m;
+    };
+    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s)
+        for (int p = b.hsegs[2 * s]; p <= b.hsegs[2 * s + 1]; p += b.sparsity * b.Y) put(p, plain);
+    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s)
+        for (int p = b.vsegs[2 * s]; p <= b.vsegs[2 * s + 1]; p += b.sparsity) put(p, plain);
     for (int q = b.pt_off[k]; q < b.pt_off[k + 1]; ++q) {
-        const int p = b.pts[q];
-        is_reader[list_position(list_net, cell_off[p], cell_off[p + 1], k)] = 1;
+        const int px = b.pts[q] / b.Y, py = b.pts[q] % b.Y;
+        if (px > 0) put((px - 1) * b.Y + py, plain);
+        if (py > 0) put(px * b.Y + py - 1, plain);
+        put(px * b.Y + py, plain | 1u);
+        if (px + 1 < b.X) put((px + 1) * b.Y + py, plain);
     }
 }
 
-// entry_info[e] = 2 * (readers before e in its cell's list) + is_reader[e]
-__global__ __launch_bounds__(256) void entry_info_kernel(const unsigned long long* ukeys, const int* unique_count,
-                                                         const int* cell_off, const int* is_reader,
-                                                         const int* reader_scan, int* entry_info) {
+// head[i] = 1 where sorted mark i starts a new (cell, net) run; head[m] = 0.
+__global__ __launch_bounds__(256) void entry_heads_kernel(const unsigned* cell, const unsigned* tag, int m, int* head) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > m) return;
+    if (i == m) {
+        head[i] = 0;
+        return;
+    }
+    head[i] = (i == 0 || cell[i] != cell[i - 1] || (tag[i] >> 1) != (tag[i - 1] >> 1)) ? 1 : 0;
+}
+
+// One list entry per run: its net, kEntryReader / kEntryFirst flags and the
+// marker position used to find the previous reader or the list start.
+__global__ __launch_bounds__(256) void build_entries_kernel(const unsigned* cell, const unsigned* tag, const int* eidx,
+                                                            int m, int* list_net, int* einfo, int* marker) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int e = eidx[i];
+    if (eidx[i + 1] == e) return;  // not a run head
+    const unsigned c = cell[i], k = tag[i] >> 1;
+    unsigned reader = 0;
+    for (int j = i; j < m && cell[j] == c && (tag[j] >> 1) == k; ++j) reader |= tag[j] & 1u;
+    const int first = (i == 0 || cell[i - 1] != c) ? kEntryFirst : 0;
+    const int info = int(reader) * kEntryReader | first;
+    list_net[e] = int(k);
+    einfo[e] = info;
+    marker[e] = info ? e : -1;
+}
+
+// slot_of[e] = record slot of reader entry e inside its net's point range
+__global__ __launch_bounds__(256) void reader_slots_kernel(int u, const int* list_net, const int* einfo,
+                                                           const int* pt_off, int* rcount, int* slot_of) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *unique_count) return;
-    const int cell = int(ukeys[e] >> 32);
-    entry_info[e] = 2 * (reader_scan[e] - reader_scan[cell_off[cell]]) + is_reader[e];
+    if (e >= u || !(einfo[e] & kEntryReader)) return;
+    const int k = list_net[e];
+    slot_of[e] = pt_off[k] + atomicAdd(rcount + k, 1);
 }
 
-// Spin until *counter >= target with relaxed loads, then one acquire fence.
-// False if the spin limit was hit or another wave reported an error.
-__device__ __forceinline__ bool wait_count(const ColorArgs& a, const int* counter, int target) {
-    long long spins = 0;
-    int sleep = 1;
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-        if (++spins > a.spin_limit) {
-            atomicExch(a.error, (int)kDevSpinLimit);
-            return false;
-        }
-        if ((spins & 255) == 0 && __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            return false;
-        if (sleep == 1) __builtin_amdgcn_s_sleep(1);
-        else __builtin_amdgcn_s_sleep(4);
-        if (spins > 64) sleep = 4;
+__global__ __launch_bounds__(256) void reader_records_kernel(int u, const int* einfo, const int* prev_marker,
+                                                             const int* slot_of, int4* rec) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= u) return;
+    const int info = einfo[e];
+    if (!(info & kEntryReader)) return;
+    int start = e, prev = -1;
+    if (!(info & kEntryFirst)) {
+        start = prev_marker[e];  // previous reader of this cell, or its first entry
+        if (einfo[start] & kEntryReader) prev = slot_of[start];
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
+    rec[slot_of[e]] = make_int4(start, e, prev, 0);
 }
 
+namespace {
+
+constexpr unsigned long long kReady = 1ull << 63;
+constexpr unsigned long long kColorBits = kReady - 1;
+
+__device__ __forceinline__ bool spin_abort(const ColorArgs& a, long long& spins) {
+    if (++spins > a.spin_limit) {
+        atomicExch(a.error, (int)kDevSpinLimit);
+        return true;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(a.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return true;
+    __builtin_amdgcn_s_sleep(2);
+    return false;
+}
+
+// Color of an earlier net: a near (possibly stale) read first, then
+// coherent reads until it is set.
+__device__ __forceinline__ int await_color(const ColorArgs& a, int net) {
+    int c = __hip_atomic_load(a.color + net, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    long long spins = 0;
+    while (c < 0) {
+        c = __hip_atomic_load(a.color + net, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (c < 0 && spin_abort(a, spins)) return 0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ unsigned long long await_mask(const ColorArgs& a, const unsigned long long* p) {
+    unsigned long long v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    long long spins = 0;
+    while (!(v & kReady)) {
+        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(v & kReady) && spin_abort(a, spins)) return 0;
+    }
+    return v & kColorBits;
+}
+
+}  // namespace
+
+// Dynamic LDS: per lane W prefix-mask words, then W forbidden words.
 __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
-    __shared__ unsigned long long forbidden[kMaxMaskWords];
-    __shared__ int s_k, s_color;
+    extern __shared__ unsigned long long lds[];
+    __shared__ int s_off[kWave], s_start[kWave];
+    __shared__ int s_k;
     const int lane = threadIdx.x;
     const int W = a.mask_words;
+    unsigned long long* P = lds;
+    unsigned long long* forbidden = lds + kWave * W;
     while (true) {
         if (lane == 0) s_k = atomicAdd(a.next, 1);
-        if (lane < kMaxMaskWords) forbidden[lane] = 0ull;
         __syncthreads();
         const int k = s_k;
-        if (k >= a.b.n) break;
-        // reads: colors of the earlier nets that mark k's points
-        for (int q = a.b.pt_off[k] + lane; q < a.b.pt_off[k + 1]; q += kWave) {
-            const int p = a.b.pts[q];
-            const int begin = a.cell_off[p];
-            const int e = list_position(a.list_net, begin, a.cell_off[p + 1], k);
-            if (!wait_count(a, a.writes + p, e - begin)) continue;
-            for (int w = 0; w < W; ++w) {
-                const unsigned long long m = __hip_atomic_load(a.mask + (size_t)p * W + w, __ATOMIC_RELAXED,
-                                                               __HIP_MEMORY_SCOPE_AGENT);
-                if (m) atomicOr(&forbidden[w], m);
+        if (k >= a.n) break;
+        for (int w = lane; w < W; w += kWave) forbidden[w] = 0ull;
+        __syncthreads();
+        for (int i = lane; i < a.nclosed; i += kWave) {  // batches already full when k is placed
+            if (a.closed_from[i] > k) continue;
+            const int c = a.closed_color[i];
+            atomicOr(&forbidden[c / kColorsPerWord], 1ull << (c % kColorsPerWord));
+        }
+        const int r0 = a.pt_off[k];
+        const int nr = a.rcount[k];
+        for (int base = 0; base < nr; base += kWave) {
+            const bool active = base + lane < nr;
+            int4 rc = make_int4(0, 0, -1, 0);
+            if (active) rc = a.rec[r0 + base + lane];
+            // own row: the previous reader's prefix (or nothing)
+            for (int w = 0; w < W; ++w)
+                P[lane * W + w] = rc.z >= 0 ? await_mask(a, a.pm + (size_t)rc.z * W + w) : 0ull;
+            // entries rc.x .. rc.y-1 of every lane, spread over the wave
+            const int cnt = active ? rc.y - rc.x : 0;
+            int incl = cnt;
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int v = __shfl_up(incl, d, kWave);
+                if (lane >= d) incl += v;
             }
-            // reader rank r done: later writers of this cell may proceed
-            __hip_atomic_fetch_max(a.reads + p, (a.entry_info[e] >> 1) + 1, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+            const int total = __shfl(incl, kWave - 1, kWave);
+            s_off[lane] = incl - cnt;
+            s_start[lane] = rc.x;
+            __syncthreads();
+            for (int t = lane; t < total; t += kWave) {
+                int lo = 0, hi = kWave - 1;  // last lane whose range starts at or before t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int c = await_color(a, a.list_net[s_start[lo] + (t - s_off[lo])]);
+                atomicOr(&P[lo * W + c / kColorsPerWord], 1ull << (c % kColorsPerWord));
+            }
+            __syncthreads();
+            if (active) {
+                for (int w = 0; w < W; ++w) {
+                    const unsigned long long v = P[lane * W + w];
+                    __hip_atomic_store(a.pm + (size_t)(r0 + base + lane) * W + w, v | kReady, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    if (v) atomicOr(&forbidden[w], v);
+                }
+            }
+            __syncthreads();
         }
         __syncthreads();
         if (lane == 0) {
             int mex = -1;
-            for (int w = 0; w < W && mex < 0; ++w)
-                if (~forbidden[w]) mex = w * 64 + __builtin_ctzll(~forbidden[w]);
+            for (int w = 0; w < W && mex < 0; ++w) {
+                const unsigned long long free = ~forbidden[w] & kColorBits;
+                if (free) mex = w * kColorsPerWord + __builtin_ctzll(free);
+            }
             if (mex < 0) {
                 atomicExch(a.error, (int)kDevColorOverflow);
-                mex = 64 * W - 1;
+                mex = 0;
             }
-            s_color = mex;
-            a.color[k] = mex;
             atomicAdd(a.color_count + mex, 1);
             atomicMax(a.max_color, mex);
-        }
-        __syncthreads();
-        // writes: k's color into every cell it marks (each list entry once)
-        const int color = s_color;
-        const unsigned long long bit = 1ull << (color & 63);
-        for (int m = a.mark_off[k] + lane; m < a.mark_off[k + 1]; m += kWave) {
-            const int cell = int(a.keys[m] >> 32);
-            const int e = list_position(a.list_net, a.cell_off[cell], a.cell_off[cell + 1], k);
-            if (atomicExch(a.claimed + e, 1)) continue;  // duplicate mark of the same cell
-            const int info = a.entry_info[e];
-            if (!wait_count(a, a.reads + cell, (info >> 1) + (info & 1))) continue;
-            atomicOr(a.mask + (size_t)cell * W + (color >> 6), bit);
-            __hip_atomic_fetch_add(a.writes + cell, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(a.color + k, mex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
     }

@@ -93,31 +93,34 @@ struct BatchGeom {
     const int* vsegs;
 };
 __global__ void count_marks_kernel(BatchGeom b, int* mark_count);
-__global__ void emit_keys_kernel(BatchGeom b, const int* mark_off, unsigned long long* keys);
-__global__ void build_lists_kernel(const unsigned long long* ukeys, const int* unique_count, int cells, int* cell_off,
-                                   int* list_net);
-__global__ void mark_readers_kernel(BatchGeom b, const int* cell_off, const int* list_net, int* is_reader);
-__global__ void entry_info_kernel(const unsigned long long* ukeys, const int* unique_count, const int* cell_off,
-                                  const int* is_reader, const int* reader_scan, int* entry_info);
-constexpr int kMaxMaskWords = 64;  // colors 0..4095
+__global__ void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell, unsigned* tag);
+__global__ void entry_heads_kernel(const unsigned* cell, const unsigned* tag, int m, int* head);
+constexpr int kEntryReader = 1;  // the cell is one of the entry's net's points
+constexpr int kEntryFirst = 2;   // first entry of its cell's list
+__global__ void build_entries_kernel(const unsigned* cell, const unsigned* tag, const int* eidx, int m, int* list_net,
+                                     int* einfo, int* marker);
+__global__ void reader_slots_kernel(int u, const int* list_net, const int* einfo, const int* pt_off, int* rcount,
+                                    int* slot_of);
+__global__ void reader_records_kernel(int u, const int* einfo, const int* prev_marker, const int* slot_of, int4* rec);
+constexpr int kColorsPerWord = 63;  // prefix-mask word: 63 color bits + a ready bit
+constexpr int kMaxMaskWords = 32;   // colors 0..2015
 struct ColorArgs {
-    BatchGeom b;
-    const int* mark_off;
-    const unsigned long long* keys;  // marks in emission order (cell << 32 | net)
-    const int* cell_off;
-    const int* list_net;
-    const int* entry_info;     // 2 * reader rank + is_reader, per list entry
-    int* claimed;              // per list entry: write done
-    unsigned long long* mask;  // [cells * mask_words]
-    int* writes;               // [cells] completed writes
-    int* reads;                // [cells] completed readers
+    int n;
+    const int* pt_off;    // per net: first record slot
+    const int* rcount;    // per net: reader records
+    const int4* rec;      // per slot: (first entry read, own entry, previous reader's slot or -1, 0)
+    const int* list_net;  // per entry: its net
+    unsigned long long* pm;  // per slot: mask_words prefix-mask words
     int mask_words;
-    int* color;
+    int* color;           // per net, -1 until set
     int* color_count;
-    int* next;
+    int* next;            // dequeue counter
     int* max_color;
     int* error;
     long long spin_limit;
+    int nclosed;          // full batches: color closed_color[i] for nets >= closed_from[i]
+    const int* closed_color;
+    const int* closed_from;
 };
 __global__ void color_kernel(ColorArgs a);
 

@@ -93,11 +93,24 @@ int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vi
  * points: 2D cells x*Y + y; hsegs / vsegs: (first, second) pairs of 2D cells.
  * Writes the batch index of net k to color[k] and the batch count.  Batches
  * are exactly the reference's lists: batch b holds the nets with color b in
- * priority order.  Runs on the GPU (bin-grid conflict lists + in-order
- * colouring); `limit` is honoured exactly. */
+ * priority order, no batch holds more than `limit` nets, and a net skips
+ * batches that are full when it is placed.  Equals gr_batch_geometry followed
+ * by gr_batch_colors. */
 int gr_batch_plan(gr_ctx *ctx, int n, const int *pt_off, const int *pts,
                   const int *hs_off, const int *hsegs, const int *vs_off, const int *vsegs,
                   int limit, int sparsity, int *color, int *num_batches);
+
+/* Validate and upload the batching geometry of n nets (priority order) into
+ * the context, where it stays until the next call: the reference's GPU
+ * batching uploads it before its batch interval starts
+ * (src/batching/hiprt/grid_chunk.cuh:373-428). */
+int gr_batch_geometry(gr_ctx *ctx, int n, const int *pt_off, const int *pts,
+                      const int *hs_off, const int *hsegs, const int *vs_off, const int *vsegs);
+
+/* Batch the uploaded geometry: the reference's batch interval
+ * (generate_batch_ids, src/batching/hiprt/grid_chunk.cuh:519-524) plus the
+ * download of the batch ids. */
+int gr_batch_colors(gr_ctx *ctx, int limit, int sparsity, int *color, int *num_batches);
 
 /* L-shape stage topology, uploaded once per stage (src/routing/lshape.cuh:126-138):
  * net_ids[k] is the subnet solved at stage slot k; its DFS-preorder tree is

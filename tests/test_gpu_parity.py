@@ -50,6 +50,46 @@ def test_batch_plan_matches_oracle(ga, ora, config, seed):
     ctx.close()
 
 
+@pytest.mark.parametrize("limit", [40, 150, 600])
+def test_batch_limit_matches_oracle(ga, ora, limit):
+    """Batches that fill up close for later nets (generate_batches_cpu.hpp:50)."""
+    _, t = topology(1, 0)
+    print(f"batch limit={limit} nets={t['nets']}", flush=True)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    ocolor, onb = ora.batches(t["X"], t["Y"], *geom, limit=limit)
+    ctx = ga.Context(t)
+    ctx.batch_geometry(*geom)
+    color, nb = ctx.batch_colors(limit=limit)
+    assert nb == onb and np.array_equal(color, ocolor)
+    assert np.bincount(color).max() <= limit
+    color2, nb2 = ctx.batch_colors()  # same resident geometry, no limit
+    ucolor, unb = ora.batches(t["X"], t["Y"], *geom)
+    assert nb2 == unb and np.array_equal(color2, ucolor)
+    ctx.close()
+
+
+def test_batch_edge_cases(ga, ora):
+    """Nets without points or segments, one net, and wide color masks."""
+    _, t = topology(6, 0)
+    ctx = ga.Context(t)
+    n = 5
+    empty = np.zeros(n + 1, np.int32)
+    color, nb = ctx.batch_plan(empty, np.zeros(0, np.int32), empty, np.zeros(0, np.int32), empty,
+                               np.zeros(0, np.int32))
+    assert nb == 1 and np.array_equal(color, np.zeros(n, np.int32))
+    # every net on the same cell: one batch per net, beyond one mask word
+    k = 150
+    off = np.arange(k + 1, dtype=np.int32)
+    pts = np.full(k, t["Y"] * 3 + 4, np.int32)
+    color, nb = ctx.batch_plan(off, pts, np.zeros(k + 1, np.int32), np.zeros(0, np.int32),
+                               np.zeros(k + 1, np.int32), np.zeros(0, np.int32))
+    assert nb == k and np.array_equal(color, np.arange(k, dtype=np.int32))
+    with pytest.raises(ga.GrError):
+        ctx.batch_plan(off, np.full(k, t["X"] * t["Y"], np.int32), np.zeros(k + 1, np.int32),
+                       np.zeros(0, np.int32), np.zeros(k + 1, np.int32), np.zeros(0, np.int32))
+    ctx.close()
+
+
 @pytest.mark.parametrize("config,seed", [(1, 0), (6, 2)])
 def test_lshape_stage_matches_oracle(ga, ora, config, seed):
     _, t = topology(config, seed)
