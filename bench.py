@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--net-scale", type=float, default=1.0)
     ap.add_argument("--mode", type=int, default=0, help="0 plain launches, 1 HIP graph, 2 persistent")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
@@ -111,7 +113,7 @@ def main():
     ctx.set_routes(topo["pin_acc_num"])
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
 
-    def step():
+    def step(mode=args.mode):
         ctx.reset()
         u0 = time.perf_counter()
         ctx.batch_geometry(*geom)
@@ -121,7 +123,7 @@ def main():
         batch_ms = (time.perf_counter() - b0) * 1e3
         sched = ga.execution_schedule(topo, color, nb)
         ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
-        dag_ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=args.mode)
+        dag_ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
         return batch_ms, dag_ms, nb, color, sched, upload_ms
 
     for i in range(args.warmup):
@@ -129,7 +131,12 @@ def main():
         print(f"[bench] warmup {i}: batching {bm:.2f} ms (+ geometry upload {um:.2f} ms), DAG {dm:.2f} ms, "
               f"{nb} batches", file=sys.stderr, flush=True)
 
-    ctx.profile_begin()
+    # per-kernel HIP events in the timed steps (mode 0); a captured graph
+    # (mode 1) cannot carry them, so its roofline comes from one extra
+    # mode-0 step after the timed ones
+    inline_profile = args.mode == 0 and not args.no_profile
+    if inline_profile:
+        ctx.profile_begin()
     total_ms = 0.0
     batch_total = dag_total = upload_total = 0.0
     for i in range(args.steps):
@@ -140,9 +147,14 @@ def main():
         total_ms += bm + dm
         print(f"[bench] step {i}: batching {bm:.2f} ms (+ geometry upload {um:.2f} ms), DAG {dm:.2f} ms",
               file=sys.stderr, flush=True)
-    prof = ctx.profile_end()
     routes = ctx.routes()
     wl, vias = ctx.stats()
+    if inline_profile:
+        prof = ctx.profile_end()
+    else:
+        ctx.profile_begin()
+        step(mode=0)
+        prof = ctx.profile_end()
 
     nets = topo["nets"]
     value = nets * args.steps / (total_ms / 1e3)
@@ -153,7 +165,8 @@ def main():
     achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": measured_traffic(KIND_NAMES[dom], args.config) if args.mode == 0 else None,
+                "traffic": measured_traffic(KIND_NAMES[dom], args.config),
+                "events": "timed steps" if inline_profile else "one extra mode-0 step after the timed steps",
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
                 "launches": int(dom_launches),
                 "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}

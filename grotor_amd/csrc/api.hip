@@ -144,7 +144,23 @@ struct gr_ctx {
     unsigned long long prof_wire_edges = 0, prof_vias = 0;  // counts of profiled steps before a reset
     unsigned long long refreshed0 = 0;
 
+    // batch loop captured as a HIP graph (gr_lshape_run mode 1), reused
+    // while the stage topology, batch offsets and first stamp are unchanged
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<int> graph_key;
+    int stage_gen = 0;
+
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_key.clear();
+    }
+
     ~gr_ctx() {
+        drop_graph();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -198,6 +214,26 @@ void from_track_major(const DeviceGrid& g, const std::vector<T>& src, T* dst) {
     for (int l = 0; l < g.L; ++l)
         for (int x = 0; x < g.X; ++x)
             for (int y = 0; y < g.Y; ++y) dst[(size_t(l) * g.X + x) * g.Y + y] = src[cell_of(g, l, x, y)];
+}
+
+Stats read_stats(gr_ctx* c) {
+    Stats part[kStatStripes];
+    download(part, c->stats, sizeof(part), c->stream);
+    Stats s{};
+    for (const Stats& p : part) {
+        s.total_wirelength += p.total_wirelength;
+        s.total_vias += p.total_vias;
+        s.wire_edges += p.wire_edges;
+    }
+    return s;
+}
+
+unsigned long long read_refreshed(gr_ctx* c) {
+    std::vector<unsigned long long> part(size_t(kStatStripes) * kCounterStride);
+    download(part.data(), c->refreshed, sizeof(unsigned long long) * part.size(), c->stream);
+    unsigned long long sum = 0;
+    for (int i = 0; i < kStatStripes; ++i) sum += part[size_t(i) * kCounterStride];
+    return sum;
 }
 
 void check_device_error(gr_ctx* c, const char* op) {
@@ -309,8 +345,7 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
         a.count = b.small;
         a.nmax = b.nmax_small;
         timed(c, GR_K_LSHAPE, bytes_of(b.begin, b.small), [&] {
-            hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.small), dim3(kWave), lshape_scratch_bytes(c->g.L, a.nmax),
-                               c->stream, a);
+            launch_lshape_dp(a, b.small, true, c->stream);
         });
     }
     if (b.large) {
@@ -321,17 +356,30 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
         const double alg = bytes_of(b.begin + b.small, b.large);
         if (bytes <= kLdsLimit) {
             timed(c, GR_K_LSHAPE, alg, [&] {
-                hipLaunchKernelGGL(lshape_dp_lds_kernel, dim3(b.large), dim3(kWave), bytes, c->stream, a);
+                launch_lshape_dp(a, b.large, true, c->stream);
             });
         } else {
-            ensure_gscratch(c, ((bytes + 7) / 8) * 8 * size_t(b.large));
+            if (((bytes + 7) / 8) * 8 * size_t(b.large) > c->gscratch_bytes)
+                throw GrError(GR_E_STATE, "DP scratch not reserved");
             a.gscratch = c->gscratch;
             timed(c, GR_K_LSHAPE, alg, [&] {
-                hipLaunchKernelGGL(lshape_dp_global_kernel, dim3(b.large), dim3(kWave), 0, c->stream, a);
+                launch_lshape_dp(a, b.large, false, c->stream);
             });
         }
     }
     hip_check(hipGetLastError(), "lshape_dp kernels");
+}
+
+// Reserve the global DP scratch the batches will need (trees whose tables
+// do not fit in LDS) before anything is launched or captured.
+void reserve_dp_scratch(gr_ctx* c, const std::vector<BatchLaunch>& plan) {
+    size_t need = 0;
+    for (const BatchLaunch& b : plan) {
+        if (!b.large) continue;
+        const size_t bytes = lshape_scratch_bytes(c->g.L, b.nmax_large);
+        if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.large));
+    }
+    ensure_gscratch(c, need);
 }
 
 void require_stage(gr_ctx* c, int begin, int count) {
@@ -358,7 +406,7 @@ int gr_device_name(char* buffer, int size) {
 int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
     return guarded([&] {
         if (!grid || !out) throw GrError(GR_E_ARG, "null argument");
-        if (grid->L < 1 || grid->L > kMaxLayers) throw GrError(GR_E_ARG, "routing layers must be 1..16");
+        if (grid->L < 1 || grid->L >= kMaxLayers) throw GrError(GR_E_ARG, "routing layers must be 1..15");
         if (grid->X < 2 || grid->Y < 2) throw GrError(GR_E_ARG, "grid must be at least 2 x 2");
         if (size_t(grid->L) * grid->X * grid->Y > size_t(INT_MAX)) throw GrError(GR_E_ARG, "grid exceeds 2^31 cells");
         int count = 0;
@@ -395,16 +443,17 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         c->presum = c->mem.alloc<double>(c->cells);
         c->timestamp = c->mem.alloc<int>(c->cells);
         c->track_dirty = c->mem.alloc<unsigned>(c->tracks);
-        c->stats = c->mem.alloc<Stats>(1);
+        c->stats = c->mem.alloc<Stats>(kStatStripes);
         c->error = c->mem.alloc<int>(1);
-        c->refreshed = c->mem.alloc<unsigned long long>(1);
-        hip_check(hipMemsetAsync(c->refreshed, 0, sizeof(unsigned long long), c->stream), "clear counter");
+        c->refreshed = c->mem.alloc<unsigned long long>(kStatStripes * kCounterStride);
+        hip_check(hipMemsetAsync(c->refreshed, 0, sizeof(unsigned long long) * kStatStripes * kCounterStride, c->stream),
+                  "clear counter");
         std::vector<float> cap;
         to_track_major(g, grid->capacity, cap);
         upload(c->capacity, cap.data(), sizeof(float) * c->cells, c->stream);
         hip_check(hipMemsetAsync(c->demand, 0, sizeof(float) * c->cells, c->stream), "clear demand");
         hip_check(hipMemsetAsync(c->timestamp, 0, sizeof(int) * c->cells, c->stream), "clear timestamps");
-        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream), "clear statistics");
+        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats) * kStatStripes, c->stream), "clear statistics");
         hip_check(hipMemsetAsync(c->error, 0, sizeof(int), c->stream), "clear error");
         std::vector<unsigned> ones(c->tracks, 1u);
         upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
@@ -519,8 +568,7 @@ int gr_timestamp_download(gr_ctx* c, int* ts) {
 int gr_stats_download(gr_ctx* c, double* wl, long long* vias) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
-        Stats s;
-        download(&s, c->stats, sizeof(Stats), c->stream);
+        const Stats s = read_stats(c);
         if (wl) *wl = s.total_wirelength;
         if (vias) *vias = (long long)s.total_vias;
     });
@@ -548,6 +596,9 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
             c->mem.release(q);
         }
         c->net_ids = c->node_off = c->nodes = c->parents = c->slots = nullptr;
+        c->slots_h.clear();  // the new slot buffer holds nothing yet
+        c->drop_graph();
+        ++c->stage_gen;
         const int total = node_off[count];
         c->stage_count = count;
         c->node_off_h.assign(node_off, node_off + count + 1);
@@ -585,6 +636,7 @@ int gr_lshape_batch(gr_ctx* c, int begin, int count, int stamp) {
         require_stage(c, begin, count);
         std::vector<int> slots(c->stage_count);
         const BatchLaunch b = plan_batch(c, begin, count, slots);
+        reserve_dp_scratch(c, {b});
         c->slots_h = slots;
         upload(c->slots + begin, slots.data() + begin, sizeof(int) * count, c->stream);
         launch_lshape(c, b);
@@ -605,24 +657,59 @@ int gr_commit_batch(gr_ctx* c, int begin, int count, int stamp, int K) {
 int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, int mode, double* elapsed_ms) {
     return guarded([&] {
         if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
-        if (mode != 0) throw GrError(GR_E_ARG, "execution mode not available in this build");
+        if (mode != 0 && mode != 1) throw GrError(GR_E_ARG, "execution mode not available (0 = launches, 1 = HIP graph)");
         require_stage(c, 0, batch_off[nbatch]);
+        if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
         std::vector<int> slots(c->stage_count);
         std::vector<BatchLaunch> plan(nbatch);
         for (int b = 0; b < nbatch; ++b) {
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
             plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
         }
-        c->slots_h = slots;
-        upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
+        reserve_dp_scratch(c, plan);
+        auto enqueue = [&] {
+            for (int b = 0; b < nbatch; ++b) {
+                launch_refresh(c, 0);
+                launch_lshape(c, plan[b]);
+                launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
+            }
+        };
+        if (mode == 1) {
+            std::vector<int> key{c->stage_gen, first_stamp, nbatch};
+            key.insert(key.end(), batch_off, batch_off + nbatch + 1);
+            if (c->slots_h != slots) {  // the graph reads the slot permutation when it runs
+                c->slots_h = slots;
+                upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
+            }
+            if (!c->graph_exec || key != c->graph_key) {
+                c->drop_graph();
+                hip_check(hipStreamSynchronize(c->stream), "before graph capture");
+                const bool profiling = c->profiling;
+                c->profiling = false;
+                hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    enqueue();
+                } catch (...) {
+                    hipGraph_t broken = nullptr;
+                    (void)hipStreamEndCapture(c->stream, &broken);
+                    if (broken) (void)hipGraphDestroy(broken);
+                    c->profiling = profiling;
+                    throw;
+                }
+                c->profiling = profiling;
+                hip_check(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
+                hip_check(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                c->graph_key = key;
+            }
+        } else {
+            c->slots_h = slots;
+            upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
+        }
         hip_check(hipStreamSynchronize(c->stream), "stage plan");
         hip_check(hipDeviceSynchronize(), "before DAG interval");
         const auto t0 = std::chrono::steady_clock::now();
-        for (int b = 0; b < nbatch; ++b) {
-            launch_refresh(c, 0);
-            launch_lshape(c, plan[b]);
-            launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
-        }
+        if (mode == 1) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
+        else enqueue();
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -635,15 +722,14 @@ int gr_ctx_reset(gr_ctx* c) {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (c->profiling) {
             // keep the committed edge and via counts of the profiled interval
-            Stats now;
-            download(&now, c->stats, sizeof(Stats), c->stream);
+            const Stats now = read_stats(c);
             c->prof_wire_edges += now.wire_edges - c->prof_stats0.wire_edges;
             c->prof_vias += now.total_vias - c->prof_stats0.total_vias;
             c->prof_stats0 = Stats{};
         }
         hip_check(hipMemsetAsync(c->demand, 0, sizeof(float) * c->cells, c->stream), "clear demand");
         hip_check(hipMemsetAsync(c->timestamp, 0, sizeof(int) * c->cells, c->stream), "clear timestamps");
-        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats), c->stream), "clear statistics");
+        hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats) * kStatStripes, c->stream), "clear statistics");
         if (c->routes)
             hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
         std::vector<unsigned> ones(c->tracks, 1u);
@@ -658,9 +744,9 @@ int gr_profile_begin(gr_ctx* c) {
         hip_check(hipStreamSynchronize(c->stream), "gr_profile_begin");
         c->prof.clear();
         c->next_event = 0;
-        download(&c->prof_stats0, c->stats, sizeof(Stats), c->stream);
+        c->prof_stats0 = read_stats(c);
         c->prof_wire_edges = c->prof_vias = 0;
-        download(&c->refreshed0, c->refreshed, sizeof(unsigned long long), c->stream);
+        c->refreshed0 = read_refreshed(c);
         c->profiling = true;
     });
 }
@@ -679,10 +765,8 @@ int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, in
             by[r.kind] += r.bytes;
             n[r.kind] += 1;
         }
-        Stats s1;
-        unsigned long long refreshed1 = 0;
-        download(&s1, c->stats, sizeof(Stats), c->stream);
-        download(&refreshed1, c->refreshed, sizeof(unsigned long long), c->stream);
+        const Stats s1 = read_stats(c);
+        const unsigned long long refreshed1 = read_refreshed(c);
         // refresh, per cell of a refreshed track: read capacity + demand, write
         // wcost + vcost (4 B each) and presum (8 B); wcost stays in LDS here
         // but is counted, as for the reference's full refresh
@@ -843,7 +927,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             rcount = cv.template take<int>(n);
             d_color = cv.template take<int>(n);
             d_count = cv.template take<int>(kMaxColors);
-            d_misc = cv.template take<int>(2);  // dequeue counter, max color
+            d_misc = cv.template take<int>(1);  // dequeue counter
             d_closed = cv.template take<int>(2 * kMaxColors);
             tmp = cv.template take<char>(std::max(sort_bytes, std::max(sum_bytes, max_bytes)));
         });
@@ -897,8 +981,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             hipLaunchKernelGGL(fill_int_kernel, dim3((n - first + 255) / 256), dim3(256), 0, st, d_color + first, -1,
                                (long long)(n - first));
             hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
-            const int misc[2] = {first, -1};
-            upload(d_misc, misc, sizeof(misc), st);
+            upload(d_misc, &first, sizeof(int), st);
             const int nclosed = int(closed_color.size());
             if (nclosed) {
                 upload(d_closed, closed_color.data(), sizeof(int) * nclosed, st);
@@ -913,9 +996,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             ca.pm = pm;
             ca.mask_words = words;
             ca.color = d_color;
-            ca.color_count = d_count;
             ca.next = d_misc;
-            ca.max_color = d_misc + 1;
             ca.error = c->error;
             ca.spin_limit = 1ll << 22;
             ca.nclosed = nclosed;
@@ -931,9 +1012,9 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             const size_t color_rec = c->prof.size();
             timed(c, GR_K_BATCH_COLOR, color_bytes,
                   [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), lds, st, ca); });
-            hip_check(hipGetLastError(), "color_kernel");
-            int maxc[2] = {0, -1};
-            download(maxc, d_misc, sizeof(maxc), st);
+            hipLaunchKernelGGL(color_hist_kernel, dim3(std::min(1024, (n - first + 255) / 256)), dim3(256), 0, st,
+                               d_color, first, n, d_count);
+            hip_check(hipGetLastError(), "color kernels");
             int err = 0;
             download(&err, c->error, sizeof(int), st);
             if (err == kDevColorOverflow && words < kMaxMaskWords) {
@@ -947,9 +1028,9 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
                 continue;
             }
             check_device_error(c, "gr_batch_colors");
-            const int nb_run = maxc[1] + 1;
-            counts.assign(nb_run, 0);
-            download(counts.data(), d_count, sizeof(int) * nb_run, st);
+            counts.assign(kMaxColors, 0);
+            download(counts.data(), d_count, sizeof(int) * kMaxColors, st);
+            while (!counts.empty() && !counts.back()) counts.pop_back();
             bool over = false;
             for (int v : counts) over |= v > limit;
             if (!over && !first) break;

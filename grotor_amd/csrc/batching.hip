@@ -26,6 +26,8 @@
 //      wavefront owning net k waits for the prefix masks of the previous
 //      readers and for the colors of the entries in its ranges, publishes its
 //      own prefix masks, takes the minimum excluded color and publishes it.
+// Nets are dequeued kColorChunk at a time, one wave finishing them in order,
+// so the dequeue counter is not a serialization point.
 // Every value another wavefront waits on is a single word written once: a
 // net's color (-1 until set) or one 64-bit prefix-mask word whose top bit
 // says it is ready (63 colors per word).  Waits are therefore relaxed atomic
@@ -182,10 +184,15 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
     const int W = a.mask_words;
     unsigned long long* P = lds;
     unsigned long long* forbidden = lds + kWave * W;
+    int k = 0, chunk_end = 0;
     while (true) {
-        if (lane == 0) s_k = atomicAdd(a.next, 1);
-        __syncthreads();
-        const int k = s_k;
+        // nets are taken in priority order, kColorChunk at a time
+        if (k == chunk_end) {
+            if (lane == 0) s_k = atomicAdd(a.next, kColorChunk);
+            __syncthreads();
+            k = s_k;
+            chunk_end = min(k + kColorChunk, a.n);
+        }
         if (k >= a.n) break;
         for (int w = lane; w < W; w += kWave) forbidden[w] = 0ull;
         __syncthreads();
@@ -246,12 +253,23 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
                 atomicExch(a.error, (int)kDevColorOverflow);
                 mex = 0;
             }
-            atomicAdd(a.color_count + mex, 1);
-            atomicMax(a.max_color, mex);
             __hip_atomic_store(a.color + k, mex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
+        ++k;
     }
+}
+
+// counts[c] += nets of [first, n) with color c; per-block LDS histogram
+__global__ __launch_bounds__(256) void color_hist_kernel(const int* color, int first, int n, int* counts) {
+    __shared__ int h[kColorsPerWord * kMaxMaskWords];
+    for (int c = threadIdx.x; c < kColorsPerWord * kMaxMaskWords; c += blockDim.x) h[c] = 0;
+    __syncthreads();
+    for (int k = first + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+        atomicAdd(&h[color[k]], 1);
+    __syncthreads();
+    for (int c = threadIdx.x; c < kColorsPerWord * kMaxMaskWords; c += blockDim.x)
+        if (h[c]) atomicAdd(counts + c, h[c]);
 }
 
 }  // namespace grotor

@@ -23,42 +23,90 @@ __device__ __forceinline__ void add_demand(const CommitArgs& a, int cell, float 
     a.track_dirty[track_of_cell(a.g, cell)] = 1u;
 }
 
+// Wires: the unit edges of all of a net's segments are spread over the 64
+// lanes of its wavefront (segment geometry per lane, a wave prefix sum of
+// edge counts, then edge t goes to lane t mod 64), so a net costs
+// ceil(edges / 64) steps however its edges split into segments.
 __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
+    __shared__ int s_off[4][kWave], s_cell[4][kWave], s_step[4][kWave], s_lo[4][kWave];
+    __shared__ unsigned char s_flags[4][kWave];
     const int lane = threadIdx.x & (kWave - 1);
+    const int wib = threadIdx.x / kWave;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int waves = gridDim.x * blockDim.x / kWave;
     const DeviceGrid& g = a.g;
+    int* off = s_off[wib];
+    int* cell0 = s_cell[wib];
+    int* step = s_step[wib];
+    int* lo_of = s_lo[wib];
+    unsigned char* flags = s_flags[wib];  // bit 0: along y, bit 1: one track
     for (int s = wave; s < a.count; s += waves) {
         const int slot = a.slots ? a.slots[a.begin + s] : a.begin + s;
         const int* r = route_of(a.rs, a.net_ids[slot]);
-        const int end = r[0];
+        const int nseg = (r[0] - 1) / 2;
         double length = 0.0;
         long long edges = 0;
-        for (int i = 1; i < end; i += 2) {
-            const int ra = r[i], rb = r[i + 1], layer = ra / g.XY;
-            if (layer != rb / g.XY) continue;
-            const int x0 = ra / g.Y % g.X, y0 = ra % g.Y, x1 = rb / g.Y % g.X, y1 = rb % g.Y;
-            const bool along_y = x0 == x1;
-            const int lo = along_y ? min(y0, y1) : min(x0, x1);
-            const int hi = along_y ? max(y0, y1) : max(x0, x1);
-            edges += hi - lo;
-            if (lane == 0) {
-                a.timestamp[cell_of(g, layer, x0, y0)] = a.stamp;
-                a.timestamp[cell_of(g, layer, x1, y1)] = a.stamp;
+        for (int base = 0; base < nseg; base += kWave) {
+            int cnt = 0;
+            if (base + lane < nseg) {
+                const int ra = r[1 + 2 * (base + lane)], rb = r[2 + 2 * (base + lane)];
+                const int layer = ra / g.XY;
+                if (layer == rb / g.XY) {
+                    const int x0 = ra / g.Y % g.X, y0 = ra % g.Y, x1 = rb / g.Y % g.X, y1 = rb % g.Y;
+                    const bool along_y = x0 == x1;
+                    const int lo = along_y ? min(y0, y1) : min(x0, x1);
+                    cnt = (along_y ? max(y0, y1) : max(x0, x1)) - lo;
+                    // a segment along its layer's direction lies on one track
+                    const bool one_track = along_y == (bool)layer_dir(g, layer);
+                    const int c0 = along_y ? cell_of(g, layer, x0, lo) : cell_of(g, layer, lo, y0);
+                    a.timestamp[cell_of(g, layer, x0, y0)] = a.stamp;
+                    a.timestamp[cell_of(g, layer, x1, y1)] = a.stamp;
+                    if (one_track && cnt) a.track_dirty[track_of_cell(g, c0)] = 1u;
+                    cell0[lane] = c0;
+                    step[lane] = one_track ? 1 : (along_y ? g.X : g.Y);
+                    lo_of[lane] = lo;
+                    flags[lane] = (along_y ? 1 : 0) | (one_track ? 2 : 0);
+                }
             }
-            for (int pos = lo + lane; pos < hi; pos += kWave) {
-                const int cell = along_y ? cell_of(g, layer, x0, pos) : cell_of(g, layer, pos, y0);
-                add_demand(a, cell, (float)a.K);
+            int incl = cnt;
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int v = __shfl_up(incl, d, kWave);
+                if (lane >= d) incl += v;
+            }
+            const int total = __shfl(incl, kWave - 1, kWave);
+            off[lane] = incl - cnt;
+            edges += cnt;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            for (int t = lane; t < total; t += kWave) {
+                int lo = 0, hi = kWave - 1;  // last lane whose edges start at or before t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (off[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int d = t - off[lo];
+                const int cell = cell0[lo] + d * step[lo];
+                const int f = flags[lo];
+                atomicAdd(a.demand + cell, (float)a.K);
+                if (!(f & 2)) a.track_dirty[track_of_cell(g, cell)] = 1u;
                 a.timestamp[cell] = a.stamp;
-                length += along_y ? (double)g.y_edge_len[pos] : (double)g.x_edge_len[pos];
+                length += (f & 1) ? (double)g.y_edge_len[lo_of[lo] + d] : (double)g.x_edge_len[lo_of[lo] + d];
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        for (int off = kWave / 2; off; off >>= 1) length += __shfl_xor(length, off, kWave);
-        if (lane == 0 && length != 0.0) atomicAdd(&a.stats->total_wirelength, length * a.K);
-        if (lane == 0 && edges) atomicAdd(&a.stats->wire_edges, (unsigned long long)edges);
+        for (int o = kWave / 2; o; o >>= 1) {
+            length += __shfl_xor(length, o, kWave);
+            edges += __shfl_xor(edges, o, kWave);
+        }
+        Stats* st = a.stats + wave % kStatStripes;
+        if (lane == 0 && length != 0.0) atomicAdd(&st->total_wirelength, length * a.K);
+        if (lane == 0 && edges) atomicAdd(&st->wire_edges, (unsigned long long)edges);
     }
 }
 
+// Vias: one lane per route segment; the timestamps of all its layer steps
+// are loaded before any is tested, so a stack costs one memory round trip.
 __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -76,9 +124,15 @@ __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
             if (low == high) continue;
             const int x = ra / g.Y % g.X, y = ra % g.Y;
             vias += high - low;
+            unsigned stamped = 0;
+#pragma unroll
+            for (int j = 0; j < kMaxLayers - 1; ++j) {
+                const int layer = min(low + j, high - 1);  // in range for every j
+                if (a.timestamp[cell_of(g, layer, x, y)] >= a.stamp) stamped |= 1u << j;
+            }
             for (int layer = low; layer < high; ++layer) {
+                if (stamped >> (layer - low) & 1u) continue;
                 const int cell = cell_of(g, layer, x, y);
-                if (a.timestamp[cell] >= a.stamp) continue;
                 const bool along_y = layer_dir(g, layer);
                 const int position = along_y ? y : x, length = along_y ? g.Y : g.X;
                 // the previous cell on the track is cell - 1 in track-major order
@@ -91,7 +145,7 @@ __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
             }
         }
         for (int off = kWave / 2; off; off >>= 1) vias += __shfl_xor(vias, off, kWave);
-        if (lane == 0 && vias) atomicAdd(&a.stats->total_vias, (unsigned long long)(vias * a.K));
+        if (lane == 0 && vias) atomicAdd(&a.stats[wave % kStatStripes].total_vias, (unsigned long long)(vias * a.K));
     }
 }
 

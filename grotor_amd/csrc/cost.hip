@@ -91,7 +91,8 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
     }
     if (threadIdx.x == 0) {
         track_dirty[T] = 0;
-        if (refreshed) atomicAdd(refreshed, (unsigned long long)len);
+        // refreshed cells, striped like the stage statistics
+        if (refreshed) atomicAdd(refreshed + (T % kStatStripes) * kCounterStride, (unsigned long long)len);
     }
 }
 

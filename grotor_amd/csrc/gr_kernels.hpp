@@ -14,11 +14,17 @@ struct CostParams {
     double via_weight = 1.0;  // multiplies unit_via_cost (src/cost/full_update.cuh:82)
 };
 
-struct Stats {
+// Stage statistics, striped over kStatStripes cache lines: a wavefront adds
+// into stripe (wave id mod kStatStripes), so thousands of nets per launch do
+// not serialize on one address.  The host sums the stripes; every addend is
+// an integer (edge lengths, counts), so the double sums are exact in any order.
+struct alignas(128) Stats {
     double total_wirelength;
     unsigned long long total_vias;
     unsigned long long wire_edges;  // unit wire edges committed (for byte accounting)
 };
+constexpr int kStatStripes = 64;
+constexpr int kCounterStride = 16;  // 64-bit counters per 128-byte stripe
 
 // Error codes written by kernels into the context's device error word.
 enum DeviceError : int {
@@ -67,9 +73,11 @@ struct LshapeArgs {
     double* gscratch;  // global scratch when the tree does not fit in LDS
     int* error;
 };
+// Bytes of one workgroup's DP tables for trees of up to nmax nodes (0 if L
+// is not supported), and the launch of `blocks` nets with the tables in LDS
+// or in a.gscratch (false if L is not supported).
 size_t lshape_scratch_bytes(int L, int nmax);
-__global__ void lshape_dp_lds_kernel(LshapeArgs a);
-__global__ void lshape_dp_global_kernel(LshapeArgs a);
+bool launch_lshape_dp(const LshapeArgs& a, int blocks, bool lds, hipStream_t stream);
 
 struct RipupArgs {
     DeviceGrid g;
@@ -113,16 +121,16 @@ struct ColorArgs {
     unsigned long long* pm;  // per slot: mask_words prefix-mask words
     int mask_words;
     int* color;           // per net, -1 until set
-    int* color_count;
     int* next;            // dequeue counter
-    int* max_color;
     int* error;
     long long spin_limit;
     int nclosed;          // full batches: color closed_color[i] for nets >= closed_from[i]
     const int* closed_color;
     const int* closed_from;
 };
+constexpr int kColorChunk = 4;  // nets per dequeue
 __global__ void color_kernel(ColorArgs a);
+__global__ void color_hist_kernel(const int* color, int first, int n, int* counts);
 
 __global__ void fill_int_kernel(int* p, int value, long long n);
 

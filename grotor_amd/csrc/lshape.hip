@@ -11,12 +11,16 @@
 // order without floating-point atomics.  Per node the work splits into three
 // lane-parallel phases separated by workgroup barriers:
 //   A  lane c: best interval containing layer c (first minimum in
-//      (low, high) row-major order); lanes L..4L-1 issue the prefix-sum and
-//      corner via gathers for the same node;
+//      (low, high) row-major order); lanes L..3L-1 fold the corner via
+//      columns of a bent edge into per-(low, high) partial sums, in the
+//      reference's left-to-right order;
 //   B  lane c: best arrival on parent layer c (straight edge, or the first
 //      minimum over the opposite-parity child layers of a bent edge);
 //   C  lane c: interval closure of row low = c and accumulation into the
 //      parent's table.
+// All costs the sweep reads (prefix-sum differences and via costs) are
+// gathered for every node before the sweep starts, so the sequential part
+// of the net touches LDS only.  The layer count is a template parameter.
 // The interval tables live in LDS (or, for trees too large for LDS, in a
 // per-workgroup slice of global scratch); traceback recomputes the closure
 // argmin from the stored per-node diagonal, so no per-interval traceback
@@ -28,13 +32,19 @@ namespace grotor {
 
 namespace {
 
-struct Scratch {
-    double* dist;   // nmax * L * L
-    double* diag;   // nmax * L
-    double* tmpd;   // 3 * L: best, first-leg wire, second-leg wire
-    float* vs;      // nmax * L: via costs on the node's column
-    float* va;      // L: via costs at corner (x, py)
-    float* vb;      // L: via costs at corner (px, y)
+// Per-workgroup tables for trees of up to nmax nodes; L is a compile-time
+// constant so that every per-layer loop unrolls.
+template <int L>
+struct Tables {
+    double* dist;   // nmax * L * L  interval tables
+    double* diag;   // nmax * L      best arrival on each parent layer
+    double* w1;     // nmax * L      first-leg wire (child side; the whole edge when straight)
+    double* w2;     // nmax * L      second-leg wire (parent side, bent edges)
+    double* best;   // L             best interval containing each layer (current node)
+    double* fold;   // 2 * L * L     corner via sums (va, vb) of the current node
+    float* vs;      // nmax * L      via costs on the node's column
+    float* va;      // nmax * L      via costs at corner (x, py)
+    float* vb;      // nmax * L      via costs at corner (px, y)
     int* dprev;     // nmax * L
     int* nd;        // nmax
     int* par;       // nmax
@@ -44,21 +54,25 @@ struct Scratch {
     int* woff;      // nmax + 1
 };
 
-__host__ __device__ inline size_t scratch_bytes(int L, int nmax) {
-    const size_t LL = size_t(L) * L;
-    return 8 * (nmax * LL + size_t(nmax) * L + 3 * L) + 4 * (size_t(nmax) * L + 2 * L) +
+template <int L>
+__host__ __device__ constexpr size_t table_bytes(int nmax) {
+    return 8 * (size_t(nmax) * (L * L + 3 * L) + L + 2 * L * L) + 4 * (size_t(nmax) * 3 * L) +
            4 * (size_t(nmax) * L + 4 * size_t(nmax) + L + nmax + 1);
 }
 
-__device__ inline Scratch carve(double* base, int L, int nmax) {
-    Scratch s;
+template <int L>
+__device__ inline Tables<L> carve(double* base, int nmax) {
+    Tables<L> s;
     s.dist = base;
     s.diag = s.dist + size_t(nmax) * L * L;
-    s.tmpd = s.diag + size_t(nmax) * L;
-    s.vs = reinterpret_cast<float*>(s.tmpd + 3 * L);
+    s.w1 = s.diag + size_t(nmax) * L;
+    s.w2 = s.w1 + size_t(nmax) * L;
+    s.best = s.w2 + size_t(nmax) * L;
+    s.fold = s.best + L;
+    s.vs = reinterpret_cast<float*>(s.fold + 2 * L * L);
     s.va = s.vs + size_t(nmax) * L;
-    s.vb = s.va + L;
-    s.dprev = reinterpret_cast<int*>(s.vb + L);
+    s.vb = s.va + size_t(nmax) * L;
+    s.dprev = reinterpret_cast<int*>(s.vb + size_t(nmax) * L);
     s.nd = s.dprev + size_t(nmax) * L;
     s.par = s.nd + nmax;
     s.rng = s.par + nmax;
@@ -76,10 +90,12 @@ __device__ __forceinline__ double wire_cost(const DeviceGrid& g, const double* p
     return presum[cell_of(g, l, xmax, ymax)] - presum[cell_of(g, l, xmin, ymin)];
 }
 
-__device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
+template <int L>
+__device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
+    constexpr int LL = L * L;
     const int lane = threadIdx.x;
     const DeviceGrid& g = a.g;
-    const int L = g.L, LL = L * L, X = g.X, Y = g.Y;
+    const int X = g.X, Y = g.Y;
     const int slot = a.slots ? a.slots[a.begin + blockIdx.x] : a.begin + blockIdx.x;
     const int id = a.net_ids[slot];
     const int n0 = a.node_off[slot], n = a.node_off[slot + 1] - n0;
@@ -88,13 +104,34 @@ __device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
         s.par[i] = a.parents[n0 + i];
     }
     __syncthreads();
-    // via stacks (src/routing/lshape_warp.cuh:137-145)
+    // Every cost the DP reads, gathered up front for all nodes so that the
+    // bottom-up sweep below touches only LDS: the via stack at each node
+    // (src/routing/lshape_warp.cuh:137-145), the edge's wire costs per layer
+    // and the via columns at both corners of a bent edge (:18-48).
     for (int r = lane; r < n * L; r += kWave) {
-        const int i = r / L, l = r % L;
-        if (l < L - 1) {
-            const int v = s.nd[i];
-            s.vs[i * L + l] = a.vcost[cell_of(g, l, v / Y % X, v % Y)];
+        const int i = r / L, c = r % L;
+        const int v = s.nd[i];
+        const int x = v / Y % X, y = v % Y;
+        s.vs[r] = c < L - 1 ? a.vcost[cell_of(g, c, x, y)] : 0.f;
+        if (!i) continue;
+        const int q = s.nd[s.par[i]];
+        const int px = q / Y % X, py = q % Y;
+        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+        const int d = layer_dir(g, c);
+        double w1 = 0.0, w2 = 0.0;
+        float va = 0.f, vb = 0.f;
+        if (x == px || y == py) {
+            if (!((d == 0 && y != py) || (d == 1 && x != px))) w1 = wire_cost(g, a.presum, c, minx, maxx, miny, maxy);
+        } else {
+            w1 = d ? wire_cost(g, a.presum, c, x, x, miny, maxy) : wire_cost(g, a.presum, c, minx, maxx, y, y);
+            w2 = d ? wire_cost(g, a.presum, c, px, px, miny, maxy) : wire_cost(g, a.presum, c, minx, maxx, py, py);
+            va = a.vcost[cell_of(g, c, x, py)];
+            vb = a.vcost[cell_of(g, c, px, y)];
         }
+        s.w1[r] = w1;
+        s.w2[r] = w2;
+        s.va[r] = va;
+        s.vb[r] = vb;
     }
     __syncthreads();
     for (int r = lane; r < n * L; r += kWave) {
@@ -114,44 +151,36 @@ __device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
     __syncthreads();
     for (int i = n - 1; i >= 1; --i) {
         const int p = s.par[i];
-        const int x = s.nd[i] / Y % X, y = s.nd[i] % Y;
-        const int px = s.nd[p] / Y % X, py = s.nd[p] % Y;
-        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
-        const bool straight = x == px || y == py;
-        // phase A
+        const int vi = s.nd[i], vp = s.nd[p];
+        const bool straight = (vi / Y % X == vp / Y % X) || (vi % Y == vp % Y);
+        // phase A: lane c < L, best interval containing layer c, first
+        // minimum in (low, high) row-major order; lanes L..3L-1 fold the
+        // corner via columns of a bent edge
         if (lane < L) {
             const int c = lane;
             const double* nd = s.dist + size_t(i) * LL;
             double best = kInf;
             int ch = 0;
             for (int lo = 0; lo <= c; ++lo)
-                for (int hi = c; hi < L; ++hi)
-                    if (nd[lo * L + hi] < best) {
+#pragma unroll
+                for (int hi = 0; hi < L; ++hi)
+                    if (hi >= c && nd[lo * L + hi] < best) {
                         best = nd[lo * L + hi];
                         ch = lo * L + hi;
                     }
-            s.tmpd[c] = best;
+            s.best[c] = best;
             s.choice[c] = ch;
-        } else if (lane < 2 * L) {
-            const int c = lane - L;
-            const int d = layer_dir(g, c);
-            double w = 0.0;
-            if (straight) {
-                if (!((d == 0 && y != py) || (d == 1 && x != px))) w = wire_cost(g, a.presum, c, minx, maxx, miny, maxy);
-            } else {
-                w = d ? wire_cost(g, a.presum, c, x, x, miny, maxy) : wire_cost(g, a.presum, c, minx, maxx, y, y);
-            }
-            s.tmpd[L + c] = w;
-        } else if (lane < 3 * L) {
-            const int c = lane - 2 * L;
-            if (!straight)
-                s.tmpd[2 * L + c] = layer_dir(g, c) ? wire_cost(g, a.presum, c, px, px, miny, maxy)
-                                                    : wire_cost(g, a.presum, c, minx, maxx, py, py);
-        } else if (lane < 4 * L) {
-            const int c = lane - 3 * L;
-            if (!straight) {
-                s.va[c] = a.vcost[cell_of(g, c, x, py)];
-                s.vb[c] = a.vcost[cell_of(g, c, px, y)];
+        } else if (lane < 3 * L && !straight) {
+            // fold[k][lo][hi] = sum of corner[lo+1 .. hi-1], left to right
+            const int k = (lane - L) / L, lo = (lane - L) % L;
+            const float* corner = (k ? s.vb : s.va) + i * L;
+            double* f = s.fold + k * LL + lo * L;
+            double v = 0.0;
+#pragma unroll
+            for (int hi = 0; hi < L; ++hi) {
+                if (hi <= lo) continue;
+                if (hi > lo + 1) v += (double)corner[hi - 1];
+                f[hi] = v;
             }
         }
         __syncthreads();
@@ -162,18 +191,21 @@ __device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
             int bp = 0;
             if (straight) {
                 const int d = layer_dir(g, pl);
+                const int x = vi / Y % X, y = vi % Y, px = vp / Y % X, py = vp % Y;
                 if (!((d == 0 && y != py) || (d == 1 && x != px))) {
-                    best = s.tmpd[pl] + s.tmpd[L + pl];
+                    best = s.best[pl] + s.w1[i * L + pl];
                     bp = s.choice[pl] * LL + pl * L + pl;
                 }
             } else {
-                for (int cl = (pl & 1) ^ 1; cl < L; cl += 2) {
-                    const double cost = s.tmpd[cl] + s.tmpd[L + cl];
-                    const float* corner = layer_dir(g, cl) ? s.va : s.vb;
+                const double second = s.w2[i * L + pl];
+#pragma unroll
+                for (int cl = 0; cl < L; ++cl) {
+                    if (((cl ^ pl) & 1) == 0) continue;
+                    const double cost = s.best[cl] + s.w1[i * L + cl];
+                    // vertical first leg: corner at (x, py), the va column
+                    const double* f = s.fold + (layer_dir(g, cl) ? 0 : LL);
                     const int lo = min(cl, pl), hi = max(cl, pl);
-                    double cost2 = 0.0;
-                    for (int l = lo + 1; l < hi; ++l) cost2 += (double)corner[l];
-                    cost2 += s.tmpd[2 * L + pl];
+                    const double cost2 = f[lo * L + hi] + second;
                     if (cost + cost2 < best) {
                         best = cost + cost2;
                         bp = s.choice[cl] * LL + cl * L + pl;
@@ -190,7 +222,9 @@ __device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
             const double* dg = s.diag + i * L;
             double* prow = s.dist + size_t(p) * LL + lo * L;
             double c = dg[lo];
-            for (int hi = lo; hi < L; ++hi) {
+#pragma unroll
+            for (int hi = 0; hi < L; ++hi) {
+                if (hi < lo) continue;
                 if (hi > lo && !(c <= dg[hi])) c = dg[hi];
                 prow[hi] += c;
             }
@@ -278,18 +312,50 @@ __device__ void solve_net(const LshapeArgs& a, const Scratch& s) {
     if (lane == 0) routes[0] = s.woff[n];
 }
 
-}  // namespace
-
-size_t lshape_scratch_bytes(int L, int nmax) { return scratch_bytes(L, nmax); }
-
+template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_lds_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
-    solve_net(a, carve(smem, a.g.L, a.nmax));
+    solve_net<L>(a, carve<L>(smem, a.nmax));
 }
 
+template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_global_kernel(LshapeArgs a) {
-    const size_t words = (scratch_bytes(a.g.L, a.nmax) + 7) / 8;
-    solve_net(a, carve(a.gscratch + words * blockIdx.x, a.g.L, a.nmax));
+    const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
+    solve_net<L>(a, carve<L>(a.gscratch + words * blockIdx.x, a.nmax));
+}
+
+template <int L>
+void launch_for(const LshapeArgs& a, int blocks, bool lds, hipStream_t st) {
+    if (lds)
+        hipLaunchKernelGGL(lshape_dp_lds_kernel<L>, dim3(blocks), dim3(kWave), table_bytes<L>(a.nmax), st, a);
+    else
+        hipLaunchKernelGGL(lshape_dp_global_kernel<L>, dim3(blocks), dim3(kWave), 0, st, a);
+}
+
+}  // namespace
+
+size_t lshape_scratch_bytes(int L, int nmax) {
+    switch (L) {
+#define GR_L_CASE(n) \
+    case n: return table_bytes<n>(nmax);
+        GR_L_CASE(1) GR_L_CASE(2) GR_L_CASE(3) GR_L_CASE(4) GR_L_CASE(5) GR_L_CASE(6) GR_L_CASE(7) GR_L_CASE(8)
+        GR_L_CASE(9) GR_L_CASE(10) GR_L_CASE(11) GR_L_CASE(12) GR_L_CASE(13) GR_L_CASE(14) GR_L_CASE(15)
+#undef GR_L_CASE
+        default: return 0;
+    }
+}
+
+bool launch_lshape_dp(const LshapeArgs& a, int blocks, bool lds, hipStream_t st) {
+    switch (a.g.L) {
+#define GR_L_CASE(n)                     \
+    case n:                              \
+        launch_for<n>(a, blocks, lds, st); \
+        return true;
+        GR_L_CASE(1) GR_L_CASE(2) GR_L_CASE(3) GR_L_CASE(4) GR_L_CASE(5) GR_L_CASE(6) GR_L_CASE(7) GR_L_CASE(8)
+        GR_L_CASE(9) GR_L_CASE(10) GR_L_CASE(11) GR_L_CASE(12) GR_L_CASE(13) GR_L_CASE(14) GR_L_CASE(15)
+#undef GR_L_CASE
+        default: return false;
+    }
 }
 
 }  // namespace grotor

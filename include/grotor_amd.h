@@ -132,8 +132,11 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
  * batch b, stamp = first_stamp + b, refresh costs, solve, commit.
  * batch_off has nbatch + 1 entries over the uploaded stage slots.
  * mode: 0 = one launch per step (the reference's loop),
- *       1 = the per-batch launches replayed from a captured HIP graph,
- *       2 = one persistent kernel for the whole stage.
+ *       1 = the same launches captured once into a HIP graph and replayed
+ *           (the capture is reused while the stage topology, batch offsets
+ *           and first stamp stay the same; capture happens before the
+ *           timed interval),
+ *       other values are rejected with GR_E_ARG.
  * *elapsed_ms receives the synchronized interval (RoutingDagTimer,
  * include/router_state.hpp:238-265). */
 int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,

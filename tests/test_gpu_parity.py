@@ -114,6 +114,46 @@ def test_lshape_stage_matches_oracle(ga, ora, config, seed):
     ctx.close()
 
 
+def test_graph_mode_replays_match_oracle(ga, ora):
+    """Mode 1 (captured HIP graph): every replay, with resets and eager work
+    between replays, equals the oracle; a changed schedule recaptures."""
+    _, t = topology(6, 5)
+    print("graph mode config=6 seed=5", flush=True)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ora.batches(t["X"], t["Y"], *geom)
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=8)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    for rep in range(3):
+        ctx.reset()
+        ctx.batch_plan(*geom)  # eager device work between replays
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+        assert np.array_equal(ctx.routes(), routes_o), rep
+        assert np.array_equal(_bits(ctx.demand()), _bits(demand_o)), rep
+        assert np.array_equal(ctx.timestamps(), ts_o), rep
+        assert ctx.stats() == (wl_o, vias_o), rep
+    # the same stage uploaded again (fresh device buffers) with the same
+    # schedule: the slot permutation must be uploaded again, not assumed
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.reset()
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+    assert np.array_equal(ctx.routes(), routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    # a different schedule (first half of the batches only)
+    half = dict(sched)
+    half["batch_off"] = sched["batch_off"][:len(sched["batch_off"]) // 2 + 1]
+    routes_h, demand_h, _, _, _ = ora.Grid(t).lshape_stage(half, t["pin_acc_num"], threads=8)
+    ctx.reset()
+    ctx.lshape_run(half["batch_off"], first_stamp=1, mode=1)
+    assert np.array_equal(ctx.routes(), routes_h)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_h))
+    with pytest.raises(ga.GrError):
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
+    ctx.close()
+
+
 def test_single_batch_entry_points(ga, ora):
     """gr_lshape_batch + gr_commit_batch over one batch equal the oracle."""
     _, t = topology(6, 4)
