@@ -107,6 +107,27 @@ int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
                      int *stamp, double *total_wirelength, long long *total_vias,
                      int threads);
 
+/* ---- soft (log-sum-exp) L-shape DP, FP64 (soft.c) ---- */
+
+/* Smooth cost model: wcost, vcost per routing cell (IDX order) and their
+ * partial derivatives in slope and zero_slope (any output may be NULL
+ * except wcost and vcost); via_weight multiplies unit_via_cost. */
+void ora_soft_costs(const ora_grid *g, const float *capacity, const float *demand, double slope, double zero_slope,
+                    double via_weight, double *wcost, double *vcost, double *dw_ds, double *dw_dz, double *dv_ds,
+                    double *dv_dz);
+
+/* Soft DP value F of one net at temperature tau; with gw and gv non-NULL
+ * adds dF/dwcost and dF/dvcost (IDX order) into them. */
+double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double *vcost, int n, const int *nodes,
+                           const int *parents, double tau, double *gw, double *gv);
+
+/* Sum of F over count nets (CSR node_off) at one demand snapshot; with gw
+ * and gv non-NULL they receive dF/dwcost and dF/dvcost, and gparams (if
+ * non-NULL) dF/d(slope, zero_slope, via_weight). */
+double ora_soft_stage(const ora_grid *g, const float *capacity, const float *demand, double slope, double zero_slope,
+                      double via_weight, double tau, int count, const int *node_off, const int *nodes,
+                      const int *parents, double *gw, double *gv, double *gparams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,45 @@ class Grid:
         return flags[:n]
 
 
+    def soft_costs(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0):
+        """Smooth cost model: (wcost, vcost, dw_ds, dw_dz, dv_ds, dv_dz), FP64, IDX order."""
+        demand = np.ascontiguousarray(demand, np.float32)
+        out = [np.empty(self.cells, np.float64) for _ in range(6)]
+        lib().ora_soft_costs(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                             ctypes.c_double(slope), ctypes.c_double(zero_slope), ctypes.c_double(via_weight),
+                             *[_p(a, ctypes.c_double) for a in out])
+        return out
+
+    def soft_net(self, wcost, vcost, nodes, parents, tau, grads=False):
+        """Soft DP value of one net; with grads also (dF/dwcost, dF/dvcost)."""
+        wcost = np.ascontiguousarray(wcost, np.float64)
+        vcost = np.ascontiguousarray(vcost, np.float64)
+        nodes, parents = _i32(nodes), _i32(parents)
+        gw = np.zeros(self.cells, np.float64) if grads else None
+        gv = np.zeros(self.cells, np.float64) if grads else None
+        lib().ora_soft_lshape_net.restype = ctypes.c_double
+        F = lib().ora_soft_lshape_net(ctypes.byref(self.c), _p(wcost, ctypes.c_double), _p(vcost, ctypes.c_double),
+                                      ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
+                                      ctypes.c_double(tau), _p(gw, ctypes.c_double), _p(gv, ctypes.c_double))
+        return (F, gw, gv) if grads else F
+
+    def soft_stage(self, demand, node_off, nodes, parents, tau, slope=0.5, zero_slope=1.5, via_weight=1.0,
+                   grads=False):
+        """Sum of soft DP values over the nets; with grads also (gw, gv, [dslope, dzero_slope, dvia_weight])."""
+        demand = np.ascontiguousarray(demand, np.float32)
+        node_off, nodes, parents = _i32(node_off), _i32(nodes), _i32(parents)
+        gw = np.zeros(self.cells, np.float64) if grads else None
+        gv = np.zeros(self.cells, np.float64) if grads else None
+        gp = np.zeros(3, np.float64) if grads else None
+        lib().ora_soft_stage.restype = ctypes.c_double
+        F = lib().ora_soft_stage(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                                 ctypes.c_double(slope), ctypes.c_double(zero_slope), ctypes.c_double(via_weight),
+                                 ctypes.c_double(tau), ctypes.c_int(len(node_off) - 1), _p(node_off, ctypes.c_int),
+                                 _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int), _p(gw, ctypes.c_double),
+                                 _p(gv, ctypes.c_double), _p(gp, ctypes.c_double))
+        return (F, gw, gv, gp) if grads else F
+
+
 def batches(X, Y, pt_off, pts, hs_off, hsegs, vs_off, vsegs, limit=1000000, sparsity=1):
     arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
     n = len(arrs[0]) - 1
