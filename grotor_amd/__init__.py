@@ -58,6 +58,16 @@ class TopologyDesc(ctypes.Structure):
                 ("unit_length_wire_cost", ctypes.c_double), ("unit_via_cost", ctypes.c_double)]
 
 
+class ZshapeStageDesc(ctypes.Structure):
+    _fields_ = [("nets", ctypes.c_int), ("order", _i32p), ("pt_off", _i32p), ("pts", _i32p), ("hs_off", _i32p),
+                ("hsegs", _i32p), ("vs_off", _i32p), ("vsegs", _i32p)]
+
+
+class ZshapePlanDesc(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int), ("batch_off", _i32p), ("net_ids", _i32p),
+                ("rec_off", ctypes.POINTER(ctypes.c_longlong)), ("records", _i32p), ("raw_cap", _i32p)]
+
+
 def lib():
     """Load libgrotor_amd.so; raise if it has not been built."""
     global _lib
@@ -148,6 +158,41 @@ class Design:
                 unit_length_wire_cost=t.unit_length_wire_cost, unit_via_cost=t.unit_via_cost)
         return self._topo
 
+    def zshape_stage(self):
+        """Z-shape stage priority order and its batching geometry (numpy copies)."""
+        t = ZshapeStageDesc()
+        _call("gr_design_zshape_stage", self._h, ctypes.byref(t))
+        n = t.nets
+
+        def arr(p, count):
+            return np.zeros(0, np.int32) if count == 0 else np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+        pt_off, hs_off, vs_off = arr(t.pt_off, n + 1), arr(t.hs_off, n + 1), arr(t.vs_off, n + 1)
+        return dict(nets=n, order=arr(t.order, n), pt_off=pt_off, pts=arr(t.pts, int(pt_off[-1])), hs_off=hs_off,
+                    hsegs=arr(t.hsegs, 2 * int(hs_off[-1])), vs_off=vs_off, vsegs=arr(t.vsegs, 2 * int(vs_off[-1])))
+
+    def zshape_plan(self, color, nb, max_batch=250):
+        """Bounding-box batches in execution order with encoded trees."""
+        color = _i32(color)
+        p = ZshapePlanDesc()
+        _call("gr_design_zshape_plan", self._h, _ptr(color, ctypes.c_int), ctypes.c_int(nb), ctypes.c_int(max_batch),
+              ctypes.byref(p))
+        nb2 = p.batches
+        batch_off = np.ctypeslib.as_array(p.batch_off, shape=(nb2 + 1,)).copy()
+        rec_off = np.ctypeslib.as_array(p.rec_off, shape=(nb2 + 1,)).copy()
+        nets, words = int(batch_off[-1]), int(rec_off[-1])
+        take = lambda q, c: np.zeros(0, np.int32) if c == 0 else np.ctypeslib.as_array(q, shape=(c,)).copy()
+        return dict(batch_off=batch_off, net_ids=take(p.net_ids, nets), rec_off=rec_off,
+                    records=take(p.records, words), raw_cap=take(p.raw_cap, nets))
+
+    def zshape_finish(self, routes, words_per_pin=256):
+        """Promote congested segments and canonicalise, in place; returns the promotions."""
+        assert routes.dtype == np.int32 and routes.flags.c_contiguous
+        promoted = ctypes.c_int()
+        _call("gr_design_zshape_finish", self._h, _ptr(routes, ctypes.c_int), ctypes.c_int(words_per_pin),
+              ctypes.byref(promoted))
+        return promoted.value
+
     def write_output(self, routes, path):
         routes = _i32(routes)
         _call("gr_design_write_output", self._h, _ptr(routes, ctypes.c_int), path.encode())
@@ -176,6 +221,7 @@ class Context:
         self._h = h
         self.cells = self.L * self.X * self.Y
         self.route_words = 0
+        self.soft_generation = 0
 
     def close(self):
         if getattr(self, "_h", None):
@@ -186,10 +232,31 @@ class Context:
         if _lib is not None:
             self.close()
 
-    def set_routes(self, pin_acc_num):
+    def set_routes(self, pin_acc_num, words_per_pin=ROUTE_PER_PIN):
+        """Route storage of words_per_pin words per pin (12, or 256 for Z-shape stages)."""
         pin_acc_num = _i32(pin_acc_num)
+        _call("gr_ctx_set_route_stride", self._h, ctypes.c_int(words_per_pin))
         _call("gr_ctx_set_routes", self._h, ctypes.c_int(len(pin_acc_num) - 1), _ptr(pin_acc_num, ctypes.c_int))
-        self.route_words = ROUTE_PER_PIN * int(pin_acc_num[-1]) + 1
+        self.route_words = words_per_pin * int(pin_acc_num[-1]) + 1
+        self.words_per_pin = words_per_pin
+
+    def set_cost_params(self, slope=0.5, zero_slope=1.5, via_weight=1.0, integer_wire=False):
+        _call("gr_ctx_set_cost_params", self._h, ctypes.c_double(slope), ctypes.c_double(zero_slope),
+              ctypes.c_double(via_weight), ctypes.c_int(1 if integer_wire else 0))
+
+    def zshape_upload(self, plan):
+        """Upload a Design.zshape_plan() plan."""
+        self._zplan = plan
+        _call("gr_zshape_upload", self._h, ctypes.c_int(len(plan["batch_off"]) - 1),
+              _ptr(_i32(plan["batch_off"]), ctypes.c_int), _ptr(_i32(plan["net_ids"]), ctypes.c_int),
+              _ptr(np.ascontiguousarray(plan["rec_off"], np.int64), ctypes.c_longlong),
+              _ptr(_i32(plan["records"]), ctypes.c_int), _ptr(_i32(plan["raw_cap"]), ctypes.c_int))
+
+    def zshape_run(self, first_stamp=1):
+        """Run the uploaded Z-shape batches; returns the DAG interval in ms."""
+        ms = ctypes.c_double()
+        _call("gr_zshape_run", self._h, ctypes.c_int(first_stamp), ctypes.byref(ms))
+        return ms.value
 
     def refresh(self, full=False):
         _call("gr_cost_refresh_full" if full else "gr_cost_refresh", self._h)
@@ -274,10 +341,15 @@ class Context:
     def reset(self):
         _call("gr_ctx_reset", self._h)
 
-    KINDS = 7
+    KINDS = 8
 
     def profile_begin(self):
         _call("gr_profile_begin", self._h)
+
+    def profile_select(self, kinds=None):
+        """Give events only to the kernel classes listed (None: all)."""
+        mask = 0xFFFFFFFF if kinds is None else sum(1 << int(k) for k in kinds)
+        _call("gr_profile_select", self._h, ctypes.c_uint(mask))
 
     def profile_end(self):
         """Per kernel class: summed device ms, launches and algorithmic bytes."""
@@ -287,6 +359,31 @@ class Context:
         _call("gr_profile_end", self._h, _ptr(ms, ctypes.c_double), _ptr(n, ctypes.c_longlong),
               _ptr(by, ctypes.c_double), ctypes.c_int(self.KINDS))
         return {"ms": ms.tolist(), "launches": n.tolist(), "bytes": by.tolist()}
+
+    def soft_forward(self, begin, count, slope=0.5, zero_slope=1.5, via_weight=1.0, tau=1.0, precision=32,
+                     per_net=False):
+        """Soft L-shape DP over stage slots [begin, begin + count): (total, per-net values or None, ms)."""
+        total, ms = ctypes.c_double(), ctypes.c_double()
+        vals = np.empty(max(count, 1), np.float64) if per_net else None
+        self.soft_generation += 1
+        _call("gr_soft_forward", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_double(slope),
+              ctypes.c_double(zero_slope), ctypes.c_double(via_weight), ctypes.c_double(tau), ctypes.c_int(precision),
+              ctypes.byref(total), _ptr(vals, ctypes.c_double), ctypes.byref(ms))
+        return total.value, (vals[:count] if per_net else None), ms.value
+
+    def soft_backward(self):
+        """Adjoint of the last soft_forward: (d/d(slope, zero_slope, via_weight), ms)."""
+        gp = np.zeros(3, np.float64)
+        ms = ctypes.c_double()
+        _call("gr_soft_backward", self._h, _ptr(gp, ctypes.c_double), ctypes.byref(ms))
+        return gp, ms.value
+
+    def soft_edge_grads(self):
+        """Per-cell d total / d wcost and d total / d vcost (IDX order)."""
+        gw = np.empty(self.cells, np.float64)
+        gv = np.empty(self.cells, np.float64)
+        _call("gr_soft_edge_grads", self._h, _ptr(gw, ctypes.c_double), _ptr(gv, ctypes.c_double))
+        return gw, gv
 
     def ripup(self, net_count, threshold=0):
         flags = np.empty(max(net_count, 1), np.uint8)

@@ -105,6 +105,7 @@ struct gr_ctx {
     int* error = nullptr;
     CostParams cp;
     // route storage
+    int route_per_pin = kRoutePerPin;
     int net_count = 0;
     std::vector<int> pin_acc_num_h;
     int* pin_acc_num = nullptr;
@@ -113,8 +114,11 @@ struct gr_ctx {
     // L-shape stage topology
     int stage_count = 0;
     std::vector<int> node_off_h;
-    int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr, *slots = nullptr;
-    std::vector<int> slots_h;
+    int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr;
+    std::vector<int> net_ids_h;
+    std::vector<int> slots_h;      // stage slot per execution position
+    int4* desc = nullptr;          // per execution position: node begin, nodes, route word, route capacity
+    std::vector<int4> desc_h;
     double* gscratch = nullptr;
     size_t gscratch_bytes = 0;
     // rip-up
@@ -126,6 +130,22 @@ struct gr_ctx {
     bool has_geom = false;
     long long geom_points = 0, geom_hsegs = 0, geom_vsegs = 0;
     std::vector<int> geom_pt_off;  // host copy of the point offsets
+    // soft DP: cost arrays, per-cell gradients, checkpointed tables
+    Workspace soft_ws, soft_tables;
+    SoftCostArgs soft_cost{};
+    SoftArgs soft{};
+    int soft_precision = 0;
+    bool soft_ready = false;  // a forward pass whose tables are still held
+    double* soft_params = nullptr;
+    // Z-shape stage: plan on the device, shared DP map, raw words
+    Workspace z_ws;
+    int* z_map = nullptr;   // 6 planes
+    int* z_prev = nullptr;  // 16 planes
+    long long z_plane = 0;
+    int z_batches = 0;
+    std::vector<int> z_batch_off;
+    std::vector<long long> z_rec_off;
+    int *z_net_ids = nullptr, *z_records = nullptr, *z_raw_cap = nullptr, *z_raw = nullptr;
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 16;
     // kernel profiling (HIP events on the context stream)
@@ -135,6 +155,7 @@ struct gr_ctx {
         double bytes;
     };
     bool profiling = false;
+    unsigned prof_mask = ~0u;  // kernel classes that get events
     std::vector<ProfRec> prof;
     std::vector<hipEvent_t> events;
     size_t next_event = 0;
@@ -182,7 +203,7 @@ hipEvent_t take_event(gr_ctx* c) {
 // Run `launch` between two events on the context stream when profiling.
 template <class F>
 void timed(gr_ctx* c, int kind, double bytes, F&& launch) {
-    if (!c->profiling) {
+    if (!c->profiling || !(c->prof_mask >> kind & 1u)) {
         launch();
         return;
     }
@@ -246,7 +267,8 @@ void check_device_error(gr_ctx* c, const char* op) {
         case kDevRouteOverflow: throw GrError(GR_E_CAPACITY, std::string(op) + ": route storage exceeded");
         case kDevNoSolution: throw GrError(GR_E_DEVICE, std::string(op) + ": no finite layer assignment");
         case kDevSpinLimit: throw GrError(GR_E_DEVICE, std::string(op) + ": batching wait exceeded its spin limit");
-        case kDevColorOverflow: throw GrError(GR_E_CAPACITY, std::string(op) + ": more than 4096 batches");
+        case kDevColorOverflow: throw GrError(GR_E_CAPACITY, std::string(op) + ": more than 2016 batches");
+        case kDevBadCoordinate: throw GrError(GR_E_DEVICE, std::string(op) + ": route coordinate outside the grid");
         default: throw GrError(GR_E_DEVICE, std::string(op) + ": device error " + std::to_string(err));
     }
 }
@@ -267,7 +289,7 @@ CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
     a.timestamp = c->timestamp;
     a.track_dirty = c->track_dirty;
     a.stats = c->stats;
-    a.rs = {c->routes, c->pin_acc_num};
+    a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
     a.net_ids = c->net_ids;
     a.slots = nullptr;
     a.begin = begin;
@@ -277,9 +299,10 @@ CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
     return a;
 }
 
-void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K) {
+void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int* net_ids = nullptr) {
     if (!count) return;
-    const CommitArgs a = commit_args(c, begin, count, stamp, K);
+    CommitArgs a = commit_args(c, begin, count, stamp, K);
+    if (net_ids) a.net_ids = net_ids;
     const int blocks = std::min((count + 3) / 4, 4096);
     timed(c, GR_K_COMMIT_WIRES, 0.0,
           [&] { hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
@@ -326,12 +349,10 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
     a.g = c->g;
     a.vcost = c->vcost;
     a.presum = c->presum;
-    a.rs = {c->routes, c->pin_acc_num};
-    a.net_ids = c->net_ids;
-    a.node_off = c->node_off;
+    a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+    a.desc = c->desc;
     a.nodes = c->nodes;
     a.parents = c->parents;
-    a.slots = c->slots;
     a.gscratch = c->gscratch;
     a.error = c->error;
     auto bytes_of = [&](int first, int count) {
@@ -380,6 +401,22 @@ void reserve_dp_scratch(gr_ctx* c, const std::vector<BatchLaunch>& plan) {
         if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.large));
     }
     ensure_gscratch(c, need);
+}
+
+// Execution-order descriptors of the stage slots in slots[first, last):
+// each DP workgroup reads one int4 instead of chasing slot -> net -> offsets.
+void upload_desc(gr_ctx* c, const std::vector<int>& slots, int first, int last) {
+    if (c->desc_h.size() != size_t(c->stage_count)) c->desc_h.assign(c->stage_count, int4{-1, -1, -1, -1});
+    bool changed = false;
+    for (int j = first; j < last; ++j) {
+        const int s = slots[j], id = c->net_ids_h[s];
+        const int4 d{c->node_off_h[s], c->node_off_h[s + 1] - c->node_off_h[s], c->route_per_pin * c->pin_acc_num_h[id],
+                     c->route_per_pin * (c->pin_acc_num_h[id + 1] - c->pin_acc_num_h[id])};
+        int4& h = c->desc_h[j];
+        if (h.x != d.x || h.y != d.y || h.z != d.z || h.w != d.w) changed = true;
+        h = d;
+    }
+    if (changed) upload(c->desc + first, c->desc_h.data() + first, sizeof(int4) * size_t(last - first), c->stream);
 }
 
 void require_stage(gr_ctx* c, int begin, int count) {
@@ -477,12 +514,33 @@ int gr_ctx_set_routes(gr_ctx* c, int net_count, const int* pin_acc_num) {
         c->mem.release(c->of_flags);
         c->net_count = net_count;
         c->pin_acc_num_h.assign(pin_acc_num, pin_acc_num + net_count + 1);
-        c->route_words = (long long)kRoutePerPin * pin_acc_num[net_count] + 1;
+        c->route_words = (long long)c->route_per_pin * pin_acc_num[net_count] + 1;
         c->pin_acc_num = c->mem.alloc<int>(net_count + 1);
         upload(c->pin_acc_num, pin_acc_num, sizeof(int) * (net_count + 1), c->stream);
         c->routes = c->mem.alloc<int>(size_t(c->route_words));
         hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
         hip_check(hipStreamSynchronize(c->stream), "route storage");
+    });
+}
+
+int gr_ctx_set_route_stride(gr_ctx* c, int words_per_pin) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (words_per_pin < 4) throw GrError(GR_E_ARG, "route words per pin must be at least 4");
+        c->route_per_pin = words_per_pin;
+    });
+}
+
+int gr_ctx_set_cost_params(gr_ctx* c, double slope, double zero_slope, double via_weight, int integer_wire) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        c->cp.slope = slope;
+        c->cp.zero_slope = zero_slope;
+        c->cp.via_weight = via_weight;
+        c->cp.integer_wire = integer_wire ? 1 : 0;
+        std::vector<unsigned> ones(c->tracks, 1u);
+        upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "cost parameters");
     });
 }
 
@@ -591,12 +649,15 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
                 if (v < 0 || v >= (c->g.L + 1) * c->g.XY) throw GrError(GR_E_ARG, "node index outside the grid");
             }
         }
-        for (int* p : {c->net_ids, c->node_off, c->nodes, c->parents, c->slots}) {
+        for (int* p : {c->net_ids, c->node_off, c->nodes, c->parents}) {
             int* q = p;
             c->mem.release(q);
         }
-        c->net_ids = c->node_off = c->nodes = c->parents = c->slots = nullptr;
-        c->slots_h.clear();  // the new slot buffer holds nothing yet
+        c->mem.release(c->desc);
+        c->net_ids = c->node_off = c->nodes = c->parents = nullptr;
+        c->slots_h.clear();
+        c->desc_h.clear();  // the new descriptor buffer holds nothing yet
+        c->soft_ready = false;
         c->drop_graph();
         ++c->stage_gen;
         const int total = node_off[count];
@@ -606,7 +667,8 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
         c->node_off = c->mem.alloc<int>(count + 1);
         c->nodes = c->mem.alloc<int>(total);
         c->parents = c->mem.alloc<int>(total);
-        c->slots = c->mem.alloc<int>(count);
+        c->desc = c->mem.alloc<int4>(count);
+        c->net_ids_h.assign(net_ids, net_ids + count);
         upload(c->net_ids, net_ids, sizeof(int) * count, c->stream);
         upload(c->node_off, node_off, sizeof(int) * (count + 1), c->stream);
         upload(c->nodes, nodes, sizeof(int) * total, c->stream);
@@ -637,8 +699,9 @@ int gr_lshape_batch(gr_ctx* c, int begin, int count, int stamp) {
         std::vector<int> slots(c->stage_count);
         const BatchLaunch b = plan_batch(c, begin, count, slots);
         reserve_dp_scratch(c, {b});
-        c->slots_h = slots;
-        upload(c->slots + begin, slots.data() + begin, sizeof(int) * count, c->stream);
+        if (c->slots_h.size() != slots.size()) c->slots_h.assign(slots.size(), 0);
+        std::copy(slots.begin() + begin, slots.begin() + begin + count, c->slots_h.begin() + begin);
+        upload_desc(c, slots, begin, begin + count);
         launch_lshape(c, b);
         hip_check(hipStreamSynchronize(c->stream), "gr_lshape_batch");
         check_device_error(c, "gr_lshape_batch");
@@ -677,10 +740,9 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         if (mode == 1) {
             std::vector<int> key{c->stage_gen, first_stamp, nbatch};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
-            if (c->slots_h != slots) {  // the graph reads the slot permutation when it runs
-                c->slots_h = slots;
-                upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
-            }
+            // the graph reads the descriptors when it runs
+            c->slots_h = slots;
+            upload_desc(c, slots, 0, c->stage_count);
             if (!c->graph_exec || key != c->graph_key) {
                 c->drop_graph();
                 hip_check(hipStreamSynchronize(c->stream), "before graph capture");
@@ -703,7 +765,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
             }
         } else {
             c->slots_h = slots;
-            upload(c->slots, slots.data(), sizeof(int) * c->stage_count, c->stream);
+            upload_desc(c, slots, 0, c->stage_count);
         }
         hip_check(hipStreamSynchronize(c->stream), "stage plan");
         hip_check(hipDeviceSynchronize(), "before DAG interval");
@@ -751,6 +813,13 @@ int gr_profile_begin(gr_ctx* c) {
     });
 }
 
+int gr_profile_select(gr_ctx* c, unsigned kind_mask) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        c->prof_mask = kind_mask;
+    });
+}
+
 int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, int kinds) {
     return guarded([&] {
         if (!c || kinds < 0) throw GrError(GR_E_ARG, "invalid arguments");
@@ -770,11 +839,14 @@ int gr_profile_end(gr_ctx* c, double* ms, long long* launches, double* bytes, in
         // refresh, per cell of a refreshed track: read capacity + demand, write
         // wcost + vcost (4 B each) and presum (8 B); wcost stays in LDS here
         // but is counted, as for the reference's full refresh
-        by[GR_K_REFRESH] += 24.0 * double(refreshed1 - c->refreshed0);
+        const auto selected = [&](int kind) { return (c->prof_mask >> kind & 1u) != 0; };
+        if (selected(GR_K_REFRESH)) by[GR_K_REFRESH] += 24.0 * double(refreshed1 - c->refreshed0);
         // wires: 4 B atomic read-modify-write + 4 B timestamp + 4 B dirty mark per unit edge
-        by[GR_K_COMMIT_WIRES] += 12.0 * double(c->prof_wire_edges + (s1.wire_edges - c->prof_stats0.wire_edges));
+        if (selected(GR_K_COMMIT_WIRES))
+            by[GR_K_COMMIT_WIRES] += 12.0 * double(c->prof_wire_edges + (s1.wire_edges - c->prof_stats0.wire_edges));
         // vias: 4 B timestamp + 4 B demand per via layer step
-        by[GR_K_COMMIT_VIAS] += 8.0 * double(c->prof_vias + (s1.total_vias - c->prof_stats0.total_vias));
+        if (selected(GR_K_COMMIT_VIAS))
+            by[GR_K_COMMIT_VIAS] += 8.0 * double(c->prof_vias + (s1.total_vias - c->prof_stats0.total_vias));
         for (int k = 0; k < std::min(kinds, (int)GR_K_COUNT); ++k) {
             if (ms) ms[k] = t[k];
             if (launches) launches[k] = n[k];
@@ -803,9 +875,231 @@ int gr_ripup(gr_ctx* c, int threshold, unsigned char* is_of_net) {
         hipLaunchKernelGGL(overflow_prefix_kernel, dim3(c->tracks), dim3(256), 0, c->stream, a);
         if (c->net_count)
             hipLaunchKernelGGL(overflow_nets_kernel, dim3((c->net_count + 255) / 256), dim3(256), 0, c->stream, c->g,
-                               c->of_sum, RouteStore{c->routes, c->pin_acc_num}, c->net_count, c->of_flags);
+                               c->of_sum, RouteStore{c->routes, c->pin_acc_num, c->route_per_pin}, c->net_count, c->of_flags);
         hip_check(hipGetLastError(), "ripup kernels");
         download(is_of_net, c->of_flags, size_t(c->net_count), c->stream);
+    });
+}
+
+int gr_zshape_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_ids, const long long* rec_off,
+                     const int* records, const int* raw_cap) {
+    return guarded([&] {
+        if (!c || nbatch < 0 || !batch_off || !rec_off) throw GrError(GR_E_ARG, "null argument");
+        if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
+        const DeviceGrid& g = c->g;
+        const int n = std::max(g.X, g.Y), nn = n * n, L = g.L;
+        if (batch_off[0] != 0 || rec_off[0] != 0) throw GrError(GR_E_ARG, "offsets must start at 0");
+        const int nets = batch_off[nbatch];
+        const long long words = rec_off[nbatch];
+        if ((nets && (!net_ids || !raw_cap)) || (words && !records)) throw GrError(GR_E_ARG, "null argument");
+        long long max_raw = 1;
+        // every index the kernels derive from the plan is checked here
+        for (int b = 0; b < nbatch; ++b) {
+            const int begin = batch_off[b], count = batch_off[b + 1] - begin;
+            const long long r0 = rec_off[b], r1 = rec_off[b + 1];
+            if (count < 0 || r1 < r0 + count) throw GrError(GR_E_ARG, "batch offsets out of order");
+            long long raw = 0;
+            for (int k = 0; k < count; ++k) raw += raw_cap[begin + k];
+            max_raw = std::max(max_raw, raw);
+            for (int k = 0; k < count; ++k) {
+                const int id = net_ids[begin + k];
+                if (id < 0 || id >= c->net_count) throw GrError(GR_E_ARG, "net id outside route storage");
+                if (raw_cap[begin + k] < 2) throw GrError(GR_E_ARG, "raw capacity too small");
+                const long long base = r0 + records[r0 + k];
+                if (records[r0 + k] < count || base + 2 > r1) throw GrError(GR_E_ARG, "record offset outside the batch");
+                const int* rec = records + base;
+                const int nodes = rec[1];
+                if (nodes < 1 || base + 2 + 6ll * nodes > r1) throw GrError(GR_E_ARG, "record length outside the batch");
+                if (rec[0] < 0 || rec[0] + raw_cap[begin + k] > raw) throw GrError(GR_E_ARG, "raw offset outside the batch");
+                for (int i = 0; i < nodes; ++i) {
+                    const int* r = rec + 2 + 6 * i;
+                    if (r[0] < 0 || r[0] >= nn || r[0] / n >= g.X || r[0] % n >= g.Y)
+                        throw GrError(GR_E_ARG, "node position outside the grid");
+                    if (!((r[1] == -1 && r[2] == -1) || (r[1] >= 1 && r[1] <= L && r[2] >= r[1] && r[2] <= L)))
+                        throw GrError(GR_E_ARG, "node layer outside the routing layers");
+                    for (int j = 3; j < 6; ++j)
+                        if (r[j] != -1 && (r[j] < 6 * (i + 1) || r[j] >= 6 * nodes || r[j] % 6))
+                            throw GrError(GR_E_ARG, "child record must follow its parent in preorder");
+                }
+            }
+        }
+        c->z_ws.carve([&](auto& cv) {
+            c->z_net_ids = cv.template take<int>(nets);
+            c->z_records = cv.template take<int>(words);
+            c->z_raw_cap = cv.template take<int>(nets);
+            c->z_raw = cv.template take<int>(max_raw);
+        });
+        if (nets) {
+            upload(c->z_net_ids, net_ids, sizeof(int) * nets, c->stream);
+            upload(c->z_raw_cap, raw_cap, sizeof(int) * nets, c->stream);
+        }
+        if (words) upload(c->z_records, records, sizeof(int) * words, c->stream);
+        const long long plane = (long long)(L + 1) * nn;
+        if (plane != c->z_plane) {
+            c->mem.release(c->z_map);
+            c->mem.release(c->z_prev);
+            c->z_map = c->mem.alloc<int>(6 * plane);
+            c->z_prev = c->mem.alloc<int>(16 * plane);
+            hip_check(hipMemsetAsync(c->z_map, 0, sizeof(int) * 6 * plane, c->stream), "clear Z-shape map");
+            hip_check(hipMemsetAsync(c->z_prev, 0, sizeof(int) * 16 * plane, c->stream), "clear Z-shape map");
+            c->z_plane = plane;
+        }
+        c->z_batches = nbatch;
+        c->z_batch_off.assign(batch_off, batch_off + nbatch + 1);
+        c->z_rec_off.assign(rec_off, rec_off + nbatch + 1);
+        hip_check(hipStreamSynchronize(c->stream), "Z-shape upload");
+    });
+}
+
+int gr_zshape_run(gr_ctx* c, int first_stamp, double* elapsed_ms) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (!c->z_map) throw GrError(GR_E_STATE, "no Z-shape plan uploaded");
+        const DeviceGrid& g = c->g;
+        ZshapeArgs a;
+        a.g = g;
+        a.presum = c->presum;
+        a.vcost = c->vcost;
+        a.map = c->z_map;
+        a.prev = c->z_prev;
+        a.raw = c->z_raw;
+        a.n = std::max(g.X, g.Y);
+        a.layers = g.L + 1;
+        a.dirv = g.DIR ^ 1;
+        a.error = c->error;
+        const RouteStore rs{c->routes, c->pin_acc_num, c->route_per_pin};
+        hip_check(hipDeviceSynchronize(), "before DAG interval");
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int b = 0; b < c->z_batches; ++b) {
+            const int begin = c->z_batch_off[b], count = c->z_batch_off[b + 1] - begin;
+            launch_refresh(c, 0);
+            if (!count) continue;
+            a.points = c->z_records + c->z_rec_off[b];
+            a.count = count;
+            a.raw_cap = c->z_raw_cap + begin;
+            timed(c, GR_K_ZSHAPE, 0.0, [&] {
+                hipLaunchKernelGGL(zshape_reset_kernel, dim3(count), dim3(256), 0, c->stream, a);
+                hipLaunchKernelGGL(zshape_solve_kernel, dim3(count), dim3(kWave), 0, c->stream, a);
+                hipLaunchKernelGGL(zshape_convert_kernel, dim3((count + 255) / 256), dim3(256), 0, c->stream, a,
+                                   (const int*)(c->z_net_ids + begin), rs);
+            });
+            launch_commit(c, begin, count, first_stamp + b, 1, c->z_net_ids);
+            hip_check(hipGetLastError(), "Z-shape kernels");
+        }
+        hip_check(hipStreamSynchronize(c->stream), "DAG interval");
+        const auto t1 = std::chrono::steady_clock::now();
+        if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        check_device_error(c, "gr_zshape_run");
+    });
+}
+
+int gr_soft_forward(gr_ctx* c, int begin, int count, double slope, double zero_slope, double via_weight, double tau,
+                    int precision, double* total, double* per_net, double* elapsed_ms) {
+    return guarded([&] {
+        if (!c || !total) throw GrError(GR_E_ARG, "null argument");
+        if (precision != 32 && precision != 64) throw GrError(GR_E_ARG, "precision must be 32 or 64");
+        if (!(tau > 0.0)) throw GrError(GR_E_ARG, "tau must be positive");
+        require_stage(c, begin, count);
+        c->soft_ready = false;
+        const size_t cells = c->cells;
+        SoftCostArgs& sc = c->soft_cost;
+        SoftArgs& a = c->soft;
+        c->soft_ws.carve([&](auto& cv) {
+            sc.wcost = cv.template take<double>(cells);
+            sc.presum = cv.template take<double>(cells);
+            sc.vcost = cv.template take<double>(cells);
+            sc.dw_ds = cv.template take<double>(cells);
+            sc.dw_dz = cv.template take<double>(cells);
+            sc.dv_ds = cv.template take<double>(cells);
+            sc.dv_dz = cv.template take<double>(cells);
+            a.gw = cv.template take<double>(cells);
+            a.gv = cv.template take<double>(cells);
+            c->soft_params = cv.template take<double>(size_t(kStatStripes) * kCounterStride);
+            a.value = cv.template take<double>(std::max(count, 1));
+        });
+        const long long nodes = (long long)c->node_off_h[begin + count] - c->node_off_h[begin];
+        void* tables = nullptr;
+        c->soft_tables.carve(
+            [&](auto& cv) { tables = cv.template take<char>(soft_scratch_bytes(c->g.L, nodes, precision)); });
+        sc.g = c->g;
+        sc.capacity = c->capacity;
+        sc.demand = c->demand;
+        sc.slope = slope;
+        sc.zero_slope = zero_slope;
+        sc.via_weight = via_weight;
+        a.g = c->g;
+        a.presum = sc.presum;
+        a.vcost = sc.vcost;
+        a.node_off = c->node_off;
+        a.nodes = c->nodes;
+        a.parents = c->parents;
+        a.begin = begin;
+        a.count = count;
+        a.tau = tau;
+        a.scratch = tables;
+        hipEvent_t e0 = take_event(c), e1 = take_event(c);
+        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+        hipLaunchKernelGGL(soft_costs_kernel, dim3(c->tracks), dim3(256), 0, c->stream, sc);
+        if (count) launch_soft_forward(a, precision, c->stream);
+        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+        hip_check(hipGetLastError(), "soft forward kernels");
+        std::vector<double> v(std::max(count, 1));
+        download(v.data(), a.value, sizeof(double) * count, c->stream);
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (elapsed_ms) *elapsed_ms = ms;
+        c->next_event -= 2;
+        double sum = 0.0;
+        for (int k = 0; k < count; ++k) sum += v[k];
+        *total = sum;
+        if (per_net) std::memcpy(per_net, v.data(), sizeof(double) * count);
+        c->soft_precision = precision;
+        c->soft_ready = true;
+    });
+}
+
+int gr_soft_backward(gr_ctx* c, double* grad_params, double* elapsed_ms) {
+    return guarded([&] {
+        if (!c || !grad_params) throw GrError(GR_E_ARG, "null argument");
+        if (!c->soft_ready) throw GrError(GR_E_STATE, "no soft forward pass to differentiate");
+        const size_t cells = c->cells;
+        SoftArgs& a = c->soft;
+        hip_check(hipMemsetAsync(a.gw, 0, sizeof(double) * cells, c->stream), "clear gradients");
+        hip_check(hipMemsetAsync(a.gv, 0, sizeof(double) * cells, c->stream), "clear gradients");
+        hip_check(hipMemsetAsync(c->soft_params, 0, sizeof(double) * kStatStripes * kCounterStride, c->stream),
+                  "clear parameter gradients");
+        hipEvent_t e0 = take_event(c), e1 = take_event(c);
+        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+        if (a.count) launch_soft_backward(a, c->soft_precision, c->stream);
+        const int blocks = int(std::min<size_t>(1024, (cells + 255) / 256));
+        hipLaunchKernelGGL(soft_params_kernel, dim3(blocks), dim3(256), 0, c->stream, c->soft_cost, a.gw, a.gv,
+                           (long long)cells, c->soft_params);
+        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+        hip_check(hipGetLastError(), "soft backward kernels");
+        std::vector<double> part(size_t(kStatStripes) * kCounterStride);
+        download(part.data(), c->soft_params, sizeof(double) * part.size(), c->stream);
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (elapsed_ms) *elapsed_ms = ms;
+        c->next_event -= 2;
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < kStatStripes; ++i) s += part[size_t(i) * kCounterStride + j];
+            grad_params[j] = s;
+        }
+    });
+}
+
+int gr_soft_edge_grads(gr_ctx* c, double* grad_wcost, double* grad_vcost) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (!c->soft_ready) throw GrError(GR_E_STATE, "no soft pass");
+        for (auto [dst, src] : {std::pair<double*, double*>{grad_wcost, c->soft.gw}, {grad_vcost, c->soft.gv}}) {
+            if (!dst) continue;
+            std::vector<double> h(c->cells);
+            download(h.data(), src, sizeof(double) * c->cells, c->stream);
+            from_track_major(c->g, h, dst);
+        }
     });
 }
 

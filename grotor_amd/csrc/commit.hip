@@ -15,7 +15,7 @@
 namespace grotor {
 
 __device__ __forceinline__ const int* route_of(const RouteStore& rs, int id) {
-    return rs.routes + (size_t)rs.pin_acc_num[id] * kRoutePerPin;
+    return rs.routes + (size_t)rs.pin_acc_num[id] * rs.per_pin;
 }
 
 __device__ __forceinline__ void add_demand(const CommitArgs& a, int cell, float value) {

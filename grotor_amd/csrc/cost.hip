@@ -57,6 +57,7 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
             // wcost: src/cost/full_update.cuh:44-46
             float w = (float)inc_1;
             if (p < len - 1) w = (float)((double)w + g.unit_wire * (double)elen[p]);
+            if (cp.integer_wire) w = (float)(long long)fminf(fmaxf(w, 0.f), 1.e7f);
             if (wcost_out) wcost_out[base + p] = w;
             if (p < len - 1) fv[p + 1] = (double)w;
             // vcost: src/cost/full_update.cuh:67-82

@@ -12,6 +12,7 @@ struct CostParams {
     double slope = 0.5;       // of_cost_scaled, capacity > 0 (src/cost/full_update.cuh:11)
     double zero_slope = 1.5;  // of_cost_scaled, capacity == 0 (src/cost/full_update.cuh:12)
     double via_weight = 1.0;  // multiplies unit_via_cost (src/cost/full_update.cuh:82)
+    int integer_wire = 0;     // Z-shape build: wcost rounded to an integer in [0, 1e7] (src/cost/full_update.cuh:47-50)
 };
 
 // Stage statistics, striped over kStatStripes cache lines: a wavefront adds
@@ -33,6 +34,7 @@ enum DeviceError : int {
     kDevNoSolution = 2,
     kDevSpinLimit = 3,
     kDevColorOverflow = 4,
+    kDevBadCoordinate = 5,
 };
 
 __global__ void refresh_tracks_kernel(DeviceGrid g, const float* capacity, const float* demand, float* vcost,
@@ -42,6 +44,7 @@ __global__ void refresh_tracks_kernel(DeviceGrid g, const float* capacity, const
 struct RouteStore {
     int* routes;
     const int* pin_acc_num;
+    int per_pin;  // route words per pin: 12 (include/router_state.hpp:31), 256 for Z-shape stages
 };
 
 struct CommitArgs {
@@ -63,11 +66,9 @@ struct LshapeArgs {
     const float* vcost;
     const double* presum;
     RouteStore rs;
-    const int* net_ids;
-    const int* node_off;
+    const int4* desc;  // per execution position: node begin, nodes, first route word, route capacity
     const int* nodes;
     const int* parents;
-    const int* slots;  // permuted stage slots
     int begin, count;
     int nmax;          // node capacity of one workgroup's scratch
     double* gscratch;  // global scratch when the tree does not fit in LDS
@@ -131,6 +132,55 @@ struct ColorArgs {
 constexpr int kColorChunk = 4;  // nets per dequeue
 __global__ void color_kernel(ColorArgs a);
 __global__ void color_hist_kernel(const int* color, int first, int n, int* counts);
+
+// Z-shape
+struct ZshapeArgs {
+    DeviceGrid g;
+    const double* presum;
+    const float* vcost;
+    int* map;    // 6 planes of layers * n * n
+    int* prev;   // 16 planes
+    const int* points;   // batch records: per-net record offsets, then records
+    int count;
+    int* raw;            // raw route words of the batch
+    const int* raw_cap;  // raw words per net
+    int n, layers, dirv;
+    int* error;
+};
+__global__ void zshape_reset_kernel(ZshapeArgs a);
+__global__ void zshape_solve_kernel(ZshapeArgs a);
+__global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteStore rs);
+
+// Soft DP
+struct SoftCostArgs {
+    DeviceGrid g;
+    const float* capacity;
+    const float* demand;
+    double slope, zero_slope, via_weight;
+    double* wcost;   // smooth wire cost per cell
+    double* presum;  // exclusive prefix of wcost along each track
+    double* vcost;   // smooth via cost per cell
+    double *dw_ds, *dw_dz, *dv_ds, *dv_dz;  // partials in slope / zero_slope
+};
+struct SoftArgs {
+    DeviceGrid g;
+    const double* presum;
+    const double* vcost;
+    const int* node_off;  // stage topology (uploaded slots)
+    const int* nodes;
+    const int* parents;
+    int begin, count;
+    double tau;
+    void* scratch;   // checkpointed tables of the nets [begin, begin + count)
+    double* value;   // soft value per net
+    double* gw;      // dF/dwcost per cell
+    double* gv;      // dF/dvcost per cell
+};
+__global__ void soft_costs_kernel(SoftCostArgs a);
+__global__ void soft_params_kernel(SoftCostArgs c, const double* gw, const double* gv, long long cells, double* out);
+size_t soft_scratch_bytes(int L, long long nodes, int precision);
+void launch_soft_forward(const SoftArgs& a, int precision, hipStream_t stream);
+void launch_soft_backward(const SoftArgs& a, int precision, hipStream_t stream);
 
 __global__ void fill_int_kernel(int* p, int value, long long n);
 

@@ -36,7 +36,7 @@ namespace {
 // constant so that every per-layer loop unrolls.
 template <int L>
 struct Tables {
-    double* dist;   // nmax * L * L  interval tables
+    double* dist;   // nmax * T      interval tables, upper triangle (low <= high) row by row
     double* diag;   // nmax * L      best arrival on each parent layer
     double* w1;     // nmax * L      first-leg wire (child side; the whole edge when straight)
     double* w2;     // nmax * L      second-leg wire (parent side, bent edges)
@@ -54,9 +54,19 @@ struct Tables {
     int* woff;      // nmax + 1
 };
 
+// entries of an interval table: intervals low <= high of L layers
+template <int L>
+constexpr int kTri = L * (L + 1) / 2;
+
+// position of interval (lo, hi), lo <= hi, in a triangular table
+template <int L>
+__host__ __device__ constexpr int tri(int lo, int hi) {
+    return lo * (2 * L - lo + 1) / 2 + (hi - lo);
+}
+
 template <int L>
 __host__ __device__ constexpr size_t table_bytes(int nmax) {
-    return 8 * (size_t(nmax) * (L * L + 3 * L) + L + 2 * L * L) + 4 * (size_t(nmax) * 3 * L) +
+    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + L + 2 * L * L) + 4 * (size_t(nmax) * 3 * L) +
            4 * (size_t(nmax) * L + 4 * size_t(nmax) + L + nmax + 1);
 }
 
@@ -64,7 +74,7 @@ template <int L>
 __device__ inline Tables<L> carve(double* base, int nmax) {
     Tables<L> s;
     s.dist = base;
-    s.diag = s.dist + size_t(nmax) * L * L;
+    s.diag = s.dist + size_t(nmax) * kTri<L>;
     s.w1 = s.diag + size_t(nmax) * L;
     s.w2 = s.w1 + size_t(nmax) * L;
     s.best = s.w2 + size_t(nmax) * L;
@@ -92,13 +102,12 @@ __device__ __forceinline__ double wire_cost(const DeviceGrid& g, const double* p
 
 template <int L>
 __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
-    constexpr int LL = L * L;
+    constexpr int LL = L * L, T = kTri<L>;
     const int lane = threadIdx.x;
     const DeviceGrid& g = a.g;
     const int X = g.X, Y = g.Y;
-    const int slot = a.slots ? a.slots[a.begin + blockIdx.x] : a.begin + blockIdx.x;
-    const int id = a.net_ids[slot];
-    const int n0 = a.node_off[slot], n = a.node_off[slot + 1] - n0;
+    const int4 desc = a.desc[a.begin + blockIdx.x];
+    const int n0 = desc.x, n = desc.y;
     for (int i = lane; i < n; i += kWave) {
         s.nd[i] = a.nodes[n0 + i];
         s.par[i] = a.parents[n0 + i];
@@ -137,16 +146,16 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
     for (int r = lane; r < n * L; r += kWave) {
         const int i = r / L, lo = r % L;
         const int fixed = s.nd[i] / g.XY;
-        double* row = s.dist + size_t(i) * LL + lo * L;
+        double* row = s.dist + size_t(i) * T + tri<L>(lo, lo);  // row[h - lo] = interval (lo, h)
         double v = 0.0;
-        row[lo] = 0.0;
+        row[0] = 0.0;
         for (int hi = lo + 1; hi < L; ++hi) {
             v = v + (double)s.vs[i * L + hi - 1];
-            row[hi] = v;
+            row[hi - lo] = v;
         }
         if (fixed < L)
             for (int hi = lo; hi < L; ++hi)
-                if (lo > fixed || hi < fixed) row[hi] = kInf;
+                if (lo > fixed || hi < fixed) row[hi - lo] = kInf;
     }
     __syncthreads();
     for (int i = n - 1; i >= 1; --i) {
@@ -158,14 +167,14 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
         // corner via columns of a bent edge
         if (lane < L) {
             const int c = lane;
-            const double* nd = s.dist + size_t(i) * LL;
+            const double* nd = s.dist + size_t(i) * T;
             double best = kInf;
             int ch = 0;
             for (int lo = 0; lo <= c; ++lo)
 #pragma unroll
                 for (int hi = 0; hi < L; ++hi)
-                    if (hi >= c && nd[lo * L + hi] < best) {
-                        best = nd[lo * L + hi];
+                    if (hi >= c && nd[tri<L>(lo, hi)] < best) {
+                        best = nd[tri<L>(lo, hi)];
                         ch = lo * L + hi;
                     }
             s.best[c] = best;
@@ -220,28 +229,31 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
         if (lane < L) {
             const int lo = lane;
             const double* dg = s.diag + i * L;
-            double* prow = s.dist + size_t(p) * LL + lo * L;
+            double* prow = s.dist + size_t(p) * T + tri<L>(lo, lo);
             double c = dg[lo];
 #pragma unroll
             for (int hi = 0; hi < L; ++hi) {
                 if (hi < lo) continue;
                 if (hi > lo && !(c <= dg[hi])) c = dg[hi];
-                prow[hi] += c;
+                prow[hi - lo] += c;
             }
         }
         __syncthreads();
     }
     // root choice and traceback (src/routing/lshape_warp.cuh:193-213)
-    const int base_word = a.rs.pin_acc_num[id] * kRoutePerPin;
-    const int cap = kRoutePerPin * (a.rs.pin_acc_num[id + 1] - a.rs.pin_acc_num[id]);
+    const int base_word = desc.z, cap = desc.w;
     if (lane == 0) {
         int ch = 0;
+        double cv = s.dist[0];
         for (int lo = 0; lo < L; ++lo)
             for (int hi = lo; hi < L; ++hi)
-                if (s.dist[lo * L + hi] < s.dist[ch]) ch = lo * L + hi;
+                if (s.dist[tri<L>(lo, hi)] < cv) {
+                    cv = s.dist[tri<L>(lo, hi)];
+                    ch = lo * L + hi;
+                }
         s.rng[0] = ch;
         s.path[0] = 0;
-        bool ok = s.dist[ch] < kInf;
+        bool ok = cv < kInf;
         for (int i = 1; i < n; ++i) {
             const int pr = s.rng[s.par[i]], plo = pr / L, phi = pr % L;
             const double* dg = s.diag + i * L;
@@ -255,7 +267,7 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
                 }
             s.path[i] = pth;
             s.rng[i] = pth / LL;
-            if (!(s.dist[size_t(i) * LL + s.rng[i]] < kInf)) ok = false;
+            if (!(s.dist[size_t(i) * T + tri<L>(s.rng[i] / L, s.rng[i] % L)] < kInf)) ok = false;
         }
         int total = 1;
         for (int i = 0; i < n; ++i) {

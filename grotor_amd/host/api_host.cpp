@@ -6,6 +6,7 @@
 
 #include "../csrc/gr_status.hpp"
 #include "design.hpp"
+#include "zshape_host.hpp"
 
 namespace grotor {
 namespace {
@@ -23,6 +24,9 @@ struct gr_design {
     std::vector<float> capacity;  // routing layers, IDX order
     std::vector<double> short_costs;
     std::vector<int> net_pin_off, net_pins, net_extra_off, net_extra, net_sub_off, net_subs;
+    // Z-shape stage
+    std::vector<int> zorder, zpt_off, zpts, zhs_off, zhsegs, zvs_off, zvsegs;
+    ZshapePlan zplan;
 };
 
 extern "C" {
@@ -133,6 +137,66 @@ int gr_design_topology(gr_design* g, gr_topology* t) {
         t->y_edge_len = gr.y_edge_len.data();
         t->unit_length_wire_cost = gr.unit_length_wire_cost;
         t->unit_via_cost = gr.unit_via_cost;
+    });
+}
+
+int gr_design_zshape_stage(gr_design* g, gr_zshape_stage* t) {
+    return guarded([&] {
+        if (!g || !t) throw GrError(GR_E_ARG, "null argument");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        g->zorder = zshape_order(g->d);
+        g->zpt_off.assign(1, 0), g->zhs_off.assign(1, 0), g->zvs_off.assign(1, 0);
+        g->zpts.clear(), g->zhsegs.clear(), g->zvsegs.clear();
+        for (int id : g->zorder) {
+            const Subnet& s = g->d.subnets[id];
+            g->zpts.insert(g->zpts.end(), s.points.begin(), s.points.end());
+            g->zpt_off.push_back(int(g->zpts.size()));
+            for (auto& e : s.hsegs) g->zhsegs.push_back(e.first), g->zhsegs.push_back(e.second);
+            g->zhs_off.push_back(int(g->zhsegs.size() / 2));
+            for (auto& e : s.vsegs) g->zvsegs.push_back(e.first), g->zvsegs.push_back(e.second);
+            g->zvs_off.push_back(int(g->zvsegs.size() / 2));
+        }
+        t->nets = int(g->zorder.size());
+        t->order = g->zorder.data();
+        t->pt_off = g->zpt_off.data();
+        t->pts = g->zpts.data();
+        t->hs_off = g->zhs_off.data();
+        t->hsegs = g->zhsegs.data();
+        t->vs_off = g->zvs_off.data();
+        t->vsegs = g->zvsegs.data();
+    });
+}
+
+int gr_design_zshape_plan(gr_design* g, const int* color, int num_batches, int max_batch, gr_zshape_plan* p) {
+    return guarded([&] {
+        if (!g || !color || !p || num_batches < 0 || max_batch < 1) throw GrError(GR_E_ARG, "invalid arguments");
+        if (g->zorder.empty() && !g->d.subnets.empty())
+            throw GrError(GR_E_STATE, "no Z-shape stage order (gr_design_zshape_stage)");
+        try {
+            g->zplan = zshape_plan(g->d, g->zorder, color, num_batches, max_batch);
+        } catch (const std::runtime_error& e) {
+            throw GrError(GR_E_ARG, e.what());
+        }
+        const ZshapePlan& z = g->zplan;
+        p->batches = int(z.batch_off.size()) - 1;
+        p->batch_off = z.batch_off.data();
+        p->net_ids = z.net_ids.data();
+        p->rec_off = z.rec_off.data();
+        p->records = z.records.data();
+        p->raw_cap = z.raw_cap.data();
+    });
+}
+
+int gr_design_zshape_finish(gr_design* g, int* routes, int words_per_pin, int* promoted) {
+    return guarded([&] {
+        if (!g || !routes || words_per_pin < 4) throw GrError(GR_E_ARG, "invalid arguments");
+        const int p = zshape_promote(g->d, words_per_pin, routes);
+        try {
+            zshape_canonicalize(g->d, words_per_pin, routes);
+        } catch (const std::runtime_error& e) {
+            throw GrError(GR_E_CAPACITY, e.what());
+        }
+        if (promoted) *promoted = p;
     });
 }
 

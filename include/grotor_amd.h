@@ -70,6 +70,18 @@ int gr_ctx_destroy(gr_ctx *ctx);
  * 12 * pin_acc_num[i] (include/router_state.hpp:31, src/preprocess.hpp:1699-1704). */
 int gr_ctx_set_routes(gr_ctx *ctx, int net_count, const int *pin_acc_num);
 
+/* Route words per pin for the next gr_ctx_set_routes: 12 by default
+ * (ROUTE_PER_PIN, include/router_state.hpp:31); the Z-shape pipeline uses
+ * 256 (src/pipeline_ggr.hpp:10). */
+int gr_ctx_set_route_stride(gr_ctx *ctx, int words_per_pin);
+
+/* Cost-model parameters of the refresh: overflow slopes (0.5 and 1.5,
+ * src/cost/full_update.cuh:9-14), via weight (graph::via_weight = 1, :7) and
+ * the Z-shape build's integer wire costs (:47-50, 0 or 1).  Marks every
+ * track for refresh. */
+int gr_ctx_set_cost_params(gr_ctx *ctx, double slope, double zero_slope, double via_weight,
+                           int integer_wire);
+
 /* Cost refresh: wcost, vcost and the Fenwick-ordered prefix presum of every
  * track whose demand changed since the last refresh
  * (graph::refresh_cost, src/cost/incremental_update.cuh:98-132). */
@@ -156,7 +168,8 @@ enum {
     GR_K_BATCH_MARKS = 4,  /* batching: mark expansion and per-cell lists */
     GR_K_BATCH_COLOR = 5,  /* batching: in-order colouring */
     GR_K_RIPUP = 6,
-    GR_K_COUNT = 7
+    GR_K_ZSHAPE = 7,       /* Z-shape DP (reset, solve, convert) */
+    GR_K_COUNT = 8
 };
 
 /* Start timing every kernel launch of this context with HIP events on its
@@ -164,6 +177,10 @@ enum {
  * (ms), the launch count and the algorithmic HBM bytes of those launches.
  * No reference counterpart (the reference prints stage wall times only). */
 int gr_profile_begin(gr_ctx *ctx);
+/* Restrict the events of gr_profile_begin to the kernel classes whose bit
+ * (1 << GR_K_*) is set in kind_mask (all classes by default); the others run
+ * without events and report nothing.  Fewer events, less host overhead. */
+int gr_profile_select(gr_ctx *ctx, unsigned kind_mask);
 int gr_profile_end(gr_ctx *ctx, double *ms, long long *launches, double *bytes, int kinds);
 
 /* Copy `words` route words (from word 0) to the host (RouterState::download_routes,
@@ -174,6 +191,47 @@ int gr_routes_download(gr_ctx *ctx, int *routes, long long words);
  * (graph::ripup, src/routing/congestion.cuh:114-136): is_of_net[i] = 1 if net
  * i's route crosses more than 10 edges with capacity + threshold <= demand. */
 int gr_ripup(gr_ctx *ctx, int threshold, unsigned char *is_of_net);
+
+/* ---- Z-shape (GGR) stage on the device (src/pipeline_ggr.hpp:88-127) ----
+ * The route storage must hold 256 words per pin (gr_ctx_set_route_stride
+ * before gr_ctx_set_routes) and the cost refresh the Z-shape build's
+ * integer wire costs (gr_ctx_set_cost_params). */
+
+/* Upload a plan from gr_design_zshape_plan (validated on the host). */
+int gr_zshape_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_ids,
+                     const long long *rec_off, const int *records, const int *raw_cap);
+
+/* Run the uploaded Z-shape batches: per batch b (stamp first_stamp + b)
+ * refresh, reset_nodes, solve, convert_routes and the wire-then-via commit;
+ * *elapsed_ms = the synchronized interval (DAG_TIMING). */
+int gr_zshape_run(gr_ctx *ctx, int first_stamp, double *elapsed_ms);
+
+/* ---- soft (log-sum-exp) L-shape DP ----
+ * No counterpart in the reference: a differentiable relaxation of the
+ * L-shape DP above (every minimum becomes smin_tau(x) = -tau log sum
+ * exp(-x/tau)) over the smooth form of the cost model
+ * (src/cost/full_update.cuh:9-83 with exact exp), at the context's current
+ * demand.  Its gradient is the expected wire and via usage of every cell,
+ * and the derivative in the cost-model parameters. */
+
+/* Soft DP of the uploaded stage slots [begin, begin + count) with overflow
+ * slopes (slope, zero_slope; 0.5 and 1.5 in the reference), via weight
+ * (1.0) and temperature tau > 0, in FP32 (precision 32) or FP64 (64)
+ * tables.  *total = sum of the nets' soft values; per_net (may be NULL)
+ * receives each value.  The tables are checkpointed for gr_soft_backward
+ * until the next forward or stage upload.  *elapsed_ms: device time. */
+int gr_soft_forward(gr_ctx *ctx, int begin, int count, double slope, double zero_slope,
+                    double via_weight, double tau, int precision, double *total,
+                    double *per_net, double *elapsed_ms);
+
+/* Adjoint of the last gr_soft_forward: grad_params[0..2] = d total /
+ * d(slope, zero_slope, via_weight); per-cell gradients are kept for
+ * gr_soft_edge_grads.  *elapsed_ms: device time. */
+int gr_soft_backward(gr_ctx *ctx, double *grad_params, double *elapsed_ms);
+
+/* d total / d wcost and d total / d vcost per routing cell (IDX order):
+ * the expected use of each wire edge and via landing.  Either may be NULL. */
+int gr_soft_edge_grads(gr_ctx *ctx, double *grad_wcost, double *grad_vcost);
 
 /* ---- host front end and pipeline (src/main.cu, src/pipeline.hpp) ---- */
 
@@ -226,6 +284,40 @@ int gr_design_nets(gr_design *design, int *count, const int **pin_off, const int
  * out (*npoints points, *nhsegs / *nvsegs segment pairs). */
 int gr_tree_geometry(int Y, int n, const int *positions, const int *adj_off, const int *adj,
                      int *points, int *npoints, int *hsegs, int *nhsegs, int *vsegs, int *nvsegs);
+
+/* ---- Z-shape (GGR) stage, src/pipeline_ggr.hpp ---- */
+
+/* Priority order of the Z-shape stage (hpwl descending, larger subnet id
+ * first on ties, src/pipeline_ggr.hpp:31-33) and the batching geometry in
+ * that order, for gr_batch_plan.  Arrays are owned by the design. */
+typedef struct gr_zshape_stage {
+    int nets;
+    const int *order;
+    const int *pt_off, *pts, *hs_off, *hsegs, *vs_off, *vsegs;
+} gr_zshape_stage;
+int gr_design_zshape_stage(gr_design *design, gr_zshape_stage *out);
+
+/* Execution plan from the stage's batch colors: every RSMT batch refined
+ * into bounding-box-disjoint batches of at most max_batch nets (250 in the
+ * reference), the list reversed (src/pipeline_ggr.hpp:49-59,
+ * generate_bbox_batches src/routing/zshape_topology.hpp:7-58), and each
+ * net's tree encoded for the kernel (encode_batch, :59-121). */
+typedef struct gr_zshape_plan {
+    int batches;
+    const int *batch_off;        /* [batches+1] over net_ids */
+    const int *net_ids;          /* subnet per execution position */
+    const long long *rec_off;    /* [batches+1] into records */
+    const int *records;          /* per batch: per-net record offsets, then the records */
+    const int *raw_cap;          /* raw route words per net (pins * 256) */
+} gr_zshape_plan;
+int gr_design_zshape_plan(gr_design *design, const int *color, int num_batches, int max_batch,
+                          gr_zshape_plan *out);
+
+/* Host post-processing of downloaded Z-shape routes (IDX words,
+ * words_per_pin per pin): layer promotion of congested segments and
+ * canonical merged segments (promote_congested_segments /
+ * canonicalize_routes, src/routing/zshape_topology.hpp:122-232). */
+int gr_design_zshape_finish(gr_design *design, int *routes, int words_per_pin, int *promoted);
 
 /* Write the .out solution for the prepared design given its route words
  * (graph::output_nets, src/output.hpp:79-112). */

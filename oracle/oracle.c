@@ -120,6 +120,11 @@ void ora_dense_prefix_track(const float *w, int len, double *presum) {
  * and the Fenwick prefix of the track (src/cost/fenwick.cuh:129-168).  Tracks
  * are independent: a cell's costs read only itself and its predecessor on
  * the track, and presum reads only the track's wcost. */
+/* Z-shape (GGR) builds round each wire cost to an integer in [0, 1e7]
+ * (src/cost/full_update.cuh:47-50). */
+static int integer_wire = 0;
+void ora_set_integer_wire(int on) { integer_wire = on; }
+
 static void refresh_track(const ora_grid *g, const float *capacity, const float *demand,
                           double slope, double zero_slope, double via_weight, int l, int t,
                           float *w, float *vcost, double *presum, float *tw, double *tp) {
@@ -135,6 +140,7 @@ static void refresh_track(const ora_grid *g, const float *capacity, const float 
             wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->x_edge_len[x]);
         if (dir == 1 && y < Y - 1)
             wc = (float)((double)wc + g->unit_length_wire_cost * (double)g->y_edge_len[y]);
+        if (integer_wire) wc = (float)(long long)fminf(fmaxf(wc, 0.f), 1.e7f);
         if (w) w[i] = wc;
         tw[p] = wc;
         if (vcost) {
@@ -550,5 +556,58 @@ int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
     }
     free(dirty);
     free(rof);
+    return status;
+}
+
+/* One Z-shape stage (src/pipeline_ggr.hpp:114-127): per batch, refresh the
+ * tracks whose demand changed, reset the batch's tree-node cells, solve,
+ * convert the raw words into route words, commit wires then vias.  records
+ * holds each batch's encoded nets from rec_off[b]; raw_cap[e] is the raw
+ * word capacity of the net at execution position e.  Returns 0 or the
+ * convert error code. */
+int ora_zshape_stage(const ora_grid *g, const float *capacity, float *demand, int *timestamp, float *wcost,
+                     float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
+                     const long long *rec_off, const int *records, const int *raw_cap,
+                     const long long *route_off, const int *route_cap, int *routes, int *stamp,
+                     double *total_wirelength, long long *total_vias, int threads) {
+    const int n = g->X > g->Y ? g->X : g->Y;
+    const size_t plane = (size_t)(g->L + 1) * n * n;
+    int *map = (int *)calloc(6 * plane, sizeof(int));
+    int *prev = (int *)calloc(16 * plane, sizeof(int));
+    long long maxraw = 1;
+    int maxcount = 1;
+    for (int b = 0; b < nbatch; ++b) {
+        long long s = 0;
+        for (int e = batch_off[b]; e < batch_off[b + 1]; ++e) s += raw_cap[e];
+        if (s > maxraw) maxraw = s;
+        if (batch_off[b + 1] - batch_off[b] > maxcount) maxcount = batch_off[b + 1] - batch_off[b];
+    }
+    int *raw = (int *)calloc((size_t)maxraw, sizeof(int));
+    int *raw_off = (int *)malloc(sizeof(int) * (size_t)maxcount);
+    int **rof = (int **)malloc(sizeof(int *) * (size_t)maxcount);
+    int tracks = 0;
+    for (int l = 0; l < g->L; ++l) tracks += track_count(g, l);
+    unsigned char *dirty = (unsigned char *)malloc((size_t)tracks);
+    memset(dirty, 1, (size_t)tracks);
+    int status = 0;
+    for (int b = 0; b < nbatch; ++b) {
+        const int begin = batch_off[b], count = batch_off[b + 1] - begin;
+        const int *pts = records + rec_off[b];
+        const int s = ++*stamp;
+        refresh_dirty(g, capacity, demand, 0.5, 1.5, 1.0, dirty, wcost, vcost, presum, threads);
+        ora_zshape_reset(g, map, pts, count);
+        ora_zshape_solve(g, presum, vcost, map, prev, pts, count, raw, NULL);
+        for (int k = 0; k < count; ++k) raw_off[k] = pts[pts[k]];
+        const int err = ora_zshape_convert(g, raw, raw_off, net_ids + begin, count, route_off, route_cap, routes);
+        if (err) status = err;
+        for (int k = 0; k < count; ++k) rof[k] = routes + route_off[net_ids[begin + k]];
+        ora_commit_batch(g, demand, timestamp, count, rof, s, 1, total_wirelength, total_vias, dirty);
+    }
+    free(map);
+    free(prev);
+    free(raw);
+    free(raw_off);
+    free(rof);
+    free(dirty);
     return status;
 }

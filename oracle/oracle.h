@@ -107,6 +107,35 @@ int ora_lshape_stage(const ora_grid *g, const float *capacity, float *demand,
                      int *stamp, double *total_wirelength, long long *total_vias,
                      int threads);
 
+/* Wire costs rounded to integers in [0, 1e7] after the refresh, as the
+ * Z-shape (GGR) build does (src/cost/full_update.cuh:47-50); off by default. */
+void ora_set_integer_wire(int on);
+
+/* ---- Z-shape DP (zshape.c), src/routing/zshape.cuh:42-404 ---- */
+
+/* reset_nodes: planes 0-3 of every tree-node cell of the batch to infinity. */
+void ora_zshape_reset(const ora_grid *g, int *map, const int *points, int count);
+/* solve: points = [per-net record offsets (count), records]; map has
+ * 6 * (L+1) * n * n ints and prev 16 * (L+1) * n * n (n = max(X, Y)); raw
+ * receives each net's words at its record's offset; cost (may be NULL)
+ * each net's DP optimum. */
+void ora_zshape_solve(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
+                      const int *points, int count, int *raw, int *cost);
+/* convert_routes: raw words of net k (at raw_off[k]) into the route words
+ * of net_ids[k] (at route_off, capacity route_cap); returns the last error
+ * code (0 ok, 1 storage, 2 no route, 3 bad coordinate). */
+int ora_zshape_convert(const ora_grid *g, const int *raw, const int *raw_off, const int *net_ids, int count,
+                       const long long *route_off, const int *route_cap, int *routes);
+
+/* One Z-shape stage over a batch schedule (src/pipeline_ggr.hpp:114-127):
+ * refresh, reset, solve, convert and commit per batch; stamps are
+ * ++*stamp per batch.  Returns 0 or the last convert error code. */
+int ora_zshape_stage(const ora_grid *g, const float *capacity, float *demand, int *timestamp, float *wcost,
+                     float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
+                     const long long *rec_off, const int *records, const int *raw_cap,
+                     const long long *route_off, const int *route_cap, int *routes, int *stamp,
+                     double *total_wirelength, long long *total_vias, int threads);
+
 /* ---- soft (log-sum-exp) L-shape DP, FP64 (soft.c) ---- */
 
 /* Smooth cost model: wcost, vcost per routing cell (IDX order) and their

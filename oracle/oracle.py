@@ -70,15 +70,19 @@ class Grid:
                          _p(self.xlen, ctypes.c_int), _p(self.ylen, ctypes.c_int))
         self.cells = self.L * self.X * self.Y
 
-    def refresh(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0):
+    def refresh(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0, integer_wire=False):
         demand = np.ascontiguousarray(demand, np.float32)
         w = np.empty(self.cells, np.float32)
         v = np.empty(self.cells, np.float32)
         p = np.empty(self.cells, np.float64)
-        lib().ora_cost_refresh_slopes(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float),
-                                      _p(demand, ctypes.c_float), ctypes.c_double(slope), ctypes.c_double(zero_slope),
-                                      ctypes.c_double(via_weight), _p(w, ctypes.c_float), _p(v, ctypes.c_float),
-                                      _p(p, ctypes.c_double))
+        lib().ora_set_integer_wire(1 if integer_wire else 0)
+        try:
+            lib().ora_cost_refresh_slopes(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float),
+                                          _p(demand, ctypes.c_float), ctypes.c_double(slope),
+                                          ctypes.c_double(zero_slope), ctypes.c_double(via_weight),
+                                          _p(w, ctypes.c_float), _p(v, ctypes.c_float), _p(p, ctypes.c_double))
+        finally:
+            lib().ora_set_integer_wire(0)
         return w, v, p
 
     def lshape_net(self, vcost, presum, nodes, parents, cap=10**6, with_cost=False):
@@ -135,6 +139,52 @@ class Grid:
                         ctypes.c_int(threshold), ctypes.c_int(n), ptrs, _p(flags, ctypes.c_ubyte))
         return flags[:n]
 
+
+    def zshape_stage(self, plan, pin_acc_num, words_per_pin=256, threads=1, first_stamp=0):
+        """Z-shape stage over a plan; returns (routes, demand, timestamps, wl, vias, status)."""
+        pin_acc_num = _i32(pin_acc_num)
+        route_off = (words_per_pin * pin_acc_num[:-1].astype(np.int64)).astype(np.int64)
+        route_cap = (words_per_pin * np.diff(pin_acc_num)).astype(np.int32)
+        routes = np.zeros(words_per_pin * int(pin_acc_num[-1]) + 1, np.int32)
+        demand = np.zeros(self.cells, np.float32)
+        ts = np.zeros(self.cells, np.int32)
+        w = np.empty(self.cells, np.float32)
+        v = np.empty(self.cells, np.float32)
+        p = np.empty(self.cells, np.float64)
+        stamp = ctypes.c_int(first_stamp)
+        wl = ctypes.c_double(0)
+        vias = ctypes.c_longlong(0)
+        bo = _i32(plan["batch_off"])
+        ro = np.ascontiguousarray(plan["rec_off"], np.int64)
+        lib().ora_set_integer_wire(1)
+        try:
+            rc = lib().ora_zshape_stage(
+                ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                _p(ts, ctypes.c_int), _p(w, ctypes.c_float), _p(v, ctypes.c_float), _p(p, ctypes.c_double),
+                ctypes.c_int(len(bo) - 1), _p(bo, ctypes.c_int), _p(_i32(plan["net_ids"]), ctypes.c_int),
+                _p(ro, ctypes.c_longlong), _p(_i32(plan["records"]), ctypes.c_int), _p(_i32(plan["raw_cap"]), ctypes.c_int),
+                _p(route_off, ctypes.c_longlong), _p(route_cap, ctypes.c_int), _p(routes, ctypes.c_int),
+                ctypes.byref(stamp), ctypes.byref(wl), ctypes.byref(vias), ctypes.c_int(threads))
+        finally:
+            lib().ora_set_integer_wire(0)
+        return routes, demand, ts, wl.value, vias.value, rc
+
+    def zshape_batch(self, presum, vcost, records, count, raw_words):
+        """One Z-shape batch on fresh maps: (raw words, DP optimum per net)."""
+        n = max(self.X, self.Y)
+        plane = (self.L + 1) * n * n
+        zmap = np.zeros(6 * plane, np.int32)
+        zprev = np.zeros(16 * plane, np.int32)
+        raw = np.zeros(max(raw_words, 1), np.int32)
+        cost = np.zeros(max(count, 1), np.int32)
+        records = _i32(records)
+        presum = np.ascontiguousarray(presum, np.float64)
+        vcost = np.ascontiguousarray(vcost, np.float32)
+        lib().ora_zshape_reset(ctypes.byref(self.c), _p(zmap, ctypes.c_int), _p(records, ctypes.c_int), ctypes.c_int(count))
+        lib().ora_zshape_solve(ctypes.byref(self.c), _p(presum, ctypes.c_double), _p(vcost, ctypes.c_float),
+                               _p(zmap, ctypes.c_int), _p(zprev, ctypes.c_int), _p(records, ctypes.c_int),
+                               ctypes.c_int(count), _p(raw, ctypes.c_int), _p(cost, ctypes.c_int))
+        return raw, cost[:count]
 
     def soft_costs(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0):
         """Smooth cost model: (wcost, vcost, dw_ds, dw_dz, dv_ds, dv_dz), FP64, IDX order."""

@@ -1,0 +1,104 @@
+"""GPU soft (log-sum-exp) L-shape DP against the FP64 oracle (oracle/soft.c),
+on the 64x64x5, 2k-net instance: values, per-cell gradients, parameter
+gradients against FP64 central finite differences (h = 1e-4), and the
+torch.autograd.Function."""
+import numpy as np
+import pytest
+
+from tests._instances import random_demand, topology
+
+pytestmark = pytest.mark.gpu
+
+TAUS = (0.1, 1.0, 10.0)
+THETA = (0.5, 1.5, 1.0)
+
+
+def _setup(ga, seed=0):
+    _, t = topology(6, seed)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(t["order"], t["node_off"], t["nodes"], t["parents"])
+    demand = random_demand(t, seed + 11, high=4)
+    ctx.upload_demand(demand)
+    return t, ctx, demand
+
+
+def _fd(grid, t, demand, tau, j, h=1e-4):
+    tp, tm = list(THETA), list(THETA)
+    tp[j] += h
+    tm[j] -= h
+    args = (demand, t["node_off"], t["nodes"], t["parents"], tau)
+    return (grid.soft_stage(*args, *tp) - grid.soft_stage(*args, *tm)) / (2 * h)
+
+
+@pytest.mark.parametrize("tau", TAUS)
+def test_soft_fp64_matches_oracle(ga, ora, tau):
+    t, ctx, demand = _setup(ga)
+    n = t["nets"]
+    print(f"soft fp64 tau={tau} nets={n}", flush=True)
+    grid = ora.Grid(t)
+    F_o, gw_o, gv_o, gp_o = grid.soft_stage(demand, t["node_off"], t["nodes"], t["parents"], tau, *THETA, grads=True)
+    total, per_net, _ = ctx.soft_forward(0, n, *THETA, tau=tau, precision=64, per_net=True)
+    assert abs(total - F_o) <= 1e-10 * abs(F_o)
+    gp, _ = ctx.soft_backward()
+    gw, gv = ctx.soft_edge_grads()
+    assert np.allclose(gw, gw_o, rtol=1e-8, atol=1e-10)
+    assert np.allclose(gv, gv_o, rtol=1e-8, atol=1e-10)
+    assert np.allclose(gp, gp_o, rtol=1e-8, atol=1e-10)
+    for j in range(3):
+        fd = _fd(grid, t, demand, tau, j)
+        assert abs(gp[j] - fd) <= 1e-4 * abs(fd), (j, gp[j], fd)
+    ctx.close()
+
+
+@pytest.mark.parametrize("tau", TAUS)
+def test_soft_fp32_backward_matches_finite_differences(ga, ora, tau):
+    """FP32 tables: relative error of d total / d params against FP64 central
+    differences of the oracle, bound 1e-4."""
+    t, ctx, demand = _setup(ga, 1)
+    n = t["nets"]
+    print(f"soft fp32 tau={tau} nets={n}", flush=True)
+    grid = ora.Grid(t)
+    F_o = grid.soft_stage(demand, t["node_off"], t["nodes"], t["parents"], tau, *THETA)
+    total, _, _ = ctx.soft_forward(0, n, *THETA, tau=tau, precision=32)
+    assert abs(total - F_o) <= 1e-4 * abs(F_o)
+    gp, _ = ctx.soft_backward()
+    errs = []
+    for j in range(3):
+        fd = _fd(grid, t, demand, tau, j)
+        errs.append(abs(gp[j] - fd) / abs(fd))
+    print("fp32 relative errors", errs, flush=True)
+    assert max(errs) <= 1e-4, errs
+    ctx.close()
+
+
+def test_soft_autograd_function(ga, ora):
+    torch = pytest.importorskip("torch")
+    from grotor_amd.soft import soft_lshape_cost
+    t, ctx, demand = _setup(ga, 2)
+    n = t["nets"]
+    params = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
+    total = soft_lshape_cost(ctx, params, 0, n, tau=1.0, precision=64)
+    total.backward()
+    grid = ora.Grid(t)
+    _, _, _, gp_o = grid.soft_stage(demand, t["node_off"], t["nodes"], t["parents"], 1.0, *THETA, grads=True)
+    assert np.allclose(params.grad.numpy(), gp_o, rtol=1e-8)
+    # a later forward invalidates an earlier graph's tables
+    p2 = torch.tensor(THETA, dtype=torch.float64, requires_grad=True)
+    first = soft_lshape_cost(ctx, p2, 0, n, tau=1.0, precision=32)
+    soft_lshape_cost(ctx, p2.detach(), 0, n, tau=1.0, precision=32)
+    with pytest.raises(RuntimeError):
+        first.backward()
+    ctx.close()
+
+
+def test_soft_batches_sum_to_stage(ga, ora):
+    """Forward over two halves equals the whole (per-net values are independent)."""
+    t, ctx, demand = _setup(ga, 3)
+    n = t["nets"]
+    whole, per, _ = ctx.soft_forward(0, n, *THETA, tau=1.0, precision=64, per_net=True)
+    a, pa, _ = ctx.soft_forward(0, n // 2, *THETA, tau=1.0, precision=64, per_net=True)
+    b, pb, _ = ctx.soft_forward(n // 2, n - n // 2, *THETA, tau=1.0, precision=64, per_net=True)
+    assert np.array_equal(np.concatenate([pa, pb]), per)
+    assert abs(a + b - whole) <= 1e-12 * abs(whole)
+    ctx.close()
