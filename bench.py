@@ -126,16 +126,29 @@ def main():
         dag_ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
         return batch_ms, dag_ms, nb, color, sched, upload_ms
 
+    # The last warmup step runs in mode 0 with HIP events on every kernel:
+    # it gives the per-class breakdown and names the dominant class.  The
+    # timed steps then carry events on that class only (mode 0), so the
+    # events add little host work to the timed region; a captured graph
+    # (mode 1) cannot carry them, so its roofline comes from one extra
+    # mode-0 step after the timed ones.
+    breakdown = None
     for i in range(args.warmup):
-        bm, dm, nb, _, _, um = step()
+        last = i == args.warmup - 1
+        if last:
+            ctx.profile_select(None)
+            ctx.profile_begin()
+        bm, dm, nb, _, _, um = step(mode=0 if last else args.mode)
+        if last:
+            breakdown = ctx.profile_end()
         print(f"[bench] warmup {i}: batching {bm:.2f} ms (+ geometry upload {um:.2f} ms), DAG {dm:.2f} ms, "
               f"{nb} batches", file=sys.stderr, flush=True)
-
-    # per-kernel HIP events in the timed steps (mode 0); a captured graph
-    # (mode 1) cannot carry them, so its roofline comes from one extra
-    # mode-0 step after the timed ones
+    dom_kind = None
+    if breakdown is not None:
+        dom_kind = max(range(len(breakdown["ms"])), key=lambda k: breakdown["ms"][k])
     inline_profile = args.mode == 0 and not args.no_profile
     if inline_profile:
+        ctx.profile_select(None if dom_kind is None else [dom_kind])
         ctx.profile_begin()
     total_ms = 0.0
     batch_total = dag_total = upload_total = 0.0
@@ -152,9 +165,12 @@ def main():
     if inline_profile:
         prof = ctx.profile_end()
     else:
+        ctx.profile_select(None if dom_kind is None else [dom_kind])
         ctx.profile_begin()
         step(mode=0)
         prof = ctx.profile_end()
+    if breakdown is None:
+        breakdown = prof
 
     nets = topo["nets"]
     value = nets * args.steps / (total_ms / 1e3)
@@ -166,13 +182,15 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
                 "traffic": measured_traffic(KIND_NAMES[dom], args.config),
-                "events": "timed steps" if inline_profile else "one extra mode-0 step after the timed steps",
+                "events": "timed steps (this class only)" if inline_profile
+                          else "one extra mode-0 step after the timed steps",
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
                 "launches": int(dom_launches),
                 "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}
-    kernels = {KIND_NAMES[k]: {"ms": round(prof["ms"][k], 4), "launches": int(prof["launches"][k]),
-                               "GBs": round(prof["bytes"][k] / (prof["ms"][k] / 1e3) / 1e9, 2) if prof["ms"][k] else 0}
-               for k in range(len(prof["ms"])) if prof["launches"][k]}
+    kernels = {KIND_NAMES[k]: {"ms": round(breakdown["ms"][k], 4), "launches": int(breakdown["launches"][k]),
+                               "GBs": round(breakdown["bytes"][k] / (breakdown["ms"][k] / 1e3) / 1e9, 2)
+                               if breakdown["ms"][k] else 0}
+               for k in range(len(breakdown["ms"])) if breakdown["launches"][k]}
 
     cpu = None
     if not args.no_cpu:
@@ -216,7 +234,7 @@ def main():
                    "geometry_upload_ms_per_step": round(upload_total / args.steps, 4),
                    "dag_ms_per_step": round(dag_total / args.steps, 4),
                    "total_wirelength": wl, "total_vias": vias, "parallelism": "single GPU",
-                   "kernels": kernels},
+                   "kernel_breakdown": {"steps": 1 if args.warmup else args.steps, "classes": kernels}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
