@@ -86,7 +86,7 @@ def test_zshape_trace_costs_the_optimum(ga, ora, config, seed):
     """The traced route lies in the DP's route space, so it never costs less
     than the DP optimum, and it costs exactly the optimum unless the optimum
     climbs more than one layer at the bend next to the parent node: the
-    traceback (src/routing/zshape.cuh:337-343) takes the middle leg's layer
+    traceback (src/routing/zshape.cuh:311-315) takes the middle leg's layer
     from the one via step recorded for that bend, so such routes come back
     with the middle leg on the adjacent layer."""
     d, t, z, color, nb, plan = _plan(ga, ora, config, seed)
