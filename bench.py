@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
+    ap.add_argument("--stage", choices=("lshape", "zshape", "soft"), default="lshape",
+                    help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
     return ap.parse_args()
@@ -109,6 +111,10 @@ def main():
     print(f"[bench] instance config={args.config} subnets={topo['nets']} nodes={int(topo['node_off'][-1])} "
           f"prepare={prep_s:.2f}s device={ga.device_name()}", file=sys.stderr, flush=True)
 
+    if args.stage == "zshape":
+        return bench_zshape(args, ga, design, topo)
+    if args.stage == "soft":
+        return bench_soft(args, ga, design, topo)
     ctx = ga.Context(topo)
     ctx.set_routes(topo["pin_acc_num"])
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
@@ -239,6 +245,107 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_zshape(args, ga, design, topo):
+    """Z-shape stage (src/pipeline_ggr.hpp): GPU RSMT batching plus the
+    host bounding-box refinement and tree encoding as the batching call,
+    then the synchronized Z-shape batch loop."""
+    import numpy as np
+    z = design.zshape_stage()
+    geom = (z["pt_off"], z["pts"], z["hs_off"], z["hsegs"], z["vs_off"], z["vsegs"])
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"], 256)
+    ctx.set_cost_params(integer_wire=True)
+    nets = z["nets"]
+
+    def step():
+        ctx.reset()
+        ctx.batch_geometry(*geom)
+        b0 = time.perf_counter()
+        color, nb = ctx.batch_colors()
+        plan = design.zshape_plan(color, nb)
+        batch_ms = (time.perf_counter() - b0) * 1e3
+        ctx.zshape_upload(plan)
+        dag_ms = ctx.zshape_run(first_stamp=1)
+        return batch_ms, dag_ms, plan, color, nb
+
+    for i in range(args.warmup):
+        bm, dm, plan, _, _ = step()
+        print(f"[bench] zshape warmup {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms, "
+              f"{len(plan['batch_off']) - 1} batches", file=sys.stderr, flush=True)
+    bt = dt = 0.0
+    for i in range(args.steps):
+        bm, dm, plan, color, nb = step()
+        bt += bm
+        dt += dm
+        print(f"[bench] zshape step {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms", file=sys.stderr, flush=True)
+    routes = ctx.routes()
+    wl, vias = ctx.stats()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        c0 = time.perf_counter()
+        ocolor, onb = ora.batches(topo["X"], topo["Y"], *geom)
+        oplan = design.zshape_plan(ocolor, onb)
+        cb = time.perf_counter() - c0
+        c1 = time.perf_counter()
+        oroutes, _, _, owl, ovias, _ = ora.Grid(topo).zshape_stage(oplan, topo["pin_acc_num"], threads=args.threads)
+        cd = time.perf_counter() - c1
+        same = bool(np.array_equal(ocolor, color) and np.array_equal(oroutes, routes) and owl == wl and ovias == vias)
+        cpu = {"value": round(nets / (cb + cd), 1), "unit": "nets/s", "cores": 1, "kind": "port",
+               "sample": f"whole workload: {nets} subnets; CPU oracle batching + bbox refinement "
+                         f"({cb * 1e3:.1f} ms) + sequential Z-shape loop ({cd * 1e3:.1f} ms)",
+               "routes_identical_to_gpu": same}
+    ms = (bt + dt) / args.steps
+    print(json.dumps({
+        "metric": "routed nets/s (batching+DAG), Z-shape stage", "value": round(nets / (ms / 1e3), 1),
+        "unit": "nets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} Z-shape stage ({CONFIG_TEXT.get(args.config, '')}, "
+                               f"{nets} routed subnets)", "batches": len(plan["batch_off"]) - 1,
+                   "batching_ms_per_step": round(bt / args.steps, 4), "dag_ms_per_step": round(dt / args.steps, 4),
+                   "total_wirelength": wl, "total_vias": vias},
+        "cpu_baseline": cpu}), flush=True)
+    ctx.close()
+
+
+def bench_soft(args, ga, design, topo):
+    """Soft-DP forward + adjoint over every routed net at the demand the
+    L-shape stage leaves, for tau in {0.1, 1, 10}, FP32 tables."""
+    from oracle import oracle as ora  # noqa: F401  (the CPU leg below)
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"])
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+    color, nb = ctx.batch_plan(*geom)
+    sched = ga.execution_schedule(topo, color, nb)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1)
+    ctx.lshape_upload(topo["order"], topo["node_off"], topo["nodes"], topo["parents"])
+    nets = topo["nets"]
+    per_tau = {}
+    for tau in (0.1, 1.0, 10.0):
+        for _ in range(args.warmup):
+            ctx.soft_forward(0, nets, tau=tau, precision=32)
+            ctx.soft_backward()
+        f = b = 0.0
+        for _ in range(args.steps):
+            total, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
+            gp, bm = ctx.soft_backward()
+            f += fm
+            b += bm
+        per_tau[str(tau)] = {"fwd_ms": round(f / args.steps, 3), "bwd_ms": round(b / args.steps, 3),
+                             "total": total, "grad_params": [float(x) for x in gp]}
+        print(f"[bench] soft tau={tau}: fwd {f / args.steps:.2f} ms, bwd {b / args.steps:.2f} ms",
+              file=sys.stderr, flush=True)
+    ms = sum(v["fwd_ms"] + v["bwd_ms"] for v in per_tau.values()) / len(per_tau)
+    print(json.dumps({
+        "metric": "soft-DP fwd+bwd ms per stage", "value": round(ms, 3), "unit": "ms", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} soft L-shape DP over {nets} routed subnets at the "
+                               f"L-shape stage's final demand", "per_tau": per_tau}}), flush=True)
     ctx.close()
 
 
