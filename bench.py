@@ -32,7 +32,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KIND_NAMES = ["refresh_tracks_kernel", "lshape_dp_kernel", "commit_wires_kernel", "commit_vias_kernel",
-              "batch_marks_kernels", "color_kernel", "ripup_kernels"]
+              "batch_marks_kernels", "color_kernel", "ripup_kernels", "zshape_kernels", "flt_edge_kernel",
+              "flt_route_kernel"]
 CONFIG_TEXT = {
     1: "128x128x5, 20k nets",
     2: "512x512x9, 300k nets",

@@ -12,6 +12,9 @@ Reference entry points mirrored here (reference tree file:line):
   Context.commit_batch     graph::commit_batch_demand  src/routing/demand.cuh:313
   Context.lshape_run       L-shape batch loop          src/routing/lshape.cuh:148-180
   Context.ripup            graph::ripup                src/routing/congestion.cuh:114
+  Context.congestion_view  extract_congestionView*     src/routing/congestion.cuh:60-112
+  Context.flt_run          FLT batch loop              src/routing/flt.cuh:141-168
+  Design.flt_stage         prepare_detour_topology     src/routing/flt.cuh:48-73
   Design.prepare           prepare_routing_nets etc.   src/preprocess.hpp:1448
 """
 import ctypes
@@ -61,6 +64,18 @@ class TopologyDesc(ctypes.Structure):
 class ZshapeStageDesc(ctypes.Structure):
     _fields_ = [("nets", ctypes.c_int), ("order", _i32p), ("pt_off", _i32p), ("pts", _i32p), ("hs_off", _i32p),
                 ("hsegs", _i32p), ("vs_off", _i32p), ("vsegs", _i32p)]
+
+
+class FltStageDesc(ctypes.Structure):
+    _fields_ = [("nets", ctypes.c_int), ("order", _i32p), ("pt_off", _i32p), ("pts", _i32p), ("hs_off", _i32p),
+                ("hsegs", _i32p), ("vs_off", _i32p), ("vsegs", _i32p), ("pin_off2", _i32p),
+                ("pins2", ctypes.c_longlong), ("nodes", ctypes.c_longlong), ("edges", ctypes.c_longlong)]
+
+
+class FltPlanDesc(ctypes.Structure):
+    _fields_ = [("batches", ctypes.c_int), ("batch_off", _i32p), ("net_ids", _i32p),
+                ("node_off", ctypes.POINTER(ctypes.c_longlong)), ("nodes", _i32p),
+                ("edge_off", ctypes.POINTER(ctypes.c_longlong)), ("edges", _i32p)]
 
 
 class ZshapePlanDesc(ctypes.Structure):
@@ -193,6 +208,41 @@ class Design:
               ctypes.byref(promoted))
         return promoted.value
 
+    def flt_stage(self, reroute, xsum=None, ysum=None, threads=8):
+        """FLT stage input for the subnets flagged in reroute: priority order,
+        batching geometry, phase-two pin offsets (numpy copies)."""
+        reroute = np.ascontiguousarray(reroute, np.uint8)
+        xs = None if xsum is None else np.ascontiguousarray(xsum, np.float32)
+        ys = None if ysum is None else np.ascontiguousarray(ysum, np.float32)
+        t = FltStageDesc()
+        _call("gr_design_flt_stage", self._h, _ptr(reroute, ctypes.c_ubyte), _ptr(xs, ctypes.c_float),
+              _ptr(ys, ctypes.c_float), ctypes.c_int(threads), ctypes.byref(t))
+        n = t.nets
+
+        def arr(p, count):
+            return np.zeros(0, np.int32) if count == 0 else np.ctypeslib.as_array(p, shape=(count,)).copy()
+
+        pt_off, hs_off, vs_off = arr(t.pt_off, n + 1), arr(t.hs_off, n + 1), arr(t.vs_off, n + 1)
+        return dict(nets=n, order=arr(t.order, n), pt_off=pt_off, pts=arr(t.pts, int(pt_off[-1])), hs_off=hs_off,
+                    hsegs=arr(t.hsegs, 2 * int(hs_off[-1])), vs_off=vs_off, vsegs=arr(t.vsegs, 2 * int(vs_off[-1])),
+                    pin_off2=arr(t.pin_off2, len(reroute)), pins2=int(t.pins2), nodes=int(t.nodes),
+                    edges=int(t.edges))
+
+    def flt_plan(self, color, nb):
+        """FLT execution plan (batches reversed) with the DAG records of every net."""
+        color = _i32(color)
+        p = FltPlanDesc()
+        _call("gr_design_flt_plan", self._h, _ptr(color, ctypes.c_int), ctypes.c_int(nb), ctypes.byref(p))
+        nb2 = p.batches
+        batch_off = np.ctypeslib.as_array(p.batch_off, shape=(nb2 + 1,)).copy()
+        nets = int(batch_off[-1])
+        node_off = np.ctypeslib.as_array(p.node_off, shape=(nets + 1,)).copy()
+        edge_off = np.ctypeslib.as_array(p.edge_off, shape=(nets + 1,)).copy()
+        take = lambda q, c: np.zeros(0, np.int32) if c == 0 else np.ctypeslib.as_array(q, shape=(c,)).copy()
+        return dict(batch_off=batch_off, net_ids=take(p.net_ids, nets), node_off=node_off,
+                    nodes=take(p.nodes, 4 * int(node_off[-1])), edge_off=edge_off,
+                    edges=take(p.edges, 2 * int(edge_off[-1])))
+
     def write_output(self, routes, path):
         routes = _i32(routes)
         _call("gr_design_write_output", self._h, _ptr(routes, ctypes.c_int), path.encode())
@@ -243,6 +293,44 @@ class Context:
     def set_cost_params(self, slope=0.5, zero_slope=1.5, via_weight=1.0, integer_wire=False):
         _call("gr_ctx_set_cost_params", self._h, ctypes.c_double(slope), ctypes.c_double(zero_slope),
               ctypes.c_double(via_weight), ctypes.c_int(1 if integer_wire else 0))
+
+    def set_routes_phase2(self, pin_off2, pins2, words_per_pin=100):
+        """Phase-two route storage (ROUTE_PER_PIN_PHASE2 = 100 words per pin)."""
+        pin_off2 = _i32(pin_off2)
+        _call("gr_ctx_set_routes_phase2", self._h, _ptr(pin_off2, ctypes.c_int), ctypes.c_longlong(pins2),
+              ctypes.c_int(words_per_pin))
+        self.route2_words = words_per_pin * int(pins2) + 1
+        self.words_per_pin2 = words_per_pin
+
+    def routes2(self):
+        r = np.empty(self.route2_words, np.int32)
+        _call("gr_routes2_download", self._h, _ptr(r, ctypes.c_int), ctypes.c_longlong(self.route2_words))
+        return r
+
+    def congestion_view(self):
+        """(congested bool[X*Y], xsum float[X*Y], ysum float[X*Y]) in x*Y+y order."""
+        n = self.X * self.Y
+        cong, xs, ys = np.empty(n, np.uint8), np.empty(n, np.float32), np.empty(n, np.float32)
+        _call("gr_congestion_view", self._h, _ptr(cong, ctypes.c_ubyte), _ptr(xs, ctypes.c_float),
+              _ptr(ys, ctypes.c_float))
+        return cong.astype(bool), xs, ys
+
+    def flt_upload(self, plan):
+        """Upload a Design.flt_plan() plan."""
+        self._fplan = plan
+        _call("gr_flt_upload", self._h, ctypes.c_int(len(plan["batch_off"]) - 1),
+              _ptr(_i32(plan["batch_off"]), ctypes.c_int), _ptr(_i32(plan["net_ids"]), ctypes.c_int),
+              _ptr(np.ascontiguousarray(plan["node_off"], np.int64), ctypes.c_longlong),
+              _ptr(_i32(plan["nodes"]), ctypes.c_int),
+              _ptr(np.ascontiguousarray(plan["edge_off"], np.int64), ctypes.c_longlong),
+              _ptr(_i32(plan["edges"]), ctypes.c_int))
+
+    def flt_run(self, first_stamp=1, remove_phase_one=True):
+        """Run the uploaded FLT batches; returns the DAG interval in ms."""
+        ms = ctypes.c_double()
+        _call("gr_flt_run", self._h, ctypes.c_int(first_stamp), ctypes.c_int(1 if remove_phase_one else 0),
+              ctypes.byref(ms))
+        return ms.value
 
     def zshape_upload(self, plan):
         """Upload a Design.zshape_plan() plan."""
@@ -341,7 +429,7 @@ class Context:
     def reset(self):
         _call("gr_ctx_reset", self._h)
 
-    KINDS = 8
+    KINDS = 10
 
     def profile_begin(self):
         _call("gr_profile_begin", self._h)

@@ -146,6 +146,20 @@ struct gr_ctx {
     std::vector<int> z_batch_off;
     std::vector<long long> z_rec_off;
     int *z_net_ids = nullptr, *z_records = nullptr, *z_raw_cap = nullptr, *z_raw = nullptr;
+    // FLT stage: phase-two route words, resident DAG records, batch tables
+    int route2_per_pin = 100;
+    int* pin_off2 = nullptr;
+    int* routes2 = nullptr;
+    long long route2_words = 0;
+    std::vector<int> pin_off2_h;
+    Workspace f_ws, f_tables;
+    FltArgs f{};
+    int f_batches = 0;
+    std::vector<int> f_batch_off;
+    std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
+    // congestion view (2D)
+    unsigned char* cview = nullptr;
+    float *cxsum = nullptr, *cysum = nullptr;
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 16;
     // kernel profiling (HIP events on the context stream)
@@ -299,10 +313,12 @@ CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
     return a;
 }
 
-void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int* net_ids = nullptr) {
+void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int* net_ids = nullptr,
+                   const RouteStore* rs = nullptr) {
     if (!count) return;
     CommitArgs a = commit_args(c, begin, count, stamp, K);
     if (net_ids) a.net_ids = net_ids;
+    if (rs) a.rs = *rs;
     const int blocks = std::min((count + 3) / 4, 4096);
     timed(c, GR_K_COMMIT_WIRES, 0.0,
           [&] { hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
@@ -990,6 +1006,239 @@ int gr_zshape_run(gr_ctx* c, int first_stamp, double* elapsed_ms) {
         const auto t1 = std::chrono::steady_clock::now();
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_zshape_run");
+    });
+}
+
+int gr_ctx_set_routes_phase2(gr_ctx* c, const int* pin_off2, long long pins2, int words_per_pin) {
+    return guarded([&] {
+        if (!c || !pin_off2 || pins2 < 0) throw GrError(GR_E_ARG, "invalid phase-two route arguments");
+        if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
+        if (words_per_pin < 8) throw GrError(GR_E_ARG, "phase-two route words per pin must be at least 8");
+        for (int i = 0; i < c->net_count; ++i) {
+            const int off = pin_off2[i];
+            const int pins = c->pin_acc_num_h[i + 1] - c->pin_acc_num_h[i];
+            if (off < -1 || (off >= 0 && (long long)off + pins > pins2))
+                throw GrError(GR_E_ARG, "phase-two pin offset outside the storage");
+        }
+        c->mem.release(c->routes2);
+        c->mem.release(c->pin_off2);
+        c->route2_per_pin = words_per_pin;
+        c->pin_off2_h.assign(pin_off2, pin_off2 + c->net_count);
+        c->route2_words = (long long)words_per_pin * pins2 + 1;
+        c->pin_off2 = c->mem.alloc<int>(std::max(c->net_count, 1));
+        if (c->net_count) upload(c->pin_off2, pin_off2, sizeof(int) * c->net_count, c->stream);
+        c->routes2 = c->mem.alloc<int>(size_t(c->route2_words));
+        hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
+        hip_check(hipStreamSynchronize(c->stream), "phase-two route storage");
+    });
+}
+
+int gr_routes2_download(gr_ctx* c, int* routes, long long words) {
+    return guarded([&] {
+        if (!c || !routes || words < 0) throw GrError(GR_E_ARG, "invalid arguments");
+        if (words > c->route2_words) throw GrError(GR_E_ARG, "more words than the phase-two storage holds");
+        download(routes, c->routes2, sizeof(int) * size_t(words), c->stream);
+    });
+}
+
+int gr_congestion_view(gr_ctx* c, unsigned char* congested, float* xsum, float* ysum) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        const DeviceGrid& g = c->g;
+        if (!c->cview) {
+            c->cview = c->mem.alloc<unsigned char>(g.XY);
+            c->cxsum = c->mem.alloc<float>(g.XY);
+            c->cysum = c->mem.alloc<float>(g.XY);
+        }
+        hipLaunchKernelGGL(congestion_view_kernel, dim3((g.XY + 255) / 256), dim3(256), 0, c->stream, g, c->capacity,
+                           c->demand, c->cview, c->cxsum, c->cysum);
+        hipLaunchKernelGGL(congestion_prefix_kernel, dim3(g.Y), dim3(256), sizeof(float) * g.X, c->stream, g.Y, g.X, 1,
+                           g.Y, c->cxsum);
+        hipLaunchKernelGGL(congestion_prefix_kernel, dim3(g.X), dim3(256), sizeof(float) * g.Y, c->stream, g.X, g.Y, g.Y,
+                           1, c->cysum);
+        hip_check(hipGetLastError(), "congestion view kernels");
+        if (congested) download(congested, c->cview, g.XY, c->stream);
+        if (xsum) download(xsum, c->cxsum, sizeof(float) * g.XY, c->stream);
+        if (ysum) download(ysum, c->cysum, sizeof(float) * g.XY, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "congestion view");
+    });
+}
+
+int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_ids, const long long* node_off,
+                  const int* nodes, const long long* edge_off, const int* edges) {
+    return guarded([&] {
+        if (!c || nbatch < 0 || !batch_off || !node_off || !edge_off) throw GrError(GR_E_ARG, "null argument");
+        if (!c->routes2) throw GrError(GR_E_STATE, "no phase-two route storage");
+        const DeviceGrid& g = c->g;
+        if (g.L > 9) throw GrError(GR_E_ARG, "FLT supports at most 9 routing layers");
+        if (batch_off[0] != 0 || node_off[0] != 0 || edge_off[0] != 0) throw GrError(GR_E_ARG, "offsets must start at 0");
+        for (int b = 0; b < nbatch; ++b)
+            if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets out of order");
+        const int nets = batch_off[nbatch];
+        if (nets && !net_ids) throw GrError(GR_E_ARG, "null net ids");
+        const long long total_nodes = node_off[nets], total_edges = edge_off[nets];
+        if ((total_nodes && !nodes) || (total_edges && !edges)) throw GrError(GR_E_ARG, "null records");
+        if (total_nodes > INT32_MAX / 16 || total_edges > INT32_MAX / 16) throw GrError(GR_E_ARG, "FLT stage too large");
+        std::vector<int4> nrec(static_cast<size_t>(total_nodes)), erec(static_cast<size_t>(total_edges)), desc(nets), sizes(nets);
+        std::vector<int> order(static_cast<size_t>(total_nodes)), doff;
+        std::vector<int> depth, first, count;
+        long long slots_total = 0;
+        const int max_coord = (g.L + 1) * g.XY;
+        for (int k = 0; k < nets; ++k) {
+            const int id = net_ids[k];
+            if (id < 0 || id >= c->net_count) throw GrError(GR_E_ARG, "net id outside route storage");
+            if (c->pin_off2_h[id] < 0) throw GrError(GR_E_ARG, "net without phase-two route storage");
+            const long long n0 = node_off[k], e0 = edge_off[k];
+            const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
+            if (nn < 1 || ne < 0) throw GrError(GR_E_ARG, "FLT net without nodes");
+            depth.assign(size_t(nn), 0);
+            int slot = 0, roots = 0, prev_end = 0, maxdepth = 0;
+            for (long long i = 0; i < nn; ++i) {
+                const int* r = nodes + 4 * (n0 + i);
+                if (r[0] < 0 || r[0] >= max_coord) throw GrError(GR_E_ARG, "FLT node outside the grid");
+                if (r[1] < 0 || r[1] >= 5000) throw GrError(GR_E_ARG, "FLT depth outside [0, 5000)");
+                if (r[2] < 0 || r[2] > 6) throw GrError(GR_E_ARG, "FLT node with more than 6 child slots");
+                if (r[3] < prev_end || r[3] > ne) throw GrError(GR_E_ARG, "FLT parent_end out of order");
+                if (r[3] == prev_end) {
+                    ++roots;
+                    if (r[1] != 0) throw GrError(GR_E_ARG, "FLT node without parents below depth 0");
+                }
+                depth[i] = r[1];
+                maxdepth = std::max(maxdepth, r[1]);
+                nrec[n0 + i] = make_int4(r[0], prev_end, r[3], slot * 8 + r[2]);
+                slot += r[2];
+                prev_end = r[3];
+            }
+            if (prev_end != ne) throw GrError(GR_E_ARG, "FLT edge count differs from the last parent_end");
+            if (roots != 1) throw GrError(GR_E_ARG, "FLT DAG needs exactly one root");
+            for (long long i = 0; i < nn; ++i) {
+                const int4 nr = nrec[n0 + i];
+                for (int e = nr.y; e < nr.z; ++e) {
+                    const int* r = edges + 2 * (e0 + e);
+                    const int par = r[0], s = r[1] % 10;
+                    if (par < 0 || par >= nn || r[1] / 10 != i) throw GrError(GR_E_ARG, "FLT edge record inconsistent");
+                    if (depth[par] >= depth[i]) throw GrError(GR_E_ARG, "FLT parent not shallower than its child");
+                    if (s >= (nrec[n0 + par].w & 7)) throw GrError(GR_E_ARG, "FLT edge slot outside its parent");
+                    const int pc = nrec[n0 + par].x, cc = nr.x;
+                    if (pc / g.Y % g.X != cc / g.Y % g.X && pc % g.Y != cc % g.Y)
+                        throw GrError(GR_E_ARG, "FLT edge not straight");
+                    erec[e0 + e] = make_int4(par, s, pc, cc);
+                }
+            }
+            // nodes by depth, stable (the reference's radix sort on (net, depth))
+            const int depths = maxdepth + 1;
+            count.assign(depths + 1, 0);
+            for (long long i = 0; i < nn; ++i) ++count[depth[i] + 1];
+            for (int d = 0; d < depths; ++d) count[d + 1] += count[d];
+            const int dbase = int(doff.size());
+            doff.insert(doff.end(), count.begin(), count.end());
+            first.assign(count.begin(), count.end() - 1);
+            for (long long i = 0; i < nn; ++i) order[n0 + first[depth[i]]++] = int(i);
+            desc[k] = make_int4(int(n0), int(e0), int(slots_total), dbase);
+            sizes[k] = make_int4(int(nn), int(ne), slot, depths);
+            slots_total += slot;
+            if (slots_total > INT32_MAX / 16) throw GrError(GR_E_ARG, "FLT stage too large");
+        }
+        // per batch: stage-global bases and the table sizes
+        long long max_nodes = 1, max_edges = 1, max_slots = 1;
+        c->f_base.assign(nbatch, make_int4(0, 0, 0, 0));
+        for (int b = 0; b < nbatch; ++b) {
+            const int k0 = batch_off[b], k1 = batch_off[b + 1];
+            if (k0 == k1) continue;
+            const long long nodes_b = node_off[k1] - node_off[k0], edges_b = edge_off[k1] - edge_off[k0];
+            const long long slots_b = (long long)desc[k1 - 1].z + sizes[k1 - 1].z - desc[k0].z;
+            max_nodes = std::max(max_nodes, nodes_b);
+            max_edges = std::max(max_edges, edges_b);
+            max_slots = std::max(max_slots, slots_b);
+            c->f_base[b] = make_int4(desc[k0].x, desc[k0].y, desc[k0].z, int(edges_b));
+        }
+        int4 *d_nrec, *d_erec, *d_desc, *d_sizes;
+        int *d_order, *d_doff, *d_ids;
+        c->f_ws.carve([&](auto& cv) {
+            d_nrec = cv.template take<int4>(nrec.size());
+            d_erec = cv.template take<int4>(erec.size());
+            d_order = cv.template take<int>(order.size());
+            d_doff = cv.template take<int>(doff.size());
+            d_desc = cv.template take<int4>(nets);
+            d_sizes = cv.template take<int4>(nets);
+            d_ids = cv.template take<int>(nets);
+        });
+        FltArgs& f = c->f;
+        c->f_tables.carve([&](auto& cv) {
+            f.costs = cv.template take<double>(size_t(max_nodes) * kFltStride);
+            f.packed = cv.template take<unsigned long long>(size_t(max_slots) * kFltStride);
+            f.best_path = cv.template take<unsigned long long>(size_t(max_slots) * kFltStride);
+            f.layer_output = cv.template take<int>(size_t(max_nodes));
+            f.cost_edges = cv.template take<double>(size_t(max_edges) * 81);
+            f.best_change = cv.template take<int>(size_t(max_edges) * 81);
+        });
+        if (!nrec.empty()) upload(d_nrec, nrec.data(), sizeof(int4) * nrec.size(), c->stream);
+        if (!erec.empty()) upload(d_erec, erec.data(), sizeof(int4) * erec.size(), c->stream);
+        if (!order.empty()) upload(d_order, order.data(), sizeof(int) * order.size(), c->stream);
+        if (!doff.empty()) upload(d_doff, doff.data(), sizeof(int) * doff.size(), c->stream);
+        if (nets) {
+            upload(d_desc, desc.data(), sizeof(int4) * nets, c->stream);
+            upload(d_sizes, sizes.data(), sizeof(int4) * nets, c->stream);
+            upload(d_ids, net_ids, sizeof(int) * nets, c->stream);
+        }
+        f.g = g;
+        f.node_rec = d_nrec;
+        f.edge_rec = d_erec;
+        f.order = d_order;
+        f.depth_off = d_doff;
+        f.desc = d_desc;
+        f.sizes = d_sizes;
+        f.net_ids = d_ids;
+        f.routes = RouteStore{c->routes2, c->pin_off2, c->route2_per_pin};
+        f.pin_acc_num = c->pin_acc_num;
+        f.error = c->error;
+        c->f_batches = nbatch;
+        c->f_batch_off.assign(batch_off, batch_off + nbatch + 1);
+        hip_check(hipStreamSynchronize(c->stream), "FLT upload");
+    });
+}
+
+int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed_ms) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
+        const RouteStore rs2{c->routes2, c->pin_off2, c->route2_per_pin};
+        hip_check(hipDeviceSynchronize(), "before DAG interval");
+        const auto t0 = std::chrono::steady_clock::now();
+        int stamp = first_stamp;
+        for (int b = 0; b < c->f_batches; ++b) {
+            const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
+            const int* ids = c->f.net_ids + begin;
+            if (remove_phase_one) launch_commit(c, 0, count, stamp++, -1, ids);
+            const int s = stamp++;
+            launch_refresh(c, 0);
+            if (!count) continue;
+            FltArgs a = c->f;
+            a.presum = c->presum;
+            a.vcost = c->vcost;
+            a.desc = c->f.desc + begin;
+            a.sizes = c->f.sizes + begin;
+            a.net_ids = ids;
+            a.count = count;
+            a.node_base = c->f_base[b].x;
+            a.edge_base = c->f_base[b].y;
+            a.slot_base = c->f_base[b].z;
+            a.edges = c->f_base[b].w;
+            const long long threads = (long long)a.edges * c->g.L;
+            timed(c, GR_K_FLT_EDGE, 0.0, [&] {
+                if (threads)
+                    hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
+            });
+            timed(c, GR_K_FLT, 0.0, [&] {
+                hipLaunchKernelGGL(flt_route_kernel, dim3((count + 3) / 4), dim3(256), 0, c->stream, a);
+            });
+            hip_check(hipGetLastError(), "FLT kernels");
+            launch_commit(c, 0, count, s, 1, ids, &rs2);
+        }
+        hip_check(hipStreamSynchronize(c->stream), "DAG interval");
+        const auto t1 = std::chrono::steady_clock::now();
+        if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        check_device_error(c, "gr_flt_run");
     });
 }
 

@@ -207,4 +207,40 @@ __global__ void fill_int_kernel(int* p, int value, long long n) {
         p[i] = value;
 }
 
+// extract_congestionView (src/routing/congestion.cuh:60-79): per 2D cell the
+// worst demand - capacity over the routing layers (>= 0).
+__global__ void congestion_view_kernel(DeviceGrid g, const float* capacity, const float* demand,
+                                       unsigned char* congested, float* xsum, float* ysum) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.XY) return;
+    const int x = idx / g.Y, y = idx % g.Y;
+    double worst = 0.0;
+    for (int l = 0; l < g.L; ++l) {
+        const int c = cell_of(g, l, x, y);
+        worst = fmax(worst, (double)(demand[c] - capacity[c]));
+    }
+    congested[idx] = worst > 0.0;
+    xsum[idx] = (float)worst;
+    ysum[idx] = (float)worst;
+}
+
+// extract_congestionView_xsum / _ysum (src/routing/congestion.cuh:81-112):
+// exclusive prefix of one row or column in the reference's in-place order
+// (at level d every index with bit d set adds the last element of the block
+// below it), so the float sums are bit-identical.
+__global__ void congestion_prefix_kernel(int tracks, int len, int track_stride, int pos_stride, float* sum) {
+    extern __shared__ float s[];
+    float* v = sum + (size_t)blockIdx.x * track_stride;
+    if (threadIdx.x == 0) s[0] = 0.f;
+    for (int p = threadIdx.x; p < len - 1; p += blockDim.x) s[p + 1] = v[(size_t)p * pos_stride];
+    __syncthreads();
+    for (int d = 0; (1 << d) < len; ++d) {
+        for (int i = threadIdx.x; i < len; i += blockDim.x)
+            if (i >> d & 1) s[i] += s[(i >> d << d) - 1];
+        __syncthreads();
+    }
+    for (int p = threadIdx.x; p < len; p += blockDim.x) v[(size_t)p * pos_stride] = s[p];
+    (void)tracks;
+}
+
 }  // namespace grotor

@@ -1,0 +1,356 @@
+// FLT routing-DAG DP on gfx950.
+//
+// Two launches per batch: flt_edge_kernel searches, for every DAG edge and
+// from-layer, the best layer-change position towards every same-parity
+// to-layer; flt_route_kernel gives each DAG one wavefront, which initializes
+// the DAG's tables, solves its depths bottom-up (children push packed
+// (cost, node, choice) keys into their parents' child slots with 64-bit
+// atomic min) and traces them top-down.  Between depths the wavefront waits
+// for its own stores and atomics and invalidates the CU's L1, so every lane
+// reads the previous depth's results from L2.
+//
+// Segments are appended in node order within a depth (a wavefront prefix
+// sum), so the route words are deterministic.
+//
+// Reference: precompute_cost_edge (src/routing/flt_cost.cuh:50-130),
+// solve_flt_node (src/routing/flt_block.cuh:6-142), trace_flt_node
+// (:144-331), route_flt_block (:334-355).
+#include "gr_kernels.hpp"
+
+namespace grotor {
+namespace {
+
+constexpr int kS = kFltStride;  // layer index 1..L of costs and slot tables, 0 unused
+constexpr int kStraight = 82;   // choice code of a same-layer edge
+// float 1.01e22 in the upper word, code 200000000 (src/routing/flt_cost.cuh:48)
+constexpr unsigned long long kEmpty = 7208259049189261824ULL;
+
+__device__ __forceinline__ unsigned long long pack(float cost, int code) {
+    const int i = __float_as_int(cost);
+    return ((unsigned long long)(unsigned)(i >= 0 ? i : i ^ 0x7FFFFFFF) << 32) | (unsigned)code;
+}
+__device__ __forceinline__ double unpack_cost(unsigned long long v) {
+    const int i = (int)(unsigned)(v >> 32);
+    return (double)__int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF);
+}
+
+__device__ __forceinline__ double wire(const FltArgs& a, int l, int xmin, int xmax, int ymin, int ymax) {
+    return a.presum[cell_of(a.g, l, xmax, ymax)] - a.presum[cell_of(a.g, l, xmin, ymin)];
+}
+
+// Previous depth's stores and atomics complete, then L1 invalidated.
+__device__ __forceinline__ void depth_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void raise(int* error, int code) { atomicCAS(error, 0, code); }
+
+// One DAG's tables, net-local.
+struct Dag {
+    const int4* nodes;  // {coordinate, first edge, end edge, first slot * 8 + slots}
+    const int4* edges;  // {parent, slot, parent coordinate, child coordinate}
+    double* costs;      // [node * kS + layer]
+    unsigned long long* packed;
+    unsigned long long* best;
+    int* layer;
+    const double* cost_edges;  // [edge * 81 + from * 9 + to]
+    const int* best_change;
+};
+
+// solve_flt_node (src/routing/flt_block.cuh:6-142)
+__device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
+    const DeviceGrid& g = a.g;
+    const int L = g.L;
+    const int4 r = d.nodes[v];
+    const int low_fixed = 1 + r.x / g.XY;
+    const int x = r.x / g.Y % g.X, y = r.x % g.Y;
+    const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
+    const int kids = r.w & 7;
+    unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
+    const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
+    double* cv = d.costs + (size_t)v * kS;
+    double mins[6];
+    unsigned long long paths[6];
+    for (int low = 1; low <= low_fixed; ++low) {
+        for (int k = 0; k < kids; ++k) mins[k] = kInf;
+        double via = 0.0;
+        for (int li = low; li < L + 1; ++li) {
+            if (li > low) via += (double)a.vcost[cell_of(g, li - 2, x, y)];
+            for (int k = 0; k < kids; ++k) {
+                const unsigned long long p = pk[k * kS + li];
+                const double c = unpack_cost(p);
+                if (c < mins[k]) {
+                    mins[k] = c;
+                    paths[k] = (p & 0xFFFFFFFFULL) * kS + (unsigned long long)li;
+                }
+            }
+            if (li >= high_fixed) {
+                double cost = via;
+                for (int k = 0; k < kids; ++k) cost += mins[k];
+                if (cost < kInf && cost < cv[li]) {
+                    cv[li] = cost;
+                    for (int k = 0; k < kids; ++k) bp[k * kS + li] = paths[k];
+                }
+            }
+        }
+        for (int li = L - 1; li >= low; --li)
+            if (cv[li + 1] < cv[li]) {
+                cv[li] = cv[li + 1];
+                for (int k = 0; k < kids; ++k) bp[k * kS + li] = bp[k * kS + li + 1];
+            }
+    }
+    unsigned long long cache[kS];
+    for (int e = r.y; e < r.z; ++e) {
+        const int4 er = d.edges[e];
+        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
+        const int px = er.z / g.Y % g.X, py = er.z % g.Y;
+        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+        for (int t = 1; t < L + 1; ++t) cache[t] = kEmpty;
+        for (int layer = 1; layer < L + 1; ++layer) {
+            if (((layer & 1) ^ g.DIR) == 1 && y != py) continue;
+            if (((layer & 1) ^ g.DIR) == 0 && x != px) continue;
+            const double base = cv[layer];
+            if (!(base < kInf)) {  // the reference asserts here
+                raise(a.error, kDevNoSolution);
+                continue;
+            }
+            for (int t = 1 + (1 - layer % 2); t < L + 1; t += 2) {
+                double cost2;
+                int choice = kStraight;
+                if (t == layer) {
+                    cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
+                } else if (abs(x - px) + abs(y - py) > 4) {
+                    const int seq = (layer - 1) * 9 + (t - 1);
+                    const double ce = d.cost_edges[(size_t)e * 81 + seq];
+                    if (!(ce < kInf)) continue;
+                    cost2 = base + ce;
+                    choice = seq;
+                } else {
+                    continue;
+                }
+                const unsigned long long nv = pack((float)cost2, v * 100 + choice);
+                if (nv < cache[t]) cache[t] = nv;
+            }
+        }
+        for (int t = 1; t < L + 1; ++t)
+            if (cache[t] < kEmpty) atomicMin(ppk + t, cache[t]);
+    }
+}
+
+// trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
+// segments of the edge its chosen parent uses and its via stack, into w[].
+__device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
+    const DeviceGrid& g = a.g;
+    const int L = g.L;
+    const int4 r = d.nodes[v];
+    const int l = r.x / g.XY, cx = r.x / g.Y % g.X, cy = r.x % g.Y;
+    const unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
+    int n = 0;
+    int lv;
+    if (r.y == r.z) {
+        int best = 0;
+        double bc = d.costs[(size_t)v * kS];
+        for (int layer = 1; layer < L + 1; ++layer) {
+            const double c = d.costs[(size_t)v * kS + layer];
+            if (c < bc) {
+                bc = c;
+                best = layer;
+            }
+        }
+        lv = best;
+        d.layer[v] = lv;
+    } else {
+        int par_layer = -1, par = -1, par_edge = -1;
+        lv = -1;
+        for (int e = r.y; e < r.z; ++e) {
+            const int4 er = d.edges[e];
+            const int pl = d.layer[er.x];
+            if (pl <= 0) continue;
+            par = er.x;
+            par_edge = e;
+            par_layer = pl;
+            const unsigned long long path = d.best[(size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer];
+            const unsigned long long child = path / kS / 100, choice = (path / kS) % 100;
+            if (child != (unsigned long long)v) {
+                lv = -1;
+                continue;
+            }
+            lv = choice == kStraight ? (int)(path % kS) : (int)(choice / 9 + 1);
+            const int px = er.z / g.Y % g.X, py = er.z % g.Y;
+            const int rl = lv - 1;
+            auto idx = [&](int ll, int xx, int yy) { return (ll * g.X + xx) * g.Y + yy; };
+            if (px == cx && cy != py) {
+                if (choice == kStraight) {
+                    w[n++] = idx(rl, px, min(cy, py));
+                    w[n++] = idx(rl, px, max(cy, py));
+                } else {
+                    const int fl = (int)choice / 9, tl = (int)choice % 9;
+                    const int pos = d.best_change[(size_t)e * 81 + fl * 9 + tl];
+                    w[n++] = idx(fl, cx, min(cy, pos));
+                    w[n++] = idx(fl, cx, max(cy, pos));
+                    w[n++] = idx(min(fl, tl), cx, pos);
+                    w[n++] = idx(max(fl, tl), cx, pos);
+                    w[n++] = idx(tl, cx, min(py, pos));
+                    w[n++] = idx(tl, cx, max(py, pos));
+                }
+            } else if (py == cy && cx != px) {
+                if (choice == kStraight) {
+                    w[n++] = idx(rl, min(cx, px), py);
+                    w[n++] = idx(rl, max(cx, px), py);
+                } else {
+                    const int fl = (int)choice / 9, tl = (int)choice % 9;
+                    const int pos = d.best_change[(size_t)e * 81 + fl * 9 + tl];
+                    w[n++] = idx(fl, min(cx, pos), cy);
+                    w[n++] = idx(fl, max(cx, pos), cy);
+                    w[n++] = idx(min(fl, tl), pos, cy);
+                    w[n++] = idx(max(fl, tl), pos, cy);
+                    w[n++] = idx(tl, min(px, pos), cy);
+                    w[n++] = idx(tl, max(px, pos), cy);
+                }
+            }
+            break;
+        }
+        if (par_layer == -1) {
+            d.layer[v] = -1;
+            return 0;
+        }
+        const int4 er = d.edges[par_edge];
+        const unsigned long long path = d.best[(size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer];
+        if (path / kS / 100 != (unsigned long long)v) {
+            d.layer[v] = -1;
+            return 0;
+        }
+        d.layer[v] = lv;
+    }
+    const int kids = r.w & 7;
+    int minl = l + 1;
+    int maxl = (l + 1) == L + 1 ? 1 : minl;
+    minl = min(minl, lv);
+    maxl = max(maxl, lv);
+    for (int k = 0; k < kids; ++k) {
+        const int lc = (int)(bp[k * kS + lv] % kS);
+        minl = min(minl, lc);
+        maxl = max(maxl, lc);
+    }
+    if (minl < maxl) {
+        w[n++] = ((minl - 1) * g.X + cx) * g.Y + cy;
+        w[n++] = ((maxl - 1) * g.X + cx) * g.Y + cy;
+    }
+    return n;
+}
+
+__device__ __forceinline__ int wave_inclusive_sum(int v, int lane) {
+    for (int s = 1; s < kWave; s <<= 1) {
+        const int o = __shfl_up(v, s, kWave);
+        if (lane >= s) v += o;
+    }
+    return v;
+}
+
+}  // namespace
+
+// precompute_cost_edge (src/routing/flt_cost.cuh:50-130); one thread per
+// (edge, from-layer) of the batch.
+__global__ void flt_edge_kernel(FltArgs a) {
+    const DeviceGrid& g = a.g;
+    const int L = g.L;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)a.edges * L) return;
+    const int e = (int)(t / L), f = (int)(t % L);
+    const int4 er = a.edge_rec[a.edge_base + e];
+    const int tox = er.z / g.Y % g.X, toy = er.z % g.Y;
+    const int fromx = er.w / g.Y % g.X, fromy = er.w % g.Y;
+    double* ce = a.cost_edges + (size_t)e * 81 + f * 9;
+    int* bc = a.best_change + (size_t)e * 81 + f * 9;
+    if ((layer_dir(g, f) == 0 && fromy != toy) || (layer_dir(g, f) == 1 && fromx != tox)) {
+        for (int k = 0; k < 9; ++k) ce[k] = kInf;
+        return;
+    }
+    const int distance = abs(tox - fromx) + abs(toy - fromy);
+    if (!distance) return;
+    const int dd = distance / min(distance, 20);
+    const bool vertical = fromx == tox;
+    const int start = vertical ? min(fromy, toy) : min(fromx, tox);
+    const int end = vertical ? max(fromy, toy) : max(fromx, tox);
+    for (int to = f & 1; to < L; to += 2) {
+        double best = kInf;
+        int pos = -1;
+        for (int s = start + dd; s < end; s += dd) {
+            const int sx = vertical ? fromx : s, sy = vertical ? s : fromy;
+            double cost = 0.0;
+            for (int l = min(f, to); l < max(f, to); ++l) cost += (double)a.vcost[cell_of(g, l, sx, sy)];
+            if (vertical)
+                cost += wire(a, f, fromx, fromx, min(fromy, sy), max(fromy, sy)) +
+                        wire(a, to, tox, tox, min(toy, sy), max(toy, sy));
+            else
+                cost += wire(a, f, min(fromx, sx), max(fromx, sx), fromy, fromy) +
+                        wire(a, to, min(tox, sx), max(tox, sx), toy, toy);
+            if (cost < best) {
+                best = cost;
+                pos = s;
+            }
+        }
+        ce[to] = best;
+        bc[to] = pos;
+    }
+}
+
+// route_flt_block (src/routing/flt_block.cuh:334-355) with one wavefront
+// per DAG and four DAGs per workgroup.
+__global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int k = blockIdx.x * (256 / kWave) + (threadIdx.x / kWave);
+    if (k >= a.count) return;
+    const int4 ds = a.desc[k];   // stage-global first node, edge, slot, depth offset
+    const int4 dn = a.sizes[k];  // nodes, edges, slots, depths
+    Dag d;
+    d.nodes = a.node_rec + ds.x;
+    d.edges = a.edge_rec + ds.y;
+    d.costs = a.costs + (size_t)(ds.x - a.node_base) * kS;
+    d.packed = a.packed + (size_t)(ds.z - a.slot_base) * kS;
+    d.best = a.best_path + (size_t)(ds.z - a.slot_base) * kS;
+    d.layer = a.layer_output + (ds.x - a.node_base);
+    d.cost_edges = a.cost_edges + (size_t)(ds.y - a.edge_base) * 81;
+    d.best_change = a.best_change + (size_t)(ds.y - a.edge_base) * 81;
+    const int* order = a.order + ds.x;
+    const int* doff = a.depth_off + ds.w;
+
+    // initialize_batch (src/routing/flt_workspace.cuh:60-69) for this DAG
+    for (int i = lane; i < dn.x * kS; i += kWave) d.costs[i] = kInf;
+    for (int i = lane; i < dn.z * kS; i += kWave) d.packed[i] = kEmpty;
+    for (int i = lane; i < dn.x; i += kWave) d.layer[i] = -1;
+    depth_barrier();
+    for (int dep = dn.w - 1; dep >= 0; --dep) {
+        const int end = doff[dep + 1];
+        for (int i = doff[dep] + lane; i < end; i += kWave) solve_node(a, d, order[i]);
+        depth_barrier();
+    }
+    const int id = a.net_ids[k];
+    int* routes = a.routes.routes + (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin;
+    const long long cap = (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin;
+    int count = 1;  // the root's trace sets the word count to 1
+    for (int dep = 0; dep < dn.w; ++dep) {
+        const int first = doff[dep], end = doff[dep + 1];
+        for (int i0 = first; i0 < end; i0 += kWave) {
+            int w[8];
+            const int i = i0 + lane;
+            const int n = i < end ? trace_node(a, d, order[i], w) : 0;
+            const int incl = wave_inclusive_sum(n, lane);
+            const int at = count + incl - n;
+            if (n) {
+                if (at + n - 1 >= cap) {
+                    raise(a.error, kDevRouteOverflow);
+                } else {
+                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
+                }
+            }
+            count += __shfl(incl, kWave - 1, kWave);
+        }
+        depth_barrier();
+    }
+    if (lane == 0) routes[0] = count <= cap ? count : 1;
+}
+
+}  // namespace grotor

@@ -151,6 +151,42 @@ __global__ void zshape_reset_kernel(ZshapeArgs a);
 __global__ void zshape_solve_kernel(ZshapeArgs a);
 __global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteStore rs);
 
+// FLT routing DAGs
+constexpr int kFltStride = 10;  // layer index 1..L of the FLT tables (L <= 9), 0 unused
+struct FltArgs {
+    DeviceGrid g;
+    const double* presum;
+    const float* vcost;
+    // stage records (resident), net-local indices
+    const int4* node_rec;   // {coordinate, first edge, end edge, first slot * 8 + slots}
+    const int4* edge_rec;   // {parent, slot, parent coordinate, child coordinate}
+    const int* order;       // per net: its nodes by depth, stable
+    const int* depth_off;   // per net: depths + 1 offsets into its order
+    // the batch
+    const int4* desc;       // stage-global first node, edge, slot, depth offset
+    const int4* sizes;      // nodes, edges, slots, depths
+    const int* net_ids;
+    int count, edges;
+    int node_base, edge_base, slot_base;  // stage-global index of the batch's first node, edge, slot
+    // batch tables
+    double* costs;                   // [node * kFltStride + layer]
+    unsigned long long* packed;      // [slot * kFltStride + layer]
+    unsigned long long* best_path;   // [slot * kFltStride + layer]
+    int* layer_output;
+    double* cost_edges;              // [edge * 81 + from * 9 + to]
+    int* best_change;
+    RouteStore routes;        // phase-two route words
+    const int* pin_acc_num;   // pins per net: capacity = pins * routes.per_pin
+    int* error;
+};
+__global__ void flt_edge_kernel(FltArgs a);
+__global__ void flt_route_kernel(FltArgs a);
+
+// Congestion view (src/routing/congestion.cuh:60-112)
+__global__ void congestion_view_kernel(DeviceGrid g, const float* capacity, const float* demand,
+                                       unsigned char* congested, float* xsum, float* ysum);
+__global__ void congestion_prefix_kernel(int tracks, int len, int track_stride, int pos_stride, float* sum);
+
 // Soft DP
 struct SoftCostArgs {
     DeviceGrid g;

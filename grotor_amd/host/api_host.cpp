@@ -1,11 +1,16 @@
 // C ABI of the host front end: design I/O, synthetic instances, preparation
 // of the stage topology and .out output.
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <memory>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../csrc/gr_status.hpp"
 #include "design.hpp"
+#include "flt_host.hpp"
 #include "zshape_host.hpp"
 
 namespace grotor {
@@ -27,6 +32,11 @@ struct gr_design {
     // Z-shape stage
     std::vector<int> zorder, zpt_off, zpts, zhs_off, zhsegs, zvs_off, zvsegs;
     ZshapePlan zplan;
+    // FLT stage: DAGs of the rerouted nets (priority order), phase-two offsets, plan
+    std::vector<int> forder, fpt_off, fpts, fhs_off, fhsegs, fvs_off, fvsegs, pin_off2;
+    std::vector<FltDag> dags;
+    std::vector<int> fp_batch_off, fp_net_ids, fp_nodes, fp_edges;
+    std::vector<long long> fp_node_off, fp_edge_off;
 };
 
 extern "C" {
@@ -200,6 +210,117 @@ int gr_design_zshape_finish(gr_design* g, int* routes, int words_per_pin, int* p
     });
 }
 
+int gr_design_flt_stage(gr_design* g, const unsigned char* reroute, const float* xsum, const float* ysum, int threads,
+                        gr_flt_stage* t) {
+    return guarded([&] {
+        if (!g || !reroute || !t) throw GrError(GR_E_ARG, "null argument");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        const Design& d = g->d;
+        const Grid& gr = d.grid;
+        const int n = int(d.subnets.size());
+        // prepare_detour_topology (src/routing/flt.cuh:48-73): hpwl descending
+        // with std::sort over the rerouted ids in increasing order
+        g->forder.clear();
+        g->pin_off2.assign(n, -1);
+        long long pins = 0;
+        for (int i = 0; i < n; ++i)
+            if (reroute[i]) {
+                g->forder.push_back(i);
+                g->pin_off2[i] = int(pins);  // src/pipeline.hpp:96-101
+                pins += d.pin_acc_num[i + 1] - d.pin_acc_num[i];
+            }
+        if (pins > INT32_MAX) throw GrError(GR_E_ARG, "too many phase-two pins");
+        const auto& s = d.subnets;
+        std::sort(g->forder.begin(), g->forder.end(), [&](int l, int r) { return s[l].hpwl > s[r].hpwl; });
+        FltParams p;
+        p.mode = n > 20000000 ? 2 : n > 5000000 ? 1 : 0;  // src/pipeline.hpp:88-89
+        p.X = gr.X, p.Y = gr.Y, p.L = gr.routing_layers();
+        p.xsum = xsum, p.ysum = ysum;
+        const int m = int(g->forder.size());
+        g->dags.assign(m, FltDag{});
+        std::vector<std::string> errors(std::max(threads, 1));
+        std::atomic<int> next{0};
+        std::vector<std::thread> pool;
+        for (int w = 0; w < std::max(threads, 1); ++w)
+            pool.emplace_back([&, w] {
+                for (int k; (k = next++) < m;) {
+                    try {
+                        flt_build_dag(s[g->forder[k]], p, g->dags[k]);
+                    } catch (const std::exception& e) {
+                        errors[w] = e.what();
+                        next = m;
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+        for (const auto& e : errors)
+            if (!e.empty()) throw GrError(GR_E_ARG, "FLT DAG: " + e);
+        g->fpt_off.assign(1, 0), g->fhs_off.assign(1, 0), g->fvs_off.assign(1, 0);
+        g->fpts.clear(), g->fhsegs.clear(), g->fvsegs.clear();
+        long long nodes = 0, edges = 0;
+        for (int k = 0; k < m; ++k) {
+            const Subnet& sn = s[g->forder[k]];
+            const FltDag& dag = g->dags[k];
+            g->fpts.insert(g->fpts.end(), dag.points.begin(), dag.points.end());
+            g->fpt_off.push_back(int(g->fpts.size()));
+            for (auto& e : sn.hsegs) g->fhsegs.push_back(e.first), g->fhsegs.push_back(e.second);
+            g->fhs_off.push_back(int(g->fhsegs.size() / 2));
+            for (auto& e : sn.vsegs) g->fvsegs.push_back(e.first), g->fvsegs.push_back(e.second);
+            g->fvs_off.push_back(int(g->fvsegs.size() / 2));
+            nodes += dag.nodes.size() / 4;
+            edges += dag.edges.size() / 2;
+        }
+        t->nets = m;
+        t->order = g->forder.data();
+        t->pt_off = g->fpt_off.data();
+        t->pts = g->fpts.data();
+        t->hs_off = g->fhs_off.data();
+        t->hsegs = g->fhsegs.data();
+        t->vs_off = g->fvs_off.data();
+        t->vsegs = g->fvsegs.data();
+        t->pin_off2 = g->pin_off2.data();
+        t->pins2 = pins;
+        t->nodes = nodes;
+        t->edges = edges;
+    });
+}
+
+int gr_design_flt_plan(gr_design* g, const int* color, int num_batches, gr_flt_plan* out) {
+    return guarded([&] {
+        if (!g || !out || num_batches < 0 || (!color && !g->forder.empty())) throw GrError(GR_E_ARG, "invalid arguments");
+        const int m = int(g->forder.size());
+        std::vector<std::vector<int>> batches(num_batches);
+        for (int k = 0; k < m; ++k) {
+            if (color[k] < 0 || color[k] >= num_batches) throw GrError(GR_E_ARG, "batch index out of range");
+            batches[color[k]].push_back(k);
+        }
+        // flt::Plans (include/flt_topology.hpp:92-127): batches in reverse
+        // order, empty ones skipped, nets in priority order inside a batch
+        g->fp_batch_off.assign(1, 0);
+        g->fp_net_ids.clear(), g->fp_nodes.clear(), g->fp_edges.clear();
+        g->fp_node_off.assign(1, 0), g->fp_edge_off.assign(1, 0);
+        for (int b = num_batches - 1; b >= 0; --b) {
+            if (batches[b].empty()) continue;
+            for (int k : batches[b]) {
+                const FltDag& dag = g->dags[k];
+                g->fp_net_ids.push_back(g->forder[k]);
+                g->fp_nodes.insert(g->fp_nodes.end(), dag.nodes.begin(), dag.nodes.end());
+                g->fp_edges.insert(g->fp_edges.end(), dag.edges.begin(), dag.edges.end());
+                g->fp_node_off.push_back((long long)g->fp_nodes.size() / 4);
+                g->fp_edge_off.push_back((long long)g->fp_edges.size() / 2);
+            }
+            g->fp_batch_off.push_back(int(g->fp_net_ids.size()));
+        }
+        out->batches = int(g->fp_batch_off.size()) - 1;
+        out->batch_off = g->fp_batch_off.data();
+        out->net_ids = g->fp_net_ids.data();
+        out->node_off = g->fp_node_off.data();
+        out->nodes = g->fp_nodes.data();
+        out->edge_off = g->fp_edge_off.data();
+        out->edges = g->fp_edges.data();
+    });
+}
+
 int gr_design_nets(gr_design* g, int* count, const int** pin_off, const int** pins, const int** extra_off,
                    const int** extra, const int** subnet_off, const int** subnets) {
     return guarded([&] {
@@ -241,6 +362,34 @@ int gr_tree_geometry(int Y, int n, const int* positions, const int* adj_off, con
         for (size_t i = 0; i < hs.size(); ++i) hsegs[2 * i] = hs[i].first, hsegs[2 * i + 1] = hs[i].second;
         for (size_t i = 0; i < vs.size(); ++i) vsegs[2 * i] = vs[i].first, vsegs[2 * i + 1] = vs[i].second;
         *npoints = int(pts.size()), *nhsegs = int(hs.size()), *nvsegs = int(vs.size());
+    });
+}
+
+int gr_flt_dag(int X, int Y, int L, int mode, int n, const int* positions, const int* adj_off, const int* adj,
+               const float* xsum, const float* ysum, int* nodes, int* nnodes, int* edges, int* nedges, int* points,
+               int* npoints) {
+    return guarded([&] {
+        if (n < 1 || X < 1 || Y < 1 || L < 1 || !positions || !adj_off || !adj || !nodes || !nnodes || !edges ||
+            !nedges || !points || !npoints)
+            throw GrError(GR_E_ARG, "invalid arguments");
+        Subnet net;
+        net.rsmt.resize(n + 1);
+        for (int i = 0; i < n; ++i) net.rsmt[i].assign(adj + adj_off[i], adj + adj_off[i + 1]);
+        net.rsmt[n].assign(positions, positions + n);
+        FltParams p;
+        p.mode = mode, p.X = X, p.Y = Y, p.L = L, p.xsum = xsum, p.ysum = ysum;
+        FltDag dag;
+        try {
+            flt_build_dag(net, p, dag);
+        } catch (const std::runtime_error& e) {
+            throw GrError(GR_E_ARG, e.what());
+        }
+        const int nn = int(dag.nodes.size() / 4), ne = int(dag.edges.size() / 2), np = int(dag.points.size());
+        if (nn > *nnodes || ne > *nedges || np > *npoints) throw GrError(GR_E_CAPACITY, "DAG output buffers too small");
+        std::copy(dag.nodes.begin(), dag.nodes.end(), nodes);
+        std::copy(dag.edges.begin(), dag.edges.end(), edges);
+        std::copy(dag.points.begin(), dag.points.end(), points);
+        *nnodes = nn, *nedges = ne, *npoints = np;
     });
 }
 

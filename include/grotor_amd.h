@@ -169,7 +169,9 @@ enum {
     GR_K_BATCH_COLOR = 5,  /* batching: in-order colouring */
     GR_K_RIPUP = 6,
     GR_K_ZSHAPE = 7,       /* Z-shape DP (reset, solve, convert) */
-    GR_K_COUNT = 8
+    GR_K_FLT_EDGE = 8,     /* FLT layer-change search per DAG edge */
+    GR_K_FLT = 9,          /* FLT DAG DP (solve and trace) */
+    GR_K_COUNT = 10
 };
 
 /* Start timing every kernel launch of this context with HIP events on its
@@ -205,6 +207,48 @@ int gr_zshape_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *n
  * refresh, reset_nodes, solve, convert_routes and the wire-then-via commit;
  * *elapsed_ms = the synchronized interval (DAG_TIMING). */
 int gr_zshape_run(gr_ctx *ctx, int first_stamp, double *elapsed_ms);
+
+/* ---- FLT stage on the device (src/routing/flt.cuh:75-182) ----
+ * Needs at most 9 routing layers (the reference's MAX_LAYER = 10). */
+
+/* Phase-two route storage (src/pipeline.hpp:96-108): pin_off2[net] is the
+ * net's offset in pins among the rerouted nets (pin_acc_num_phase2), -1 for
+ * the others; pins2 their total; words_per_pin = ROUTE_PER_PIN_PHASE2 (100,
+ * include/router_state.hpp:33).  Net i's words start at pin_off2[i] *
+ * words_per_pin, capacity words_per_pin * its pin count.  Cleared to zero. */
+int gr_ctx_set_routes_phase2(gr_ctx *ctx, const int *pin_off2, long long pins2, int words_per_pin);
+
+/* Copy `words` phase-two route words (from word 0) to the host
+ * (RouterState::download_routes(true), include/router_state.hpp:215-222). */
+int gr_routes2_download(gr_ctx *ctx, int *routes, long long words);
+
+/* The congestion view the FLT DAG builder reads (extract_congestionView,
+ * _xsum, _ysum, src/routing/congestion.cuh:60-112), X*Y cells in x*Y+y
+ * order: congested = worst demand - capacity over the layers > 0; xsum and
+ * ysum its exclusive prefix along x and along y.  Any output may be NULL. */
+int gr_congestion_view(gr_ctx *ctx, unsigned char *congested, float *xsum, float *ysum);
+
+/* Upload an FLT stage in execution order (flt::Plans, include/flt_topology.hpp:87-129,
+ * and Workspace::upload, src/routing/flt_workspace.cuh:156-173): batch b
+ * is nets [batch_off[b], batch_off[b+1]); net k (id net_ids[k]) has node
+ * records nodes[4*i..] = {coordinate, depth, child slots, parent_end} for i
+ * in [node_off[k], node_off[k+1]) and edge records edges[2*j..] = {parent,
+ * node * 10 + slot} for j in [edge_off[k], edge_off[k+1]), net-local
+ * indices, as flt::Topology::emit writes them (:52-70).  Every net needs
+ * phase-two storage, exactly one root (no parents, depth 0), parents
+ * shallower than their children, at most 6 child slots per node and
+ * straight edges; violations return GR_E_ARG. */
+int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_ids,
+                  const long long *node_off, const int *nodes, const long long *edge_off,
+                  const int *edges);
+
+/* Run the uploaded FLT batches (the loop of src/routing/flt.cuh:141-168):
+ * per batch, if remove_phase_one, the nets' phase-one routes are removed
+ * (commit K = -1 at the next stamp); then at the next stamp the cost
+ * refresh, precompute_cost_edge, the DAG DP and the commit of the
+ * phase-two routes.  Stamps start at first_stamp and advance by one per
+ * commit.  *elapsed_ms = the synchronized interval (DAG_TIMING). */
+int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, double *elapsed_ms);
 
 /* ---- soft (log-sum-exp) L-shape DP ----
  * No counterpart in the reference: a differentiable relaxation of the
@@ -272,6 +316,51 @@ typedef struct gr_topology {
     double unit_length_wire_cost, unit_via_cost;
 } gr_topology;
 int gr_design_topology(gr_design *design, gr_topology *out);
+
+/* FLT stage input (prepare_detour_topology, src/routing/flt.cuh:48-73):
+ * the nets flagged in reroute[subnet] in priority order (hpwl descending,
+ * std::sort over increasing ids), one DAG per net built by
+ * net::generate_detours_reconstruct (src/preprocess.hpp:769-1436) from the
+ * congestion view's prefix sums (gr_congestion_view; NULL = uncongested),
+ * their batching geometry (DAG points, tree segments) and the phase-two
+ * route offsets of src/pipeline.hpp:96-101.  Arrays are owned by the design. */
+typedef struct gr_flt_stage {
+    int nets;
+    const int *order;            /* [nets] subnet ids */
+    const int *pt_off, *pts;     /* batching points (2D) */
+    const int *hs_off, *hsegs;
+    const int *vs_off, *vsegs;
+    const int *pin_off2;         /* [subnets] offset in pins, -1 if not rerouted */
+    long long pins2;             /* pins of the rerouted nets */
+    long long nodes, edges;      /* DAG totals */
+} gr_flt_stage;
+int gr_design_flt_stage(gr_design *design, const unsigned char *reroute, const float *xsum,
+                        const float *ysum, int threads, gr_flt_stage *out);
+
+/* Execution plan of the FLT stage from a batch colouring of its nets
+ * (color[k] for priority slot k, gr_batch_plan with limit 300000): batches
+ * in reverse order, empty ones skipped (flt::Plans, include/flt_topology.hpp:92-127),
+ * with the DAG records in the layout gr_flt_upload takes. */
+typedef struct gr_flt_plan {
+    int batches;
+    const int *batch_off;        /* [batches+1] */
+    const int *net_ids;          /* subnet ids, execution order */
+    const long long *node_off;   /* [nets+1] */
+    const int *nodes;            /* 4 ints per node */
+    const long long *edge_off;   /* [nets+1] */
+    const int *edges;            /* 2 ints per edge */
+} gr_flt_plan;
+int gr_design_flt_plan(gr_design *design, const int *color, int num_batches, gr_flt_plan *out);
+
+/* The FLT DAG of one tree (net::generate_detours_reconstruct,
+ * src/preprocess.hpp:769-1436, with `mode` of src/pipeline.hpp:88-89):
+ * positions are 3D (pin layer, or L for Steiner nodes), adjacency in CSR,
+ * xsum/ysum the congestion view (NULL = uncongested).  Capacities in, counts
+ * out: *nnodes node records of 4 ints, *nedges edge records of 2 ints (as
+ * gr_flt_upload takes them), *npoints batching cells. */
+int gr_flt_dag(int X, int Y, int L, int mode, int n, const int *positions, const int *adj_off, const int *adj,
+               const float *xsum, const float *ysum, int *nodes, int *nnodes, int *edges, int *nedges,
+               int *points, int *npoints);
 
 /* Per design net of a prepared design: selected pins (3D metal indices),
  * extra_routes pairs and routed subnet ids, as CSR arrays owned by the design

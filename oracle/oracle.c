@@ -611,3 +611,74 @@ int ora_zshape_stage(const ora_grid *g, const float *capacity, float *demand, in
     free(dirty);
     return status;
 }
+
+int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *timestamp, float *wcost,
+                  float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
+                  const long long *node_off, const int *node_rec, const long long *edge_off, const int *edge_rec,
+                  const long long *route_off1, int *routes1, const long long *route_off2, const int *route_cap2,
+                  int *routes2, int remove_phase_one, int *stamp, double *total_wirelength, long long *total_vias,
+                  int threads) {
+    int maxcount = 1;
+    for (int b = 0; b < nbatch; ++b)
+        if (batch_off[b + 1] - batch_off[b] > maxcount) maxcount = batch_off[b + 1] - batch_off[b];
+    int **rof = (int **)malloc(sizeof(int *) * (size_t)maxcount);
+    int tracks = 0;
+    for (int l = 0; l < g->L; ++l) tracks += track_count(g, l);
+    unsigned char *dirty = (unsigned char *)malloc((size_t)tracks);
+    memset(dirty, 1, (size_t)tracks);
+    int status = 0;
+    for (int b = 0; b < nbatch; ++b) {
+        const int begin = batch_off[b], count = batch_off[b + 1] - begin;
+        if (remove_phase_one) {
+            for (int k = 0; k < count; ++k) rof[k] = routes1 + route_off1[net_ids[begin + k]];
+            const int s1 = ++*stamp;
+            ora_commit_batch(g, demand, timestamp, count, rof, s1, -1, total_wirelength, total_vias, dirty);
+        }
+        const int s = ++*stamp;
+        refresh_dirty(g, capacity, demand, 0.5, 1.5, 1.0, dirty, wcost, vcost, presum, threads);
+        const int err = ora_flt_batch(g, presum, vcost, count, net_ids + begin, node_off + begin, node_rec,
+                                      edge_off + begin, edge_rec, route_off2, route_cap2, routes2, NULL);
+        if (err) status = err;
+        for (int k = 0; k < count; ++k) rof[k] = routes2 + route_off2[net_ids[begin + k]];
+        ora_commit_batch(g, demand, timestamp, count, rof, s, 1, total_wirelength, total_vias, dirty);
+    }
+    free(rof);
+    free(dirty);
+    return status;
+}
+
+/* extract_congestionView, _xsum, _ysum (src/routing/congestion.cuh:60-112) */
+static void view_prefix(float *s, int len) {
+    for (int d = 0; (1 << d) < len; ++d)
+        for (int i = 0; i < len; ++i)
+            if (i >> d & 1) s[i] += s[(i >> d << d) - 1];
+}
+
+void ora_congestion_view(const ora_grid *g, const float *capacity, const float *demand, unsigned char *congested,
+                         float *xsum, float *ysum) {
+    const int X = g->X, Y = g->Y, XY = X * Y;
+    for (int idx = 0; idx < XY; ++idx) {
+        double worst = 0.0;
+        for (int l = 0; l < g->L; ++l) {
+            const double d = (double)(demand[(size_t)l * XY + idx] - capacity[(size_t)l * XY + idx]);
+            worst = fmax(worst, d);
+        }
+        if (congested) congested[idx] = worst > 0.0;
+        xsum[idx] = (float)worst;
+        ysum[idx] = (float)worst;
+    }
+    float *s = (float *)malloc(sizeof(float) * (size_t)(X > Y ? X : Y));
+    for (int x = 0; x < X; ++x) {
+        s[0] = 0.f;
+        for (int y = 0; y + 1 < Y; ++y) s[y + 1] = ysum[x * Y + y];
+        view_prefix(s, Y);
+        for (int y = 0; y < Y; ++y) ysum[x * Y + y] = s[y];
+    }
+    for (int y = 0; y < Y; ++y) {
+        s[0] = 0.f;
+        for (int x = 0; x + 1 < X; ++x) s[x + 1] = xsum[x * Y + y];
+        view_prefix(s, X);
+        for (int x = 0; x < X; ++x) xsum[x * Y + y] = s[x];
+    }
+    free(s);
+}

@@ -157,6 +157,42 @@ double ora_soft_stage(const ora_grid *g, const float *capacity, const float *dem
                       double via_weight, double tau, int count, const int *node_off, const int *nodes,
                       const int *parents, double *gw, double *gv, double *gparams);
 
+/* Congestion view (src/routing/congestion.cuh:60-112), X*Y cells in x*Y+y
+ * order: worst demand - capacity over the layers, its exclusive prefix
+ * along x (xsum) and along y (ysum) in the reference's summation order.
+ * congested may be NULL. */
+void ora_congestion_view(const ora_grid *g, const float *capacity, const float *demand, unsigned char *congested,
+                         float *xsum, float *ysum);
+
+/* ---- FLT DAG DP (flt.c), src/routing/flt_cost.cuh, flt_block.cuh ---- */
+
+/* Packed FLT candidate: order-preserving float bits above a 32-bit code
+ * (src/routing/flt_cost.cuh:25-40). */
+unsigned long long ora_flt_pack(float cost, int code);
+
+/* One FLT batch (src/routing/flt.cuh:142-159 without the commits): count
+ * DAGs in execution order.  Net k's compact records are node_rec[4 * i] for
+ * i in [node_off[k], node_off[k+1]) = {coordinate, depth, child slots,
+ * parent_end} and edge_rec[2 * j] for j in [edge_off[k], edge_off[k+1]) =
+ * {parent, node * 10 + slot} (net-local indices, include/flt_topology.hpp:15-16).
+ * Routes of net_ids[k] go to routes + route_off[net] (capacity route_cap[net]
+ * words).  layer_output (may be NULL) receives every node's chosen layer.
+ * Returns 0, 1 (route storage overflow) or 2 (a node without a finite cost
+ * was reached). */
+int ora_flt_batch(const ora_grid *g, const double *presum, const float *vcost, int count, const int *net_ids,
+                  const long long *node_off, const int *node_rec, const long long *edge_off, const int *edge_rec,
+                  const long long *route_off, const int *route_cap, int *routes, int *layer_output);
+
+/* One FLT stage (src/routing/flt.cuh:141-168): per batch, optionally remove
+ * the nets' phase-one routes (K = -1 at ++*stamp), then at ++*stamp refresh,
+ * route and commit the phase-two routes.  Returns the last batch error. */
+int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *timestamp, float *wcost,
+                  float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
+                  const long long *node_off, const int *node_rec, const long long *edge_off, const int *edge_rec,
+                  const long long *route_off1, int *routes1, const long long *route_off2, const int *route_cap2,
+                  int *routes2, int remove_phase_one, int *stamp, double *total_wirelength, long long *total_vias,
+                  int threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,6 +140,67 @@ class Grid:
         return flags[:n]
 
 
+    def congestion_view(self, demand):
+        """(congested, xsum, ysum) over X*Y cells in x*Y+y order."""
+        demand = np.ascontiguousarray(demand, np.float32)
+        n = self.X * self.Y
+        cong, xs, ys = np.zeros(n, np.uint8), np.zeros(n, np.float32), np.zeros(n, np.float32)
+        lib().ora_congestion_view(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+                                  _p(cong, ctypes.c_ubyte), _p(xs, ctypes.c_float), _p(ys, ctypes.c_float))
+        return cong.astype(bool), xs, ys
+
+    def flt_batch(self, presum, vcost, plan, begin, count, pin_acc_num, pin_off2, words_per_pin=100, routes=None):
+        """One FLT batch of plan nets [begin, begin + count) at the given costs;
+        returns (phase-two routes, layer_output of the batch nodes, status)."""
+        pin_acc_num, pin_off2 = _i32(pin_acc_num), _i32(pin_off2)
+        route_off = (words_per_pin * pin_off2.astype(np.int64)).astype(np.int64)
+        route_cap = (words_per_pin * np.diff(pin_acc_num)).astype(np.int32)
+        if routes is None:
+            routes = np.zeros(words_per_pin * (int(np.max(pin_off2 + np.diff(pin_acc_num), initial=0))) + 1, np.int32)
+        node_off = np.ascontiguousarray(plan["node_off"], np.int64)
+        edge_off = np.ascontiguousarray(plan["edge_off"], np.int64)
+        nodes_b = int(node_off[begin + count] - node_off[begin])
+        layer = np.zeros(max(nodes_b, 1), np.int32)
+        presum = np.ascontiguousarray(presum, np.float64)
+        vcost = np.ascontiguousarray(vcost, np.float32)
+        rc = lib().ora_flt_batch(ctypes.byref(self.c), _p(presum, ctypes.c_double), _p(vcost, ctypes.c_float),
+                                 ctypes.c_int(count), _p(_i32(plan["net_ids"][begin:begin + count]), ctypes.c_int),
+                                 _p(node_off[begin:], ctypes.c_longlong), _p(_i32(plan["nodes"]), ctypes.c_int),
+                                 _p(edge_off[begin:], ctypes.c_longlong), _p(_i32(plan["edges"]), ctypes.c_int),
+                                 _p(route_off, ctypes.c_longlong), _p(route_cap, ctypes.c_int),
+                                 _p(routes, ctypes.c_int), _p(layer, ctypes.c_int))
+        return routes, layer[:nodes_b], rc
+
+    def flt_stage(self, plan, pin_acc_num, pin_off2, pins2, demand, ts, routes1, first_stamp, remove_phase_one=True,
+                  words_per_pin=100, wl=0.0, vias=0, threads=1):
+        """FLT stage continuing from a phase-one state (demand, timestamps,
+        phase-one routes at 12 words per pin, updated in place copies);
+        returns (routes2, demand, ts, wl, vias, status, last stamp)."""
+        pin_acc_num, pin_off2 = _i32(pin_acc_num), _i32(pin_off2)
+        route_off1 = (ROUTE_PER_PIN * pin_acc_num[:-1].astype(np.int64)).astype(np.int64)
+        route_off2 = (words_per_pin * pin_off2.astype(np.int64)).astype(np.int64)
+        route_cap2 = (words_per_pin * np.diff(pin_acc_num)).astype(np.int32)
+        routes1 = _i32(routes1).copy()
+        routes2 = np.zeros(words_per_pin * int(pins2) + 1, np.int32)
+        demand = np.ascontiguousarray(demand, np.float32).copy()
+        ts = _i32(ts).copy()
+        w = np.empty(self.cells, np.float32)
+        v = np.empty(self.cells, np.float32)
+        p = np.empty(self.cells, np.float64)
+        stamp = ctypes.c_int(first_stamp - 1)
+        wlc, vc = ctypes.c_double(wl), ctypes.c_longlong(vias)
+        bo = _i32(plan["batch_off"])
+        rc = lib().ora_flt_stage(
+            ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float), _p(ts, ctypes.c_int),
+            _p(w, ctypes.c_float), _p(v, ctypes.c_float), _p(p, ctypes.c_double), ctypes.c_int(len(bo) - 1),
+            _p(bo, ctypes.c_int), _p(_i32(plan["net_ids"]), ctypes.c_int),
+            _p(np.ascontiguousarray(plan["node_off"], np.int64), ctypes.c_longlong), _p(_i32(plan["nodes"]), ctypes.c_int),
+            _p(np.ascontiguousarray(plan["edge_off"], np.int64), ctypes.c_longlong), _p(_i32(plan["edges"]), ctypes.c_int),
+            _p(route_off1, ctypes.c_longlong), _p(routes1, ctypes.c_int), _p(route_off2, ctypes.c_longlong),
+            _p(route_cap2, ctypes.c_int), _p(routes2, ctypes.c_int), ctypes.c_int(1 if remove_phase_one else 0),
+            ctypes.byref(stamp), ctypes.byref(wlc), ctypes.byref(vc), ctypes.c_int(threads))
+        return routes2, demand, ts, wlc.value, vc.value, rc, stamp.value
+
     def zshape_stage(self, plan, pin_acc_num, words_per_pin=256, threads=1, first_stamp=0):
         """Z-shape stage over a plan; returns (routes, demand, timestamps, wl, vias, status)."""
         pin_acc_num = _i32(pin_acc_num)

@@ -10,12 +10,18 @@
 // then prints one JSON object per line:
 //   {"grid": ...}, {"net": ...} per design net, {"sub": ...} per subnet,
 //   {"order": [...]}, {"batches": [[...], ...]}.
+// With a congestion-view file (X*Y float32 xsum, then X*Y float32 ysum, in
+// x*Y+y order) it then runs net::generate_detours_reconstruct
+// (src/preprocess.hpp:769-1436) on every subnet tree of two or more nodes
+// and prints {"dag": subnet, "nodes", "depth", "kids", "npar", "par",
+// "slot", "points"} per subnet: the FLT builder's arrays.
 //
-// usage: ref_harness <cap> <net> <flute_lut_dir> [sparsity]
+// usage: ref_harness <cap> <net> <flute_lut_dir> [sparsity [view_file]]
 #include <unistd.h>
 
 #include "batching/generate_batches_cpu.hpp"
 #include "preprocess.hpp"
+#include "flt_topology.hpp"
 
 static void print_json_string(const std::string& s) {
     putchar('"');
@@ -105,5 +111,46 @@ int main(int argc, char** argv) {
         print_list(batches[b]);
     }
     printf("]}\n");
+    if (argc > 5) {
+        const size_t cells = size_t(v.X) * v.Y;
+        std::vector<float> xsum(cells), ysum(cells);
+        std::vector<char> busy(cells, 0);
+        FILE* f = fopen(argv[5], "rb");
+        if (!f || fread(xsum.data(), sizeof(float), cells, f) != cells ||
+            fread(ysum.data(), sizeof(float), cells, f) != cells)
+            return 5;
+        fclose(f);
+        bool* busy_view = reinterpret_cast<bool*>(busy.data());
+        float* xs = xsum.data();
+        float* ys = ysum.data();
+        grotor::flt::Builder b;
+        for (size_t i = 0; i < nets.size(); ++i) {
+            auto& n = nets[i];
+            if (n.rsmt.size() < 3) continue;
+            int saved_out = dup(1);
+            if (!freopen("/dev/null", "w", stdout)) return 3;
+            n.generate_detours_reconstruct(state, b, busy_view, xs, ys, false, false);
+            fflush(stdout);
+            dup2(saved_out, 1);
+            close(saved_out);
+            stdout = fdopen(1, "w");
+            const int cnt = b.node_index_cnt;
+            printf("{\"dag\":%zu,\"nodes\":", i);
+            print_list(std::vector<int>(b.nodes_cpu.begin(), b.nodes_cpu.begin() + cnt));
+            printf(",\"depth\":");
+            print_list(std::vector<int>(b.node_depth_cpu.begin(), b.node_depth_cpu.begin() + cnt));
+            printf(",\"kids\":");
+            print_list(std::vector<int>(b.child_num_cpu.begin(), b.child_num_cpu.begin() + cnt));
+            printf(",\"npar\":");
+            print_list(std::vector<int>(b.par_num_cpu.begin(), b.par_num_cpu.begin() + cnt));
+            printf(",\"par\":");
+            print_list(b.par_nodes_cpu);
+            printf(",\"slot\":");
+            print_list(b.currentChildIDX_cpu);
+            printf(",\"points\":");
+            print_list(n.points);
+            printf("}\n");
+        }
+    }
     return 0;
 }

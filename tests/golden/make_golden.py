@@ -13,6 +13,12 @@ Two instances, both with 10 metal layers because the reference reads only
 For each, the CAP/NET pair and the output of oracle/_ref/ref_harness (the
 reference's own host code: pin selection, FLUTE trees, subnet split, hpwl
 order and CPU batch lists, one JSON object per line) are written gzipped.
+
+`--flt` adds {name}.flt.jsonl.gz from the committed CAP/NET pairs: a
+congestion view (first line, {"view": {"xsum", "ysum"}}: a fixed pattern of
+congested cells and its exclusive prefix sums along x and y) and the FLT
+DAG that net::generate_detours_reconstruct (src/preprocess.hpp:769-1436)
+builds from it for every subnet tree of two or more nodes.
 """
 import gzip
 import os
@@ -82,8 +88,52 @@ def run_harness(cap_text, net_text, name):
     return res.stdout
 
 
+def flt_view(X, Y):
+    """Congested cells on a fixed pattern (worst overflow 1 or 2) and the
+    exclusive prefix sums along x and along y, float32, x*Y+y order."""
+    import numpy as np
+    x, y = np.meshgrid(np.arange(X), np.arange(Y), indexing="ij")
+    worst = np.where((x * 131 + y * 71) % 17 == 0, 1.0 + (x + y) % 2, 0.0).astype(np.float32)
+    xsum = np.zeros((X, Y), np.float32)
+    ysum = np.zeros((X, Y), np.float32)
+    for i in range(1, X):
+        xsum[i] = xsum[i - 1] + worst[i - 1]
+    for j in range(1, Y):
+        ysum[:, j] = ysum[:, j - 1] + worst[:, j - 1]
+    return xsum.reshape(-1), ysum.reshape(-1)
+
+
+def make_flt(name):
+    import gzip as gz
+    import json
+    with gz.open(os.path.join(HERE, f"{name}.cap.gz"), "rt") as f:
+        cap = f.read()
+    with gz.open(os.path.join(HERE, f"{name}.net.gz"), "rt") as f:
+        net = f.read()
+    _, X, Y = map(int, cap.split("\n")[0].split())
+    xsum, ysum = flt_view(X, Y)
+    tmp = f"/tmp/golden_{name}"
+    with open(tmp + ".cap", "w") as f:
+        f.write(cap)
+    with open(tmp + ".net", "w") as f:
+        f.write(net)
+    with open(tmp + ".view", "wb") as f:
+        f.write(xsum.tobytes() + ysum.tobytes())
+    res = subprocess.run([os.path.join(ROOT, "oracle/_ref/ref_harness"), tmp + ".cap", tmp + ".net",
+                          os.path.join(REF, "resources/flute"), "1", tmp + ".view"],
+                         capture_output=True, text=True, check=True)
+    dags = [line for line in res.stdout.split("\n") if line.startswith('{"dag"')]
+    head = json.dumps({"view": {"xsum": [float(v) for v in xsum], "ysum": [float(v) for v in ysum]}})
+    write_gz(os.path.join(HERE, f"{name}.flt.jsonl.gz"), head + "\n" + "\n".join(dags) + "\n")
+    print(name, "FLT DAGs", len(dags))
+
+
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if "--flt" in sys.argv:
+        for name in ("syn10", "mempool"):
+            make_flt(name)
+        return
     import grotor_amd as ga
     d = ga.Design.generate(7, seed=1)
     d.write("/tmp/golden_syn10_src.cap", "/tmp/golden_syn10_src.net")
