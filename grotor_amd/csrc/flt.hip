@@ -161,6 +161,10 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
         }
         lv = best;
         d.layer[v] = lv;
+        if (!lv) {  // no finite cost at any layer: nothing to trace
+            raise(a.error, kDevNoSolution);
+            return 0;
+        }
     } else {
         int par_layer = -1, par = -1, par_edge = -1;
         lv = -1;
@@ -330,13 +334,19 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     const int id = a.net_ids[k];
     int* routes = a.routes.routes + (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin;
     const long long cap = (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin;
+    const int g_cells = a.g.L * a.g.XY;
     int count = 1;  // the root's trace sets the word count to 1
     for (int dep = 0; dep < dn.w; ++dep) {
         const int first = doff[dep], end = doff[dep + 1];
         for (int i0 = first; i0 < end; i0 += kWave) {
             int w[8];
             const int i = i0 + lane;
-            const int n = i < end ? trace_node(a, d, order[i], w) : 0;
+            int n = i < end ? trace_node(a, d, order[i], w) : 0;
+            for (int j = 0; j < n; ++j)
+                if (w[j] < 0 || w[j] >= g_cells) {
+                    raise(a.error, kDevBadCoordinate);
+                    n = 0;
+                }
             const int incl = wave_inclusive_sum(n, lane);
             const int at = count + incl - n;
             if (n) {

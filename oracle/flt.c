@@ -229,6 +229,10 @@ static void flt_trace(flt_ctx *c, int v, int *routes, int cap) {
             }
         lo[v] = best;
         routes[0] = 1;
+        if (!best) { /* no finite cost at any layer: nothing to trace */
+            c->error = 2;
+            return;
+        }
     } else {
         int par_layer = -1, par_idx = -1, par_seq = -1;
         for (int p = 0; p < parents; ++p) {
