@@ -56,8 +56,9 @@ def parse():
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
-    ap.add_argument("--stage", choices=("lshape", "zshape", "soft"), default="lshape",
-                    help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd")
+    ap.add_argument("--stage", choices=("lshape", "zshape", "soft", "flt"), default="lshape",
+                    help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd; "
+                         "flt: the FLT rip-up stage after the L-shape stage")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
     return ap.parse_args()
@@ -116,6 +117,8 @@ def main():
         return bench_zshape(args, ga, design, topo)
     if args.stage == "soft":
         return bench_soft(args, ga, design, topo)
+    if args.stage == "flt":
+        return bench_flt(args, ga, design, topo)
     ctx = ga.Context(topo)
     ctx.set_routes(topo["pin_acc_num"])
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
@@ -246,6 +249,136 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def bench_flt(args, ga, design, topo):
+    """FLT rip-up stage (src/routing/flt.cuh:75-182): after the L-shape
+    stage, rip-up and the congestion view, GPU batching of the rerouted
+    nets (limit 300000) and the FLT batch loop (phase-one removal, cost
+    refresh, layer-change search, DAG DP, phase-two commit).  The L-shape
+    stage that rebuilds the phase-one state before every step is untimed."""
+    import numpy as np
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"])
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+    nsub = len(topo["pin_acc_num"]) - 1
+
+    def phase_one():
+        ctx.reset()
+        ctx.batch_geometry(*geom)
+        color, nb = ctx.batch_colors()
+        sched = ga.execution_schedule(topo, color, nb)
+        ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+        ctx.lshape_run(sched["batch_off"], first_stamp=1)
+        return nb
+
+    nb1 = phase_one()
+    flags = ctx.ripup(nsub)
+    cong, xs, ys = ctx.congestion_view()
+    t0 = time.perf_counter()
+    fs = design.flt_stage(flags, xs, ys, threads=args.threads)
+    topo_ms = (time.perf_counter() - t0) * 1e3
+    fgeom = (fs["pt_off"], fs["pts"], fs["hs_off"], fs["hsegs"], fs["vs_off"], fs["vsegs"])
+    ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
+    nets = fs["nets"]
+    print(f"[bench] flt: {nets} rerouted nets, {fs['nodes']} DAG nodes, {fs['edges']} DAG edges, "
+          f"DAG build {topo_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    def step(profile_kinds=False):
+        phase_one()
+        ctx.batch_geometry(*fgeom)
+        if profile_kinds is not False:
+            ctx.profile_select(profile_kinds)
+            ctx.profile_begin()
+        b0 = time.perf_counter()
+        color, nb = ctx.batch_colors(limit=300000)
+        batch_ms = (time.perf_counter() - b0) * 1e3
+        u0 = time.perf_counter()
+        plan = design.flt_plan(color, nb)  # flt::Plans, outside the reference's DAG interval too
+        ctx.flt_upload(plan)
+        upload_ms = (time.perf_counter() - u0) * 1e3
+        dag_ms = ctx.flt_run(first_stamp=nb1 + 1, remove_phase_one=True)
+        prof = ctx.profile_end() if profile_kinds is not False else None
+        return batch_ms, dag_ms, upload_ms, plan, prof
+
+    breakdown = None
+    for i in range(args.warmup):
+        last = i == args.warmup - 1
+        bm, dm, um, plan, prof = step(None if last else False)
+        if last:
+            breakdown = prof
+        print(f"[bench] flt warmup {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms (+ plan and upload {um:.2f} ms), "
+              f"{len(plan['batch_off']) - 1} batches", file=sys.stderr, flush=True)
+    dom_kind = None
+    if breakdown is not None:
+        dom_kind = max(range(len(breakdown["ms"])), key=lambda k: breakdown["ms"][k])
+    bt = dt = ut = 0.0
+    prof_sum = None
+    for i in range(args.steps):
+        bm, dm, um, plan, prof = step(False if args.no_profile or dom_kind is None else [dom_kind])
+        bt += bm
+        dt += dm
+        ut += um
+        if prof is not None:
+            prof_sum = prof if prof_sum is None else {k: [a + b for a, b in zip(prof_sum[k], prof[k])]
+                                                      for k in prof}
+        print(f"[bench] flt step {i}: batching {bm:.2f} ms, DAG {dm:.2f} ms (+ plan and upload {um:.2f} ms)",
+              file=sys.stderr, flush=True)
+    routes2 = ctx.routes2()
+    wl, vias = ctx.stats()
+    if prof_sum is None:
+        _, _, _, _, prof_sum = step(None if dom_kind is None else [dom_kind])
+    if breakdown is None:
+        breakdown = prof_sum
+    dom = max(range(len(prof_sum["ms"])), key=lambda k: prof_sum["ms"][k])
+    dom_ms, dom_launches = prof_sum["ms"][dom], prof_sum["launches"][dom]
+    achieved = prof_sum["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(KIND_NAMES[dom], args.config),
+                "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
+                "launches": int(dom_launches), "alg_bytes_per_launch": round(prof_sum["bytes"][dom] / max(dom_launches, 1), 1)}
+    kernels = {KIND_NAMES[k]: {"ms": round(breakdown["ms"][k], 4), "launches": int(breakdown["launches"][k]),
+                               "GBs": round(breakdown["bytes"][k] / (breakdown["ms"][k] / 1e3) / 1e9, 2)
+                               if breakdown["ms"][k] else 0}
+               for k in range(len(breakdown["ms"])) if breakdown["launches"][k]}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        # the oracle continues from the GPU's phase-one state (bit-identical to its own)
+        phase_one()
+        routes1, demand1, ts1 = ctx.routes(), ctx.demand(), ctx.timestamps()
+        wl1, vias1 = ctx.stats()
+        c0 = time.perf_counter()
+        ocolor, onb = ora.batches(topo["X"], topo["Y"], *fgeom, limit=300000)
+        oplan = design.flt_plan(ocolor, onb)
+        cb = time.perf_counter() - c0
+        c1 = time.perf_counter()
+        o2, _, _, owl, ovias, rc, _ = ora.Grid(topo).flt_stage(oplan, topo["pin_acc_num"], fs["pin_off2"], fs["pins2"],
+                                                              demand1, ts1, routes1, nb1 + 1, wl=wl1, vias=vias1,
+                                                              threads=args.threads)
+        cd = time.perf_counter() - c1
+        same = bool(rc == 0 and np.array_equal(o2, routes2) and owl == wl and ovias == vias)
+        cpu = {"value": round(nets / (cb + cd), 1), "unit": "nets/s", "cores": 1, "kind": "port",
+               "sample": f"whole FLT stage: {nets} rerouted nets; CPU oracle batching ({cb * 1e3:.1f} ms) + "
+                         f"sequential FLT loop ({cd * 1e3:.1f} ms, cost refresh on {args.threads} threads)",
+               "routes_identical_to_gpu": same}
+        print(f"[bench] flt cpu: batching {cb * 1e3:.1f} ms, DAG {cd * 1e3:.1f} ms, identical={same}",
+              file=sys.stderr, flush=True)
+    ms = (bt + dt) / args.steps
+    print(json.dumps({
+        "metric": "routed nets/s (batching+DAG) at equal evaluator cost",
+        "value": round(nets / (ms / 1e3), 1), "unit": "nets/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} FLT rip-up stage ({CONFIG_TEXT.get(args.config, '')}): "
+                               f"{nets} rerouted nets after the L-shape stage; GPU batching + FLT batch loop",
+                   "rerouted_nets": nets, "dag_nodes": fs["nodes"], "dag_edges": fs["edges"],
+                   "batches": len(plan["batch_off"]) - 1, "batching_ms_per_step": round(bt / args.steps, 4),
+                   "dag_ms_per_step": round(dt / args.steps, 4), "plan_upload_ms_per_step": round(ut / args.steps, 4),
+                   "dag_build_ms": round(topo_ms, 2), "total_wirelength": wl, "total_vias": vias,
+                   "parallelism": "single GPU", "kernel_breakdown": {"steps": 1, "classes": kernels}},
+        "roofline": roofline, "cpu_baseline": cpu}), flush=True)
     ctx.close()
 
 

@@ -23,7 +23,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgrotor_amd.so")
+LIB_PATH = os.environ.get("GROTOR_AMD_LIB") or os.path.join(_HERE, "libgrotor_amd.so")
 ROUTE_PER_PIN = 12
 
 GR_OK, GR_E_ARG, GR_E_HIP, GR_E_CAPACITY, GR_E_STATE, GR_E_DEVICE, GR_E_IO = range(7)
@@ -243,9 +243,16 @@ class Design:
                     nodes=take(p.nodes, 4 * int(node_off[-1])), edge_off=edge_off,
                     edges=take(p.edges, 2 * int(edge_off[-1])))
 
-    def write_output(self, routes, path):
+    def write_output(self, routes, path, routes2=None, pin_off2=None, words_per_pin2=100):
+        """.out solution; with routes2/pin_off2 the rerouted subnets print
+        their phase-two words."""
         routes = _i32(routes)
-        _call("gr_design_write_output", self._h, _ptr(routes, ctypes.c_int), path.encode())
+        if routes2 is None:
+            _call("gr_design_write_output", self._h, _ptr(routes, ctypes.c_int), path.encode())
+            return
+        routes2, pin_off2 = _i32(routes2), _i32(pin_off2)
+        _call("gr_design_write_output2", self._h, _ptr(routes, ctypes.c_int), _ptr(routes2, ctypes.c_int),
+              _ptr(pin_off2, ctypes.c_int), ctypes.c_int(words_per_pin2), path.encode())
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -157,6 +157,7 @@ struct gr_ctx {
     int f_batches = 0;
     std::vector<int> f_batch_off;
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
+    std::vector<double> f_edge_bytes, f_route_bytes;  // algorithmic bytes per batch launch
     // congestion view (2D)
     unsigned char* cview = nullptr;
     float *cxsum = nullptr, *cysum = nullptr;
@@ -810,6 +811,8 @@ int gr_ctx_reset(gr_ctx* c) {
         hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats) * kStatStripes, c->stream), "clear statistics");
         if (c->routes)
             hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
+        if (c->routes2)
+            hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
         std::vector<unsigned> ones(c->tracks, 1u);
         upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "gr_ctx_reset");
@@ -1152,6 +1155,28 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             max_slots = std::max(max_slots, slots_b);
             c->f_base[b] = make_int4(desc[k0].x, desc[k0].y, desc[k0].z, int(edges_b));
         }
+        // Algorithmic bytes per launch, the FLT unit cost of one DAG: per
+        // edge 12 * L * (len + 1) for the layer-change search (presum and via
+        // cost gathers along the edge on every layer) and 8 for the edge
+        // record in the DP; per node 16 + 8 * L (record, layer costs).  The
+        // 81-wide per-edge tables are scratch and not counted.
+        const int L = g.L;
+        c->f_edge_bytes.assign(nbatch, 0.0);
+        c->f_route_bytes.assign(nbatch, 0.0);
+        for (int b = 0; b < nbatch; ++b) {
+            double eb = 0, rb = 0;
+            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k) {
+                const int4 ds = desc[k], dn = sizes[k];
+                rb += dn.x * (16.0 + 8.0 * L) + 8.0 * dn.y;
+                for (int e = 0; e < dn.y; ++e) {
+                    const int4 er = erec[ds.y + e];
+                    const int len = std::abs(er.z / g.Y % g.X - er.w / g.Y % g.X) + std::abs(er.z % g.Y - er.w % g.Y);
+                    eb += 12.0 * L * (len + 1);
+                }
+            }
+            c->f_edge_bytes[b] = eb;
+            c->f_route_bytes[b] = rb;
+        }
         int4 *d_nrec, *d_erec, *d_desc, *d_sizes;
         int *d_order, *d_doff, *d_ids;
         c->f_ws.carve([&](auto& cv) {
@@ -1225,12 +1250,12 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed
             a.slot_base = c->f_base[b].z;
             a.edges = c->f_base[b].w;
             const long long threads = (long long)a.edges * c->g.L;
-            timed(c, GR_K_FLT_EDGE, 0.0, [&] {
+            timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] {
                 if (threads)
                     hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
             });
-            timed(c, GR_K_FLT, 0.0, [&] {
-                hipLaunchKernelGGL(flt_route_kernel, dim3((count + 3) / 4), dim3(256), 0, c->stream, a);
+            timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
+                launch_flt_route(a, (count + 3) / 4, c->stream);
             });
             hip_check(hipGetLastError(), "FLT kernels");
             launch_commit(c, 0, count, s, 1, ids, &rs2);

@@ -6,8 +6,8 @@
 // the DAG's tables, solves its depths bottom-up (children push packed
 // (cost, node, choice) keys into their parents' child slots with 64-bit
 // atomic min) and traces them top-down.  Between depths the wavefront waits
-// for its own stores and atomics and invalidates the CU's L1, so every lane
-// reads the previous depth's results from L2.
+// for its own stores and atomics, and lanes read what other lanes wrote with
+// loads that bypass L1.
 //
 // Segments are appended in node order within a depth (a wavefront prefix
 // sum), so the route words are deterministic.
@@ -38,11 +38,18 @@ __device__ __forceinline__ double wire(const FltArgs& a, int l, int xmin, int xm
     return a.presum[cell_of(a.g, l, xmax, ymax)] - a.presum[cell_of(a.g, l, xmin, ymin)];
 }
 
-// Previous depth's stores and atomics complete, then L1 invalidated.
+// Between depths: every store and atomic of this wavefront has reached L2.
+// Values another lane wrote (child slots lowered by atomics, best paths,
+// layer outputs) are then read with L1-bypassing loads (l2_load), so no
+// cache invalidation is needed.
 __device__ __forceinline__ void depth_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __builtin_amdgcn_wave_barrier();
+}
+
+template <class T>
+__device__ __forceinline__ T l2_load(const T* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ void raise(int* error, int code) { atomicCAS(error, 0, code); }
@@ -59,7 +66,8 @@ struct Dag {
     const int* best_change;
 };
 
-// solve_flt_node (src/routing/flt_block.cuh:6-142)
+// solve_flt_node (src/routing/flt_block.cuh:6-142), tables in global memory
+// (any number of child slots).
 __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
@@ -71,6 +79,7 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
     unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
     const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
     double* cv = d.costs + (size_t)v * kS;
+    for (int li = 0; li < kS; ++li) cv[li] = kInf;
     double mins[6];
     unsigned long long paths[6];
     for (int low = 1; low <= low_fixed; ++low) {
@@ -79,7 +88,7 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
         for (int li = low; li < L + 1; ++li) {
             if (li > low) via += (double)a.vcost[cell_of(g, li - 2, x, y)];
             for (int k = 0; k < kids; ++k) {
-                const unsigned long long p = pk[k * kS + li];
+                const unsigned long long p = l2_load(pk + k * kS + li);
                 const double c = unpack_cost(p);
                 if (c < mins[k]) {
                     mins[k] = c;
@@ -139,6 +148,135 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
     }
 }
 
+// solve_flt_node for nodes of at most kFastKids child slots with the
+// node's tables in registers: the slot values and via costs are loaded
+// once (one memory latency instead of one per inner step) and the layer
+// costs live in registers.  Instead of copying best-path rows, the passes
+// keep, per layer, the slot positions (4 bits per child slot) of the
+// running first minima that produced the layer's cost; the best-path rows
+// are written once at the end from those positions, which is what the
+// reference's rows hold after its copies.
+constexpr int kFastKids = 4;
+
+template <int L>
+__device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int4 r) {
+    const DeviceGrid& g = a.g;
+    const int kids = r.w & 7;
+    const int low_fixed = 1 + r.x / g.XY;
+    const int x = r.x / g.Y % g.X, y = r.x % g.Y;
+    const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
+    const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
+    float pv[kFastKids][L + 1];  // slot costs (the packed key's float)
+#pragma unroll
+    for (int k = 0; k < kFastKids; ++k)
+#pragma unroll
+        for (int li = 1; li <= L; ++li) pv[k][li] = k < kids ? (float)unpack_cost(l2_load(pk + k * kS + li)) : 0.f;
+    float vc[L];
+#pragma unroll
+    for (int j = 0; j + 1 < L; ++j) vc[j] = a.vcost[cell_of(g, j, x, y)];
+    double cv[L + 1];
+    unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
+#pragma unroll
+    for (int li = 0; li <= L; ++li) cv[li] = kInf, at[li] = 0;
+#pragma unroll
+    for (int low = 1; low <= L; ++low) {
+        if (low <= low_fixed) {
+            double mins[kFastKids];
+#pragma unroll
+            for (int k = 0; k < kFastKids; ++k) mins[k] = kInf;
+            unsigned run = 0;
+            double via = 0.0;
+#pragma unroll
+            for (int li = low; li <= L; ++li) {
+                if (li > low) via += (double)vc[li - 2];
+#pragma unroll
+                for (int k = 0; k < kFastKids; ++k)
+                    if (k < kids) {
+                        const double c = (double)pv[k][li];
+                        if (c < mins[k]) {
+                            mins[k] = c;
+                            run = (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
+                        }
+                    }
+                if (li >= high_fixed) {
+                    double cost = via;
+#pragma unroll
+                    for (int k = 0; k < kFastKids; ++k)
+                        if (k < kids) cost += mins[k];
+                    if (cost < kInf && cost < cv[li]) {
+                        cv[li] = cost;
+                        at[li] = run;
+                    }
+                }
+            }
+#pragma unroll
+            for (int li = L - 1; li >= low; --li)
+                if (cv[li + 1] < cv[li]) {
+                    cv[li] = cv[li + 1];
+                    at[li] = at[li + 1];
+                }
+        }
+    }
+    unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
+#pragma unroll
+    for (int li = 1; li <= L; ++li) {
+        if (!(cv[li] < kInf)) continue;
+#pragma unroll
+        for (int k = 0; k < kFastKids; ++k) {
+            if (k >= kids) continue;
+            const int j = (at[li] >> (4 * k)) & 15;
+            bp[k * kS + li] = (l2_load(pk + k * kS + j) & 0xFFFFFFFFULL) * kS + (unsigned long long)j;
+        }
+    }
+    if (r.y == r.z) {  // the root's trace reads its layer costs
+        double* cg = d.costs + (size_t)v * kS;
+        cg[0] = kInf;
+#pragma unroll
+        for (int li = 1; li <= L; ++li) cg[li] = cv[li];
+    }
+    for (int e = r.y; e < r.z; ++e) {
+        const int4 er = d.edges[e];
+        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
+        const int px = er.z / g.Y % g.X, py = er.z % g.Y;
+        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+        const bool far = abs(x - px) + abs(y - py) > 4;
+        unsigned long long cache[L + 1];
+#pragma unroll
+        for (int t = 1; t <= L; ++t) cache[t] = kEmpty;
+#pragma unroll
+        for (int layer = 1; layer <= L; ++layer) {
+            if (((layer & 1) ^ g.DIR) == 1 && y != py) continue;
+            if (((layer & 1) ^ g.DIR) == 0 && x != px) continue;
+            const double base = cv[layer];
+            if (!(base < kInf)) {  // the reference asserts here
+                raise(a.error, kDevNoSolution);
+                continue;
+            }
+#pragma unroll
+            for (int t = 1 + (1 - layer % 2); t <= L; t += 2) {
+                double cost2;
+                int choice = kStraight;
+                if (t == layer) {
+                    cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
+                } else if (far) {
+                    const int seq = (layer - 1) * 9 + (t - 1);
+                    const double ce = d.cost_edges[(size_t)e * 81 + seq];
+                    if (!(ce < kInf)) continue;
+                    cost2 = base + ce;
+                    choice = seq;
+                } else {
+                    continue;
+                }
+                const unsigned long long nv = pack((float)cost2, v * 100 + choice);
+                if (nv < cache[t]) cache[t] = nv;
+            }
+        }
+#pragma unroll
+        for (int t = 1; t <= L; ++t)
+            if (cache[t] < kEmpty) atomicMin(ppk + t, cache[t]);
+    }
+}
+
 // trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
 // segments of the edge its chosen parent uses and its via stack, into w[].
 __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
@@ -170,12 +308,12 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
         lv = -1;
         for (int e = r.y; e < r.z; ++e) {
             const int4 er = d.edges[e];
-            const int pl = d.layer[er.x];
+            const int pl = l2_load(d.layer + er.x);
             if (pl <= 0) continue;
             par = er.x;
             par_edge = e;
             par_layer = pl;
-            const unsigned long long path = d.best[(size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer];
+            const unsigned long long path = l2_load(d.best + (size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer);
             const unsigned long long child = path / kS / 100, choice = (path / kS) % 100;
             if (child != (unsigned long long)v) {
                 lv = -1;
@@ -221,7 +359,7 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
             return 0;
         }
         const int4 er = d.edges[par_edge];
-        const unsigned long long path = d.best[(size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer];
+        const unsigned long long path = l2_load(d.best + (size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer);
         if (path / kS / 100 != (unsigned long long)v) {
             d.layer[v] = -1;
             return 0;
@@ -234,7 +372,7 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
     minl = min(minl, lv);
     maxl = max(maxl, lv);
     for (int k = 0; k < kids; ++k) {
-        const int lc = (int)(bp[k * kS + lv] % kS);
+        const int lc = (int)(l2_load(bp + k * kS + lv) % kS);
         minl = min(minl, lc);
         maxl = max(maxl, lc);
     }
@@ -303,6 +441,7 @@ __global__ void flt_edge_kernel(FltArgs a) {
 
 // route_flt_block (src/routing/flt_block.cuh:334-355) with one wavefront
 // per DAG and four DAGs per workgroup.
+template <int L>
 __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     const int k = blockIdx.x * (256 / kWave) + (threadIdx.x / kWave);
@@ -321,14 +460,22 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     const int* order = a.order + ds.x;
     const int* doff = a.depth_off + ds.w;
 
-    // initialize_batch (src/routing/flt_workspace.cuh:60-69) for this DAG
-    for (int i = lane; i < dn.x * kS; i += kWave) d.costs[i] = kInf;
+    // initialize_batch (src/routing/flt_workspace.cuh:60-69) for this DAG:
+    // only the child slots need it here (a node's layer costs start at
+    // infinity in its solve, and every node's trace sets its layer output
+    // before any child reads it)
     for (int i = lane; i < dn.z * kS; i += kWave) d.packed[i] = kEmpty;
-    for (int i = lane; i < dn.x; i += kWave) d.layer[i] = -1;
     depth_barrier();
     for (int dep = dn.w - 1; dep >= 0; --dep) {
         const int end = doff[dep + 1];
-        for (int i = doff[dep] + lane; i < end; i += kWave) solve_node(a, d, order[i]);
+        for (int i = doff[dep] + lane; i < end; i += kWave) {
+            const int v = order[i];
+            const int4 r = d.nodes[v];
+            if ((r.w & 7) <= kFastKids)
+                solve_node_fast<L>(a, d, v, r);
+            else
+                solve_node(a, d, v);
+        }
         depth_barrier();
     }
     const int id = a.net_ids[k];
@@ -361,6 +508,24 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
         depth_barrier();
     }
     if (lane == 0) routes[0] = count <= cap ? count : 1;
+}
+
+void launch_flt_route(const FltArgs& a, int blocks, hipStream_t stream) {
+    switch (a.g.L) {
+#define GR_FLT_CASE(n) \
+    case n: hipLaunchKernelGGL(flt_route_kernel<n>, dim3(blocks), dim3(256), 0, stream, a); break;
+        GR_FLT_CASE(1)
+        GR_FLT_CASE(2)
+        GR_FLT_CASE(3)
+        GR_FLT_CASE(4)
+        GR_FLT_CASE(5)
+        GR_FLT_CASE(6)
+        GR_FLT_CASE(7)
+        GR_FLT_CASE(8)
+        GR_FLT_CASE(9)
+#undef GR_FLT_CASE
+        default: break;
+    }
 }
 
 }  // namespace grotor

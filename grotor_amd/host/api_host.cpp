@@ -393,6 +393,25 @@ int gr_flt_dag(int X, int Y, int L, int mode, int n, const int* positions, const
     });
 }
 
+int gr_design_write_output2(const gr_design* g, const int* routes, const int* routes2, const int* pin_off2,
+                            int words_per_pin2, const char* out_path) {
+    return guarded([&] {
+        if (!g || !routes || !out_path || (pin_off2 && (!routes2 || words_per_pin2 < 1)))
+            throw GrError(GR_E_ARG, "invalid arguments");
+        if (!g->prepared) throw GrError(GR_E_STATE, "design not prepared");
+        const auto& acc = g->d.pin_acc_num;
+        // src/output.hpp:98-106: rerouted subnets print their phase-two words
+        const std::string text = format_output(g->d, [&](int id) {
+            if (pin_off2 && pin_off2[id] >= 0) return routes2 + (size_t)pin_off2[id] * words_per_pin2;
+            return routes + (size_t)acc[id] * 12;
+        });
+        FILE* f = fopen(out_path, "w");
+        if (!f) throw GrError(GR_E_IO, std::string("cannot open output: ") + out_path);
+        const size_t written = fwrite(text.data(), 1, text.size(), f);
+        if (fclose(f) || written != text.size()) throw GrError(GR_E_IO, "failed to write routing output");
+    });
+}
+
 int gr_design_write_output(const gr_design* g, const int* routes, const char* out_path) {
     return guarded([&] {
         if (!g || !routes || !out_path) throw GrError(GR_E_ARG, "null argument");

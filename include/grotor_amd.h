@@ -411,6 +411,12 @@ int gr_design_zshape_finish(gr_design *design, int *routes, int words_per_pin, i
 /* Write the .out solution for the prepared design given its route words
  * (graph::output_nets, src/output.hpp:79-112). */
 int gr_design_write_output(const gr_design *design, const int *routes, const char *out_path);
+/* As gr_design_write_output after an FLT stage: subnets with pin_off2[id]
+ * >= 0 print their phase-two words (routes2 at pin_off2[id] *
+ * words_per_pin2), the others their phase-one words (graph::output_nets,
+ * src/output.hpp:98-106). */
+int gr_design_write_output2(const gr_design *design, const int *routes, const int *routes2, const int *pin_off2,
+                            int words_per_pin2, const char *out_path);
 
 #ifdef __cplusplus
 }
