@@ -123,6 +123,18 @@ def main():
     ctx.set_routes(topo["pin_acc_num"])
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
 
+    # autotuner: nets per wavefront of the DP launches, once, before any step
+    ctx.batch_geometry(*geom)
+    tcolor, tnb = ctx.batch_colors()
+    tsched = ga.execution_schedule(topo, tcolor, tnb)
+    ctx.lshape_upload(tsched["net_ids"], tsched["node_off"], tsched["nodes"], tsched["parents"])
+    t0 = time.perf_counter()
+    per_wave, tune_ms = ctx.lshape_tune(tsched["batch_off"])
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] autotuner: nets per wavefront {per_wave} (classes <=16 nodes, larger); "
+          f"DP ms per candidate 1/2/4/8: {[[round(v, 2) for v in row] for row in tune_ms]} ({tune_s:.2f} s)",
+          file=sys.stderr, flush=True)
+
     def step(mode=args.mode):
         ctx.reset()
         u0 = time.perf_counter()
@@ -240,6 +252,9 @@ def main():
         "config": {"workload": f"configuration {args.config} L-shape stage ({CONFIG_TEXT.get(args.config, '')}, "
                                f"{nets} routed subnets): GPU batching + L-shape batch loop",
                    "routed_subnets": nets, "batches": nb, "mode": args.mode,
+                   "nets_per_wavefront": {"classes": ["<=16 nodes", "larger"],
+                                          "chosen": per_wave,
+                                          "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in tune_ms]},
                    "batching_ms_per_step": round(batch_total / args.steps, 4),
                    "geometry_upload_ms_per_step": round(upload_total / args.steps, 4),
                    "dag_ms_per_step": round(dag_total / args.steps, 4),

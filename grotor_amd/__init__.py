@@ -428,6 +428,21 @@ class Context:
               ctypes.c_int(first_stamp), ctypes.c_int(mode), ctypes.byref(ms))
         return ms.value
 
+    def set_nets_per_wave(self, per_wave):
+        """L-shape DP nets per wavefront for the tree size classes (<=16 nodes, larger)."""
+        pw = _i32(per_wave)
+        _call("gr_ctx_set_nets_per_wave", self._h, _ptr(pw, ctypes.c_int))
+
+    def lshape_tune(self, batch_off):
+        """Autotune nets per wavefront over the uploaded stage's batches:
+        (chosen per class, ms[class][candidate 1, 2, 4, 8] with -1 where not tried)."""
+        batch_off = _i32(batch_off)
+        pw = np.zeros(2, np.int32)
+        ms = np.zeros(8, np.float64)
+        _call("gr_lshape_tune", self._h, ctypes.c_int(len(batch_off) - 1), _ptr(batch_off, ctypes.c_int),
+              _ptr(pw, ctypes.c_int), _ptr(ms, ctypes.c_double))
+        return pw.tolist(), ms.reshape(2, 4).tolist()
+
     def routes(self):
         r = np.empty(self.route_words, np.int32)
         _call("gr_routes_download", self._h, _ptr(r, ctypes.c_int), ctypes.c_longlong(self.route_words))

@@ -161,6 +161,7 @@ struct gr_ctx {
     // congestion view (2D)
     unsigned char* cview = nullptr;
     float *cxsum = nullptr, *cysum = nullptr;
+    int per_wave[kTreeClasses] = {1, 1};  // L-shape DP nets per wavefront by tree size class (gr_lshape_tune)
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 16;
     // kernel profiling (HIP events on the context stream)
@@ -328,27 +329,36 @@ void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int*
     hip_check(hipGetLastError(), "commit kernels");
 }
 
-// Per batch: stage slots reordered so that small trees come first, and the
-// node capacity needed by each of the two groups.
+// Per batch: stage slots grouped by tree size class (up to kSmallTree nodes,
+// larger), in stage order inside a class, and the node capacity each class
+// needs.  Each class is one DP launch with its own nets per wavefront.
+// (Finer classes were measured slower: every extra launch per batch costs
+// more in launch and tail time than the smaller tables save.)
+constexpr int kClassNodes[kTreeClasses - 1] = {kSmallTree};
+
 struct BatchLaunch {
-    int begin, small, large, nmax_small, nmax_large;
+    int begin;
+    int count[kTreeClasses];
+    int nmax[kTreeClasses];
 };
 
+int tree_class(int n) {
+    int k = 0;
+    while (k < kTreeClasses - 1 && n > kClassNodes[k]) ++k;
+    return k;
+}
+
 BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots) {
-    BatchLaunch b{begin, 0, 0, 1, 1};
-    std::vector<int> big;
+    BatchLaunch b{begin, {0, 0}, {1, 1}};
     for (int s = begin; s < begin + count; ++s) {
-        const int n = c->node_off_h[s + 1] - c->node_off_h[s];
-        if (n <= kSmallTree) {
-            slots[begin + b.small++] = s;
-            b.nmax_small = std::max(b.nmax_small, n);
-        } else {
-            big.push_back(s);
-            b.nmax_large = std::max(b.nmax_large, n);
-        }
+        const int n = c->node_off_h[s + 1] - c->node_off_h[s], k = tree_class(n);
+        ++b.count[k];
+        b.nmax[k] = std::max(b.nmax[k], n);
     }
-    b.large = int(big.size());
-    for (int i = 0; i < b.large; ++i) slots[begin + b.small + i] = big[i];
+    int at[kTreeClasses];
+    at[0] = begin;
+    for (int k = 1; k < kTreeClasses; ++k) at[k] = at[k - 1] + b.count[k - 1];
+    for (int s = begin; s < begin + count; ++s) slots[at[tree_class(c->node_off_h[s + 1] - c->node_off_h[s])]++] = s;
     return b;
 }
 
@@ -361,7 +371,19 @@ void ensure_gscratch(gr_ctx* c, size_t bytes) {
     c->gscratch_bytes = bytes;
 }
 
-void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
+// Nets per wavefront for a class launch: the tuned value, halved until the
+// workgroup's tables fit in LDS; 0 when one net's tables do not fit (the
+// global-scratch kernel then runs with one net per wavefront).
+int per_wave_for(gr_ctx* c, int k, int nmax) {
+    const size_t bytes = (lshape_scratch_bytes(c->g.L, nmax) + 7) / 8 * 8;
+    if (bytes > kLdsLimit) return 0;
+    int p = c->per_wave[k];
+    while (p > 1 && bytes * p > kLdsLimit) p /= 2;
+    return p;
+}
+
+void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_override = 0) {
+    if (!b.count[k]) return;
     LshapeArgs a;
     a.g = c->g;
     a.vcost = c->vcost;
@@ -372,51 +394,40 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
     a.parents = c->parents;
     a.gscratch = c->gscratch;
     a.error = c->error;
-    auto bytes_of = [&](int first, int count) {
-        double sum = 0;
-        if (c->profiling)
-            for (int i = first; i < first + count; ++i) sum += c->slot_dp_bytes[c->slots_h[i]];
-        return sum;
-    };
-    if (b.small) {
-        a.begin = b.begin;
-        a.count = b.small;
-        a.nmax = b.nmax_small;
-        timed(c, GR_K_LSHAPE, bytes_of(b.begin, b.small), [&] {
-            launch_lshape_dp(a, b.small, true, c->stream);
-        });
-    }
-    if (b.large) {
-        a.begin = b.begin + b.small;
-        a.count = b.large;
-        a.nmax = b.nmax_large;
+    a.begin = b.begin;
+    for (int j = 0; j < k; ++j) a.begin += b.count[j];
+    a.count = b.count[k];
+    a.nmax = b.nmax[k];
+    double alg = 0;
+    if (c->profiling)
+        for (int i = a.begin; i < a.begin + a.count; ++i) alg += c->slot_dp_bytes[c->slots_h[i]];
+    const int p = per_wave_override ? per_wave_override : per_wave_for(c, k, a.nmax);
+    if (p > 0) {
+        a.per_wave = p;
+        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, c->stream); });
+    } else {
         const size_t bytes = lshape_scratch_bytes(c->g.L, a.nmax);
-        const double alg = bytes_of(b.begin + b.small, b.large);
-        if (bytes <= kLdsLimit) {
-            timed(c, GR_K_LSHAPE, alg, [&] {
-                launch_lshape_dp(a, b.large, true, c->stream);
-            });
-        } else {
-            if (((bytes + 7) / 8) * 8 * size_t(b.large) > c->gscratch_bytes)
-                throw GrError(GR_E_STATE, "DP scratch not reserved");
-            a.gscratch = c->gscratch;
-            timed(c, GR_K_LSHAPE, alg, [&] {
-                launch_lshape_dp(a, b.large, false, c->stream);
-            });
-        }
+        if (((bytes + 7) / 8) * 8 * size_t(a.count) > c->gscratch_bytes) throw GrError(GR_E_STATE, "DP scratch not reserved");
+        a.per_wave = 1;
+        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, a.count, false, c->stream); });
     }
     hip_check(hipGetLastError(), "lshape_dp kernels");
+}
+
+void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
+    for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k);
 }
 
 // Reserve the global DP scratch the batches will need (trees whose tables
 // do not fit in LDS) before anything is launched or captured.
 void reserve_dp_scratch(gr_ctx* c, const std::vector<BatchLaunch>& plan) {
     size_t need = 0;
-    for (const BatchLaunch& b : plan) {
-        if (!b.large) continue;
-        const size_t bytes = lshape_scratch_bytes(c->g.L, b.nmax_large);
-        if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.large));
-    }
+    for (const BatchLaunch& b : plan)
+        for (int k = 0; k < kTreeClasses; ++k) {
+            if (!b.count[k]) continue;
+            const size_t bytes = lshape_scratch_bytes(c->g.L, b.nmax[k]);
+            if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.count[k]));
+        }
     ensure_gscratch(c, need);
 }
 
@@ -755,7 +766,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
             }
         };
         if (mode == 1) {
-            std::vector<int> key{c->stage_gen, first_stamp, nbatch};
+            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1]};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
             // the graph reads the descriptors when it runs
             c->slots_h = slots;
@@ -793,6 +804,76 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         const auto t1 = std::chrono::steady_clock::now();
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_lshape_run");
+    });
+}
+
+int gr_ctx_set_nets_per_wave(gr_ctx* c, const int* per_wave) {
+    return guarded([&] {
+        if (!c || !per_wave) throw GrError(GR_E_ARG, "null argument");
+        for (int k = 0; k < kTreeClasses; ++k)
+            if (per_wave[k] != 1 && per_wave[k] != 2 && per_wave[k] != 4 && per_wave[k] != 8)
+                throw GrError(GR_E_ARG, "nets per wavefront must be 1, 2, 4 or 8");
+        std::copy(per_wave, per_wave + kTreeClasses, c->per_wave);
+    });
+}
+
+int gr_lshape_tune(gr_ctx* c, int nbatch, const int* batch_off, int* per_wave, double* ms) {
+    return guarded([&] {
+        if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
+        require_stage(c, 0, batch_off[nbatch]);
+        if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
+        std::vector<int> slots(c->stage_count);
+        std::vector<BatchLaunch> plan(nbatch);
+        for (int b = 0; b < nbatch; ++b) {
+            if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
+            plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+        }
+        reserve_dp_scratch(c, plan);
+        c->slots_h = slots;
+        upload_desc(c, slots, 0, c->stage_count);
+        launch_refresh(c, 0);
+        const bool profiling = c->profiling;
+        c->profiling = false;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int candidates[4] = {1, 2, 4, 8};
+        for (int k = 0; k < kTreeClasses; ++k) {
+            int nmax = 1;
+            bool any = false;
+            for (const BatchLaunch& b : plan)
+                if (b.count[k]) any = true, nmax = std::max(nmax, b.nmax[k]);
+            double best = 1e30;
+            int choice = c->per_wave[k];
+            for (int ci = 0; ci < 4; ++ci) {
+                const int p = candidates[ci];
+                const size_t bytes = (lshape_scratch_bytes(c->g.L, nmax) + 7) / 8 * 8;
+                double t = -1;
+                if (any && bytes * p <= kLdsLimit) {
+                    // every batch's launch of this class at the stage's current
+                    // costs; the routes it writes are rewritten by the stage
+                    t = 1e30;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+                        for (const BatchLaunch& b : plan) launch_lshape_class(c, b, k, p);
+                        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+                        hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                        float f = 0;
+                        hip_check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+                        t = std::min(t, double(f));
+                    }
+                    if (t < best) best = t, choice = p;
+                }
+                if (ms) ms[k * 4 + ci] = t;
+            }
+            c->per_wave[k] = choice;
+            if (per_wave) per_wave[k] = choice;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->profiling = profiling;
+        c->drop_graph();
+        check_device_error(c, "gr_lshape_tune");
     });
 }
 
@@ -1159,7 +1240,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // edge 12 * L * (len + 1) for the layer-change search (presum and via
         // cost gathers along the edge on every layer) and 8 for the edge
         // record in the DP; per node 16 + 8 * L (record, layer costs).  The
-        // 81-wide per-edge tables are scratch and not counted.
+        // per-edge change tables are scratch and not counted.
         const int L = g.L;
         c->f_edge_bytes.assign(nbatch, 0.0);
         c->f_route_bytes.assign(nbatch, 0.0);
@@ -1194,8 +1275,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             f.packed = cv.template take<unsigned long long>(size_t(max_slots) * kFltStride);
             f.best_path = cv.template take<unsigned long long>(size_t(max_slots) * kFltStride);
             f.layer_output = cv.template take<int>(size_t(max_nodes));
-            f.cost_edges = cv.template take<double>(size_t(max_edges) * 81);
-            f.best_change = cv.template take<int>(size_t(max_edges) * 81);
+            f.cost_edges = cv.template take<double>(size_t(max_edges) * 9 * kFltChangeWidth);
+            f.best_change = cv.template take<int>(size_t(max_edges) * 9 * kFltChangeWidth);
         });
         if (!nrec.empty()) upload(d_nrec, nrec.data(), sizeof(int4) * nrec.size(), c->stream);
         if (!erec.empty()) upload(d_erec, erec.data(), sizeof(int4) * erec.size(), c->stream);
@@ -1249,7 +1330,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed
             a.edge_base = c->f_base[b].y;
             a.slot_base = c->f_base[b].z;
             a.edges = c->f_base[b].w;
-            const long long threads = (long long)a.edges * c->g.L;
+            const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
             timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] {
                 if (threads)
                     hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);

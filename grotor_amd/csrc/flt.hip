@@ -54,6 +54,13 @@ __device__ __forceinline__ T l2_load(const T* p) {
 
 __device__ __forceinline__ void raise(int* error, int code) { atomicCAS(error, 0, code); }
 
+// Layer-change tables: per edge and from-layer f the five to-layers of f's
+// parity (the only ones the DP reads), dense, instead of the reference's
+// 9 x 9 rows (src/routing/flt_cost.cuh:71, 127).
+__host__ __device__ __forceinline__ size_t change_slot(int e, int f, int t) {
+    return ((size_t)e * 9 + f) * kFltChangeWidth + (t - (f & 1)) / 2;
+}
+
 // One DAG's tables, net-local.
 struct Dag {
     const int4* nodes;  // {coordinate, first edge, end edge, first slot * 8 + slots}
@@ -62,7 +69,7 @@ struct Dag {
     unsigned long long* packed;
     unsigned long long* best;
     int* layer;
-    const double* cost_edges;  // [edge * 81 + from * 9 + to]
+    const double* cost_edges;  // [change_slot(edge, from, to)]
     const int* best_change;
 };
 
@@ -132,7 +139,7 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
                     cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
                 } else if (abs(x - px) + abs(y - py) > 4) {
                     const int seq = (layer - 1) * 9 + (t - 1);
-                    const double ce = d.cost_edges[(size_t)e * 81 + seq];
+                    const double ce = d.cost_edges[change_slot(e, layer - 1, t - 1)];
                     if (!(ce < kInf)) continue;
                     cost2 = base + ce;
                     choice = seq;
@@ -260,7 +267,7 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
                     cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
                 } else if (far) {
                     const int seq = (layer - 1) * 9 + (t - 1);
-                    const double ce = d.cost_edges[(size_t)e * 81 + seq];
+                    const double ce = d.cost_edges[change_slot(e, layer - 1, t - 1)];
                     if (!(ce < kInf)) continue;
                     cost2 = base + ce;
                     choice = seq;
@@ -329,7 +336,7 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
                     w[n++] = idx(rl, px, max(cy, py));
                 } else {
                     const int fl = (int)choice / 9, tl = (int)choice % 9;
-                    const int pos = d.best_change[(size_t)e * 81 + fl * 9 + tl];
+                    const int pos = d.best_change[change_slot(e, fl, tl)];
                     w[n++] = idx(fl, cx, min(cy, pos));
                     w[n++] = idx(fl, cx, max(cy, pos));
                     w[n++] = idx(min(fl, tl), cx, pos);
@@ -343,7 +350,7 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
                     w[n++] = idx(rl, max(cx, px), py);
                 } else {
                     const int fl = (int)choice / 9, tl = (int)choice % 9;
-                    const int pos = d.best_change[(size_t)e * 81 + fl * 9 + tl];
+                    const int pos = d.best_change[change_slot(e, fl, tl)];
                     w[n++] = idx(fl, min(cx, pos), cy);
                     w[n++] = idx(fl, max(cx, pos), cy);
                     w[n++] = idx(min(fl, tl), pos, cy);
@@ -394,49 +401,48 @@ __device__ __forceinline__ int wave_inclusive_sum(int v, int lane) {
 }  // namespace
 
 // precompute_cost_edge (src/routing/flt_cost.cuh:50-130); one thread per
-// (edge, from-layer) of the batch.
+// (edge, from-layer, to-layer of the same parity) of the batch.
 __global__ void flt_edge_kernel(FltArgs a) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)a.edges * L) return;
-    const int e = (int)(t / L), f = (int)(t % L);
+    if (t >= (long long)a.edges * L * kFltChangeWidth) return;
+    const int e = (int)(t / (L * kFltChangeWidth));
+    const int r = (int)(t % (L * kFltChangeWidth));
+    const int f = r / kFltChangeWidth, to = (f & 1) + 2 * (r % kFltChangeWidth);
+    // to == f is the straight edge, which the DP prices itself
+    if (to >= L || to == f) return;
     const int4 er = a.edge_rec[a.edge_base + e];
     const int tox = er.z / g.Y % g.X, toy = er.z % g.Y;
     const int fromx = er.w / g.Y % g.X, fromy = er.w % g.Y;
-    double* ce = a.cost_edges + (size_t)e * 81 + f * 9;
-    int* bc = a.best_change + (size_t)e * 81 + f * 9;
-    if ((layer_dir(g, f) == 0 && fromy != toy) || (layer_dir(g, f) == 1 && fromx != tox)) {
-        for (int k = 0; k < 9; ++k) ce[k] = kInf;
-        return;
-    }
+    // a from-layer across the edge: the reference fills its row with
+    // infinity; the DP never reads it (its direction test comes first)
+    if ((layer_dir(g, f) == 0 && fromy != toy) || (layer_dir(g, f) == 1 && fromx != tox)) return;
     const int distance = abs(tox - fromx) + abs(toy - fromy);
     if (!distance) return;
     const int dd = distance / min(distance, 20);
     const bool vertical = fromx == tox;
     const int start = vertical ? min(fromy, toy) : min(fromx, tox);
     const int end = vertical ? max(fromy, toy) : max(fromx, tox);
-    for (int to = f & 1; to < L; to += 2) {
-        double best = kInf;
-        int pos = -1;
-        for (int s = start + dd; s < end; s += dd) {
-            const int sx = vertical ? fromx : s, sy = vertical ? s : fromy;
-            double cost = 0.0;
-            for (int l = min(f, to); l < max(f, to); ++l) cost += (double)a.vcost[cell_of(g, l, sx, sy)];
-            if (vertical)
-                cost += wire(a, f, fromx, fromx, min(fromy, sy), max(fromy, sy)) +
-                        wire(a, to, tox, tox, min(toy, sy), max(toy, sy));
-            else
-                cost += wire(a, f, min(fromx, sx), max(fromx, sx), fromy, fromy) +
-                        wire(a, to, min(tox, sx), max(tox, sx), toy, toy);
-            if (cost < best) {
-                best = cost;
-                pos = s;
-            }
+    double best = kInf;
+    int pos = -1;
+    for (int s = start + dd; s < end; s += dd) {
+        const int sx = vertical ? fromx : s, sy = vertical ? s : fromy;
+        double cost = 0.0;
+        for (int l = min(f, to); l < max(f, to); ++l) cost += (double)a.vcost[cell_of(g, l, sx, sy)];
+        if (vertical)
+            cost += wire(a, f, fromx, fromx, min(fromy, sy), max(fromy, sy)) +
+                    wire(a, to, tox, tox, min(toy, sy), max(toy, sy));
+        else
+            cost += wire(a, f, min(fromx, sx), max(fromx, sx), fromy, fromy) +
+                    wire(a, to, min(tox, sx), max(tox, sx), toy, toy);
+        if (cost < best) {
+            best = cost;
+            pos = s;
         }
-        ce[to] = best;
-        bc[to] = pos;
     }
+    a.cost_edges[change_slot(e, f, to)] = best;
+    a.best_change[change_slot(e, f, to)] = pos;
 }
 
 // route_flt_block (src/routing/flt_block.cuh:334-355) with one wavefront
@@ -455,8 +461,8 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     d.packed = a.packed + (size_t)(ds.z - a.slot_base) * kS;
     d.best = a.best_path + (size_t)(ds.z - a.slot_base) * kS;
     d.layer = a.layer_output + (ds.x - a.node_base);
-    d.cost_edges = a.cost_edges + (size_t)(ds.y - a.edge_base) * 81;
-    d.best_change = a.best_change + (size_t)(ds.y - a.edge_base) * 81;
+    d.cost_edges = a.cost_edges + change_slot(ds.y - a.edge_base, 0, 0);
+    d.best_change = a.best_change + change_slot(ds.y - a.edge_base, 0, 0);
     const int* order = a.order + ds.x;
     const int* doff = a.depth_off + ds.w;
 

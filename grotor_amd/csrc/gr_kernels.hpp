@@ -70,14 +70,16 @@ struct LshapeArgs {
     const int* nodes;
     const int* parents;
     int begin, count;
-    int nmax;          // node capacity of one workgroup's scratch
+    int nmax;          // node capacity of one net's tables
+    int per_wave;      // nets per wavefront: 1, 2, 4 or 8 (each gets 64 / per_wave lanes)
     double* gscratch;  // global scratch when the tree does not fit in LDS
     int* error;
 };
-// Bytes of one workgroup's DP tables for trees of up to nmax nodes (0 if L
-// is not supported), and the launch of `blocks` nets with the tables in LDS
-// or in a.gscratch (false if L is not supported).
+// Bytes of one net's DP tables for trees of up to nmax nodes (0 if L is not
+// supported), and the launch of `blocks` workgroups of a.per_wave nets each
+// with the tables in LDS or in a.gscratch (false if L is not supported).
 size_t lshape_scratch_bytes(int L, int nmax);
+constexpr int kTreeClasses = 2;  // L-shape DP launches per batch: trees up to 16 nodes, larger
 bool launch_lshape_dp(const LshapeArgs& a, int blocks, bool lds, hipStream_t stream);
 
 struct RipupArgs {
@@ -153,6 +155,7 @@ __global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteSto
 
 // FLT routing DAGs
 constexpr int kFltStride = 10;  // layer index 1..L of the FLT tables (L <= 9), 0 unused
+constexpr int kFltChangeWidth = 5;  // same-parity to-layers per (edge, from-layer)
 struct FltArgs {
     DeviceGrid g;
     const double* presum;
@@ -173,14 +176,14 @@ struct FltArgs {
     unsigned long long* packed;      // [slot * kFltStride + layer]
     unsigned long long* best_path;   // [slot * kFltStride + layer]
     int* layer_output;
-    double* cost_edges;              // [edge * 81 + from * 9 + to]
+    double* cost_edges;              // [(edge * 9 + from) * 5 + to / 2]
     int* best_change;
     RouteStore routes;        // phase-two route words
     const int* pin_acc_num;   // pins per net: capacity = pins * routes.per_pin
     int* error;
 };
 __global__ void flt_edge_kernel(FltArgs a);
-__global__ void flt_route_kernel(FltArgs a);
+void launch_flt_route(const FltArgs& a, int blocks, hipStream_t stream);  // L = 1..9
 
 // Congestion view (src/routing/congestion.cuh:60-112)
 __global__ void congestion_view_kernel(DeviceGrid g, const float* capacity, const float* demand,

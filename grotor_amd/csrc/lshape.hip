@@ -4,12 +4,15 @@
 // (src/routing/lshape_warp.cuh:8-247) and its scalar twin Lshape_route_cuda
 // (src/routing/lshape_thread.cuh:18-171).
 //
-// One 64-lane wavefront (one workgroup) owns one net.  Lanes run across
-// layers instead of across tree nodes: nodes are solved one at a time in
-// descending preorder index, which is the scalar kernel's order and makes
-// every parent accumulate its children's contributions in descending child
-// order without floating-point atomics.  Per node the work splits into three
-// lane-parallel phases separated by workgroup barriers:
+// One 64-lane wavefront (one workgroup) owns 1, 2, 4 or 8 nets (nets per
+// wavefront, a launch parameter chosen by the autotuner): each net gets a
+// group of 64/P lanes and its own tables.  Lanes run across layers instead
+// of across tree nodes: nodes are solved one at a time in descending
+// preorder index, which is the scalar kernel's order and makes every parent
+// accumulate its children's contributions in descending child order without
+// floating-point atomics.  Per node the work splits into three lane-parallel
+// phases separated by workgroup barriers (the nets of a wavefront step
+// through their nodes together; a net with fewer nodes idles):
 //   A  lane c: best interval containing layer c (first minimum in
 //      (low, high) row-major order); lanes L..3L-1 fold the corner via
 //      columns of a bent edge into per-(low, high) partial sums, in the
@@ -101,14 +104,20 @@ __device__ __forceinline__ double wire_cost(const DeviceGrid& g, const double* p
 }
 
 template <int L>
-__device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
+__device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     constexpr int LL = L * L, T = kTri<L>;
-    const int lane = threadIdx.x;
+    const int P = a.per_wave, W = kWave / P;
+    const int sub = threadIdx.x / W, lane = threadIdx.x % W;  // lane within the net's group
+    const int slot = blockIdx.x * P + sub;
+    const bool live = slot < a.count;
+    const Tables<L> s = carve<L>(base + words * sub, a.nmax);
     const DeviceGrid& g = a.g;
     const int X = g.X, Y = g.Y;
-    const int4 desc = a.desc[a.begin + blockIdx.x];
+    const int4 desc = live ? a.desc[a.begin + slot] : make_int4(0, 0, 0, 0);
     const int n0 = desc.x, n = desc.y;
-    for (int i = lane; i < n; i += kWave) {
+    int steps = n;  // nodes of the wavefront's largest net
+    for (int off = kWave / 2; off; off >>= 1) steps = max(steps, __shfl_xor(steps, off, kWave));
+    for (int i = lane; i < n; i += W) {
         s.nd[i] = a.nodes[n0 + i];
         s.par[i] = a.parents[n0 + i];
     }
@@ -117,7 +126,7 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
     // bottom-up sweep below touches only LDS: the via stack at each node
     // (src/routing/lshape_warp.cuh:137-145), the edge's wire costs per layer
     // and the via columns at both corners of a bent edge (:18-48).
-    for (int r = lane; r < n * L; r += kWave) {
+    for (int r = lane; r < n * L; r += W) {
         const int i = r / L, c = r % L;
         const int v = s.nd[i];
         const int x = v / Y % X, y = v % Y;
@@ -143,7 +152,7 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
         s.vb[r] = vb;
     }
     __syncthreads();
-    for (int r = lane; r < n * L; r += kWave) {
+    for (int r = lane; r < n * L; r += W) {
         const int i = r / L, lo = r % L;
         const int fixed = s.nd[i] / g.XY;
         double* row = s.dist + size_t(i) * T + tri<L>(lo, lo);  // row[h - lo] = interval (lo, h)
@@ -158,15 +167,18 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
                 if (lo > fixed || hi < fixed) row[hi - lo] = kInf;
     }
     __syncthreads();
-    for (int i = n - 1; i >= 1; --i) {
-        const int p = s.par[i];
-        const int vi = s.nd[i], vp = s.nd[p];
+    for (int t = 1; t < steps; ++t) {
+        const int i = n - t;  // this net's node at this step (none once i < 1)
+        const bool act = i >= 1;
+        const int p = act ? s.par[i] : 0;
+        const int vi = act ? s.nd[i] : 0, vp = act ? s.nd[p] : 0;
         const bool straight = (vi / Y % X == vp / Y % X) || (vi % Y == vp % Y);
         // phase A: lane c < L, best interval containing layer c, first
         // minimum in (low, high) row-major order; lanes L..3L-1 fold the
         // corner via columns of a bent edge
-        if (lane < L) {
-            const int c = lane;
+        for (int q = act ? lane : 3 * L; q < 3 * L; q += W) {
+          if (q < L) {
+            const int c = q;
             const double* nd = s.dist + size_t(i) * T;
             double best = kInf;
             int ch = 0;
@@ -179,9 +191,9 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
                     }
             s.best[c] = best;
             s.choice[c] = ch;
-        } else if (lane < 3 * L && !straight) {
+          } else if (!straight) {
             // fold[k][lo][hi] = sum of corner[lo+1 .. hi-1], left to right
-            const int k = (lane - L) / L, lo = (lane - L) % L;
+            const int k = (q - L) / L, lo = (q - L) % L;
             const float* corner = (k ? s.vb : s.va) + i * L;
             double* f = s.fold + k * LL + lo * L;
             double v = 0.0;
@@ -191,11 +203,11 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
                 if (hi > lo + 1) v += (double)corner[hi - 1];
                 f[hi] = v;
             }
+          }
         }
         __syncthreads();
         // phase B (src/routing/lshape_warp.cuh:18-48)
-        if (lane < L) {
-            const int pl = lane;
+        for (int pl = act ? lane : L; pl < L; pl += W) {
             double best = kInf;
             int bp = 0;
             if (straight) {
@@ -226,8 +238,7 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
         }
         __syncthreads();
         // phase C (src/routing/lshape_warp.cuh:165-189)
-        if (lane < L) {
-            const int lo = lane;
+        for (int lo = act ? lane : L; lo < L; lo += W) {
             const double* dg = s.diag + i * L;
             double* prow = s.dist + size_t(p) * T + tri<L>(lo, lo);
             double c = dg[lo];
@@ -242,7 +253,7 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
     }
     // root choice and traceback (src/routing/lshape_warp.cuh:193-213)
     const int base_word = desc.z, cap = desc.w;
-    if (lane == 0) {
+    if (lane == 0 && live) {
         int ch = 0;
         double cv = s.dist[0];
         for (int lo = 0; lo < L; ++lo)
@@ -285,10 +296,10 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
         if (total > cap) atomicExch(a.error, (int)kDevRouteOverflow);
     }
     __syncthreads();
-    if (s.woff[n] > cap) return;
+    if (!live || s.woff[n] > cap) return;
     int* routes = a.rs.routes + base_word;
     // emission (src/routing/lshape_warp.cuh:51-100)
-    for (int i = lane; i < n; i += kWave) {
+    for (int i = lane; i < n; i += W) {
         int* out = routes + s.woff[i];
         int c = 0;
         const int lo = s.rng[i] / L, hi = s.rng[i] % L;
@@ -327,19 +338,20 @@ __device__ void solve_net(const LshapeArgs& a, const Tables<L>& s) {
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_lds_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
-    solve_net<L>(a, carve<L>(smem, a.nmax));
+    solve_net<L>(a, smem, (table_bytes<L>(a.nmax) + 7) / 8);
 }
 
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_global_kernel(LshapeArgs a) {
     const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
-    solve_net<L>(a, carve<L>(a.gscratch + words * blockIdx.x, a.nmax));
+    solve_net<L>(a, a.gscratch + words * a.per_wave * blockIdx.x, words);
 }
 
 template <int L>
 void launch_for(const LshapeArgs& a, int blocks, bool lds, hipStream_t st) {
     if (lds)
-        hipLaunchKernelGGL(lshape_dp_lds_kernel<L>, dim3(blocks), dim3(kWave), table_bytes<L>(a.nmax), st, a);
+        hipLaunchKernelGGL(lshape_dp_lds_kernel<L>, dim3(blocks), dim3(kWave),
+                           ((table_bytes<L>(a.nmax) + 7) / 8) * 8 * a.per_wave, st, a);
     else
         hipLaunchKernelGGL(lshape_dp_global_kernel<L>, dim3(blocks), dim3(kWave), 0, st, a);
 }

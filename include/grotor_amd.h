@@ -154,6 +154,20 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
 int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,
                   int mode, double *elapsed_ms);
 
+/* Nets per wavefront of the L-shape DP launches (1, 2, 4 or 8; each net gets
+ * 64 / n lanes and its own tables) for the two tree size classes of a batch:
+ * up to 16 nodes, and larger (per_wave[2]).  Results do not depend on it.
+ * No reference counterpart (the reference runs one net per 32-lane warp). */
+int gr_ctx_set_nets_per_wave(gr_ctx *ctx, const int *per_wave);
+
+/* Autotuner: times every size class's DP launches over the given batches of
+ * the uploaded stage at the current costs for 1, 2, 4 and 8 nets per
+ * wavefront, keeps the fastest per class in the context and returns it in
+ * per_wave[2]; ms (may be NULL) receives [class * 4 + candidate] times
+ * (-1 where a candidate's tables do not fit in LDS).  The route words it
+ * writes are rewritten by the next stage run.  No reference counterpart. */
+int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave, double *ms);
+
 /* Reset the routing state to that of a fresh context: demand, timestamps,
  * statistics and route words cleared, every track marked for refresh (the
  * DAG-throughput benchmark's reset, src/experiments/lshape_benchmark.cuh:84-97). */

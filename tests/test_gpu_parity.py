@@ -114,6 +114,38 @@ def test_lshape_stage_matches_oracle(ga, ora, config, seed):
     ctx.close()
 
 
+@pytest.mark.parametrize("per_wave", [(1, 1), (8, 8), (4, 2)])
+def test_nets_per_wavefront_do_not_change_results(ga, ora, per_wave):
+    """Any nets-per-wavefront setting of the two tree size classes gives the
+    oracle's routes, demand, timestamps and statistics; the autotuner picks
+    a valid setting and the stage still matches after it ran."""
+    _, t = topology(6, 2)
+    print(f"nets per wavefront {per_wave}", flush=True)
+    color, nb = ora.batches(t["X"], t["Y"], t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=8)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.set_nets_per_wave(per_wave)
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=0)
+    assert np.array_equal(ctx.routes(), routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    assert np.array_equal(ctx.timestamps(), ts_o)
+    assert ctx.stats() == (wl_o, vias_o)
+    if per_wave == (4, 2):
+        chosen, ms = ctx.lshape_tune(sched["batch_off"])
+        assert all(p in (1, 2, 4, 8) for p in chosen)
+        assert any(v > 0 for row in ms for v in row)
+        ctx.reset()
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+        assert np.array_equal(ctx.routes(), routes_o)
+        assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    with pytest.raises(ga.GrError):
+        ctx.set_nets_per_wave((3, 1))
+    ctx.close()
+
+
 def test_graph_mode_replays_match_oracle(ga, ora):
     """Mode 1 (captured HIP graph): every replay, with resets and eager work
     between replays, equals the oracle; a changed schedule recaptures."""
