@@ -297,6 +297,14 @@ def bench_flt(args, ga, design, topo):
     fgeom = (fs["pt_off"], fs["pts"], fs["hs_off"], fs["hsegs"], fs["vs_off"], fs["vsegs"])
     ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
     nets = fs["nets"]
+    # autotuner: DAGs per wavefront of the FLT DP, once, on the first plan
+    ctx.batch_geometry(*fgeom)
+    tcolor, tnb = ctx.batch_colors(limit=300000)
+    ctx.flt_upload(design.flt_plan(tcolor, tnb))
+    per_wave, tune_ms = ctx.flt_tune()
+    print(f"[bench] flt autotuner: DAGs per wavefront {per_wave} (batches <512, >=512 DAGs); "
+          f"DP ms per candidate 1/2/4/8: {[[round(v, 2) for v in row] for row in tune_ms]}",
+          file=sys.stderr, flush=True)
     print(f"[bench] flt: {nets} rerouted nets, {fs['nodes']} DAG nodes, {fs['edges']} DAG edges, "
           f"DAG build {topo_ms:.1f} ms", file=sys.stderr, flush=True)
 
@@ -392,6 +400,8 @@ def bench_flt(args, ga, design, topo):
                    "batches": len(plan["batch_off"]) - 1, "batching_ms_per_step": round(bt / args.steps, 4),
                    "dag_ms_per_step": round(dt / args.steps, 4), "plan_upload_ms_per_step": round(ut / args.steps, 4),
                    "dag_build_ms": round(topo_ms, 2), "total_wirelength": wl, "total_vias": vias,
+                   "dags_per_wavefront": {"classes": ["batches < 512 DAGs", ">= 512 DAGs"], "chosen": per_wave,
+                                          "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in tune_ms]},
                    "parallelism": "single GPU", "kernel_breakdown": {"steps": 1, "classes": kernels}},
         "roofline": roofline, "cpu_baseline": cpu}), flush=True)
     ctx.close()

@@ -339,6 +339,18 @@ class Context:
               ctypes.byref(ms))
         return ms.value
 
+    def set_flt_per_wave(self, per_wave):
+        """FLT DP DAGs per wavefront for batches of <512 and >=512 DAGs."""
+        pw = _i32(per_wave)
+        _call("gr_ctx_set_flt_per_wave", self._h, _ptr(pw, ctypes.c_int))
+
+    def flt_tune(self):
+        """Autotune FLT DAGs per wavefront: (chosen per class, ms[class][candidate 1, 2, 4, 8])."""
+        pw = np.zeros(2, np.int32)
+        ms = np.zeros(8, np.float64)
+        _call("gr_flt_tune", self._h, _ptr(pw, ctypes.c_int), _ptr(ms, ctypes.c_double))
+        return pw.tolist(), ms.reshape(2, 4).tolist()
+
     def zshape_upload(self, plan):
         """Upload a Design.zshape_plan() plan."""
         self._zplan = plan

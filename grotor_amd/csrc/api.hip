@@ -158,6 +158,7 @@ struct gr_ctx {
     std::vector<int> f_batch_off;
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
     std::vector<double> f_edge_bytes, f_route_bytes;  // algorithmic bytes per batch launch
+    int f_per_wave[2] = {1, 1};  // FLT DAGs per wavefront: batches of fewer / at least kFltWideBatch DAGs
     // congestion view (2D)
     unsigned char* cview = nullptr;
     float *cxsum = nullptr, *cysum = nullptr;
@@ -1258,6 +1259,22 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             c->f_edge_bytes[b] = eb;
             c->f_route_bytes[b] = rb;
         }
+        // the DAGs of a wavefront step through their depths together: order
+        // each batch by decreasing depth, then size (results do not depend
+        // on the order of nets inside a batch)
+        std::vector<int> ids(net_ids, net_ids + nets);
+        for (int b = 0; b < nbatch; ++b) {
+            const int k0 = batch_off[b], k1 = batch_off[b + 1];
+            std::vector<int> perm(k1 - k0);
+            for (int i = 0; i < k1 - k0; ++i) perm[i] = k0 + i;
+            std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+                return sizes[x].w != sizes[y].w ? sizes[x].w > sizes[y].w : sizes[x].x > sizes[y].x;
+            });
+            std::vector<int4> dp(k1 - k0), sp(k1 - k0);
+            std::vector<int> ip(k1 - k0);
+            for (int i = 0; i < k1 - k0; ++i) dp[i] = desc[perm[i]], sp[i] = sizes[perm[i]], ip[i] = net_ids[perm[i]];
+            for (int i = 0; i < k1 - k0; ++i) desc[k0 + i] = dp[i], sizes[k0 + i] = sp[i], ids[k0 + i] = ip[i];
+        }
         int4 *d_nrec, *d_erec, *d_desc, *d_sizes;
         int *d_order, *d_doff, *d_ids;
         c->f_ws.carve([&](auto& cv) {
@@ -1285,7 +1302,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         if (nets) {
             upload(d_desc, desc.data(), sizeof(int4) * nets, c->stream);
             upload(d_sizes, sizes.data(), sizeof(int4) * nets, c->stream);
-            upload(d_ids, net_ids, sizeof(int) * nets, c->stream);
+            upload(d_ids, ids.data(), sizeof(int) * nets, c->stream);
         }
         f.g = g;
         f.node_rec = d_nrec;
@@ -1335,8 +1352,9 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed
                 if (threads)
                     hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
             });
+            a.per_wave = c->f_per_wave[count >= kFltWideBatch];
             timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
-                launch_flt_route(a, (count + 3) / 4, c->stream);
+                launch_flt_route(a, (count + 4 * a.per_wave - 1) / (4 * a.per_wave), c->stream);
             });
             hip_check(hipGetLastError(), "FLT kernels");
             launch_commit(c, 0, count, s, 1, ids, &rs2);
@@ -1345,6 +1363,82 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed
         const auto t1 = std::chrono::steady_clock::now();
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_flt_run");
+    });
+}
+
+int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
+        launch_refresh(c, 0);
+        const bool profiling = c->profiling;
+        c->profiling = false;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int candidates[4] = {1, 2, 4, 8};
+        double t[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        bool seen[2] = {false, false};
+        for (int b = 0; b < c->f_batches; ++b) {
+            const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
+            if (!count) continue;
+            FltArgs a = c->f;
+            a.presum = c->presum;
+            a.vcost = c->vcost;
+            a.desc = c->f.desc + begin;
+            a.sizes = c->f.sizes + begin;
+            a.net_ids = c->f.net_ids + begin;
+            a.count = count;
+            a.node_base = c->f_base[b].x;
+            a.edge_base = c->f_base[b].y;
+            a.slot_base = c->f_base[b].z;
+            a.edges = c->f_base[b].w;
+            const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
+            if (threads)
+                hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
+            const int cls = count >= kFltWideBatch;
+            seen[cls] = true;
+            for (int ci = 0; ci < 4; ++ci) {
+                // the DP of this batch at the current costs; its words are
+                // rewritten when the stage runs
+                a.per_wave = candidates[ci];
+                hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+                launch_flt_route(a, (count + 4 * a.per_wave - 1) / (4 * a.per_wave), c->stream);
+                hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float f = 0;
+                hip_check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+                t[cls][ci] += f;
+            }
+        }
+        hip_check(hipGetLastError(), "FLT tuning launches");
+        // the tuning runs wrote phase-two words: start the stage from clear storage
+        hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
+        hip_check(hipStreamSynchronize(c->stream), "FLT tuning");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->profiling = profiling;
+        for (int cls = 0; cls < 2; ++cls) {
+            int best = 0;
+            for (int ci = 1; ci < 4; ++ci)
+                if (t[cls][ci] < t[cls][best]) best = ci;
+            if (seen[cls]) c->f_per_wave[cls] = candidates[best];
+            if (per_wave) per_wave[cls] = c->f_per_wave[cls];
+            if (ms)
+                for (int ci = 0; ci < 4; ++ci) ms[cls * 4 + ci] = seen[cls] ? t[cls][ci] : -1.0;
+        }
+        check_device_error(c, "gr_flt_tune");
+    });
+}
+
+int gr_ctx_set_flt_per_wave(gr_ctx* c, const int* per_wave) {
+    return guarded([&] {
+        if (!c || !per_wave) throw GrError(GR_E_ARG, "null argument");
+        for (int k = 0; k < 2; ++k)
+            if (per_wave[k] != 1 && per_wave[k] != 2 && per_wave[k] != 4 && per_wave[k] != 8)
+                throw GrError(GR_E_ARG, "DAGs per wavefront must be 1, 2, 4 or 8");
+        c->f_per_wave[0] = per_wave[0];
+        c->f_per_wave[1] = per_wave[1];
     });
 }
 

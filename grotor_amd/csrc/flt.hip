@@ -390,13 +390,6 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
     return n;
 }
 
-__device__ __forceinline__ int wave_inclusive_sum(int v, int lane) {
-    for (int s = 1; s < kWave; s <<= 1) {
-        const int o = __shfl_up(v, s, kWave);
-        if (lane >= s) v += o;
-    }
-    return v;
-}
 
 }  // namespace
 
@@ -445,15 +438,21 @@ __global__ void flt_edge_kernel(FltArgs a) {
     a.best_change[change_slot(e, f, to)] = pos;
 }
 
-// route_flt_block (src/routing/flt_block.cuh:334-355) with one wavefront
-// per DAG and four DAGs per workgroup.
+// route_flt_block (src/routing/flt_block.cuh:334-355): a.per_wave DAGs per
+// wavefront (1, 2, 4 or 8; each gets 64 / per_wave lanes), four wavefronts
+// per workgroup.  The DAGs of a wavefront step through their depths
+// together, so a batch is uploaded with its DAGs in decreasing depth order.
 template <int L>
 __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int k = blockIdx.x * (256 / kWave) + (threadIdx.x / kWave);
-    if (k >= a.count) return;
-    const int4 ds = a.desc[k];   // stage-global first node, edge, slot, depth offset
-    const int4 dn = a.sizes[k];  // nodes, edges, slots, depths
+    const int P = a.per_wave, W = kWave / P;
+    const int wl = threadIdx.x & (kWave - 1);
+    const int sub = wl / W, lane = wl % W;  // lane within this DAG's group
+    const int first_dag = (blockIdx.x * (256 / kWave) + threadIdx.x / kWave) * P;
+    if (first_dag >= a.count) return;  // the whole wavefront
+    const int k = first_dag + sub;
+    const bool live = k < a.count;
+    const int4 ds = live ? a.desc[k] : make_int4(0, 0, 0, 0);  // stage-global first node, edge, slot, depth offset
+    const int4 dn = live ? a.sizes[k] : make_int4(0, 0, 0, 0);  // nodes, edges, slots, depths
     Dag d;
     d.nodes = a.node_rec + ds.x;
     d.edges = a.edge_rec + ds.y;
@@ -465,42 +464,53 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     d.best_change = a.best_change + change_slot(ds.y - a.edge_base, 0, 0);
     const int* order = a.order + ds.x;
     const int* doff = a.depth_off + ds.w;
+    int depths = dn.w;  // the wavefront's deepest DAG
+    for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
 
     // initialize_batch (src/routing/flt_workspace.cuh:60-69) for this DAG:
     // only the child slots need it here (a node's layer costs start at
     // infinity in its solve, and every node's trace sets its layer output
     // before any child reads it)
-    for (int i = lane; i < dn.z * kS; i += kWave) d.packed[i] = kEmpty;
+    for (int i = lane; i < dn.z * kS; i += W) d.packed[i] = kEmpty;
     depth_barrier();
-    for (int dep = dn.w - 1; dep >= 0; --dep) {
-        const int end = doff[dep + 1];
-        for (int i = doff[dep] + lane; i < end; i += kWave) {
-            const int v = order[i];
-            const int4 r = d.nodes[v];
-            if ((r.w & 7) <= kFastKids)
-                solve_node_fast<L>(a, d, v, r);
-            else
-                solve_node(a, d, v);
+    for (int dep = depths - 1; dep >= 0; --dep) {
+        if (dep < dn.w) {
+            const int end = doff[dep + 1];
+            for (int i = doff[dep] + lane; i < end; i += W) {
+                const int v = order[i];
+                const int4 r = d.nodes[v];
+                if ((r.w & 7) <= kFastKids)
+                    solve_node_fast<L>(a, d, v, r);
+                else
+                    solve_node(a, d, v);
+            }
         }
         depth_barrier();
     }
-    const int id = a.net_ids[k];
-    int* routes = a.routes.routes + (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin;
-    const long long cap = (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin;
+    const int id = live ? a.net_ids[k] : 0;
+    int* routes = a.routes.routes + (live ? (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin : 0);
+    const long long cap = live ? (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin : 0;
     const int g_cells = a.g.L * a.g.XY;
     int count = 1;  // the root's trace sets the word count to 1
-    for (int dep = 0; dep < dn.w; ++dep) {
-        const int first = doff[dep], end = doff[dep + 1];
-        for (int i0 = first; i0 < end; i0 += kWave) {
+    for (int dep = 0; dep < depths; ++dep) {
+        const int first = dep < dn.w ? doff[dep] : 0, end = dep < dn.w ? doff[dep + 1] : 0;
+        int chunks = (end - first + W - 1) / W;
+        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+        for (int ch = 0; ch < chunks; ++ch) {
             int w[8];
-            const int i = i0 + lane;
+            const int i = first + ch * W + lane;
             int n = i < end ? trace_node(a, d, order[i], w) : 0;
             for (int j = 0; j < n; ++j)
                 if (w[j] < 0 || w[j] >= g_cells) {
                     raise(a.error, kDevBadCoordinate);
                     n = 0;
                 }
-            const int incl = wave_inclusive_sum(n, lane);
+            // inclusive prefix of the word counts within this DAG's lanes
+            int incl = n;
+            for (int s = 1; s < W; s <<= 1) {
+                const int o = __shfl_up(incl, s, W);
+                if (lane >= s) incl += o;
+            }
             const int at = count + incl - n;
             if (n) {
                 if (at + n - 1 >= cap) {
@@ -509,11 +519,11 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
                     for (int j = 0; j < n; ++j) routes[at + j] = w[j];
                 }
             }
-            count += __shfl(incl, kWave - 1, kWave);
+            count += __shfl(incl, W - 1, W);
         }
         depth_barrier();
     }
-    if (lane == 0) routes[0] = count <= cap ? count : 1;
+    if (live && lane == 0) routes[0] = count <= cap ? count : 1;
 }
 
 void launch_flt_route(const FltArgs& a, int blocks, hipStream_t stream) {

@@ -156,6 +156,7 @@ __global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteSto
 // FLT routing DAGs
 constexpr int kFltStride = 10;  // layer index 1..L of the FLT tables (L <= 9), 0 unused
 constexpr int kFltChangeWidth = 5;  // same-parity to-layers per (edge, from-layer)
+constexpr int kFltWideBatch = 512;  // FLT batches of at least this many DAGs form the second tuning class
 struct FltArgs {
     DeviceGrid g;
     const double* presum;
@@ -170,6 +171,7 @@ struct FltArgs {
     const int4* sizes;      // nodes, edges, slots, depths
     const int* net_ids;
     int count, edges;
+    int per_wave;           // DAGs per wavefront: 1, 2, 4 or 8
     int node_base, edge_base, slot_base;  // stage-global index of the batch's first node, edge, slot
     // batch tables
     double* costs;                   // [node * kFltStride + layer]

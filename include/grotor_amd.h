@@ -264,6 +264,20 @@ int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_
  * commit.  *elapsed_ms = the synchronized interval (DAG_TIMING). */
 int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, double *elapsed_ms);
 
+/* DAGs per wavefront of the FLT DP (1, 2, 4 or 8; each DAG gets 64 / n
+ * lanes) for batches of fewer than 512 DAGs (per_wave[0]) and of at least
+ * 512 (per_wave[1]).  Results do not depend on it.  No reference
+ * counterpart (the reference runs one DAG per 128-thread block). */
+int gr_ctx_set_flt_per_wave(gr_ctx *ctx, const int *per_wave);
+
+/* Autotuner of the FLT DP: per uploaded batch, the layer-change search and
+ * then the DAG DP once per candidate (1, 2, 4, 8 DAGs per wavefront) at the
+ * current costs; keeps the fastest per batch class (as above) and returns it
+ * in per_wave[2]; ms (may be NULL) receives [class * 4 + candidate] summed
+ * DP times (-1 for a class without batches).  Phase-two route storage is
+ * cleared afterwards.  No reference counterpart. */
+int gr_flt_tune(gr_ctx *ctx, int *per_wave, double *ms);
+
 /* ---- soft (log-sum-exp) L-shape DP ----
  * No counterpart in the reference: a differentiable relaxation of the
  * L-shape DP above (every minimum becomes smin_tau(x) = -tau log sum

@@ -55,6 +55,33 @@ def test_flt_stage_matches_oracle(ga, ora, config, seed, scale, side):
     ctx.close()
 
 
+@pytest.mark.parametrize("per_wave", [(8, 8), (2, 4), None])
+def test_flt_dags_per_wavefront_do_not_change_results(ga, ora, per_wave):
+    """Several DAGs per wavefront (or the autotuner's choice) give the
+    oracle's phase-two routes, demand and statistics."""
+    I = flt_instance(ga, ora, 3, 7, 0.01, 64)
+    t, fs, plan = I["t"], I["fs"], I["plan"]
+    print(f"flt per wave {per_wave}", flush=True)
+    ctx = _gpu_phase_one(ga, I)
+    first = len(I["sched"]["batch_off"])
+    ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
+    ctx.flt_upload(plan)
+    if per_wave is None:
+        chosen, ms = ctx.flt_tune()
+        assert all(p in (1, 2, 4, 8) for p in chosen) and max(max(r) for r in ms) > 0
+    else:
+        ctx.set_flt_per_wave(per_wave)
+    ctx.flt_run(first_stamp=first, remove_phase_one=True)
+    r2, dem, ts, wl, vias, rc, _ = I["g"].flt_stage(plan, t["pin_acc_num"], fs["pin_off2"], fs["pins2"], I["demand"],
+                                                     I["ts"], I["routes"], first, wl=I["wl"], vias=I["vias"])
+    assert rc == 0
+    assert np.array_equal(ctx.routes2(), r2)
+    assert np.array_equal(_bits(ctx.demand()), _bits(dem))
+    assert np.array_equal(ctx.timestamps(), ts)
+    assert ctx.stats() == (wl, vias)
+    ctx.close()
+
+
 def test_flt_stage_without_phase_one_removal(ga, ora):
     """remove_phase_one = 0 commits phase two on top of phase one, one stamp per batch."""
     I = flt_instance(ga, ora, 3, 7, 0.01, 64)
