@@ -165,17 +165,90 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
 // reference's rows hold after its copies.
 constexpr int kFastKids = 4;
 
+// The candidates of a solved node towards its parents (the second half of
+// solve_flt_node, :81-141) with the layer costs cv in registers.  A layer
+// can run along an edge only if its direction matches, which for an edge of
+// non-zero length selects one layer parity, so each lane walks only that
+// parity's layers (and to-layers); an edge between two nodes on the same
+// cell takes every layer.
 template <int L>
-__device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int4 r) {
+__device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, int v, int x, int y, const int4 r, const double* cv) {
+    const DeviceGrid& g = a.g;
+    constexpr int H = (L + 1) / 2;  // layers of one parity, at most
+    for (int e = r.y; e < r.z; ++e) {
+        const int4 er = d.edges[e];
+        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
+        const int px = er.z / g.Y % g.X, py = er.z % g.Y;
+        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+        const bool far = abs(x - px) + abs(y - py) > 4;
+        if (x == px && y == py) {
+            // zero length: every layer, straight only (far is false)
+#pragma unroll
+            for (int layer = 1; layer <= L; ++layer) {
+                const double base = cv[layer];
+                if (!(base < kInf)) {  // the reference asserts here
+                    raise(a.error, kDevNoSolution);
+                    continue;
+                }
+                const unsigned long long nv =
+                    pack((float)(base + wire(a, layer - 1, minx, maxx, miny, maxy)), v * 100 + kStraight);
+                atomicMin(ppk + layer, nv);
+            }
+            continue;
+        }
+        // a layer runs along y when ((layer & 1) ^ DIR) == 0 (the reference's
+        // direction tests, :101-102): odd layers do when DIR == 1
+        const bool odd_runs_y = g.DIR == 1;
+        const int par = (x == px) == odd_runs_y ? 1 : 2;  // first layer of the usable parity
+        unsigned long long cache[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m) cache[m] = kEmpty;
+#pragma unroll
+        for (int j = 0; j < H; ++j) {
+            const int layer = par + 2 * j;
+            if (layer > L) continue;
+            const double base = par == 1 ? cv[min(2 * j + 1, L)] : cv[min(2 * j + 2, L)];
+            if (!(base < kInf)) {  // the reference asserts here
+                raise(a.error, kDevNoSolution);
+                continue;
+            }
+#pragma unroll
+            for (int m = 0; m < H; ++m) {
+                const int t = par + 2 * m;
+                if (t > L) continue;
+                double cost2;
+                int choice = kStraight;
+                if (m == j) {
+                    cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
+                } else if (far) {
+                    const double ce = d.cost_edges[change_slot(e, layer - 1, t - 1)];
+                    if (!(ce < kInf)) continue;
+                    cost2 = base + ce;
+                    choice = (layer - 1) * 9 + (t - 1);
+                } else {
+                    continue;
+                }
+                const unsigned long long nv = pack((float)cost2, v * 100 + choice);
+                if (nv < cache[m]) cache[m] = nv;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < H; ++m)
+            if (cache[m] < kEmpty && par + 2 * m <= L) atomicMin(ppk + par + 2 * m, cache[m]);
+    }
+}
+
+template <int L, int K>
+__device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int4 r) {
     const DeviceGrid& g = a.g;
     const int kids = r.w & 7;
     const int low_fixed = 1 + r.x / g.XY;
     const int x = r.x / g.Y % g.X, y = r.x % g.Y;
     const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
     const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
-    float pv[kFastKids][L + 1];  // slot costs (the packed key's float)
+    float pv[K][L + 1];  // slot costs (the packed key's float)
 #pragma unroll
-    for (int k = 0; k < kFastKids; ++k)
+    for (int k = 0; k < K; ++k)
 #pragma unroll
         for (int li = 1; li <= L; ++li) pv[k][li] = k < kids ? (float)unpack_cost(l2_load(pk + k * kS + li)) : 0.f;
     float vc[L];
@@ -185,19 +258,22 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
     unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
 #pragma unroll
     for (int li = 0; li <= L; ++li) cv[li] = kInf, at[li] = 0;
+    // passes over the stack's low end stay a loop; every layer index inside
+    // is a compile-time constant, so the tables stay in registers
+#pragma unroll 1
+    for (int low = 1; low <= min(L, low_fixed); ++low) {
+        {
+            double mins[K];
 #pragma unroll
-    for (int low = 1; low <= L; ++low) {
-        if (low <= low_fixed) {
-            double mins[kFastKids];
-#pragma unroll
-            for (int k = 0; k < kFastKids; ++k) mins[k] = kInf;
+            for (int k = 0; k < K; ++k) mins[k] = kInf;
             unsigned run = 0;
             double via = 0.0;
 #pragma unroll
-            for (int li = low; li <= L; ++li) {
+            for (int li = 1; li <= L; ++li) {
+                if (li < low) continue;
                 if (li > low) via += (double)vc[li - 2];
 #pragma unroll
-                for (int k = 0; k < kFastKids; ++k)
+                for (int k = 0; k < K; ++k)
                     if (k < kids) {
                         const double c = (double)pv[k][li];
                         if (c < mins[k]) {
@@ -208,7 +284,7 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
                 if (li >= high_fixed) {
                     double cost = via;
 #pragma unroll
-                    for (int k = 0; k < kFastKids; ++k)
+                    for (int k = 0; k < K; ++k)
                         if (k < kids) cost += mins[k];
                     if (cost < kInf && cost < cv[li]) {
                         cv[li] = cost;
@@ -217,8 +293,8 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
                 }
             }
 #pragma unroll
-            for (int li = L - 1; li >= low; --li)
-                if (cv[li + 1] < cv[li]) {
+            for (int li = L - 1; li >= 1; --li)
+                if (li >= low && cv[li + 1] < cv[li]) {
                     cv[li] = cv[li + 1];
                     at[li] = at[li + 1];
                 }
@@ -229,7 +305,7 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
     for (int li = 1; li <= L; ++li) {
         if (!(cv[li] < kInf)) continue;
 #pragma unroll
-        for (int k = 0; k < kFastKids; ++k) {
+        for (int k = 0; k < K; ++k) {
             if (k >= kids) continue;
             const int j = (at[li] >> (4 * k)) & 15;
             bp[k * kS + li] = (l2_load(pk + k * kS + j) & 0xFFFFFFFFULL) * kS + (unsigned long long)j;
@@ -241,47 +317,7 @@ __device__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int
 #pragma unroll
         for (int li = 1; li <= L; ++li) cg[li] = cv[li];
     }
-    for (int e = r.y; e < r.z; ++e) {
-        const int4 er = d.edges[e];
-        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
-        const int px = er.z / g.Y % g.X, py = er.z % g.Y;
-        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
-        const bool far = abs(x - px) + abs(y - py) > 4;
-        unsigned long long cache[L + 1];
-#pragma unroll
-        for (int t = 1; t <= L; ++t) cache[t] = kEmpty;
-#pragma unroll
-        for (int layer = 1; layer <= L; ++layer) {
-            if (((layer & 1) ^ g.DIR) == 1 && y != py) continue;
-            if (((layer & 1) ^ g.DIR) == 0 && x != px) continue;
-            const double base = cv[layer];
-            if (!(base < kInf)) {  // the reference asserts here
-                raise(a.error, kDevNoSolution);
-                continue;
-            }
-#pragma unroll
-            for (int t = 1 + (1 - layer % 2); t <= L; t += 2) {
-                double cost2;
-                int choice = kStraight;
-                if (t == layer) {
-                    cost2 = base + wire(a, layer - 1, minx, maxx, miny, maxy);
-                } else if (far) {
-                    const int seq = (layer - 1) * 9 + (t - 1);
-                    const double ce = d.cost_edges[change_slot(e, layer - 1, t - 1)];
-                    if (!(ce < kInf)) continue;
-                    cost2 = base + ce;
-                    choice = seq;
-                } else {
-                    continue;
-                }
-                const unsigned long long nv = pack((float)cost2, v * 100 + choice);
-                if (nv < cache[t]) cache[t] = nv;
-            }
-        }
-#pragma unroll
-        for (int t = 1; t <= L; ++t)
-            if (cache[t] < kEmpty) atomicMin(ppk + t, cache[t]);
-    }
+    push_parents<L>(a, d, v, x, y, r, cv);
 }
 
 // trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
@@ -479,8 +515,10 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
             for (int i = doff[dep] + lane; i < end; i += W) {
                 const int v = order[i];
                 const int4 r = d.nodes[v];
-                if ((r.w & 7) <= kFastKids)
-                    solve_node_fast<L>(a, d, v, r);
+                if ((r.w & 7) <= 2)
+                    solve_node_fast<L, 2>(a, d, v, r);
+                else if ((r.w & 7) <= kFastKids)
+                    solve_node_fast<L, kFastKids>(a, d, v, r);
                 else
                     solve_node(a, d, v);
             }
