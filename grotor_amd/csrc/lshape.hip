@@ -55,6 +55,10 @@ struct Tables {
     int* path;      // nmax
     int* choice;    // L
     int* woff;      // nmax + 1
+    int* xy;        // nmax          node position x << 16 | y
+    int* lay;       // nmax          node layer (L for Steiner nodes)
+    double* rmin;   // L * L         row suffix minima of the current node's table
+    int* rarg;      // L * L         their high ends
 };
 
 // entries of an interval table: intervals low <= high of L layers
@@ -69,8 +73,8 @@ __host__ __device__ constexpr int tri(int lo, int hi) {
 
 template <int L>
 __host__ __device__ constexpr size_t table_bytes(int nmax) {
-    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + L + 2 * L * L) + 4 * (size_t(nmax) * 3 * L) +
-           4 * (size_t(nmax) * L + 4 * size_t(nmax) + L + nmax + 1);
+    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + L + 3 * L * L) + 4 * (size_t(nmax) * 3 * L) +
+           4 * (size_t(nmax) * L + 4 * size_t(nmax) + L + nmax + 1) + 4 * (2 * size_t(nmax) + L * L) + 8;
 }
 
 template <int L>
@@ -92,6 +96,12 @@ __device__ inline Tables<L> carve(double* base, int nmax) {
     s.path = s.rng + nmax;
     s.choice = s.path + nmax;
     s.woff = s.choice + L;
+    s.xy = s.woff + nmax + 1;
+    s.lay = s.xy + nmax;
+    s.rarg = s.lay + nmax;
+    // doubles after the ints, 8-byte aligned
+    const size_t ints = size_t(s.rarg + L * L - reinterpret_cast<int*>(base));
+    s.rmin = base + (ints + 1) / 2;
     return s;
 }
 
@@ -118,8 +128,11 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     int steps = n;  // nodes of the wavefront's largest net
     for (int off = kWave / 2; off; off >>= 1) steps = max(steps, __shfl_xor(steps, off, kWave));
     for (int i = lane; i < n; i += W) {
-        s.nd[i] = a.nodes[n0 + i];
+        const int v = a.nodes[n0 + i];
+        s.nd[i] = v;
         s.par[i] = a.parents[n0 + i];
+        s.xy[i] = (v / Y % X) << 16 | (v % Y);  // decoded once: no divisions below
+        s.lay[i] = v / g.XY;
     }
     __syncthreads();
     // Every cost the DP reads, gathered up front for all nodes so that the
@@ -128,12 +141,11 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     // and the via columns at both corners of a bent edge (:18-48).
     for (int r = lane; r < n * L; r += W) {
         const int i = r / L, c = r % L;
-        const int v = s.nd[i];
-        const int x = v / Y % X, y = v % Y;
+        const int x = s.xy[i] >> 16, y = s.xy[i] & 0xFFFF;
         s.vs[r] = c < L - 1 ? a.vcost[cell_of(g, c, x, y)] : 0.f;
         if (!i) continue;
-        const int q = s.nd[s.par[i]];
-        const int px = q / Y % X, py = q % Y;
+        const int q = s.xy[s.par[i]];
+        const int px = q >> 16, py = q & 0xFFFF;
         const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
         const int d = layer_dir(g, c);
         double w1 = 0.0, w2 = 0.0;
@@ -154,7 +166,7 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     __syncthreads();
     for (int r = lane; r < n * L; r += W) {
         const int i = r / L, lo = r % L;
-        const int fixed = s.nd[i] / g.XY;
+        const int fixed = s.lay[i];
         double* row = s.dist + size_t(i) * T + tri<L>(lo, lo);  // row[h - lo] = interval (lo, h)
         double v = 0.0;
         row[0] = 0.0;
@@ -171,26 +183,34 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
         const int i = n - t;  // this net's node at this step (none once i < 1)
         const bool act = i >= 1;
         const int p = act ? s.par[i] : 0;
-        const int vi = act ? s.nd[i] : 0, vp = act ? s.nd[p] : 0;
-        const bool straight = (vi / Y % X == vp / Y % X) || (vi % Y == vp % Y);
-        // phase A: lane c < L, best interval containing layer c, first
-        // minimum in (low, high) row-major order; lanes L..3L-1 fold the
-        // corner via columns of a bent edge
+        const int vi = act ? s.xy[i] : 0, vp = act ? s.xy[p] : 0;
+        const int x = vi >> 16, y = vi & 0xFFFF, px = vp >> 16, py = vp & 0xFFFF;
+        const bool straight = x == px || y == py;
+        // phase A: the best interval containing each layer c, first minimum
+        // in (low, high) row-major order, in two steps.  A1: lane lo < L
+        // takes the suffix minima of row lo (the smallest high end among
+        // equal values); lanes L..3L-1 fold the corner via columns of a bent
+        // edge.  A2: lane c takes the first minimum over lo <= c of row lo's
+        // suffix minimum at c.
+        const double* nd = s.dist + size_t(act ? i : 0) * T;
         for (int q = act ? lane : 3 * L; q < 3 * L; q += W) {
           if (q < L) {
-            const int c = q;
-            const double* nd = s.dist + size_t(i) * T;
-            double best = kInf;
-            int ch = 0;
-            for (int lo = 0; lo <= c; ++lo)
+            const int lo = q;
+            double m = nd[tri<L>(lo, L - 1)];
+            int mh = L - 1;
+            s.rmin[lo * L + L - 1] = m;
+            s.rarg[lo * L + L - 1] = mh;
 #pragma unroll
-                for (int hi = 0; hi < L; ++hi)
-                    if (hi >= c && nd[tri<L>(lo, hi)] < best) {
-                        best = nd[tri<L>(lo, hi)];
-                        ch = lo * L + hi;
-                    }
-            s.best[c] = best;
-            s.choice[c] = ch;
+            for (int hi = L - 2; hi >= 0; --hi) {
+                if (hi < lo) continue;
+                const double v = nd[tri<L>(lo, hi)];
+                if (v <= m) {
+                    m = v;
+                    mh = hi;
+                }
+                s.rmin[lo * L + hi] = m;
+                s.rarg[lo * L + hi] = mh;
+            }
           } else if (!straight) {
             // fold[k][lo][hi] = sum of corner[lo+1 .. hi-1], left to right
             const int k = (q - L) / L, lo = (q - L) % L;
@@ -206,13 +226,28 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
           }
         }
         __syncthreads();
+        for (int c = act ? lane : L; c < L; c += W) {
+            double best = kInf;
+            int ch = 0;
+#pragma unroll
+            for (int lo = 0; lo < L; ++lo) {
+                if (lo > c) continue;
+                const double v = s.rmin[lo * L + c];
+                if (v < best) {
+                    best = v;
+                    ch = lo * L + s.rarg[lo * L + c];
+                }
+            }
+            s.best[c] = best;
+            s.choice[c] = ch;
+        }
+        __syncthreads();
         // phase B (src/routing/lshape_warp.cuh:18-48)
         for (int pl = act ? lane : L; pl < L; pl += W) {
             double best = kInf;
             int bp = 0;
             if (straight) {
                 const int d = layer_dir(g, pl);
-                const int x = vi / Y % X, y = vi % Y, px = vp / Y % X, py = vp % Y;
                 if (!((d == 0 && y != py) || (d == 1 && x != px))) {
                     best = s.best[pl] + s.w1[i * L + pl];
                     bp = s.choice[pl] * LL + pl * L + pl;
@@ -253,15 +288,32 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     }
     // root choice and traceback (src/routing/lshape_warp.cuh:193-213)
     const int base_word = desc.z, cap = desc.w;
+    // the root's first minimum in row-major order: per row its first
+    // minimum over high (lanes), then the first minimum over the rows
+    for (int lo = live ? lane : L; lo < L; lo += W) {
+        double m = s.dist[tri<L>(lo, lo)];
+        int mh = lo;
+#pragma unroll
+        for (int hi = 1; hi < L; ++hi) {
+            if (hi <= lo) continue;
+            const double v = s.dist[tri<L>(lo, hi)];
+            if (v < m) {
+                m = v;
+                mh = hi;
+            }
+        }
+        s.rmin[lo] = m;
+        s.rarg[lo] = mh;
+    }
+    __syncthreads();
     if (lane == 0 && live) {
-        int ch = 0;
-        double cv = s.dist[0];
-        for (int lo = 0; lo < L; ++lo)
-            for (int hi = lo; hi < L; ++hi)
-                if (s.dist[tri<L>(lo, hi)] < cv) {
-                    cv = s.dist[tri<L>(lo, hi)];
-                    ch = lo * L + hi;
-                }
+        int ch = s.rarg[0];
+        double cv = s.rmin[0];
+        for (int lo = 1; lo < L; ++lo)
+            if (s.rmin[lo] < cv) {
+                cv = s.rmin[lo];
+                ch = lo * L + s.rarg[lo];
+            }
         s.rng[0] = ch;
         s.path[0] = 0;
         bool ok = cv < kInf;
@@ -285,8 +337,8 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
             s.woff[i] = total;
             int w = s.rng[i] / L < s.rng[i] % L ? 2 : 0;
             if (i) {
-                const int v = s.nd[i], q = s.nd[s.par[i]];
-                const bool st = (v / Y % X == q / Y % X) || (v % Y == q % Y);
+                const int v = s.xy[i], q = s.xy[s.par[i]];
+                const bool st = (v >> 16) == (q >> 16) || (v & 0xFFFF) == (q & 0xFFFF);
                 w += st ? 2 : 6;
             }
             total += w;
@@ -303,14 +355,14 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
         int* out = routes + s.woff[i];
         int c = 0;
         const int lo = s.rng[i] / L, hi = s.rng[i] % L;
-        const int x = s.nd[i] / Y % X, y = s.nd[i] % Y;
+        const int x = s.xy[i] >> 16, y = s.xy[i] & 0xFFFF;
         if (lo < hi) {
             out[c++] = idx3(g, lo, x, y);
             out[c++] = idx3(g, hi, x, y);
         }
         if (i) {
-            const int q = s.nd[s.par[i]];
-            const int px = q / Y % X, py = q % Y;
+            const int q = s.xy[s.par[i]];
+            const int px = q >> 16, py = q & 0xFFFF;
             const int cl = s.path[i] / L % L, pl = s.path[i] % L;
             if (px == x || py == y) {
                 out[c++] = idx3(g, cl, min(x, px), min(y, py));
