@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--net-scale", type=float, default=1.0)
-    ap.add_argument("--mode", type=int, default=0, help="0 plain launches, 1 HIP graph, 2 persistent")
+    ap.add_argument("--mode", type=int, default=1, help="0 plain launches, 1 the batch loop captured once in a HIP graph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")

@@ -93,6 +93,10 @@ struct Workspace {
 struct gr_ctx {
     DeviceArena mem;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for the large-tree DP launch of a
+    // batch, which runs beside the small-tree launch; fork/join events
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int cus = 0;
     DeviceGrid g{};
     int tracks = 0, maxlen = 0;
@@ -200,6 +204,9 @@ struct gr_ctx {
     ~gr_ctx() {
         drop_graph();
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+        if (side) (void)hipStreamDestroy(side);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -217,17 +224,19 @@ hipEvent_t take_event(gr_ctx* c) {
     return c->events[c->next_event++];
 }
 
-// Run `launch` between two events on the context stream when profiling.
+// Run `launch` between two events on `stream` (the context stream unless
+// given) when profiling.
 template <class F>
-void timed(gr_ctx* c, int kind, double bytes, F&& launch) {
+void timed(gr_ctx* c, int kind, double bytes, F&& launch, hipStream_t stream = nullptr) {
     if (!c->profiling || !(c->prof_mask >> kind & 1u)) {
         launch();
         return;
     }
+    if (!stream) stream = c->stream;
     hipEvent_t a = take_event(c), b = take_event(c);
-    hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+    hip_check(hipEventRecord(a, stream), "hipEventRecord");
     launch();
-    hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+    hip_check(hipEventRecord(b, stream), "hipEventRecord");
     c->prof.push_back({kind, a, b, bytes});
 }
 
@@ -383,8 +392,10 @@ int per_wave_for(gr_ctx* c, int k, int nmax) {
     return p;
 }
 
-void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_override = 0) {
+void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_override = 0,
+                         hipStream_t stream = nullptr) {
     if (!b.count[k]) return;
+    if (!stream) stream = c->stream;
     LshapeArgs a;
     a.g = c->g;
     a.vcost = c->vcost;
@@ -405,18 +416,29 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
     const int p = per_wave_override ? per_wave_override : per_wave_for(c, k, a.nmax);
     if (p > 0) {
         a.per_wave = p;
-        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, c->stream); });
+        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, stream); }, stream);
     } else {
         const size_t bytes = lshape_scratch_bytes(c->g.L, a.nmax);
         if (((bytes + 7) / 8) * 8 * size_t(a.count) > c->gscratch_bytes) throw GrError(GR_E_STATE, "DP scratch not reserved");
         a.per_wave = 1;
-        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, a.count, false, c->stream); });
+        timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, a.count, false, stream); }, stream);
     }
     hip_check(hipGetLastError(), "lshape_dp kernels");
 }
 
+// The two size classes of a batch are independent: the large trees (the
+// batch's critical path) run on the side stream beside the small ones.
 void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
-    for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k);
+    if (!b.count[0] || !b.count[1]) {
+        for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k);
+        return;
+    }
+    hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
+    launch_lshape_class(c, b, 1, 0, c->side);
+    launch_lshape_class(c, b, 0, 0, c->stream);
+    hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
 }
 
 // Reserve the global DP scratch the batches will need (trees whose tables
@@ -484,6 +506,9 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
         c->cus = prop.multiProcessorCount;
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming), "hipEventCreate");
         DeviceGrid& g = c->g;
         g.L = grid->L, g.X = grid->X, g.Y = grid->Y, g.DIR = grid->DIR & 1, g.XY = grid->X * grid->Y;
         g.unit_wire = grid->unit_length_wire_cost;
