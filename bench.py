@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the nets-per-wavefront autotuner (one net / DAG per wavefront), e.g. for counter runs")
     ap.add_argument("--stage", choices=("lshape", "zshape", "soft", "flt"), default="lshape",
                     help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd; "
                          "flt: the FLT rip-up stage after the L-shape stage")
@@ -129,7 +131,7 @@ def main():
     tsched = ga.execution_schedule(topo, tcolor, tnb)
     ctx.lshape_upload(tsched["net_ids"], tsched["node_off"], tsched["nodes"], tsched["parents"])
     t0 = time.perf_counter()
-    per_wave, tune_ms = ctx.lshape_tune(tsched["batch_off"])
+    per_wave, tune_ms = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.lshape_tune(tsched["batch_off"])
     tune_s = time.perf_counter() - t0
     print(f"[bench] autotuner: nets per wavefront {per_wave} (classes <=16 nodes, larger); "
           f"DP ms per candidate 1/2/4/8: {[[round(v, 2) for v in row] for row in tune_ms]} ({tune_s:.2f} s)",
@@ -301,7 +303,7 @@ def bench_flt(args, ga, design, topo):
     ctx.batch_geometry(*fgeom)
     tcolor, tnb = ctx.batch_colors(limit=300000)
     ctx.flt_upload(design.flt_plan(tcolor, tnb))
-    per_wave, tune_ms = ctx.flt_tune()
+    per_wave, tune_ms = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.flt_tune()
     print(f"[bench] flt autotuner: DAGs per wavefront {per_wave} (batches <512, >=512 DAGs); "
           f"DP ms per candidate 1/2/4/8: {[[round(v, 2) for v in row] for row in tune_ms]}",
           file=sys.stderr, flush=True)
