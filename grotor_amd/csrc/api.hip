@@ -1695,7 +1695,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             rcount = cv.template take<int>(n);
             d_color = cv.template take<int>(n);
             d_count = cv.template take<int>(kMaxColors);
-            d_misc = cv.template take<int>(1);  // dequeue counter
+            d_misc = cv.template take<int>(kColorQueues * kColorQueueStride);  // dequeue counters
             d_closed = cv.template take<int>(2 * kMaxColors);
             tmp = cv.template take<char>(std::max(sort_bytes, std::max(sum_bytes, max_bytes)));
         });
@@ -1749,7 +1749,8 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             hipLaunchKernelGGL(fill_int_kernel, dim3((n - first + 255) / 256), dim3(256), 0, st, d_color + first, -1,
                                (long long)(n - first));
             hip_check(hipMemsetAsync(d_count, 0, sizeof(int) * kMaxColors, st), "clear color counts");
-            upload(d_misc, &first, sizeof(int), st);
+            hip_check(hipMemsetAsync(d_misc, 0, sizeof(int) * kColorQueues * kColorQueueStride, st),
+                      "clear dequeue counters");
             const int nclosed = int(closed_color.size());
             if (nclosed) {
                 upload(d_closed, closed_color.data(), sizeof(int) * nclosed, st);
@@ -1765,6 +1766,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             ca.mask_words = words;
             ca.color = d_color;
             ca.next = d_misc;
+            ca.first = first;
             ca.error = c->error;
             ca.spin_limit = 1ll << 22;
             ca.nclosed = nclosed;

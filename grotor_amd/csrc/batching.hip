@@ -26,15 +26,24 @@
 //      wavefront owning net k waits for the prefix masks of the previous
 //      readers and for the colors of the entries in its ranges, publishes its
 //      own prefix masks, takes the minimum excluded color and publishes it.
-// Nets are dequeued kColorChunk at a time, one wave finishing them in order,
-// so the dequeue counter is not a serialization point.
+// A wave holds one net at a time: a wave that held several consecutive nets
+// would finish them one after the other, and every net waiting on a later one
+// of them would wait for the whole run.  Nets are split over kColorQueues
+// queues by residue (net - first) % kColorQueues, each handed out in
+// increasing order by its own counter, and a wave dequeues its next net while
+// it colors the current one, so the dequeue latency is hidden.
 // Every value another wavefront waits on is a single word written once: a
 // net's color (-1 until set) or one 64-bit prefix-mask word whose top bit
 // says it is ready (63 colors per word).  Waits are therefore relaxed atomic
 // loads of that one word, with no fences: nothing is published through a
-// second location.  Each wait is on a strictly earlier net, which was
-// dequeued earlier, so the earliest unfinished net always progresses and
-// every wave exits once the dequeue counter passes n.  Each list entry is
+// second location.  Progress: each wait is on a strictly earlier net.  Let
+// k be the earliest uncolored net, in queue q.  If a wave holds k as its
+// current net, every net k waits on is colored, so k completes; if a wave
+// holds it as its next net, that wave's current net is earlier, hence
+// colored, and the wave moves on to k.  Otherwise every net that queue q's
+// waves hold is earlier than k, so they are colored and the next dequeue of
+// queue q returns k.  The first kColorQueues waves of the grid cover every
+// queue, and a wave exits only after its queue passes n.  Each list entry is
 // read by at most one reader, so the work is linear in points + marks.
 // The batch size limit is applied by the host: colors stay exact up to the
 // first net that would overfill a batch; from that net on the batch is
@@ -184,16 +193,19 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
     const int W = a.mask_words;
     unsigned long long* P = lds;
     unsigned long long* forbidden = lds + kWave * W;
-    int k = 0, chunk_end = 0;
-    while (true) {
-        // nets are taken in priority order, kColorChunk at a time
-        if (k == chunk_end) {
-            if (lane == 0) s_k = atomicAdd(a.next, kColorChunk);
-            __syncthreads();
-            k = s_k;
-            chunk_end = min(k + kColorChunk, a.n);
-        }
-        if (k >= a.n) break;
+    const int q = blockIdx.x % kColorQueues;
+    int* counter = a.next + q * kColorQueueStride;
+    // net of the c-th dequeue of queue q, a.n once the queue is exhausted
+    auto net_of = [&](int c) {
+        const long long k = (long long)a.first + q + (long long)kColorQueues * c;
+        return k < a.n ? (int)k : a.n;
+    };
+    if (lane == 0) s_k = atomicAdd(counter, 1);
+    __syncthreads();
+    int k = net_of(s_k);
+    while (k < a.n) {
+        int next = 0;
+        if (lane == 0) next = atomicAdd(counter, 1);  // result used after net k
         for (int w = lane; w < W; w += kWave) forbidden[w] = 0ull;
         __syncthreads();
         for (int i = lane; i < a.nclosed; i += kWave) {  // batches already full when k is placed
@@ -254,9 +266,10 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
                 mex = 0;
             }
             __hip_atomic_store(a.color + k, mex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_k = next;
         }
         __syncthreads();
-        ++k;
+        k = net_of(s_k);
     }
 }
 

@@ -124,14 +124,18 @@ struct ColorArgs {
     unsigned long long* pm;  // per slot: mask_words prefix-mask words
     int mask_words;
     int* color;           // per net, -1 until set
-    int* next;            // dequeue counter
+    int* next;            // kColorQueues dequeue counters, kColorQueueStride ints apart, zeroed
+    int first;            // first net to color
     int* error;
     long long spin_limit;
     int nclosed;          // full batches: color closed_color[i] for nets >= closed_from[i]
     const int* closed_color;
     const int* closed_from;
 };
-constexpr int kColorChunk = 4;  // nets per dequeue
+// Wave w takes nets first + q + kColorQueues * c, q = w % kColorQueues, c from
+// queue q's counter, one net per dequeue; the queues spread the atomics.
+constexpr int kColorQueues = 16;
+constexpr int kColorQueueStride = 32;  // one 128-byte line per counter
 __global__ void color_kernel(ColorArgs a);
 __global__ void color_hist_kernel(const int* color, int first, int n, int* counts);
 
