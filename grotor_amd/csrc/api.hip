@@ -119,6 +119,7 @@ struct gr_ctx {
     int stage_count = 0;
     std::vector<int> node_off_h;
     int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr;
+    int *lvl = nullptr, *link = nullptr;  // L-shape DP level order and child lists, per stage node
     std::vector<int> net_ids_h;
     std::vector<int> slots_h;      // stage slot per execution position
     int4* desc = nullptr;          // per execution position: node begin, nodes, route word, route capacity
@@ -214,6 +215,7 @@ struct gr_ctx {
 namespace {
 
 constexpr int kSmallTree = 16;
+constexpr int kMaxTreeNodes = 32767;  // node indices and heights are packed in 16 bits
 
 hipEvent_t take_event(gr_ctx* c) {
     if (c->next_event == c->events.size()) {
@@ -404,6 +406,8 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
     a.desc = c->desc;
     a.nodes = c->nodes;
     a.parents = c->parents;
+    a.lvl = c->lvl;
+    a.link = c->link;
     a.gscratch = c->gscratch;
     a.error = c->error;
     a.begin = b.begin;
@@ -694,6 +698,7 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
             if (net_ids[k] < 0 || net_ids[k] >= c->net_count) throw GrError(GR_E_ARG, "net id outside route storage");
             const int n = node_off[k + 1] - node_off[k];
             if (n < 1) throw GrError(GR_E_ARG, "empty tree");
+            if (n > kMaxTreeNodes) throw GrError(GR_E_ARG, "tree has more than 32767 nodes");
             if (parents[node_off[k]] != -1) throw GrError(GR_E_ARG, "tree root must have parent -1");
             for (int i = 1; i < n; ++i)
                 if (parents[node_off[k] + i] < 0 || parents[node_off[k] + i] >= i)
@@ -703,12 +708,12 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
                 if (v < 0 || v >= (c->g.L + 1) * c->g.XY) throw GrError(GR_E_ARG, "node index outside the grid");
             }
         }
-        for (int* p : {c->net_ids, c->node_off, c->nodes, c->parents}) {
+        for (int* p : {c->net_ids, c->node_off, c->nodes, c->parents, c->lvl, c->link}) {
             int* q = p;
             c->mem.release(q);
         }
         c->mem.release(c->desc);
-        c->net_ids = c->node_off = c->nodes = c->parents = nullptr;
+        c->net_ids = c->node_off = c->nodes = c->parents = c->lvl = c->link = nullptr;
         c->slots_h.clear();
         c->desc_h.clear();  // the new descriptor buffer holds nothing yet
         c->soft_ready = false;
@@ -727,6 +732,35 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
         upload(c->node_off, node_off, sizeof(int) * (count + 1), c->stream);
         upload(c->nodes, nodes, sizeof(int) * total, c->stream);
         upload(c->parents, parents, sizeof(int) * total, c->stream);
+        {
+            // per net: node heights (leaves 0), the nodes ordered by (height,
+            // index), and each node's children in descending index as a list
+            std::vector<int> lvl(total), link(total), height, cnt, first, last;
+            for (int k = 0; k < count; ++k) {
+                const int b = node_off[k], n = node_off[k + 1] - b;
+                height.assign(n, 0);
+                first.assign(n, -1);
+                last.assign(n, -1);
+                std::vector<int> next(n, -1);
+                for (int i = n - 1; i >= 1; --i) {
+                    const int p = parents[b + i];
+                    height[p] = std::max(height[p], height[i] + 1);
+                    if (first[p] < 0) first[p] = i;
+                    else next[last[p]] = i;
+                    last[p] = i;
+                }
+                cnt.assign(n + 1, 0);
+                for (int i = 0; i < n; ++i) ++cnt[height[i] + 1];
+                for (int h = 0; h < n; ++h) cnt[h + 1] += cnt[h];
+                for (int i = 0; i < n; ++i) lvl[b + cnt[height[i]]++] = i | height[i] << 16;
+                for (int i = 0; i < n; ++i) link[b + i] = (first[i] + 1) | (next[i] + 1) << 16;
+            }
+            c->lvl = c->mem.alloc<int>(total);
+            c->link = c->mem.alloc<int>(total);
+            upload(c->lvl, lvl.data(), sizeof(int) * total, c->stream);
+            upload(c->link, link.data(), sizeof(int) * total, c->stream);
+            hip_check(hipStreamSynchronize(c->stream), "stage upload");
+        }
         // algorithmic bytes of the DP per net: tree input 8n, via stacks
         // 4*L*n, straight edges 16*L, bent edges 40*L, route words 32n
         const int L = c->g.L, X = c->g.X, Y = c->g.Y;

@@ -69,6 +69,8 @@ struct LshapeArgs {
     const int4* desc;  // per execution position: node begin, nodes, first route word, route capacity
     const int* nodes;
     const int* parents;
+    const int* lvl;    // per net position q: node of the level order | its height << 16
+    const int* link;   // per node: (first child + 1) | (next smaller sibling + 1) << 16
     int begin, count;
     int nmax;          // node capacity of one net's tables
     int per_wave;      // nets per wavefront: 1, 2, 4 or 8 (each gets 64 / per_wave lanes)

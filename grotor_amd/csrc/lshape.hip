@@ -5,37 +5,41 @@
 // (src/routing/lshape_thread.cuh:18-171).
 //
 // One 64-lane wavefront (one workgroup) owns 1, 2, 4 or 8 nets (nets per
-// wavefront, a launch parameter chosen by the autotuner): each net gets a
-// group of 64/P lanes and its own tables.  Lanes run across layers instead
-// of across tree nodes: nodes are solved one at a time in descending
-// preorder index, which is the scalar kernel's order and makes every parent
-// accumulate its children's contributions in descending child order without
-// floating-point atomics.  Per node the work splits into three lane-parallel
-// phases separated by workgroup barriers (the nets of a wavefront step
-// through their nodes together; a net with fewer nodes idles):
+// wavefront, a launch parameter chosen by the autotuner): each net gets
+// 64/P lanes and its own tables.  Within a net, lanes run across layers and
+// across tree nodes: the net's lanes form G = max(1, (64/P) / L) groups and
+// every group solves one node, lane c of a group taking layer c.  Nodes are
+// solved level by level, a node's level being its height (leaves 0, a parent
+// one more than its highest child), so all nodes of a level are independent.
+// Per node (lshape_warp.cuh:103-189):
+//   G  lane lo: the node's interval table row lo receives its children's
+//      closures (each child's best arrival per layer, closed over [lo, hi]
+//      with the first minimum), children in descending preorder index, which
+//      is the scalar kernel's order of accumulation, so every double is formed
+//      by the same additions in the same order;
 //   A  lane c: best interval containing layer c (first minimum in
-//      (low, high) row-major order); lanes L..3L-1 fold the corner via
-//      columns of a bent edge into per-(low, high) partial sums, in the
-//      reference's left-to-right order;
+//      (low, high) row-major order);
 //   B  lane c: best arrival on parent layer c (straight edge, or the first
-//      minimum over the opposite-parity child layers of a bent edge);
-//   C  lane c: interval closure of row low = c and accumulation into the
-//      parent's table.
+//      minimum over the opposite-parity child layers of a bent edge, with the
+//      corner via column summed left to right as the reference does).
+// Three wavefront barriers per level instead of four per node.  The
+// traceback walks the levels top down, one lane per node, and the route
+// words are placed by a wavefront prefix sum.  Heights, the level order and
+// the child lists are computed on the host when the stage is uploaded.
 // All costs the sweep reads (prefix-sum differences and via costs) are
 // gathered for every node before the sweep starts, so the sequential part
 // of the net touches LDS only.  The layer count is a template parameter.
 // The interval tables live in LDS (or, for trees too large for LDS, in a
 // per-workgroup slice of global scratch); traceback recomputes the closure
-// argmin from the stored per-node diagonal, so no per-interval traceback
-// table is kept.  Results equal the reference's bit for bit: every double
-// is formed by the same operations in the same order.
+// argmin from the stored per-node best arrivals, so no per-interval
+// traceback table is kept.  Results equal the reference's bit for bit.
 #include "gr_kernels.hpp"
 
 namespace grotor {
 
 namespace {
 
-// Per-workgroup tables for trees of up to nmax nodes; L is a compile-time
+// Per-net tables for trees of up to nmax nodes; L is a compile-time
 // constant so that every per-layer loop unrolls.
 template <int L>
 struct Tables {
@@ -43,22 +47,22 @@ struct Tables {
     double* diag;   // nmax * L      best arrival on each parent layer
     double* w1;     // nmax * L      first-leg wire (child side; the whole edge when straight)
     double* w2;     // nmax * L      second-leg wire (parent side, bent edges)
-    double* best;   // L             best interval containing each layer (current node)
-    double* fold;   // 2 * L * L     corner via sums (va, vb) of the current node
+    double* gbest;  // kWave         per lane group: best interval containing each layer
     float* vs;      // nmax * L      via costs on the node's column
     float* va;      // nmax * L      via costs at corner (x, py)
     float* vb;      // nmax * L      via costs at corner (px, y)
     int* dprev;     // nmax * L
+    int* gchoice;   // kWave         per lane group: that interval, low * L + high
     int* nd;        // nmax
     int* par;       // nmax
     int* rng;       // nmax
     int* path;      // nmax
-    int* choice;    // L
-    int* woff;      // nmax + 1
     int* xy;        // nmax          node position x << 16 | y
     int* lay;       // nmax          node layer (L for Steiner nodes)
-    double* rmin;   // L * L         row suffix minima of the current node's table
-    int* rarg;      // L * L         their high ends
+    int* order;     // nmax          nodes by (height, index)
+    int* link;      // nmax          (first child + 1) | (next smaller sibling + 1) << 16
+    int* woff;      // nmax + 1
+    int* lstart;    // nmax + 1      first order position of each level
 };
 
 // entries of an interval table: intervals low <= high of L layers
@@ -73,8 +77,8 @@ __host__ __device__ constexpr int tri(int lo, int hi) {
 
 template <int L>
 __host__ __device__ constexpr size_t table_bytes(int nmax) {
-    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + L + 3 * L * L) + 4 * (size_t(nmax) * 3 * L) +
-           4 * (size_t(nmax) * L + 4 * size_t(nmax) + L + nmax + 1) + 4 * (2 * size_t(nmax) + L * L) + 8;
+    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + kWave) + 4 * (size_t(nmax) * 3 * L) +
+           4 * (size_t(nmax) * L + kWave + 8 * size_t(nmax) + 2 * (size_t(nmax) + 1));
 }
 
 template <int L>
@@ -84,24 +88,22 @@ __device__ inline Tables<L> carve(double* base, int nmax) {
     s.diag = s.dist + size_t(nmax) * kTri<L>;
     s.w1 = s.diag + size_t(nmax) * L;
     s.w2 = s.w1 + size_t(nmax) * L;
-    s.best = s.w2 + size_t(nmax) * L;
-    s.fold = s.best + L;
-    s.vs = reinterpret_cast<float*>(s.fold + 2 * L * L);
+    s.gbest = s.w2 + size_t(nmax) * L;
+    s.vs = reinterpret_cast<float*>(s.gbest + kWave);
     s.va = s.vs + size_t(nmax) * L;
     s.vb = s.va + size_t(nmax) * L;
     s.dprev = reinterpret_cast<int*>(s.vb + size_t(nmax) * L);
-    s.nd = s.dprev + size_t(nmax) * L;
+    s.gchoice = s.dprev + size_t(nmax) * L;
+    s.nd = s.gchoice + kWave;
     s.par = s.nd + nmax;
     s.rng = s.par + nmax;
     s.path = s.rng + nmax;
-    s.choice = s.path + nmax;
-    s.woff = s.choice + L;
-    s.xy = s.woff + nmax + 1;
+    s.xy = s.path + nmax;
     s.lay = s.xy + nmax;
-    s.rarg = s.lay + nmax;
-    // doubles after the ints, 8-byte aligned
-    const size_t ints = size_t(s.rarg + L * L - reinterpret_cast<int*>(base));
-    s.rmin = base + (ints + 1) / 2;
+    s.order = s.lay + nmax;
+    s.link = s.order + nmax;
+    s.woff = s.link + nmax;
+    s.lstart = s.woff + nmax + 1;
     return s;
 }
 
@@ -113,11 +115,44 @@ __device__ __forceinline__ double wire_cost(const DeviceGrid& g, const double* p
     return presum[cell_of(g, l, xmax, ymax)] - presum[cell_of(g, l, xmin, ymin)];
 }
 
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = kWave / 2; off; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// Row lo of node i's table receives the closures of i's children, in
+// descending child index (lshape_warp.cuh:165-189).  The loop runs over
+// every high end with selects instead of branches (row[h - lo] for h < lo
+// still lies inside node i's table), so the row stays in registers.
+template <int L>
+__device__ __forceinline__ void gather_children(const Tables<L>& s, int i, int lo) {
+    double* row = s.dist + size_t(i) * kTri<L> + tri<L>(lo, lo);  // row[h - lo] = interval (lo, h)
+    double acc[L];
+#pragma unroll
+    for (int hi = 0; hi < L; ++hi) acc[hi] = row[hi - lo];
+    for (int k = (s.link[i] & 0xFFFF) - 1; k >= 0; k = (s.link[k] >> 16) - 1) {
+        const double* dg = s.diag + k * L;
+        double c = dg[lo];
+#pragma unroll
+        for (int hi = 0; hi < L; ++hi) {
+            const double d = dg[hi];
+            if (hi > lo && !(c <= d)) c = d;
+            const double sum = acc[hi] + c;
+            acc[hi] = hi >= lo ? sum : acc[hi];
+        }
+    }
+#pragma unroll
+    for (int hi = 0; hi < L; ++hi)
+        if (hi >= lo) row[hi - lo] = acc[hi];
+}
+
 template <int L>
 __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     constexpr int LL = L * L, T = kTri<L>;
     const int P = a.per_wave, W = kWave / P;
-    const int sub = threadIdx.x / W, lane = threadIdx.x % W;  // lane within the net's group
+    const int sub = threadIdx.x / W, lane = threadIdx.x % W;  // lane within the net's lanes
+    const int G = max(1, W / L), GL = W / G;                   // lane groups of the net, lanes per group
+    const int grp = lane / GL, gl = lane % GL;                 // grp >= G: an idle lane
     const int slot = blockIdx.x * P + sub;
     const bool live = slot < a.count;
     const Tables<L> s = carve<L>(base + words * sub, a.nmax);
@@ -125,15 +160,21 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
     const int X = g.X, Y = g.Y;
     const int4 desc = live ? a.desc[a.begin + slot] : make_int4(0, 0, 0, 0);
     const int n0 = desc.x, n = desc.y;
-    int steps = n;  // nodes of the wavefront's largest net
-    for (int off = kWave / 2; off; off >>= 1) steps = max(steps, __shfl_xor(steps, off, kWave));
+    // height of the root = number of levels below it
+    const int H = live ? a.lvl[n0 + n - 1] >> 16 : 0;
+    const int hmax = wave_max(H);
     for (int i = lane; i < n; i += W) {
         const int v = a.nodes[n0 + i];
         s.nd[i] = v;
         s.par[i] = a.parents[n0 + i];
         s.xy[i] = (v / Y % X) << 16 | (v % Y);  // decoded once: no divisions below
         s.lay[i] = v / g.XY;
+        s.link[i] = a.link[n0 + i];
+        const int q = a.lvl[n0 + i], h = q >> 16;
+        s.order[i] = q & 0xFFFF;
+        if (i == 0 || (a.lvl[n0 + i - 1] >> 16) != h) s.lstart[h] = i;
     }
+    if (live && lane == 0) s.lstart[H + 1] = n;
     __syncthreads();
     // Every cost the DP reads, gathered up front for all nodes so that the
     // bottom-up sweep below touches only LDS: the via stack at each node
@@ -179,117 +220,99 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
                 if (lo > fixed || hi < fixed) row[hi - lo] = kInf;
     }
     __syncthreads();
-    for (int t = 1; t < steps; ++t) {
-        const int i = n - t;  // this net's node at this step (none once i < 1)
-        const bool act = i >= 1;
-        const int p = act ? s.par[i] : 0;
-        const int vi = act ? s.xy[i] : 0, vp = act ? s.xy[p] : 0;
-        const int x = vi >> 16, y = vi & 0xFFFF, px = vp >> 16, py = vp & 0xFFFF;
-        const bool straight = x == px || y == py;
-        // phase A: the best interval containing each layer c, first minimum
-        // in (low, high) row-major order, in two steps.  A1: lane lo < L
-        // takes the suffix minima of row lo (the smallest high end among
-        // equal values); lanes L..3L-1 fold the corner via columns of a bent
-        // edge.  A2: lane c takes the first minimum over lo <= c of row lo's
-        // suffix minimum at c.
-        const double* nd = s.dist + size_t(act ? i : 0) * T;
-        for (int q = act ? lane : 3 * L; q < 3 * L; q += W) {
-          if (q < L) {
-            const int lo = q;
-            double m = nd[tri<L>(lo, L - 1)];
-            int mh = L - 1;
-            s.rmin[lo * L + L - 1] = m;
-            s.rarg[lo * L + L - 1] = mh;
-#pragma unroll
-            for (int hi = L - 2; hi >= 0; --hi) {
-                if (hi < lo) continue;
-                const double v = nd[tri<L>(lo, hi)];
-                if (v <= m) {
-                    m = v;
-                    mh = hi;
-                }
-                s.rmin[lo * L + hi] = m;
-                s.rarg[lo * L + hi] = mh;
+    for (int lv = 0; lv < hmax; ++lv) {
+        const bool lact = live && lv < H;
+        const int q0 = lact ? s.lstart[lv] : 0, cnt = lact ? s.lstart[lv + 1] - q0 : 0;
+        const int chunks = wave_max((cnt + G - 1) / G);
+        for (int ch = 0; ch < chunks; ++ch) {
+            const int k = ch * G + grp;
+            const bool act = lact && grp < G && k < cnt;
+            const int i = act ? s.order[q0 + k] : 0;
+            if (lv > 0) {  // level-0 nodes are leaves
+                if (act)
+                    for (int lo = gl; lo < L; lo += GL) gather_children<L>(s, i, lo);
+                __syncthreads();
             }
-          } else if (!straight) {
-            // fold[k][lo][hi] = sum of corner[lo+1 .. hi-1], left to right
-            const int k = (q - L) / L, lo = (q - L) % L;
-            const float* corner = (k ? s.vb : s.va) + i * L;
-            double* f = s.fold + k * LL + lo * L;
-            double v = 0.0;
+            // phase A: the best interval containing each layer c, first
+            // minimum in (low, high) row-major order
+            if (act) {
+                const double* nd = s.dist + size_t(i) * T;
+                for (int c = gl; c < L; c += GL) {
+                    double best = kInf;
+                    int chs = 0;
 #pragma unroll
-            for (int hi = 0; hi < L; ++hi) {
-                if (hi <= lo) continue;
-                if (hi > lo + 1) v += (double)corner[hi - 1];
-                f[hi] = v;
-            }
-          }
-        }
-        __syncthreads();
-        for (int c = act ? lane : L; c < L; c += W) {
-            double best = kInf;
-            int ch = 0;
+                    for (int lo = 0; lo < L; ++lo)
 #pragma unroll
-            for (int lo = 0; lo < L; ++lo) {
-                if (lo > c) continue;
-                const double v = s.rmin[lo * L + c];
-                if (v < best) {
-                    best = v;
-                    ch = lo * L + s.rarg[lo * L + c];
+                        for (int hi = lo; hi < L; ++hi) {
+                            const double v = nd[tri<L>(lo, hi)];
+                            if (lo <= c && c <= hi && v < best) {
+                                best = v;
+                                chs = lo * L + hi;
+                            }
+                        }
+                    s.gbest[grp * L + c] = best;
+                    s.gchoice[grp * L + c] = chs;
                 }
             }
-            s.best[c] = best;
-            s.choice[c] = ch;
-        }
-        __syncthreads();
-        // phase B (src/routing/lshape_warp.cuh:18-48)
-        for (int pl = act ? lane : L; pl < L; pl += W) {
-            double best = kInf;
-            int bp = 0;
-            if (straight) {
-                const int d = layer_dir(g, pl);
-                if (!((d == 0 && y != py) || (d == 1 && x != px))) {
-                    best = s.best[pl] + s.w1[i * L + pl];
-                    bp = s.choice[pl] * LL + pl * L + pl;
-                }
-            } else {
-                const double second = s.w2[i * L + pl];
+            __syncthreads();
+            // phase B (src/routing/lshape_warp.cuh:18-48)
+            if (act) {
+                const int vi = s.xy[i], vp = s.xy[s.par[i]];
+                const int x = vi >> 16, y = vi & 0xFFFF, px = vp >> 16, py = vp & 0xFFFF;
+                const double* gb = s.gbest + grp * L;
+                const int* gc = s.gchoice + grp * L;
+                for (int pl = gl; pl < L; pl += GL) {
+                    double best = kInf;
+                    int bp = 0;
+                    if (x == px || y == py) {
+                        const int d = layer_dir(g, pl);
+                        if (!((d == 0 && y != py) || (d == 1 && x != px))) {
+                            best = gb[pl] + s.w1[i * L + pl];
+                            bp = gc[pl] * LL + pl * L + pl;
+                        }
+                    } else {
+                        const double second = s.w2[i * L + pl];
+                        // every candidate layer has the opposite parity, hence
+                        // one direction and one corner column: (x, py) for a
+                        // vertical first leg, (px, y) otherwise
+                        const float* corner = (layer_dir(g, pl ^ 1) ? s.va : s.vb) + i * L;
+                        float cf[L];
 #pragma unroll
-                for (int cl = 0; cl < L; ++cl) {
-                    if (((cl ^ pl) & 1) == 0) continue;
-                    const double cost = s.best[cl] + s.w1[i * L + cl];
-                    // vertical first leg: corner at (x, py), the va column
-                    const double* f = s.fold + (layer_dir(g, cl) ? 0 : LL);
-                    const int lo = min(cl, pl), hi = max(cl, pl);
-                    const double cost2 = f[lo * L + hi] + second;
-                    if (cost + cost2 < best) {
-                        best = cost + cost2;
-                        bp = s.choice[cl] * LL + cl * L + pl;
+                        for (int j = 0; j < L; ++j) cf[j] = corner[j];
+                        double up = 0.0;  // corner[pl+1 .. cl-1], left to right
+#pragma unroll
+                        for (int cl = 0; cl < L; ++cl) {
+                            if (cl >= pl + 2) up += (double)cf[cl - 1];
+                            double fold = up;
+                            if (cl < pl) {  // corner[cl+1 .. pl-1], left to right
+                                double v = 0.0;
+#pragma unroll
+                                for (int j = cl + 1; j < L; ++j)
+                                    if (j < pl) v += (double)cf[j];
+                                fold = v;
+                            }
+                            if (((cl ^ pl) & 1) == 0) continue;
+                            const double cost = gb[cl] + s.w1[i * L + cl];
+                            const double cost2 = fold + second;
+                            if (cost + cost2 < best) {
+                                best = cost + cost2;
+                                bp = gc[cl] * LL + cl * L + pl;
+                            }
+                        }
                     }
+                    s.diag[i * L + pl] = best;
+                    s.dprev[i * L + pl] = bp;
                 }
             }
-            s.diag[i * L + pl] = best;
-            s.dprev[i * L + pl] = bp;
+            __syncthreads();
         }
-        __syncthreads();
-        // phase C (src/routing/lshape_warp.cuh:165-189)
-        for (int lo = act ? lane : L; lo < L; lo += W) {
-            const double* dg = s.diag + i * L;
-            double* prow = s.dist + size_t(p) * T + tri<L>(lo, lo);
-            double c = dg[lo];
-#pragma unroll
-            for (int hi = 0; hi < L; ++hi) {
-                if (hi < lo) continue;
-                if (hi > lo && !(c <= dg[hi])) c = dg[hi];
-                prow[hi - lo] += c;
-            }
-        }
-        __syncthreads();
     }
-    // root choice and traceback (src/routing/lshape_warp.cuh:193-213)
-    const int base_word = desc.z, cap = desc.w;
-    // the root's first minimum in row-major order: per row its first
-    // minimum over high (lanes), then the first minimum over the rows
+    // root: its children's closures, then its first minimum in row-major
+    // order (src/routing/lshape_warp.cuh:193-199): per row the first minimum
+    // over high (lanes), then the first minimum over the rows
+    if (live && hmax > 0)
+        for (int lo = lane; lo < L; lo += W) gather_children<L>(s, 0, lo);
+    if (hmax > 0) __syncthreads();
     for (int lo = live ? lane : L; lo < L; lo += W) {
         double m = s.dist[tri<L>(lo, lo)];
         int mh = lo;
@@ -302,52 +325,74 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
                 mh = hi;
             }
         }
-        s.rmin[lo] = m;
-        s.rarg[lo] = mh;
+        s.gbest[lo] = m;
+        s.gchoice[lo] = mh;
     }
     __syncthreads();
     if (lane == 0 && live) {
-        int ch = s.rarg[0];
-        double cv = s.rmin[0];
+        int ch = s.gchoice[0];
+        double cv = s.gbest[0];
         for (int lo = 1; lo < L; ++lo)
-            if (s.rmin[lo] < cv) {
-                cv = s.rmin[lo];
-                ch = lo * L + s.rarg[lo];
+            if (s.gbest[lo] < cv) {
+                cv = s.gbest[lo];
+                ch = lo * L + s.gchoice[lo];
             }
         s.rng[0] = ch;
         s.path[0] = 0;
-        bool ok = cv < kInf;
-        for (int i = 1; i < n; ++i) {
-            const int pr = s.rng[s.par[i]], plo = pr / L, phi = pr % L;
-            const double* dg = s.diag + i * L;
-            const int* dp = s.dprev + i * L;
-            double c = dg[plo];
-            int pth = dp[plo];
-            for (int h = plo + 1; h <= phi; ++h)
-                if (!(c <= dg[h])) {
-                    c = dg[h];
-                    pth = dp[h];
-                }
-            s.path[i] = pth;
-            s.rng[i] = pth / LL;
-            if (!(s.dist[size_t(i) * T + tri<L>(s.rng[i] / L, s.rng[i] % L)] < kInf)) ok = false;
-        }
-        int total = 1;
-        for (int i = 0; i < n; ++i) {
-            s.woff[i] = total;
-            int w = s.rng[i] / L < s.rng[i] % L ? 2 : 0;
+        if (!(cv < kInf)) atomicExch(a.error, (int)kDevNoSolution);
+    }
+    __syncthreads();
+    // traceback (src/routing/lshape_warp.cuh:200-213), top level first: a
+    // node's interval follows from its parent's
+    for (int lv = hmax - 1; lv >= 0; --lv) {
+        if (live && lv < H)
+            for (int q = s.lstart[lv] + lane; q < s.lstart[lv + 1]; q += W) {
+                const int i = s.order[q];
+                const int pr = s.rng[s.par[i]], plo = pr / L, phi = pr % L;
+                const double* dg = s.diag + i * L;
+                const int* dp = s.dprev + i * L;
+                double c = dg[plo];
+                int pth = dp[plo];
+                for (int h = plo + 1; h <= phi; ++h)
+                    if (!(c <= dg[h])) {
+                        c = dg[h];
+                        pth = dp[h];
+                    }
+                s.path[i] = pth;
+                const int r = pth / LL;
+                s.rng[i] = r;
+                if (!(s.dist[size_t(i) * T + tri<L>(r / L, r % L)] < kInf)) atomicExch(a.error, (int)kDevNoSolution);
+            }
+        __syncthreads();
+    }
+    // route words per node, placed by a prefix sum over the net's lanes
+    const int nchunks = wave_max(live ? (n + W - 1) / W : 0);
+    int carry = 1;
+    for (int b = 0; b < nchunks; ++b) {
+        const int i = b * W + lane;
+        int w = 0;
+        if (live && i < n) {
+            w = s.rng[i] / L < s.rng[i] % L ? 2 : 0;
             if (i) {
                 const int v = s.xy[i], q = s.xy[s.par[i]];
                 const bool st = (v >> 16) == (q >> 16) || (v & 0xFFFF) == (q & 0xFFFF);
                 w += st ? 2 : 6;
             }
-            total += w;
         }
-        s.woff[n] = total;
-        if (!ok) atomicExch(a.error, (int)kDevNoSolution);
-        if (total > cap) atomicExch(a.error, (int)kDevRouteOverflow);
+        int incl = w;
+        for (int d = 1; d < W; d <<= 1) {
+            const int u = __shfl_up(incl, d, W);
+            if (lane >= d) incl += u;
+        }
+        if (live && i < n) s.woff[i] = carry + incl - w;
+        carry += __shfl(incl, W - 1, W);
+    }
+    if (live && lane == 0) {
+        s.woff[n] = carry;
+        if (carry > desc.w) atomicExch(a.error, (int)kDevRouteOverflow);
     }
     __syncthreads();
+    const int base_word = desc.z, cap = desc.w;
     if (!live || s.woff[n] > cap) return;
     int* routes = a.rs.routes + base_word;
     // emission (src/routing/lshape_warp.cuh:51-100)
