@@ -120,6 +120,7 @@ struct gr_ctx {
     std::vector<int> node_off_h;
     int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr;
     int *lvl = nullptr, *link = nullptr;  // L-shape DP level order and child lists, per stage node
+    std::vector<int> height_h;            // per stage slot: height of the tree's root
     std::vector<int> net_ids_h;
     std::vector<int> slots_h;      // stage slot per execution position
     int4* desc = nullptr;          // per execution position: node begin, nodes, route word, route capacity
@@ -371,6 +372,11 @@ BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots)
     at[0] = begin;
     for (int k = 1; k < kTreeClasses; ++k) at[k] = at[k - 1] + b.count[k - 1];
     for (int s = begin; s < begin + count; ++s) slots[at[tree_class(c->node_off_h[s + 1] - c->node_off_h[s])]++] = s;
+    // tallest trees first inside a class: the nets with the longest level
+    // chains start first and the launch's tail is short
+    for (int k = 0, lo = begin; k < kTreeClasses; lo += b.count[k++])
+        std::stable_sort(slots.begin() + lo, slots.begin() + lo + b.count[k],
+                         [&](int u, int v) { return c->height_h[u] > c->height_h[v]; });
     return b;
 }
 
@@ -736,6 +742,7 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
             // per net: node heights (leaves 0), the nodes ordered by (height,
             // index), and each node's children in descending index as a list
             std::vector<int> lvl(total), link(total), height, cnt, first, last;
+            c->height_h.assign(count, 0);
             for (int k = 0; k < count; ++k) {
                 const int b = node_off[k], n = node_off[k + 1] - b;
                 height.assign(n, 0);
@@ -754,6 +761,7 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
                 for (int h = 0; h < n; ++h) cnt[h + 1] += cnt[h];
                 for (int i = 0; i < n; ++i) lvl[b + cnt[height[i]]++] = i | height[i] << 16;
                 for (int i = 0; i < n; ++i) link[b + i] = (first[i] + 1) | (next[i] + 1) << 16;
+                c->height_h[k] = height[0];
             }
             c->lvl = c->mem.alloc<int>(total);
             c->link = c->mem.alloc<int>(total);

@@ -241,7 +241,7 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
                     double best = kInf;
                     int chs = 0;
 #pragma unroll
-                    for (int lo = 0; lo < L; ++lo)
+                    for (int lo = 0; lo < L; ++lo) {
 #pragma unroll
                         for (int hi = lo; hi < L; ++hi) {
                             const double v = nd[tri<L>(lo, hi)];
@@ -250,6 +250,8 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
                                 chs = lo * L + hi;
                             }
                         }
+                        if (lo % 2 == 1) asm volatile("" ::: "memory");  // bounds the loads in flight
+                    }
                     s.gbest[grp * L + c] = best;
                     s.gchoice[grp * L + c] = chs;
                 }
