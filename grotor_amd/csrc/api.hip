@@ -1665,6 +1665,12 @@ int gr_batch_geometry(gr_ctx* c, int n, const int* pt_off, const int* pts, const
     });
 }
 
+// run-head flags of the sorted marks as a scan input, without storing them
+hipcub::TransformInputIterator<int, RunHead, hipcub::CountingInputIterator<int>> run_heads(const unsigned* cell,
+                                                                                          const unsigned* tag, int m) {
+    return {hipcub::CountingInputIterator<int>(0), RunHead{cell, tag, m}};
+}
+
 int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_batches) {
     return guarded([&] {
         if (!c || !color || !num_batches) throw GrError(GR_E_ARG, "null argument");
@@ -1705,7 +1711,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         int cell_bits = 1;
         while ((1ll << cell_bits) < XY) ++cell_bits;
         unsigned *cell = nullptr, *tag = nullptr, *cell2 = nullptr, *tag2 = nullptr;
-        int *head = nullptr, *eidx = nullptr, *list_net = nullptr, *einfo = nullptr, *marker = nullptr;
+        int *eidx = nullptr, *list_net = nullptr, *einfo = nullptr, *marker = nullptr;
         int *prev_marker = nullptr, *slot_of = nullptr, *rcount = nullptr, *d_color = nullptr, *d_count = nullptr;
         int *d_misc = nullptr, *d_closed = nullptr;
         int4* rec = nullptr;
@@ -1715,7 +1721,8 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             hipcub::DoubleBuffer<unsigned> k0(cell, cell2), v0(tag, tag2);
             hip_check(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, k0, v0, std::max(M, 1), 0, cell_bits, st),
                       "sort size");
-            hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, sum_bytes, head, eidx, M + 1, st), "scan size");
+            hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, sum_bytes, run_heads(nullptr, nullptr, M), eidx, M + 1, st),
+                      "scan size");
             hip_check(hipcub::DeviceScan::ExclusiveScan(nullptr, max_bytes, marker, prev_marker, hipcub::Max(), -1,
                                                         std::max(M, 1), st),
                       "scan size");
@@ -1726,7 +1733,6 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             tag = cv.template take<unsigned>(M);
             cell2 = cv.template take<unsigned>(M);
             tag2 = cv.template take<unsigned>(M);
-            head = cv.template take<int>(M + 1);
             eidx = cv.template take<int>(M + 1);
             list_net = cv.template take<int>(M);
             einfo = cv.template take<int>(M);
@@ -1745,7 +1751,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         hip_check(hipMemsetAsync(rcount, 0, sizeof(int) * n, st), "clear reader counts");
         if (M) {
             timed(c, GR_K_BATCH_MARKS, 8.0 * M, [&] {
-                hipLaunchKernelGGL(emit_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_off, cell, tag);
+                hipLaunchKernelGGL(emit_marks_kernel, dim3((n + 3) / 4), dim3(256), 0, st, b, mark_off, cell, tag);
             });
             hipcub::DoubleBuffer<unsigned> keys(cell, cell2), vals(tag, tag2);
             timed(c, GR_K_BATCH_MARKS, 32.0 * M, [&] {
@@ -1755,8 +1761,8 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             const unsigned* sc = keys.Current();
             const unsigned* sv = vals.Current();
             timed(c, GR_K_BATCH_MARKS, 12.0 * M, [&] {
-                hipLaunchKernelGGL(entry_heads_kernel, dim3(grid_m), dim3(256), 0, st, sc, sv, M, head);
-                hip_check(hipcub::DeviceScan::ExclusiveSum(tmp, sum_bytes, head, eidx, M + 1, st), "scan entries");
+                hip_check(hipcub::DeviceScan::ExclusiveSum(tmp, sum_bytes, run_heads(sc, sv, M), eidx, M + 1, st),
+                          "scan entries");
                 hipLaunchKernelGGL(build_entries_kernel, dim3(grid_m), dim3(256), 0, st, sc, sv, eidx, M, list_net,
                                    einfo, marker);
             });

@@ -69,55 +69,106 @@ __global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark
     mark_count[k] = m;
 }
 
+// One wavefront per net.  The stencil centres of the net's points (its
+// reader marks) come first, so that after the stable sort the first mark of
+// every (cell, net) run is a reader mark whenever the run has one; then the
+// segment cells, spread over the lanes (a wave prefix sum of per-segment
+// counts, then mark t goes to lane t mod 64); then the other stencil cells.
 __global__ __launch_bounds__(256) void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell,
                                                          unsigned* tag) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= b.n) return;
-    int m = mark_off[k];
+    __shared__ int s_off[4][kWave], s_start[4][kWave], s_step[4][kWave];
+    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
+    const int k = blockIdx.x * 4 + wib;
+    if (k >= b.n) return;  // uniform per wavefront
+    int* off = s_off[wib];
+    int* start = s_start[wib];
+    int* step = s_step[wib];
     const unsigned plain = unsigned(k) << 1;
-    auto put = [&](int c, unsigned t) {
-        cell[m] = unsigned(c);
-        tag[m] = t;
-        ++m;
-    };
-    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s)
-        for (int p = b.hsegs[2 * s]; p <= b.hsegs[2 * s + 1]; p += b.sparsity * b.Y) put(p, plain);
-    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s)
-        for (int p = b.vsegs[2 * s]; p <= b.vsegs[2 * s + 1]; p += b.sparsity) put(p, plain);
-    for (int q = b.pt_off[k]; q < b.pt_off[k + 1]; ++q) {
-        const int px = b.pts[q] / b.Y, py = b.pts[q] % b.Y;
-        if (px > 0) put((px - 1) * b.Y + py, plain);
-        if (py > 0) put(px * b.Y + py - 1, plain);
-        put(px * b.Y + py, plain | 1u);
-        if (px + 1 < b.X) put((px + 1) * b.Y + py, plain);
+    int m = mark_off[k];
+    const int p0 = b.pt_off[k], np = b.pt_off[k + 1] - p0;
+    for (int q = lane; q < np; q += kWave) {
+        cell[m + q] = unsigned(b.pts[p0 + q]);
+        tag[m + q] = plain | 1u;
     }
-}
-
-// head[i] = 1 where sorted mark i starts a new (cell, net) run; head[m] = 0.
-__global__ __launch_bounds__(256) void entry_heads_kernel(const unsigned* cell, const unsigned* tag, int m, int* head) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > m) return;
-    if (i == m) {
-        head[i] = 0;
-        return;
+    m += np;
+    const int h0 = b.hs_off[k], nh = b.hs_off[k + 1] - h0;
+    const int v0 = b.vs_off[k], ns = nh + b.vs_off[k + 1] - v0;
+    for (int base = 0; base < ns; base += kWave) {
+        const int sg = base + lane;
+        int cnt = 0, first = 0, st = 1;
+        if (sg < ns) {
+            const int* seg = sg < nh ? b.hsegs + 2 * (h0 + sg) : b.vsegs + 2 * (v0 + sg - nh);
+            st = sg < nh ? b.sparsity * b.Y : b.sparsity;
+            first = seg[0];
+            cnt = seg_marks(seg[0], seg[1], st);
+        }
+        int incl = cnt;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int v = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += v;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        off[lane] = incl - cnt;
+        start[lane] = first;
+        step[lane] = st;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        for (int t = lane; t < total; t += kWave) {
+            int lo = 0, hi = kWave - 1;  // last lane whose marks start at or before t
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (off[mid] <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            cell[m + t] = unsigned(start[lo] + (t - off[lo]) * step[lo]);
+            tag[m + t] = plain;
+        }
+        m += total;
+        __builtin_amdgcn_wave_barrier();
     }
-    head[i] = (i == 0 || cell[i] != cell[i - 1] || (tag[i] >> 1) != (tag[i - 1] >> 1)) ? 1 : 0;
+    for (int base = 0; base < np; base += kWave) {
+        const int q = base + lane;
+        int cnt = 0, p = 0, px = 0, py = 0;
+        if (q < np) {
+            p = b.pts[p0 + q];
+            px = p / b.Y;
+            py = p % b.Y;
+            cnt = (px > 0) + (px + 1 < b.X) + (py > 0);
+        }
+        int incl = cnt;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int v = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += v;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        if (q < np) {
+            int w = m + incl - cnt;
+            auto put = [&](int c, unsigned t) {
+                cell[w] = unsigned(c);
+                tag[w] = t;
+                ++w;
+            };
+            if (px > 0) put(p - b.Y, plain);
+            if (py > 0) put(p - 1, plain);
+            if (px + 1 < b.X) put(p + b.Y, plain);
+        }
+        m += total;
+    }
 }
 
 // One list entry per run: its net, kEntryReader / kEntryFirst flags and the
-// marker position used to find the previous reader or the list start.
+// marker position used to find the previous reader or the list start.  The
+// run's first mark carries the reader flag (reader marks are emitted first).
 __global__ __launch_bounds__(256) void build_entries_kernel(const unsigned* cell, const unsigned* tag, const int* eidx,
                                                             int m, int* list_net, int* einfo, int* marker) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const int e = eidx[i];
     if (eidx[i + 1] == e) return;  // not a run head
-    const unsigned c = cell[i], k = tag[i] >> 1;
-    unsigned reader = 0;
-    for (int j = i; j < m && cell[j] == c && (tag[j] >> 1) == k; ++j) reader |= tag[j] & 1u;
+    const unsigned c = cell[i], t = tag[i];
     const int first = (i == 0 || cell[i - 1] != c) ? kEntryFirst : 0;
-    const int info = int(reader) * kEntryReader | first;
-    list_net[e] = int(k);
+    const int info = int(t & 1u) * kEntryReader | first;
+    list_net[e] = int(t >> 1);
     einfo[e] = info;
     marker[e] = info ? e : -1;
 }

@@ -107,7 +107,17 @@ struct BatchGeom {
 };
 __global__ void count_marks_kernel(BatchGeom b, int* mark_count);
 __global__ void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell, unsigned* tag);
-__global__ void entry_heads_kernel(const unsigned* cell, const unsigned* tag, int m, int* head);
+// 1 where sorted mark i starts a new (cell, net) run, 0 for i >= m: the
+// input of the entry-index scan
+struct RunHead {
+    const unsigned* cell;
+    const unsigned* tag;
+    int m;
+    __host__ __device__ int operator()(int i) const {
+        if (i >= m) return 0;
+        return (i == 0 || cell[i] != cell[i - 1] || (tag[i] >> 1) != (tag[i - 1] >> 1)) ? 1 : 0;
+    }
+};
 constexpr int kEntryReader = 1;  // the cell is one of the entry's net's points
 constexpr int kEntryFirst = 2;   // first entry of its cell's list
 __global__ void build_entries_kernel(const unsigned* cell, const unsigned* tag, const int* eidx, int m, int* list_net,
