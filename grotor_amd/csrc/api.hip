@@ -1685,27 +1685,40 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         const int XY = c->g.XY;
         const long long npts = c->geom_points;
         hipStream_t st = c->stream;
-        const int grid_n = (n + 255) / 256;
 
         // marks per net and their offsets
-        int *mark_count = nullptr, *mark_off = nullptr;
+        int *mark_count = nullptr, *mark_off = nullptr, *cand_count = nullptr;
         void* scan_tmp = nullptr;
         size_t scan_bytes = 0;
         hip_check(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, mark_count, mark_off, n + 1, st), "scan size");
         c->batch_ws1.carve([&](auto& cv) {
             mark_count = cv.template take<int>(n + 1);
             mark_off = cv.template take<int>(n + 1);
+            cand_count = cv.template take<int>(n);
             scan_tmp = cv.template take<char>(scan_bytes);
         });
         hip_check(hipMemsetAsync(mark_count + n, 0, sizeof(int), st), "clear mark count tail");
         const double geom_bytes = 4.0 * npts + 8.0 * (c->geom_hsegs + c->geom_vsegs) + 4.0 * n;
+        // the algorithmic bytes charge every candidate mark (repeats included)
+        // and are only needed when the colouring launch is being timed
+        const bool color_profiled = c->profiling && (c->prof_mask >> GR_K_BATCH_COLOR & 1u);
         timed(c, GR_K_BATCH_MARKS, geom_bytes,
-              [&] { hipLaunchKernelGGL(count_marks_kernel, dim3(grid_n), dim3(256), 0, st, b, mark_count); });
+              [&] {
+                  hipLaunchKernelGGL(count_marks_kernel, dim3((n + 3) / 4), dim3(256), 0, st, b, mark_count,
+                                     color_profiled ? cand_count : nullptr);
+              });
         hip_check(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, mark_count, mark_off, n + 1, st), "scan marks");
         int total_marks = 0;
         download(&total_marks, mark_off + n, sizeof(int), st);
         if (total_marks < 0) throw GrError(GR_E_CAPACITY, "more than 2^31 batching marks");
         const int M = total_marks;
+        double cand_marks = M;
+        if (color_profiled) {
+            std::vector<int> cand(n);
+            download(cand.data(), cand_count, sizeof(int) * n, st);
+            cand_marks = 0;
+            for (int v : cand) cand_marks += v;
+        }
 
         // per-cell lists, reader records
         int cell_bits = 1;
@@ -1826,7 +1839,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
             // per net 4 B per point, 8 B per segment and 4 B of color, plus one
             // 4-byte read-modify-write (8 B) per marked cell
             const double color_bytes =
-                4.0 * npts + 8.0 * (c->geom_hsegs + c->geom_vsegs) + 4.0 * n + 8.0 * double(M);
+                4.0 * npts + 8.0 * (c->geom_hsegs + c->geom_vsegs) + 4.0 * n + 8.0 * cand_marks;
             const size_t color_rec = c->prof.size();
             timed(c, GR_K_BATCH_COLOR, color_bytes,
                   [&] { hipLaunchKernelGGL(color_kernel, dim3(waves), dim3(kWave), lds, st, ca); });

@@ -10,8 +10,9 @@
 // Equivalently color(k) = mex( {color(j) : j < k, marks(j) contain a point of
 // k} + {batches already full when k is placed} ).  The GPU form:
 //   1. count_marks / emit_marks: every mark as (cell, net << 1 | self), where
-//      `self` flags the stencil centre of one of the net's points; a stable
-//      radix sort on the cell gives, per cell, the marks in priority order;
+//      `self` flags the stencil centre of one of the net's points, each cell
+//      of a net once; a stable radix sort on the cell gives, per cell, the
+//      marks in priority order;
 //   2. entry_heads / build_entries: one list entry per (cell, net) pair.  An
 //      entry is a reader when the cell is one of its net's points.  A reader
 //      at list position e needs the colors of every entry before it; the
@@ -56,104 +57,149 @@ __device__ __forceinline__ int seg_marks(int first, int second, int step) {
     return second >= first ? (second - first) / step + 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark_count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= b.n) return;
-    int m = 0;
-    for (int s = b.hs_off[k]; s < b.hs_off[k + 1]; ++s) m += seg_marks(b.hsegs[2 * s], b.hsegs[2 * s + 1], b.sparsity * b.Y);
-    for (int s = b.vs_off[k]; s < b.vs_off[k + 1]; ++s) m += seg_marks(b.vsegs[2 * s], b.vsegs[2 * s + 1], b.sparsity);
-    for (int q = b.pt_off[k]; q < b.pt_off[k + 1]; ++q) {
-        const int px = b.pts[q] / b.Y, py = b.pts[q] % b.Y;
-        m += 1 + (px > 0) + (px + 1 < b.X) + (py > 0);
+// A net's marks, enumerated by its wavefront 64 candidates at a time in a
+// fixed order: the stencil centres of its points (its reader marks) first,
+// so that after the stable sort the first mark of every (cell, net) run is a
+// reader mark whenever the run has one; then the segment cells, spread over
+// the lanes (a wave prefix sum of per-segment counts, then candidate t goes
+// to lane t mod 64); then the other stencil cells.  Repeated cells of one net
+// are dropped through a per-wavefront hash set in LDS (the colouring needs
+// each (cell, net) pair once; a net with more than kDedupMax candidates keeps
+// its repeats, which the entry runs then merge).  count_marks_kernel and
+// emit_marks_kernel run the same enumeration, so they keep the same marks.
+constexpr int kDedupSlots = 1024, kDedupMax = 512;
+constexpr unsigned kNoCell = 0xFFFFFFFFu;
+
+struct MarkScratch {
+    int off[kWave], start[kWave], step[kWave];
+    unsigned set[kDedupSlots];
+};
+
+// true the first time `cell` is offered (the set holds at most kDedupMax of
+// its kDedupSlots slots, so the probe ends)
+__device__ __forceinline__ bool first_visit(unsigned* set, unsigned cell) {
+    unsigned h = (cell * 2654435761u) >> 22;
+    while (true) {
+        const unsigned old = atomicCAS(&set[h], kNoCell, cell);
+        if (old == kNoCell) return true;
+        if (old == cell) return false;
+        h = (h + 1) & (kDedupSlots - 1);
     }
-    mark_count[k] = m;
 }
 
-// One wavefront per net.  The stencil centres of the net's points (its
-// reader marks) come first, so that after the stable sort the first mark of
-// every (cell, net) run is a reader mark whenever the run has one; then the
-// segment cells, spread over the lanes (a wave prefix sum of per-segment
-// counts, then mark t goes to lane t mod 64); then the other stencil cells.
-__global__ __launch_bounds__(256) void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell,
-                                                         unsigned* tag) {
-    __shared__ int s_off[4][kWave], s_start[4][kWave], s_step[4][kWave];
-    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
-    const int k = blockIdx.x * 4 + wib;
-    if (k >= b.n) return;  // uniform per wavefront
-    int* off = s_off[wib];
-    int* start = s_start[wib];
-    int* step = s_step[wib];
-    const unsigned plain = unsigned(k) << 1;
-    int m = mark_off[k];
+// chunk(keep, cell, reader) once per 64 candidates, wave-uniformly;
+// returns the number of candidates
+template <class Chunk>
+__device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane, MarkScratch& ms, Chunk&& chunk) {
     const int p0 = b.pt_off[k], np = b.pt_off[k + 1] - p0;
-    for (int q = lane; q < np; q += kWave) {
-        cell[m + q] = unsigned(b.pts[p0 + q]);
-        tag[m + q] = plain | 1u;
-    }
-    m += np;
     const int h0 = b.hs_off[k], nh = b.hs_off[k + 1] - h0;
     const int v0 = b.vs_off[k], ns = nh + b.vs_off[k + 1] - v0;
-    for (int base = 0; base < ns; base += kWave) {
-        const int sg = base + lane;
+    auto seg_of = [&](int sg, int& first, int& st) {
+        const int* seg = sg < nh ? b.hsegs + 2 * (h0 + sg) : b.vsegs + 2 * (v0 + sg - nh);
+        st = sg < nh ? b.sparsity * b.Y : b.sparsity;
+        first = seg[0];
+        return seg_marks(seg[0], seg[1], st);
+    };
+    // candidates, to decide on the dedup
+    int cand = 0;
+    for (int q = lane; q < np; q += kWave) {
+        const int p = b.pts[p0 + q], px = p / b.Y, py = p % b.Y;
+        cand += 1 + (px > 0) + (px + 1 < b.X) + (py > 0);
+    }
+    for (int sg = lane; sg < ns; sg += kWave) {
+        int first, st;
+        cand += seg_of(sg, first, st);
+    }
+    for (int o = kWave / 2; o; o >>= 1) cand += __shfl_xor(cand, o, kWave);
+    const bool dedup = cand <= kDedupMax;
+    if (dedup)
+        for (int i = lane; i < kDedupSlots; i += kWave) ms.set[i] = kNoCell;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    auto offer = [&](bool valid, unsigned cell, unsigned reader) {
+        const bool keep = valid && (!dedup || first_visit(ms.set, cell));
+        chunk(keep, cell, reader);
+    };
+    for (int base = 0; base < np; base += kWave) {
+        const int q = base + lane;
+        offer(q < np, q < np ? unsigned(b.pts[p0 + q]) : 0u, 1u);
+    }
+    for (int sbase = 0; sbase < ns; sbase += kWave) {
+        const int sg = sbase + lane;
         int cnt = 0, first = 0, st = 1;
-        if (sg < ns) {
-            const int* seg = sg < nh ? b.hsegs + 2 * (h0 + sg) : b.vsegs + 2 * (v0 + sg - nh);
-            st = sg < nh ? b.sparsity * b.Y : b.sparsity;
-            first = seg[0];
-            cnt = seg_marks(seg[0], seg[1], st);
-        }
+        if (sg < ns) cnt = seg_of(sg, first, st);
         int incl = cnt;
         for (int d = 1; d < kWave; d <<= 1) {
             const int v = __shfl_up(incl, d, kWave);
             if (lane >= d) incl += v;
         }
         const int total = __shfl(incl, kWave - 1, kWave);
-        off[lane] = incl - cnt;
-        start[lane] = first;
-        step[lane] = st;
+        ms.off[lane] = incl - cnt;
+        ms.start[lane] = first;
+        ms.step[lane] = st;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        for (int t = lane; t < total; t += kWave) {
-            int lo = 0, hi = kWave - 1;  // last lane whose marks start at or before t
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (off[mid] <= t) lo = mid;
-                else hi = mid - 1;
+        for (int base = 0; base < total; base += kWave) {
+            const int t = base + lane;
+            unsigned cell = 0;
+            if (t < total) {
+                int lo = 0, hi = kWave - 1;  // last lane whose candidates start at or before t
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (ms.off[mid] <= t) lo = mid;
+                    else hi = mid - 1;
+                }
+                cell = unsigned(ms.start[lo] + (t - ms.off[lo]) * ms.step[lo]);
             }
-            cell[m + t] = unsigned(start[lo] + (t - off[lo]) * step[lo]);
-            tag[m + t] = plain;
+            offer(t < total, cell, 0u);
         }
-        m += total;
         __builtin_amdgcn_wave_barrier();
     }
     for (int base = 0; base < np; base += kWave) {
         const int q = base + lane;
-        int cnt = 0, p = 0, px = 0, py = 0;
-        if (q < np) {
-            p = b.pts[p0 + q];
-            px = p / b.Y;
-            py = p % b.Y;
-            cnt = (px > 0) + (px + 1 < b.X) + (py > 0);
-        }
-        int incl = cnt;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int v = __shfl_up(incl, d, kWave);
-            if (lane >= d) incl += v;
-        }
-        const int total = __shfl(incl, kWave - 1, kWave);
-        if (q < np) {
-            int w = m + incl - cnt;
-            auto put = [&](int c, unsigned t) {
-                cell[w] = unsigned(c);
-                tag[w] = t;
-                ++w;
-            };
-            if (px > 0) put(p - b.Y, plain);
-            if (py > 0) put(p - 1, plain);
-            if (px + 1 < b.X) put(p + b.Y, plain);
-        }
-        m += total;
+        const bool valid = q < np;
+        const int p = valid ? b.pts[p0 + q] : 0, px = p / b.Y, py = p % b.Y;
+        offer(valid && px > 0, unsigned(p - b.Y), 0u);
+        offer(valid && py > 0, unsigned(p - 1), 0u);
+        offer(valid && px + 1 < b.X, unsigned(p + b.Y), 0u);
     }
+    return cand;
+}
+
+// mark_count[k] = marks net k keeps, cand_count[k] (if given) = its
+// candidates before the dedup; one wavefront per net
+__global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark_count, int* cand_count) {
+    __shared__ MarkScratch s_ms[4];
+    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
+    const int k = blockIdx.x * 4 + wib;
+    if (k >= b.n) return;  // uniform per wavefront
+    int m = 0;
+    const int cand = for_each_mark(b, k, lane, s_ms[wib], [&](bool keep, unsigned, unsigned) { m += __popcll(__ballot(keep)); });
+    if (lane == 0) {
+        mark_count[k] = m;
+        if (cand_count) cand_count[k] = cand;
+    }
+}
+
+// The kept marks of net k at mark_off[k], each as (cell, k << 1 | reader).
+__global__ __launch_bounds__(256) void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell,
+                                                         unsigned* tag) {
+    __shared__ MarkScratch s_ms[4];
+    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
+    const int k = blockIdx.x * 4 + wib;
+    if (k >= b.n) return;  // uniform per wavefront
+    const unsigned plain = unsigned(k) << 1;
+    const unsigned long long below = (1ull << lane) - 1;
+    int m = mark_off[k];
+    (void)for_each_mark(b, k, lane, s_ms[wib], [&](bool keep, unsigned c, unsigned reader) {
+        const unsigned long long mask = __ballot(keep);
+        if (keep) {
+            const int w = m + __popcll(mask & below);
+            cell[w] = c;
+            tag[w] = plain | reader;
+        }
+        m += __popcll(mask);
+    });
 }
 
 // One list entry per run: its net, kEntryReader / kEntryFirst flags and the

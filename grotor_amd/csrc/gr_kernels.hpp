@@ -105,7 +105,7 @@ struct BatchGeom {
     const int* vs_off;
     const int* vsegs;
 };
-__global__ void count_marks_kernel(BatchGeom b, int* mark_count);
+__global__ void count_marks_kernel(BatchGeom b, int* mark_count, int* cand_count);
 __global__ void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell, unsigned* tag);
 // 1 where sorted mark i starts a new (cell, net) run, 0 for i >= m: the
 // input of the entry-index scan
