@@ -506,6 +506,8 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         if (!grid || !out) throw GrError(GR_E_ARG, "null argument");
         if (grid->L < 1 || grid->L >= kMaxLayers) throw GrError(GR_E_ARG, "routing layers must be 1..15");
         if (grid->X < 2 || grid->Y < 2) throw GrError(GR_E_ARG, "grid must be at least 2 x 2");
+        // a track's Fenwick values live in LDS during the refresh: 64 KiB
+        if (grid->X > 8192 || grid->Y > 8192) throw GrError(GR_E_ARG, "tracks longer than 8192 cells");
         if (size_t(grid->L) * grid->X * grid->Y > size_t(INT_MAX)) throw GrError(GR_E_ARG, "grid exceeds 2^31 cells");
         int count = 0;
         hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
@@ -1779,15 +1781,14 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
                 hipLaunchKernelGGL(build_entries_kernel, dim3(grid_m), dim3(256), 0, st, sc, sv, eidx, M, list_net,
                                    einfo, marker);
             });
-            int U = 0;
-            download(&U, eidx + M, sizeof(int), st);
-            const int grid_u = U / 256 + 1;
-            timed(c, GR_K_BATCH_MARKS, 20.0 * U, [&] {
-                hip_check(hipcub::DeviceScan::ExclusiveScan(tmp, max_bytes, marker, prev_marker, hipcub::Max(), -1, U, st),
+            // entries: eidx[M] on the device; the scans and kernels below run over
+            // all M positions and ignore those past it
+            timed(c, GR_K_BATCH_MARKS, 20.0 * M, [&] {
+                hip_check(hipcub::DeviceScan::ExclusiveScan(tmp, max_bytes, marker, prev_marker, hipcub::Max(), -1, M, st),
                           "scan markers");
-                hipLaunchKernelGGL(reader_slots_kernel, dim3(grid_u), dim3(256), 0, st, U, list_net, einfo, b.pt_off,
-                                   rcount, slot_of);
-                hipLaunchKernelGGL(reader_records_kernel, dim3(grid_u), dim3(256), 0, st, U, einfo, prev_marker,
+                hipLaunchKernelGGL(reader_slots_kernel, dim3(grid_m), dim3(256), 0, st, eidx + M, list_net, einfo,
+                                   b.pt_off, rcount, slot_of);
+                hipLaunchKernelGGL(reader_records_kernel, dim3(grid_m), dim3(256), 0, st, eidx + M, einfo, prev_marker,
                                    slot_of, rec);
             });
             hip_check(hipGetLastError(), "batching list kernels");

@@ -75,15 +75,15 @@ struct MarkScratch {
     unsigned set[kDedupSlots];
 };
 
-// true the first time `cell` is offered (the set holds at most kDedupMax of
-// its kDedupSlots slots, so the probe ends)
-__device__ __forceinline__ bool first_visit(unsigned* set, unsigned cell) {
-    unsigned h = (cell * 2654435761u) >> 22;
+// true the first time `cell` is offered to a set of 2^bits slots (at least
+// twice the candidates, so the probe ends)
+__device__ __forceinline__ bool first_visit(unsigned* set, int bits, unsigned cell) {
+    unsigned h = (cell * 2654435761u) >> (32 - bits);
     while (true) {
         const unsigned old = atomicCAS(&set[h], kNoCell, cell);
         if (old == kNoCell) return true;
         if (old == cell) return false;
-        h = (h + 1) & (kDedupSlots - 1);
+        h = (h + 1) & ((1u << bits) - 1);
     }
 }
 
@@ -112,12 +112,14 @@ __device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane
     }
     for (int o = kWave / 2; o; o >>= 1) cand += __shfl_xor(cand, o, kWave);
     const bool dedup = cand <= kDedupMax;
+    int bits = 6;  // the set: at least 64 slots, at least twice the candidates
+    while ((1 << bits) < 2 * cand) ++bits;
     if (dedup)
-        for (int i = lane; i < kDedupSlots; i += kWave) ms.set[i] = kNoCell;
+        for (int i = lane; i < (1 << bits); i += kWave) ms.set[i] = kNoCell;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     auto offer = [&](bool valid, unsigned cell, unsigned reader) {
-        const bool keep = valid && (!dedup || first_visit(ms.set, cell));
+        const bool keep = valid && (!dedup || first_visit(ms.set, bits, cell));
         chunk(keep, cell, reader);
     };
     for (int base = 0; base < np; base += kWave) {
@@ -219,19 +221,20 @@ __global__ __launch_bounds__(256) void build_entries_kernel(const unsigned* cell
     marker[e] = info ? e : -1;
 }
 
-// slot_of[e] = record slot of reader entry e inside its net's point range
-__global__ __launch_bounds__(256) void reader_slots_kernel(int u, const int* list_net, const int* einfo,
+// slot_of[e] = record slot of reader entry e inside its net's point range;
+// *u (on the device) is the number of entries
+__global__ __launch_bounds__(256) void reader_slots_kernel(const int* u, const int* list_net, const int* einfo,
                                                            const int* pt_off, int* rcount, int* slot_of) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= u || !(einfo[e] & kEntryReader)) return;
+    if (e >= *u || !(einfo[e] & kEntryReader)) return;
     const int k = list_net[e];
     slot_of[e] = pt_off[k] + atomicAdd(rcount + k, 1);
 }
 
-__global__ __launch_bounds__(256) void reader_records_kernel(int u, const int* einfo, const int* prev_marker,
+__global__ __launch_bounds__(256) void reader_records_kernel(const int* u, const int* einfo, const int* prev_marker,
                                                              const int* slot_of, int4* rec) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= u) return;
+    if (e >= *u) return;
     const int info = einfo[e];
     if (!(info & kEntryReader)) return;
     int start = e, prev = -1;

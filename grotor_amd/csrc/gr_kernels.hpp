@@ -122,9 +122,10 @@ constexpr int kEntryReader = 1;  // the cell is one of the entry's net's points
 constexpr int kEntryFirst = 2;   // first entry of its cell's list
 __global__ void build_entries_kernel(const unsigned* cell, const unsigned* tag, const int* eidx, int m, int* list_net,
                                      int* einfo, int* marker);
-__global__ void reader_slots_kernel(int u, const int* list_net, const int* einfo, const int* pt_off, int* rcount,
+__global__ void reader_slots_kernel(const int* u, const int* list_net, const int* einfo, const int* pt_off, int* rcount,
                                     int* slot_of);
-__global__ void reader_records_kernel(int u, const int* einfo, const int* prev_marker, const int* slot_of, int4* rec);
+__global__ void reader_records_kernel(const int* u, const int* einfo, const int* prev_marker, const int* slot_of,
+                                      int4* rec);
 constexpr int kColorsPerWord = 63;  // prefix-mask word: 63 color bits + a ready bit
 constexpr int kMaxMaskWords = 32;   // colors 0..2015
 struct ColorArgs {
