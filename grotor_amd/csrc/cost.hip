@@ -37,18 +37,15 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
     const double sc = g.short_cost[l];
     extern __shared__ double fv[];  // Fenwick values, index = BIT position
 
+    // increments of each thread's last cell: the via cost of the next
+    // thread's first cell needs them
+    __shared__ double last_h[kRefreshThreads], last_1[kRefreshThreads];
     const int k = (len + blockDim.x - 1) / blockDim.x;
     const int p0 = threadIdx.x * k;
     const int p1 = min(len, p0 + k);
+    double first_h = 0.0, first_1 = 0.0;  // increments of cell p0
     if (p0 < len) {
-        // increments of the predecessor cell (needed by vcost of p0)
         double prev_h = 0.0, prev_1 = 0.0;
-        if (p0 > 0) {
-            const float c = capacity[base + p0 - 1], d = demand[base + p0 - 1];
-            const double e0 = of_cost(c, demand_plus(d, 0.0), cp.slope, cp.zero_slope);
-            prev_h = incr_from(e0, of_cost(c, demand_plus(d, 0.5), cp.slope, cp.zero_slope), sc);
-            prev_1 = incr_from(e0, of_cost(c, demand_plus(d, 1.0), cp.slope, cp.zero_slope), sc);
-        }
         for (int p = p0; p < p1; ++p) {
             const float c = capacity[base + p], d = demand[base + p];
             const double e0 = of_cost(c, demand_plus(d, 0.0), cp.slope, cp.zero_slope);
@@ -60,20 +57,33 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
             if (cp.integer_wire) w = (float)(long long)fminf(fmaxf(w, 0.f), 1.e7f);
             if (wcost_out) wcost_out[base + p] = w;
             if (p < len - 1) fv[p + 1] = (double)w;
-            // vcost: src/cost/full_update.cuh:67-82
-            double v;
-            if (p == 0) v = inc_1;
-            else if (p == len - 1) v = prev_1;
-            else v = inc_h + prev_h;
-            float vf = (float)v;
-            vf = (float)((double)vf + cp.via_weight * g.unit_via);
-            vcost[base + p] = vf;
+            if (p == p0) {
+                first_h = inc_h;
+                first_1 = inc_1;
+            } else {
+                // vcost: src/cost/full_update.cuh:67-82
+                const double v = p == len - 1 ? prev_1 : inc_h + prev_h;
+                float vf = (float)v;
+                vf = (float)((double)vf + cp.via_weight * g.unit_via);
+                vcost[base + p] = vf;
+            }
             prev_h = inc_h;
             prev_1 = inc_1;
         }
+        last_h[threadIdx.x] = prev_h;
+        last_1[threadIdx.x] = prev_1;
     }
     if (threadIdx.x == 0) fv[0] = 0.0;
     __syncthreads();
+    if (p0 < len) {
+        double v;
+        if (p0 == 0) v = first_1;
+        else if (p0 == len - 1) v = last_1[threadIdx.x - 1];
+        else v = first_h + last_h[threadIdx.x - 1];
+        float vf = (float)v;
+        vf = (float)((double)vf + cp.via_weight * g.unit_via);
+        vcost[base + p0] = vf;
+    }
     // Fenwick nodes: node(i) = leaf(i) + node(i-1) + node(i-2) + ... +
     // node(i - width/2) for i = (2s+1)*width - 1 (src/cost/fenwick.cuh:149-159)
     for (int width = 2; width <= len; width <<= 1) {
