@@ -51,7 +51,9 @@ def parse():
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--net-scale", type=float, default=1.0)
-    ap.add_argument("--mode", type=int, default=1, help="0 plain launches, 1 the batch loop captured once in a HIP graph")
+    ap.add_argument("--mode", type=int, default=2, choices=(0, 1, 2),
+                    help="0 plain launches, 1 the batch loop captured once in a HIP graph, "
+                         "2 persistent DP launches (one per batch, draining a work queue) in the captured loop")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
@@ -150,19 +152,19 @@ def main():
         dag_ms = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
         return batch_ms, dag_ms, nb, color, sched, upload_ms
 
-    # The last warmup step runs in mode 0 with HIP events on every kernel:
-    # it gives the per-class breakdown and names the dominant class.  The
-    # timed steps then carry events on that class only (mode 0), so the
-    # events add little host work to the timed region; a captured graph
-    # (mode 1) cannot carry them, so its roofline comes from one extra
-    # mode-0 step after the timed ones.
+    # The last warmup step runs with HIP events on every kernel: it gives the
+    # per-class breakdown and names the dominant class.  The timed steps then
+    # carry events on that class only (mode 0), so the events add little host
+    # work to the timed region; a captured graph (modes 1 and 2) cannot carry
+    # them, so its roofline comes from one extra step after the timed ones,
+    # in which the library issues the same launches directly.
     breakdown = None
     for i in range(args.warmup):
         last = i == args.warmup - 1
         if last:
             ctx.profile_select(None)
             ctx.profile_begin()
-        bm, dm, nb, _, _, um = step(mode=0 if last else args.mode)
+        bm, dm, nb, _, _, um = step()
         if last:
             breakdown = ctx.profile_end()
         print(f"[bench] warmup {i}: batching {bm:.2f} ms (+ geometry upload {um:.2f} ms), DAG {dm:.2f} ms, "
@@ -191,7 +193,7 @@ def main():
     else:
         ctx.profile_select(None if dom_kind is None else [dom_kind])
         ctx.profile_begin()
-        step(mode=0)
+        step()
         prof = ctx.profile_end()
     if breakdown is None:
         breakdown = prof
@@ -207,7 +209,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
                 "traffic": measured_traffic(KIND_NAMES[dom], args.config),
                 "events": "timed steps (this class only)" if inline_profile
-                          else "one extra mode-0 step after the timed steps",
+                          else "one extra step after the timed steps, its launches issued directly",
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
                 "launches": int(dom_launches),
                 "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}

@@ -120,6 +120,8 @@ struct gr_ctx {
     std::vector<int> node_off_h;
     int *net_ids = nullptr, *node_off = nullptr, *nodes = nullptr, *parents = nullptr;
     int *lvl = nullptr, *link = nullptr;  // L-shape DP level order and child lists, per stage node
+    int* dp_queue = nullptr;              // persistent DP: kDpQueues work-queue counters per batch
+    int dp_queue_cap = 0;
     std::vector<int> height_h;            // per stage slot: height of the tree's root
     std::vector<int> net_ids_h;
     std::vector<int> slots_h;      // stage slot per execution position
@@ -416,6 +418,8 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
     a.link = c->link;
     a.gscratch = c->gscratch;
     a.error = c->error;
+    a.queue = nullptr;
+    a.cus = c->cus;
     a.begin = b.begin;
     for (int j = 0; j < k; ++j) a.begin += b.count[j];
     a.count = b.count[k];
@@ -449,6 +453,54 @@ void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
     launch_lshape_class(c, b, 0, 0, c->stream);
     hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
+}
+
+// Persistent form of one batch: its slots in one work queue, tallest trees
+// first; BatchLaunch class 0 carries the whole batch (its node capacity is
+// the batch's largest tree) and class 1 is empty.
+BatchLaunch plan_batch_queue(gr_ctx* c, int begin, int count, std::vector<int>& slots) {
+    BatchLaunch b{begin, {count, 0}, {1, 1}};
+    for (int s = begin; s < begin + count; ++s) {
+        slots[s] = s;
+        b.nmax[0] = std::max(b.nmax[0], c->node_off_h[s + 1] - c->node_off_h[s]);
+    }
+    std::stable_sort(slots.begin() + begin, slots.begin() + begin + count,
+                     [&](int u, int v) { return c->height_h[u] > c->height_h[v]; });
+    return b;
+}
+
+// The batch's nets drained by the persistent DP kernel from its queue
+// counters (zero beforehand); trees too large for LDS take the launch form.
+void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
+    if (!b.count[0]) return;
+    const int p = per_wave_for(c, 0, b.nmax[0]);
+    if (p <= 0) {
+        launch_lshape_class(c, b, 0);
+        return;
+    }
+    LshapeArgs a;
+    a.g = c->g;
+    a.vcost = c->vcost;
+    a.presum = c->presum;
+    a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+    a.desc = c->desc;
+    a.nodes = c->nodes;
+    a.parents = c->parents;
+    a.lvl = c->lvl;
+    a.link = c->link;
+    a.gscratch = c->gscratch;
+    a.error = c->error;
+    a.queue = queue;
+    a.cus = c->cus;
+    a.begin = b.begin;
+    a.count = b.count[0];
+    a.nmax = b.nmax[0];
+    a.per_wave = p;
+    double alg = 0;
+    if (c->profiling)
+        for (int i = a.begin; i < a.begin + a.count; ++i) alg += c->slot_dp_bytes[c->slots_h[i]];
+    timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, c->stream); });
+    hip_check(hipGetLastError(), "lshape_dp queue kernel");
 }
 
 // Reserve the global DP scratch the batches will need (trees whose tables
@@ -818,25 +870,39 @@ int gr_commit_batch(gr_ctx* c, int begin, int count, int stamp, int K) {
 int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, int mode, double* elapsed_ms) {
     return guarded([&] {
         if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
-        if (mode != 0 && mode != 1) throw GrError(GR_E_ARG, "execution mode not available (0 = launches, 1 = HIP graph)");
+        if (mode < 0 || mode > 2)
+            throw GrError(GR_E_ARG, "execution mode not available (0 = launches, 1 = HIP graph, 2 = persistent + graph)");
         require_stage(c, 0, batch_off[nbatch]);
         if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
         std::vector<int> slots(c->stage_count);
         std::vector<BatchLaunch> plan(nbatch);
         for (int b = 0; b < nbatch; ++b) {
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
-            plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+            plan[b] = mode == 2 ? plan_batch_queue(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots)
+                                : plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
         }
         reserve_dp_scratch(c, plan);
+        if (mode == 2 && c->dp_queue_cap < nbatch) {
+            c->mem.release(c->dp_queue);
+            c->dp_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kDpQueues * kDpQueueStride);
+            c->dp_queue_cap = nbatch;
+        }
         auto enqueue = [&] {
+            if (mode == 2 && nbatch)  // a kernel, not a memset: the loop may be captured
+                hipLaunchKernelGGL(fill_int_kernel, dim3((nbatch * kDpQueues * kDpQueueStride + 255) / 256), dim3(256), 0,
+                                   c->stream, c->dp_queue, 0, (long long)nbatch * kDpQueues * kDpQueueStride);
             for (int b = 0; b < nbatch; ++b) {
                 launch_refresh(c, 0);
-                launch_lshape(c, plan[b]);
+                if (mode == 2) launch_lshape_queue(c, plan[b], c->dp_queue + size_t(b) * kDpQueues * kDpQueueStride);
+                else launch_lshape(c, plan[b]);
                 launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
             }
         };
-        if (mode == 1) {
-            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1]};
+        // a captured graph cannot carry profiling events: profiled runs of
+        // modes 1 and 2 issue the same launches directly
+        const bool graph = mode >= 1 && !c->profiling;
+        if (graph) {
+            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
             // the graph reads the descriptors when it runs
             c->slots_h = slots;
@@ -868,7 +934,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         hip_check(hipStreamSynchronize(c->stream), "stage plan");
         hip_check(hipDeviceSynchronize(), "before DAG interval");
         const auto t0 = std::chrono::steady_clock::now();
-        if (mode == 1) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
+        if (graph) hip_check(hipGraphLaunch(c->graph_exec, c->stream), "hipGraphLaunch");
         else enqueue();
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();

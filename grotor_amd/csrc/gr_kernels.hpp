@@ -76,11 +76,17 @@ struct LshapeArgs {
     int per_wave;      // nets per wavefront: 1, 2, 4 or 8 (each gets 64 / per_wave lanes)
     double* gscratch;  // global scratch when the tree does not fit in LDS
     int* error;
+    int* queue;        // persistent form: the batch's kDpQueues work-queue counters (nullptr: one workgroup per P nets)
+    int cus;           // compute units, for the persistent grid
 };
 // Bytes of one net's DP tables for trees of up to nmax nodes (0 if L is not
 // supported), and the launch of `blocks` workgroups of a.per_wave nets each
-// with the tables in LDS or in a.gscratch (false if L is not supported).
+// with the tables in LDS or in a.gscratch, or, with a.queue set, of the
+// persistent kernel that drains the nets with at most `blocks` resident
+// workgroups (false if L is not supported).
 size_t lshape_scratch_bytes(int L, int nmax);
+constexpr int kDpQueues = 32;       // persistent DP: work-queue counters per batch
+constexpr int kDpQueueStride = 32;  // ints between counters (one 128-byte line)
 constexpr int kTreeClasses = 2;  // L-shape DP launches per batch: trees up to 16 nodes, larger
 bool launch_lshape_dp(const LshapeArgs& a, int blocks, bool lds, hipStream_t stream);
 

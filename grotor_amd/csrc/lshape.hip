@@ -147,13 +147,13 @@ __device__ __forceinline__ void gather_children(const Tables<L>& s, int i, int l
 }
 
 template <int L>
-__device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
+__device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int first_slot) {
     constexpr int LL = L * L, T = kTri<L>;
     const int P = a.per_wave, W = kWave / P;
     const int sub = threadIdx.x / W, lane = threadIdx.x % W;  // lane within the net's lanes
     const int G = max(1, W / L), GL = W / G;                   // lane groups of the net, lanes per group
     const int grp = lane / GL, gl = lane % GL;                 // grp >= G: an idle lane
-    const int slot = blockIdx.x * P + sub;
+    const int slot = first_slot + sub;
     const bool live = slot < a.count;
     const Tables<L> s = carve<L>(base + words * sub, a.nmax);
     const DeviceGrid& g = a.g;
@@ -437,20 +437,52 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words) {
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_lds_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
-    solve_net<L>(a, smem, (table_bytes<L>(a.nmax) + 7) / 8);
+    solve_net<L>(a, smem, (table_bytes<L>(a.nmax) + 7) / 8, blockIdx.x * a.per_wave);
 }
 
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_global_kernel(LshapeArgs a) {
     const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
-    solve_net<L>(a, a.gscratch + words * a.per_wave * blockIdx.x, words);
+    solve_net<L>(a, a.gscratch + words * a.per_wave * blockIdx.x, words, blockIdx.x * a.per_wave);
+}
+
+// Persistent form: a grid no larger than what stays resident drains the
+// batch's nets from a work queue, per_wave nets per dequeue, in execution
+// order (tallest trees first).  The queue is kDpQueues counters, one
+// 128-byte line apart and zero at launch: workgroup w dequeues from counter
+// w % kDpQueues, which hands out chunks q, q + kDpQueues, ... in order (one
+// counter for all workgroups serializes the dequeues).
+template <int L>
+__global__ __launch_bounds__(kWave) void lshape_dp_queue_kernel(LshapeArgs a) {
+    extern __shared__ double smem[];
+    __shared__ int s_first;
+    const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
+    const int q = blockIdx.x % kDpQueues;
+    int* counter = a.queue + q * kDpQueueStride;
+    while (true) {
+        if (threadIdx.x == 0) s_first = (atomicAdd(counter, 1) * kDpQueues + q) * a.per_wave;
+        __syncthreads();
+        const int first = s_first;
+        __syncthreads();  // every lane has read s_first before the next dequeue
+        if (first >= a.count) break;
+        solve_net<L>(a, smem, words, first);
+        __syncthreads();  // the tables are reused by the next nets
+    }
 }
 
 template <int L>
 void launch_for(const LshapeArgs& a, int blocks, bool lds, hipStream_t st) {
-    if (lds)
-        hipLaunchKernelGGL(lshape_dp_lds_kernel<L>, dim3(blocks), dim3(kWave),
-                           ((table_bytes<L>(a.nmax) + 7) / 8) * 8 * a.per_wave, st, a);
+    const size_t lds_bytes = ((table_bytes<L>(a.nmax) + 7) / 8) * 8 * a.per_wave;
+    if (a.queue) {
+        int resident = 0;  // workgroups per compute unit; any value keeps the queue correct
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, lshape_dp_queue_kernel<L>, kWave, lds_bytes) !=
+                hipSuccess ||
+            resident < 1)
+            resident = 1;
+        const int grid = std::max(1, std::min(blocks, resident * a.cus));
+        hipLaunchKernelGGL(lshape_dp_queue_kernel<L>, dim3(grid), dim3(kWave), lds_bytes, st, a);
+    } else if (lds)
+        hipLaunchKernelGGL(lshape_dp_lds_kernel<L>, dim3(blocks), dim3(kWave), lds_bytes, st, a);
     else
         hipLaunchKernelGGL(lshape_dp_global_kernel<L>, dim3(blocks), dim3(kWave), 0, st, a);
 }

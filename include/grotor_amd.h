@@ -145,9 +145,13 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
  * batch_off has nbatch + 1 entries over the uploaded stage slots.
  * mode: 0 = one launch per step (the reference's loop),
  *       1 = the same launches captured once into a HIP graph and replayed
- *           (the capture is reused while the stage topology, batch offsets
- *           and first stamp stay the same; capture happens before the
- *           timed interval),
+ *           (the capture is reused while the stage topology, batch offsets,
+ *           first stamp, nets per wavefront and mode stay the same; capture
+ *           happens before the timed interval),
+ *       2 = persistent form, captured like mode 1: per batch one DP launch
+ *           whose resident workgroups drain all of the batch's nets from a
+ *           device work queue, tallest trees first (no reference
+ *           counterpart; results equal modes 0 and 1),
  *       other values are rejected with GR_E_ARG.
  * *elapsed_ms receives the synchronized interval (RoutingDagTimer,
  * include/router_state.hpp:238-265). */

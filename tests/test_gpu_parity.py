@@ -182,7 +182,32 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     assert np.array_equal(ctx.routes(), routes_h)
     assert np.array_equal(_bits(ctx.demand()), _bits(demand_h))
     with pytest.raises(ga.GrError):
-        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=3)
+    ctx.close()
+
+
+@pytest.mark.parametrize("per_wave", [(1, 1), (4, 1)])
+def test_persistent_mode_matches_oracle(ga, ora, per_wave):
+    """Mode 2 (one persistent DP launch per batch draining a work queue,
+    the loop captured in a HIP graph): replays and a switch back and forth
+    with mode 1 equal the oracle."""
+    _, t = topology(6, 5)
+    print(f"persistent mode config=6 seed=5 per_wave={per_wave}", flush=True)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ora.batches(t["X"], t["Y"], *geom)
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=8)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.set_nets_per_wave(per_wave)
+    for rep, mode in enumerate((2, 2, 1, 2)):
+        ctx.reset()
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
+        assert np.array_equal(ctx.routes(), routes_o), (rep, mode)
+        assert np.array_equal(_bits(ctx.demand()), _bits(demand_o)), (rep, mode)
+        assert np.array_equal(ctx.timestamps(), ts_o), (rep, mode)
+        assert ctx.stats() == (wl_o, vias_o), (rep, mode)
     ctx.close()
 
 
