@@ -82,16 +82,18 @@ def copy_calibration(mib):
                                            "launches": 5}}), flush=True)
 
 
-def measured_traffic(kernel, config):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE table (profiles/hbm_traffic.txt),
-    scaled by its plain-copy calibration; None when not measured."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` (configN, or
+    configN-launches for the L-shape stage's modes 0 and 1) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE table
+    (profiles/hbm_traffic.txt), scaled by its plain-copy calibration; None
+    when not measured."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 6 and not line.startswith("#") and f[0] == kernel and f[1] == f"config{config}":
+        if len(f) == 6 and not line.startswith("#") and f[0] == kernel and f[1] == workload:
             fetch, write, fs, ws = map(float, f[2:])
             return round((fetch * fs + write * ws) * 1024.0)
     return None
@@ -207,7 +209,8 @@ def main():
     achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": measured_traffic(KIND_NAMES[dom], args.config),
+                "traffic": measured_traffic(KIND_NAMES[dom], f"config{args.config}" if args.mode == 2
+                                            else f"config{args.config}-launches"),
                 "events": "timed steps (this class only)" if inline_profile
                           else "one extra step after the timed steps, its launches issued directly",
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
@@ -362,7 +365,7 @@ def bench_flt(args, ga, design, topo):
     dom_ms, dom_launches = prof_sum["ms"][dom], prof_sum["launches"][dom]
     achieved = prof_sum["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(KIND_NAMES[dom], args.config),
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": measured_traffic(KIND_NAMES[dom], f"config{args.config}"),
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
                 "launches": int(dom_launches), "alg_bytes_per_launch": round(prof_sum["bytes"][dom] / max(dom_launches, 1), 1)}
     kernels = {KIND_NAMES[k]: {"ms": round(breakdown["ms"][k], 4), "launches": int(breakdown["launches"][k]),
