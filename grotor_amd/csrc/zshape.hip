@@ -13,8 +13,12 @@
 // leg along the child's row/column; via sweeps at every cell of that leg;
 // middle leg across the box; via sweeps on the node's row/column; last leg
 // into child slot j of the node.  Lanes take disjoint cells in every phase,
-// so the values equal a sequential run.  The node's interval combination
-// and the traceback run on lane 0, as in the reference.  Costs are integers:
+// so the values equal a sequential run; the last leg's per-layer first
+// minimum is a wavefront reduction.  Each cell's via sweep, the child's
+// value in the first leg and the node's interval combination work on values
+// loaded once into registers instead of chains of dependent global accesses.
+// The node's interval combination and the traceback run on lane 0, as in
+// the reference.  Costs are integers:
 // int(min(|presum(a) - presum(b)|, 5e8)) per wire, int(clamp(vcost, 1, 1e6))
 // per via landing.  Indices decoded from the traceback tables are checked
 // against the map before use; a bad one sets the error word.
@@ -55,32 +59,57 @@ struct Z {
     __device__ __forceinline__ int wire(long long p, long long q) const {
         return (int)fmin(fabs(wire_sum(p) - wire_sum(q)), double(kZInf / 2));
     }
+    // wire(p, q) with wire_sum(p) already loaded
+    __device__ __forceinline__ int wire_from(double wp, long long q) const {
+        return (int)fmin(fabs(wp - wire_sum(q)), double(kZInf / 2));
+    }
 };
 
+// The via sweeps at one (x, y) of plane 4: upwards then downwards through
+// the layer stack, each step taking the neighbour's value plus the via cost
+// when it is smaller.  The stack is loaded once and swept in registers, in
+// the reference's order (zshape.cuh viaSweep); only this lane touches it.
 __device__ void via_sweep(const Z& z, int x, int y) {
     int* m = z.a.map + 4 * z.P;
     int* pr = z.a.prev + 4 * z.P;
-    for (int i = 1; i < z.a.layers; i++) {
-        const long long cur = z.cell(i, x, y);
-        pr[cur + z.P] = (int)cur;
+    const int L1 = z.a.layers;
+    long long idx[kMaxLayers];
+    int mv[kMaxLayers], pv[kMaxLayers], pl[kMaxLayers], vc[kMaxLayers];
+#pragma unroll
+    for (int i = 1; i < kMaxLayers; i++) {
+        if (i >= L1) continue;
+        idx[i] = z.cell(i, x, y);
+        mv[i] = m[idx[i]];
+        pv[i] = pr[idx[i]];
+        pl[i] = (int)idx[i];
+        vc[i] = i < L1 - 1 ? z.via(idx[i]) : 0;
     }
-    for (int i = 2; i < z.a.layers; i++) {
-        const long long cur = z.cell(i, x, y), last = z.cell_swapped(i - 1, i, x, y);
-        const int v = m[last] + z.via(last);
-        if (m[cur] > v) {
-            m[cur] = v;
-            pr[cur] = pr[last];
-            pr[cur + z.P] = (int)last;
+#pragma unroll
+    for (int i = 2; i < kMaxLayers; i++) {
+        if (i >= L1) continue;
+        const int v = mv[i - 1] + vc[i - 1];
+        if (mv[i] > v) {
+            mv[i] = v;
+            pv[i] = pv[i - 1];
+            pl[i] = (int)idx[i - 1];
         }
     }
-    for (int i = z.a.layers - 2; i >= 1; i--) {
-        const long long cur = z.cell(i, x, y), last = z.cell_swapped(i + 1, i, x, y);
-        const int v = m[last] + z.via(cur);
-        if (m[cur] > v) {
-            m[cur] = v;
-            pr[cur] = pr[last];
-            pr[cur + z.P] = (int)last;
+#pragma unroll
+    for (int i = kMaxLayers - 2; i >= 1; i--) {
+        if (i > L1 - 2) continue;
+        const int v = mv[i + 1] + vc[i];
+        if (mv[i] > v) {
+            mv[i] = v;
+            pv[i] = pv[i + 1];
+            pl[i] = (int)idx[i + 1];
         }
+    }
+#pragma unroll
+    for (int i = 1; i < kMaxLayers; i++) {
+        if (i >= L1) continue;
+        m[idx[i]] = mv[i];
+        pr[idx[i]] = pv[i];
+        pr[idx[i] + z.P] = pl[i];
     }
 }
 
@@ -117,22 +146,37 @@ __device__ void zedge(const Z& z, int lane, int j, int fromx, int fromy, int tox
                 }
             }
         __syncthreads();
-        for (int i = 1; i < L1; i++) {
-            if (z.vert(i)) {
-                const long long last = i * nn + (long long)fromx * n + fromy;
-                for (int q = y0 + lane; q <= y1; q += kWave) {
-                    if (q == fromy) continue;
+        // first leg along the child's column (vertical layers) or row
+        // (horizontal layers); the child's value and prefix sum per layer are
+        // loaded once, and the layers need no barrier between them
+        {
+            int cv[kMaxLayers];
+            double cw[kMaxLayers];
+#pragma unroll
+            for (int i = 1; i < kMaxLayers; i++) {
+                if (i >= L1) continue;
+                const long long last = z.vert(i) ? i * nn + (long long)fromx * n + fromy : i * nn + (long long)fromy * n + fromx;
+                cv[i] = map[last];
+                cw[i] = z.wire_sum(last);
+            }
+            for (int q = y0 + lane; q <= y1; q += kWave) {
+                if (q == fromy) continue;
+#pragma unroll
+                for (int i = 1; i < kMaxLayers; i++) {
+                    if (i >= L1 || !z.vert(i)) continue;
                     const long long cur = i * nn + (long long)fromx * n + q;
-                    m4[cur] = map[last] + z.wire(last, cur);
-                    p4[cur] = (int)last;
+                    m4[cur] = cv[i] + z.wire_from(cw[i], cur);
+                    p4[cur] = (int)(i * nn + (long long)fromx * n + fromy);
                 }
-            } else {
-                const long long last = i * nn + (long long)fromy * n + fromx;
-                for (int q = x0 + lane; q <= x1; q += kWave) {
-                    if (q == fromx) continue;
+            }
+            for (int q = x0 + lane; q <= x1; q += kWave) {
+                if (q == fromx) continue;
+#pragma unroll
+                for (int i = 1; i < kMaxLayers; i++) {
+                    if (i >= L1 || z.vert(i)) continue;
                     const long long cur = i * nn + (long long)fromy * n + q;
-                    m4[cur] = map[last] + z.wire(last, cur);
-                    p4[cur] = (int)last;
+                    m4[cur] = cv[i] + z.wire_from(cw[i], cur);
+                    p4[cur] = (int)(i * nn + (long long)fromy * n + fromx);
                 }
             }
             __syncthreads();
@@ -143,59 +187,72 @@ __device__ void zedge(const Z& z, int lane, int j, int fromx, int fromy, int tox
         for (int q = x0 + lane; q <= x1; q += kWave)
             if (q != fromx) via_sweep(z, q, fromy);
         __syncthreads();
-        for (int i = 1; i < L1; i++) {
-            if (z.vert(i)) {
-                for (int q = x0 + lane; q <= x1; q += kWave) {
-                    const long long last = i * nn + (long long)q * n + fromy, cur = i * nn + (long long)q * n + toy;
-                    const int cost = z.wire(last, cur);
-                    if (m4[last] + cost < m4[cur]) {
-                        m4[cur] = m4[last] + cost;
-                        p4[cur] = p4[last];
-                    }
-                }
-            } else {
-                for (int q = y0 + lane; q <= y1; q += kWave) {
-                    const long long last = i * nn + (long long)q * n + fromx, cur = i * nn + (long long)q * n + tox;
-                    const int cost = z.wire(last, cur);
-                    if (m4[last] + cost < m4[cur]) {
-                        m4[cur] = m4[last] + cost;
-                        p4[cur] = p4[last];
-                    }
+        // middle leg across the box: vertical layers from row fromy to row
+        // toy, horizontal layers from column fromx to column tox; cells of
+        // different layers and lanes are disjoint, so one barrier suffices
+        for (int q = x0 + lane; q <= x1; q += kWave) {
+#pragma unroll
+            for (int i = 1; i < kMaxLayers; i++) {
+                if (i >= L1 || !z.vert(i)) continue;
+                const long long last = i * nn + (long long)q * n + fromy, cur = i * nn + (long long)q * n + toy;
+                const int cost = z.wire(last, cur), ml = m4[last], mc = m4[cur], pl = p4[last];
+                if (ml + cost < mc) {
+                    m4[cur] = ml + cost;
+                    p4[cur] = pl;
                 }
             }
-            __syncthreads();
         }
+        for (int q = y0 + lane; q <= y1; q += kWave) {
+#pragma unroll
+            for (int i = 1; i < kMaxLayers; i++) {
+                if (i >= L1 || z.vert(i)) continue;
+                const long long last = i * nn + (long long)q * n + fromx, cur = i * nn + (long long)q * n + tox;
+                const int cost = z.wire(last, cur), ml = m4[last], mc = m4[cur], pl = p4[last];
+                if (ml + cost < mc) {
+                    m4[cur] = ml + cost;
+                    p4[cur] = pl;
+                }
+            }
+        }
+        __syncthreads();
         for (int q = y0 + lane; q <= y1; q += kWave)
             if (q != toy) via_sweep(z, tox, q);
         __syncthreads();
         for (int q = x0 + lane; q <= x1; q += kWave)
             if (q != tox) via_sweep(z, q, toy);
         __syncthreads();
-        for (int i = 1 + lane; i < L1; i += kWave) {
-            if (z.vert(i)) {
-                const long long cur = i * nn + (long long)tox * n + toy;
-                for (int q = y0; q <= y1; q++) {
-                    if (q == toy) continue;
-                    const long long last = i * nn + (long long)tox * n + q;
-                    const int cost = z.wire(last, cur);
-                    if (m4[last] + cost < map[cur + j * P]) {
-                        map[cur + j * P] = m4[last] + cost;
-                        prev[cur + j * P] = p4[last];
-                        prev2[cur + j * P] = prev[5 * P + last];
-                    }
+        // last leg into child slot j of the node, per layer the first
+        // minimum over the leg's cells in ascending order: each lane takes
+        // every 64th cell, then a wavefront reduction keeps the smallest
+        // value and, among equal values, the smallest cell
+#pragma unroll 1
+        for (int i = 1; i < L1; i++) {
+            const bool v = z.vert(i);
+            const long long cur = v ? i * nn + (long long)tox * n + toy : i * nn + (long long)toy * n + tox;
+            const int lo = v ? y0 : x0, hi = v ? y1 : x1, skip = v ? toy : tox;
+            const double wc = z.wire_sum(cur);
+            int best = kZInf * 2, bq = -1;  // above every candidate (at most kZInf + kZInf / 2)
+            for (int q = lo + lane; q <= hi; q += kWave) {
+                if (q == skip) continue;
+                const long long last = v ? i * nn + (long long)tox * n + q : i * nn + (long long)toy * n + q;
+                const int cand = m4[last] + (int)fmin(fabs(z.wire_sum(last) - wc), double(kZInf / 2));
+                if (cand < best) {
+                    best = cand;
+                    bq = q;
                 }
-            } else {
-                const long long cur = i * nn + (long long)toy * n + tox;
-                for (int q = x0; q <= x1; q++) {
-                    if (q == tox) continue;
-                    const long long last = i * nn + (long long)toy * n + q;
-                    const int cost = z.wire(last, cur);
-                    if (m4[last] + cost < map[cur + j * P]) {
-                        map[cur + j * P] = m4[last] + cost;
-                        prev[cur + j * P] = p4[last];
-                        prev2[cur + j * P] = prev[5 * P + last];
-                    }
+            }
+            for (int o = kWave / 2; o; o >>= 1) {
+                const int ob = __shfl_xor(best, o, kWave), oq = __shfl_xor(bq, o, kWave);
+                if (ob < best || (ob == best && oq >= 0 && (bq < 0 || oq < bq))) {
+                    best = ob;
+                    bq = oq;
                 }
+            }
+            if (lane == 0 && bq >= 0 && best < map[cur + j * P]) {
+                const long long last = v ? i * nn + (long long)tox * n + bq : i * nn + (long long)toy * n + bq;
+                map[cur + j * P] = best;
+                prev[cur + j * P] = p4[last];
+                prev2[cur + j * P] = prev[5 * P + last];
             }
         }
     } else if (fromx == tox) {
@@ -228,32 +285,61 @@ __device__ void zedge(const Z& z, int lane, int j, int fromx, int fromy, int tox
     __syncthreads();
 }
 
+// The node's interval combination (zshape.cuh, lane 0 as in the
+// reference): for every layer interval [lower, upper] the cheapest entry of
+// each child slot inside it plus the vias of the interval, kept where it
+// lowers the node's plane-0 value.  The node's cells are loaded once and
+// combined in registers in the reference's order.
 __device__ void znode(const Z& z, const int* rec) {
     const ZshapeArgs& a = z.a;
     const int tox = rec[0] / a.n, toy = rec[0] % a.n, minlayer = rec[1], maxlayer = rec[2];
     const int L1 = a.layers;
+    const bool has[3] = {rec[3] != -1, rec[4] != -1, rec[5] != -1};
+    long long idx[kMaxLayers];
+    int slot[3][kMaxLayers], mv[kMaxLayers], pv[kMaxLayers], via[kMaxLayers];
+#pragma unroll
+    for (int i = 0; i < kMaxLayers; i++) {
+        if (i >= L1) continue;
+        idx[i] = z.cell(i, tox, toy);
+#pragma unroll
+        for (int j = 0; j < 3; j++) slot[j][i] = has[j] ? a.map[idx[i] + (j + 1) * z.P] : 0;
+        mv[i] = a.map[idx[i]];
+        pv[i] = a.prev[idx[i]];
+        via[i] = z.via(idx[i]);
+    }
+#pragma unroll 1
     for (int lower = 0; lower < L1; lower++) {
         int via_sum = 0, min_cost = kZInf, bestpos[3] = {0, 0, 0};
-        int costs[3] = {rec[3] == -1 ? 0 : kZInf, rec[4] == -1 ? 0 : kZInf, rec[5] == -1 ? 0 : kZInf};
-        for (int upper = lower; upper < L1; upper++) {
-            const long long idx = z.cell(upper, tox, toy);
+        int costs[3] = {has[0] ? kZInf : 0, has[1] ? kZInf : 0, has[2] ? kZInf : 0};
+#pragma unroll
+        for (int upper = 0; upper < kMaxLayers; upper++) {
+            if (upper < lower || upper >= L1) continue;
+#pragma unroll
             for (int j = 0; j < 3; j++)
-                if (rec[3 + j] != -1 && a.map[idx + (j + 1) * z.P] < costs[j]) {
-                    costs[j] = a.map[idx + (j + 1) * z.P];
+                if (has[j] && slot[j][upper] < costs[j]) {
+                    costs[j] = slot[j][upper];
                     bestpos[j] = upper;
                 }
             if (1LL * costs[0] + costs[1] + costs[2] < min_cost) min_cost = costs[0] + costs[1] + costs[2];
             if (minlayer == -1 || (lower <= minlayer && maxlayer <= upper)) {
-                for (int i = lower; i <= upper; i++) {
-                    const long long index = z.cell(i, tox, toy);
-                    if (a.map[index] > min_cost + via_sum) {
-                        a.map[index] = min_cost + via_sum;
-                        a.prev[index] = (((lower * L1 + upper) * L1 + bestpos[2]) * L1 + bestpos[1]) * L1 + bestpos[0];
+                const int code = (((lower * L1 + upper) * L1 + bestpos[2]) * L1 + bestpos[1]) * L1 + bestpos[0];
+#pragma unroll
+                for (int i = 0; i < kMaxLayers; i++) {
+                    if (i < lower || i > upper) continue;
+                    if (mv[i] > min_cost + via_sum) {
+                        mv[i] = min_cost + via_sum;
+                        pv[i] = code;
                     }
                 }
             }
-            via_sum += z.via(idx);
+            via_sum += via[upper];
         }
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxLayers; i++) {
+        if (i >= L1) continue;
+        a.map[idx[i]] = mv[i];
+        a.prev[idx[i]] = pv[i];
     }
 }
 
