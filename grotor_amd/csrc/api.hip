@@ -1733,12 +1733,6 @@ int gr_batch_geometry(gr_ctx* c, int n, const int* pt_off, const int* pts, const
     });
 }
 
-// run-head flags of the sorted marks as a scan input, without storing them
-hipcub::TransformInputIterator<int, RunHead, hipcub::CountingInputIterator<int>> run_heads(const unsigned* cell,
-                                                                                          const unsigned* tag, int m) {
-    return {hipcub::CountingInputIterator<int>(0), RunHead{cell, tag, m}};
-}
-
 int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_batches) {
     return guarded([&] {
         if (!c || !color || !num_batches) throw GrError(GR_E_ARG, "null argument");
@@ -1748,6 +1742,11 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         *num_batches = 0;
         BatchGeom b = c->geom;
         b.sparsity = sparsity;
+        // run-head flags of the sorted marks as a scan input, without storing them
+        auto run_heads = [](const unsigned* cell, const unsigned* tag, int m) {
+            return hipcub::TransformInputIterator<int, RunHead, hipcub::CountingInputIterator<int>>(
+                hipcub::CountingInputIterator<int>(0), RunHead{cell, tag, m});
+        };
         const int n = b.n;
         if (!n) return;
         const int XY = c->g.XY;

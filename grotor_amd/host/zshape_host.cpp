@@ -7,10 +7,14 @@
 #include "zshape_host.hpp"
 
 #include <algorithm>
+#include <atomic>
+#include <exception>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <set>
 #include <stdexcept>
+#include <thread>
 #include <tuple>
 
 namespace grotor {
@@ -28,52 +32,76 @@ std::vector<int> zshape_order(const Design& design) {
 std::vector<std::vector<int>> bbox_batches(const std::vector<int>& group, const Design& design, int max_batch) {
     // generate_bbox_batches (src/routing/zshape_topology.hpp:7-58): largest
     // boxes first (larger id first on ties), each pass a greedy set of nets
-    // with pairwise disjoint boxes, at most max_batch nets per pass
+    // with pairwise disjoint boxes, at most max_batch nets per pass.  Nets
+    // taken by a pass leave the pending list; a pass clears only the boxes
+    // it marked.
     const auto& s = design.subnets;
-    const int X = design.grid.X, Y = design.grid.Y;
+    const int Y = design.grid.Y;
     auto area = [&](int id) { return (long long)(s[id].maxx - s[id].minx + 1) * (s[id].maxy - s[id].miny + 1); };
     std::vector<int> pending = group;
     std::sort(pending.begin(), pending.end(),
               [&](int a, int b) { return area(a) == area(b) ? a > b : area(a) > area(b); });
     std::vector<std::vector<int>> batches;
-    std::vector<unsigned char> occupied(size_t(X) * Y);
-    size_t routed = 0;
-    while (routed < pending.size()) {
-        std::fill(occupied.begin(), occupied.end(), 0);
-        std::vector<int> batch;
-        for (int& id : pending) {
-            if (id < 0) continue;
-            const Subnet& n = s[id];
-            bool clash = false;
-            for (int x = n.minx; x <= n.maxx && !clash; ++x)
-                for (int y = n.miny; y <= n.maxy; ++y)
-                    if (occupied[size_t(x) * Y + y]) {
-                        clash = true;
-                        break;
-                    }
-            if (clash) continue;
-            for (int x = n.minx; x <= n.maxx; ++x)
-                for (int y = n.miny; y <= n.maxy; ++y) occupied[size_t(x) * Y + y] = 1;
-            batch.push_back(id);
-            id = -1;
-            ++routed;
-            if (int(batch.size()) == max_batch) break;
+    // occupancy as one bit per cell, rows of x with 64 cells of y per word
+    const int W = (Y + 63) / 64;
+    std::vector<unsigned long long> occupied(size_t(design.grid.X) * W, 0ull);
+    auto span = [](int w, int lo, int hi) {  // bits of word w covering cells lo..hi
+        const int a = std::max(lo - 64 * w, 0), b = std::min(hi - 64 * w, 63);
+        return (b == 63 ? ~0ull : (2ull << b) - 1) & ~((1ull << a) - 1);
+    };
+    auto clashes = [&](const Subnet& n) {
+        for (int x = n.minx; x <= n.maxx; ++x) {
+            const unsigned long long* row = &occupied[size_t(x) * W];
+            for (int w = n.miny / 64; w <= n.maxy / 64; ++w)
+                if (row[w] & span(w, n.miny, n.maxy)) return true;
         }
+        return false;
+    };
+    auto mark = [&](const Subnet& n, bool set) {
+        for (int x = n.minx; x <= n.maxx; ++x) {
+            unsigned long long* row = &occupied[size_t(x) * W];
+            for (int w = n.miny / 64; w <= n.maxy / 64; ++w) {
+                if (set) row[w] |= span(w, n.miny, n.maxy);
+                else row[w] &= ~span(w, n.miny, n.maxy);
+            }
+        }
+    };
+    while (!pending.empty()) {
+        std::vector<int> batch;
+        size_t keep = 0;
+        size_t k = 0;
+        for (; k < pending.size(); ++k) {
+            const int id = pending[k];
+            const Subnet& n = s[id];
+            if (clashes(n)) {
+                pending[keep++] = id;
+                continue;
+            }
+            mark(n, true);
+            batch.push_back(id);
+            if (int(batch.size()) == max_batch) {
+                ++k;
+                break;
+            }
+        }
+        for (; k < pending.size(); ++k) pending[keep++] = pending[k];
+        pending.resize(keep);
         if (batch.empty()) break;
+        for (int id : batch) mark(s[id], false);
         batches.push_back(std::move(batch));
     }
     return batches;
 }
 
-void encode_zshape_net(const Subnet& net, int raw_offset, int X, int Y, int L, std::vector<int>& out) {
+void encode_zshape_net(const Subnet& net, int raw_offset, int X, int Y, int L, int* out) {
     // encode_batch (src/routing/zshape_topology.hpp:59-121): rooted at the
     // first leaf, preorder with children in adjacency order, records of six
-    // words per node
+    // words per node, after the raw-word offset and the node count
     const auto& graph = net.rsmt;
     const int nodes = int(graph.size()) - 1;
     const int n = std::max(X, Y);
-    out.push_back(raw_offset);
-    out.push_back(nodes);
+    out[0] = raw_offset;
+    out[1] = nodes;
     int root = 0;
     for (int i = 0; i < nodes; ++i)
         if (graph[i].size() == 1) {
@@ -95,12 +123,11 @@ void encode_zshape_net(const Subnet& net, int raw_offset, int X, int Y, int L, s
     if (int(order.size()) != nodes) throw std::runtime_error("Z-shape tree is not connected");
     std::vector<int> index(nodes, -1);
     for (int i = 0; i < nodes; ++i) index[order[i]] = i;
-    const size_t base = out.size();
-    out.resize(base + size_t(nodes) * 6, -1);
+    std::fill(out + 2, out + 2 + size_t(nodes) * 6, -1);
     for (int i = 0; i < nodes; ++i) {
         const int v = order[i];
         const int pos = graph.back()[v], layer = pos / (X * Y), pos2 = pos % (X * Y);
-        int* r = out.data() + base + size_t(i) * 6;
+        int* r = out + 2 + size_t(i) * 6;
         r[0] = (pos2 / Y) * n + pos2 % Y;
         if (layer < L) r[1] = r[2] = layer + 1;
         int slot = 3;
@@ -121,32 +148,82 @@ ZshapePlan zshape_plan(const Design& design, const std::vector<int>& order, cons
         if (color[k] < 0 || color[k] >= num_batches) throw std::runtime_error("batch index out of range");
         groups[color[k]].push_back(order[k]);
     }
-    std::vector<std::vector<int>> batches;
-    for (const auto& gr : groups) {
-        auto refined = bbox_batches(gr, design, max_batch);
-        for (auto& b : refined) batches.push_back(std::move(b));
+    // the RSMT batches refine independently: spread them over threads
+    std::vector<std::vector<std::vector<int>>> refined(groups.size());
+    {
+        const int workers = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w)
+            pool.emplace_back([&] {
+                for (size_t i = next++; i < groups.size(); i = next++) refined[i] = bbox_batches(groups[i], design, max_batch);
+            });
+        for (auto& th : pool) th.join();
     }
+    std::vector<std::vector<int>> batches;
+    for (auto& r : refined)
+        for (auto& b : r) batches.push_back(std::move(b));
     std::reverse(batches.begin(), batches.end());
     const Grid& g = design.grid;
     const int L = g.routing_layers();
+    // layout: per batch its nets' record offsets, then their records; every
+    // net's place is known up front, so the nets are encoded in parallel
     ZshapePlan plan;
     plan.batch_off.push_back(0);
     plan.rec_off.push_back(0);
+    std::vector<long long> where;  // per net: its record's position in plan.records
+    long long words = 0;
     for (const auto& b : batches) {
-        std::vector<int> points(b.size());
+        const long long start = words;
+        words += (long long)b.size();
         int raw = 0;
         for (size_t i = 0; i < b.size(); ++i) {
-            points[i] = int(points.size());
             const Subnet& s = design.subnets[b[i]];
-            encode_zshape_net(s, raw, g.X, g.Y, L, points);
+            where.push_back(words);
+            words += 2 + 6 * ((long long)s.rsmt.size() - 1);
             const int cap = int(s.pins.size()) * kZshapeRoutePerPin;
             plan.raw_cap.push_back(cap);
             raw += cap;
             plan.net_ids.push_back(b[i]);
         }
-        plan.records.insert(plan.records.end(), points.begin(), points.end());
+        (void)start;
         plan.batch_off.push_back(int(plan.net_ids.size()));
-        plan.rec_off.push_back((long long)plan.records.size());
+        plan.rec_off.push_back(words);
+    }
+    plan.records.assign(size_t(words), 0);
+    for (size_t bi = 0, k = 0; bi < batches.size(); ++bi) {
+        const long long start = plan.rec_off[bi];
+        for (size_t i = 0; i < batches[bi].size(); ++i, ++k) plan.records[size_t(start + i)] = int(where[k] - start);
+    }
+    {
+        const size_t nets = plan.net_ids.size();
+        const int workers = int(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+        std::atomic<size_t> next{0};
+        std::exception_ptr failure;
+        std::mutex failure_lock;
+        std::vector<int> raw_off(nets);
+        for (size_t bi = 0, k = 0; bi < batches.size(); ++bi) {
+            int raw = 0;
+            for (size_t i = 0; i < batches[bi].size(); ++i, ++k) {
+                raw_off[k] = raw;
+                raw += plan.raw_cap[k];
+            }
+        }
+        std::vector<std::thread> pool;
+        for (int w = 0; w < workers; ++w)
+            pool.emplace_back([&] {
+                try {
+                    for (size_t k = next.fetch_add(256); k < nets; k = next.fetch_add(256))
+                        for (size_t e = std::min(nets, k + 256); k < e; ++k)
+                            encode_zshape_net(design.subnets[plan.net_ids[k]], raw_off[k], g.X, g.Y, L,
+                                              plan.records.data() + where[k]);
+                } catch (...) {
+                    std::lock_guard<std::mutex> guard(failure_lock);
+                    if (!failure) failure = std::current_exception();
+                }
+            });
+        for (auto& th : pool) th.join();
+        if (failure) std::rethrow_exception(failure);
     }
     return plan;
 }

@@ -17,9 +17,9 @@ std::vector<int> zshape_order(const Design& design);
 // nets per refined batch, pairwise disjoint boxes).
 std::vector<std::vector<int>> bbox_batches(const std::vector<int>& group, const Design& design, int max_batch);
 
-// Encoded records of one net, appended to out: raw offset, node count, six
-// words per node in preorder from the first leaf.
-void encode_zshape_net(const Subnet& net, int raw_offset, int X, int Y, int L, std::vector<int>& out);
+// Encoded records of one net, written to out (2 + 6 * nodes words): raw
+// offset, node count, six words per node in preorder from the first leaf.
+void encode_zshape_net(const Subnet& net, int raw_offset, int X, int Y, int L, int* out);
 
 struct ZshapePlan {
     std::vector<int> batch_off{};  // [batches + 1] over net_ids, execution order
