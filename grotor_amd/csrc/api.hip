@@ -1771,7 +1771,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         const bool color_profiled = c->profiling && (c->prof_mask >> GR_K_BATCH_COLOR & 1u);
         timed(c, GR_K_BATCH_MARKS, geom_bytes,
               [&] {
-                  hipLaunchKernelGGL(count_marks_kernel, dim3((n + 3) / 4), dim3(256), 0, st, b, mark_count,
+                  hipLaunchKernelGGL(count_marks_kernel, dim3((n + 15) / 16), dim3(256), 0, st, b, mark_count,
                                      color_profiled ? cand_count : nullptr);
               });
         hip_check(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, mark_count, mark_off, n + 1, st), "scan marks");
@@ -1831,7 +1831,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
         hip_check(hipMemsetAsync(rcount, 0, sizeof(int) * n, st), "clear reader counts");
         if (M) {
             timed(c, GR_K_BATCH_MARKS, 8.0 * M, [&] {
-                hipLaunchKernelGGL(emit_marks_kernel, dim3((n + 3) / 4), dim3(256), 0, st, b, mark_off, cell, tag);
+                hipLaunchKernelGGL(emit_marks_kernel, dim3((n + 15) / 16), dim3(256), 0, st, b, mark_off, cell, tag);
             });
             hipcub::DoubleBuffer<unsigned> keys(cell, cell2), vals(tag, tag2);
             timed(c, GR_K_BATCH_MARKS, 32.0 * M, [&] {

@@ -67,11 +67,14 @@ __device__ __forceinline__ int seg_marks(int first, int second, int step) {
 // each (cell, net) pair once; a net with more than kDedupMax candidates keeps
 // its repeats, which the entry runs then merge).  count_marks_kernel and
 // emit_marks_kernel run the same enumeration, so they keep the same marks.
-constexpr int kDedupSlots = 1024, kDedupMax = 512;
+constexpr int kDedupSlots = 512, kDedupMax = 256;
 constexpr unsigned kNoCell = 0xFFFFFFFFu;
+// Nets share a wavefront in groups of kMarkLanes lanes: most nets have a few
+// dozen candidate marks, which a group covers in a few steps.
+constexpr int kMarkLanes = 16, kMarkGroups = kWave / kMarkLanes;
 
 struct MarkScratch {
-    int off[kWave], start[kWave], step[kWave];
+    int off[kMarkLanes], start[kMarkLanes], step[kMarkLanes];
     unsigned set[kDedupSlots];
 };
 
@@ -87,10 +90,17 @@ __device__ __forceinline__ bool first_visit(unsigned* set, int bits, unsigned ce
     }
 }
 
-// chunk(keep, cell, reader) once per 64 candidates, wave-uniformly;
-// returns the number of candidates
+// The group's bits of a wavefront ballot.
+__device__ __forceinline__ unsigned group_bits(bool pred, int group) {
+    return unsigned(__ballot(pred) >> (group * kMarkLanes)) & ((1u << kMarkLanes) - 1);
+}
+
+// chunk(keep, cell, reader) once per kMarkLanes candidates, uniformly over
+// the net's group of lanes (gl = lane within the group); returns the number
+// of candidates
 template <class Chunk>
-__device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane, MarkScratch& ms, Chunk&& chunk) {
+__device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int gl, MarkScratch& ms, Chunk&& chunk) {
+    constexpr int GW = kMarkLanes;
     const int p0 = b.pt_off[k], np = b.pt_off[k + 1] - p0;
     const int h0 = b.hs_off[k], nh = b.hs_off[k + 1] - h0;
     const int v0 = b.vs_off[k], ns = nh + b.vs_off[k + 1] - v0;
@@ -102,50 +112,50 @@ __device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane
     };
     // candidates, to decide on the dedup
     int cand = 0;
-    for (int q = lane; q < np; q += kWave) {
+    for (int q = gl; q < np; q += GW) {
         const int p = b.pts[p0 + q], px = p / b.Y, py = p % b.Y;
         cand += 1 + (px > 0) + (px + 1 < b.X) + (py > 0);
     }
-    for (int sg = lane; sg < ns; sg += kWave) {
+    for (int sg = gl; sg < ns; sg += GW) {
         int first, st;
         cand += seg_of(sg, first, st);
     }
-    for (int o = kWave / 2; o; o >>= 1) cand += __shfl_xor(cand, o, kWave);
+    for (int o = GW / 2; o; o >>= 1) cand += __shfl_xor(cand, o, GW);
     const bool dedup = cand <= kDedupMax;
-    int bits = 6;  // the set: at least 64 slots, at least twice the candidates
+    int bits = 5;  // the set: at least 32 slots, at least twice the candidates
     while ((1 << bits) < 2 * cand) ++bits;
     if (dedup)
-        for (int i = lane; i < (1 << bits); i += kWave) ms.set[i] = kNoCell;
+        for (int i = gl; i < (1 << bits); i += GW) ms.set[i] = kNoCell;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     auto offer = [&](bool valid, unsigned cell, unsigned reader) {
         const bool keep = valid && (!dedup || first_visit(ms.set, bits, cell));
         chunk(keep, cell, reader);
     };
-    for (int base = 0; base < np; base += kWave) {
-        const int q = base + lane;
+    for (int base = 0; base < np; base += GW) {
+        const int q = base + gl;
         offer(q < np, q < np ? unsigned(b.pts[p0 + q]) : 0u, 1u);
     }
-    for (int sbase = 0; sbase < ns; sbase += kWave) {
-        const int sg = sbase + lane;
+    for (int sbase = 0; sbase < ns; sbase += GW) {
+        const int sg = sbase + gl;
         int cnt = 0, first = 0, st = 1;
         if (sg < ns) cnt = seg_of(sg, first, st);
         int incl = cnt;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int v = __shfl_up(incl, d, kWave);
-            if (lane >= d) incl += v;
+        for (int d = 1; d < GW; d <<= 1) {
+            const int v = __shfl_up(incl, d, GW);
+            if (gl >= d) incl += v;
         }
-        const int total = __shfl(incl, kWave - 1, kWave);
-        ms.off[lane] = incl - cnt;
-        ms.start[lane] = first;
-        ms.step[lane] = st;
+        const int total = __shfl(incl, GW - 1, GW);
+        ms.off[gl] = incl - cnt;
+        ms.start[gl] = first;
+        ms.step[gl] = st;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        for (int base = 0; base < total; base += kWave) {
-            const int t = base + lane;
+        for (int base = 0; base < total; base += GW) {
+            const int t = base + gl;
             unsigned cell = 0;
             if (t < total) {
-                int lo = 0, hi = kWave - 1;  // last lane whose candidates start at or before t
+                int lo = 0, hi = GW - 1;  // last lane whose candidates start at or before t
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (ms.off[mid] <= t) lo = mid;
@@ -157,8 +167,8 @@ __device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane
         }
         __builtin_amdgcn_wave_barrier();
     }
-    for (int base = 0; base < np; base += kWave) {
-        const int q = base + lane;
+    for (int base = 0; base < np; base += GW) {
+        const int q = base + gl;
         const bool valid = q < np;
         const int p = valid ? b.pts[p0 + q] : 0, px = p / b.Y, py = p % b.Y;
         offer(valid && px > 0, unsigned(p - b.Y), 0u);
@@ -169,15 +179,16 @@ __device__ __forceinline__ int for_each_mark(const BatchGeom& b, int k, int lane
 }
 
 // mark_count[k] = marks net k keeps, cand_count[k] (if given) = its
-// candidates before the dedup; one wavefront per net
+// candidates before the dedup; kMarkGroups nets per wavefront
 __global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark_count, int* cand_count) {
-    __shared__ MarkScratch s_ms[4];
-    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
-    const int k = blockIdx.x * 4 + wib;
-    if (k >= b.n) return;  // uniform per wavefront
+    __shared__ MarkScratch s_ms[4 * kMarkGroups];
+    const int g = threadIdx.x / kMarkLanes, gl = threadIdx.x % kMarkLanes, grp = g % kMarkGroups;
+    const int k = blockIdx.x * 4 * kMarkGroups + g;
+    if (k >= b.n) return;  // uniform per group
     int m = 0;
-    const int cand = for_each_mark(b, k, lane, s_ms[wib], [&](bool keep, unsigned, unsigned) { m += __popcll(__ballot(keep)); });
-    if (lane == 0) {
+    const int cand = for_each_mark(b, k, gl, s_ms[g],
+                                   [&](bool keep, unsigned, unsigned) { m += __popc(group_bits(keep, grp)); });
+    if (gl == 0) {
         mark_count[k] = m;
         if (cand_count) cand_count[k] = cand;
     }
@@ -186,21 +197,21 @@ __global__ __launch_bounds__(256) void count_marks_kernel(BatchGeom b, int* mark
 // The kept marks of net k at mark_off[k], each as (cell, k << 1 | reader).
 __global__ __launch_bounds__(256) void emit_marks_kernel(BatchGeom b, const int* mark_off, unsigned* cell,
                                                          unsigned* tag) {
-    __shared__ MarkScratch s_ms[4];
-    const int lane = threadIdx.x & (kWave - 1), wib = threadIdx.x / kWave;
-    const int k = blockIdx.x * 4 + wib;
-    if (k >= b.n) return;  // uniform per wavefront
+    __shared__ MarkScratch s_ms[4 * kMarkGroups];
+    const int g = threadIdx.x / kMarkLanes, gl = threadIdx.x % kMarkLanes, grp = g % kMarkGroups;
+    const int k = blockIdx.x * 4 * kMarkGroups + g;
+    if (k >= b.n) return;  // uniform per group
     const unsigned plain = unsigned(k) << 1;
-    const unsigned long long below = (1ull << lane) - 1;
+    const unsigned below = (1u << gl) - 1;
     int m = mark_off[k];
-    (void)for_each_mark(b, k, lane, s_ms[wib], [&](bool keep, unsigned c, unsigned reader) {
-        const unsigned long long mask = __ballot(keep);
+    (void)for_each_mark(b, k, gl, s_ms[g], [&](bool keep, unsigned c, unsigned reader) {
+        const unsigned mask = group_bits(keep, grp);
         if (keep) {
-            const int w = m + __popcll(mask & below);
+            const int w = m + __popc(mask & below);
             cell[w] = c;
             tag[w] = plain | reader;
         }
-        m += __popcll(mask);
+        m += __popc(mask);
     });
 }
 
