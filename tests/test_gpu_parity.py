@@ -211,6 +211,49 @@ def test_persistent_mode_matches_oracle(ga, ora, per_wave):
     ctx.close()
 
 
+def _large_tree_stage(t, sizes, seed):
+    """A stage of random trees (pins as leaves on layer 0, Steiner points
+    inside), some too large for one workgroup's LDS tables."""
+    rng = np.random.default_rng(seed)
+    X, Y, L = t["X"], t["Y"], t["L"]
+    nodes, parents, node_off = [], [], [0]
+    for n in sizes:
+        par = np.array([-1] + [int(rng.integers(0, i)) for i in range(1, n)], np.int32)
+        leaf = np.ones(n, bool)
+        leaf[par[1:]] = False
+        leaf[0] = True  # the root is a pin
+        x = rng.integers(0, X, n)
+        y = rng.integers(0, Y, n)
+        lay = np.where(leaf, 0, L)
+        nodes.append(((lay * X + x) * Y + y).astype(np.int32))
+        parents.append(par)
+        node_off.append(node_off[-1] + n)
+    return dict(net_ids=np.arange(len(sizes), dtype=np.int32), node_off=np.array(node_off, np.int32),
+                nodes=np.concatenate(nodes), parents=np.concatenate(parents))
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_trees_beyond_lds_match_oracle(ga, ora, mode):
+    """Trees whose DP tables do not fit in LDS (300 and 240 nodes on four
+    layers) take the global-scratch kernel, in the launch and persistent
+    forms alike, and equal the oracle."""
+    _, t = topology(6, 2)
+    print(f"large trees mode={mode}", flush=True)
+    sched = _large_tree_stage(t, [300, 12, 240, 60, 3, 97], 11)
+    sched["batch_off"] = np.array([0, 3, 6], np.int32)
+    pin_acc_num = np.arange(0, 250 * 7, 250, dtype=np.int32)  # 3,000 route words per net
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, pin_acc_num, threads=8)
+    ctx = ga.Context(t)
+    ctx.set_routes(pin_acc_num)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
+    assert np.array_equal(ctx.routes(), routes_o)
+    assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
+    assert np.array_equal(ctx.timestamps(), ts_o)
+    assert ctx.stats() == (wl_o, vias_o)
+    ctx.close()
+
+
 def test_single_batch_entry_points(ga, ora):
     """gr_lshape_batch + gr_commit_batch over one batch equal the oracle."""
     _, t = topology(6, 4)
