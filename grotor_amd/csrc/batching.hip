@@ -295,6 +295,15 @@ __device__ __forceinline__ unsigned long long await_mask(const ColorArgs& a, con
 
 }  // namespace
 
+// The colouring workgroup is one wavefront: its LDS exchanges need only the
+// wavefront's own ordering, not a workgroup barrier (which would also wait
+// for every outstanding global load and store).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Dynamic LDS: per lane W prefix-mask words, then W forbidden words.
 __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
     extern __shared__ unsigned long long lds[];
@@ -312,13 +321,13 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
         return k < a.n ? (int)k : a.n;
     };
     if (lane == 0) s_k = atomicAdd(counter, 1);
-    __syncthreads();
+    wave_sync();
     int k = net_of(s_k);
     while (k < a.n) {
         int next = 0;
         if (lane == 0) next = atomicAdd(counter, 1);  // result used after net k
         for (int w = lane; w < W; w += kWave) forbidden[w] = 0ull;
-        __syncthreads();
+        wave_sync();
         for (int i = lane; i < a.nclosed; i += kWave) {  // batches already full when k is placed
             if (a.closed_from[i] > k) continue;
             const int c = a.closed_color[i];
@@ -343,7 +352,7 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
             const int total = __shfl(incl, kWave - 1, kWave);
             s_off[lane] = incl - cnt;
             s_start[lane] = rc.x;
-            __syncthreads();
+            wave_sync();
             for (int t = lane; t < total; t += kWave) {
                 int lo = 0, hi = kWave - 1;  // last lane whose range starts at or before t
                 while (lo < hi) {
@@ -354,7 +363,7 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
                 const int c = await_color(a, a.list_net[s_start[lo] + (t - s_off[lo])]);
                 atomicOr(&P[lo * W + c / kColorsPerWord], 1ull << (c % kColorsPerWord));
             }
-            __syncthreads();
+            wave_sync();
             if (active) {
                 for (int w = 0; w < W; ++w) {
                     const unsigned long long v = P[lane * W + w];
@@ -363,9 +372,9 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
                     if (v) atomicOr(&forbidden[w], v);
                 }
             }
-            __syncthreads();
+            wave_sync();
         }
-        __syncthreads();
+        wave_sync();
         if (lane == 0) {
             int mex = -1;
             for (int w = 0; w < W && mex < 0; ++w) {
@@ -379,7 +388,7 @@ __global__ __launch_bounds__(kWave) void color_kernel(ColorArgs a) {
             __hip_atomic_store(a.color + k, mex, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_k = next;
         }
-        __syncthreads();
+        wave_sync();
         k = net_of(s_k);
     }
 }
