@@ -246,11 +246,17 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
     const int x = r.x / g.Y % g.X, y = r.x % g.Y;
     const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
     const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
+    // slot keys, loaded once: the costs for the pass below, the codes for
+    // the best-path rows
+    unsigned long long pkv[K][L + 1];
     float pv[K][L + 1];  // slot costs (the packed key's float)
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
-        for (int li = 1; li <= L; ++li) pv[k][li] = k < kids ? (float)unpack_cost(l2_load(pk + k * kS + li)) : 0.f;
+        for (int li = 1; li <= L; ++li) {
+            pkv[k][li] = k < kids ? l2_load(pk + k * kS + li) : 0ull;
+            pv[k][li] = k < kids ? (float)unpack_cost(pkv[k][li]) : 0.f;
+        }
     float vc[L];
 #pragma unroll
     for (int j = 0; j + 1 < L; ++j) vc[j] = a.vcost[cell_of(g, j, x, y)];
@@ -308,7 +314,11 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
         for (int k = 0; k < K; ++k) {
             if (k >= kids) continue;
             const int j = (at[li] >> (4 * k)) & 15;
-            bp[k * kS + li] = (l2_load(pk + k * kS + j) & 0xFFFFFFFFULL) * kS + (unsigned long long)j;
+            unsigned long long key = 0;  // pkv[k][j], j a run-time layer
+#pragma unroll
+            for (int lj = 1; lj <= L; ++lj)
+                if (lj == j) key = pkv[k][lj];
+            bp[k * kS + li] = (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j;
         }
     }
     if (r.y == r.z) {  // the root's trace reads its layer costs
