@@ -48,9 +48,10 @@ __global__ __launch_bounds__(kRefreshThreads) void refresh_tracks_kernel(
         double prev_h = 0.0, prev_1 = 0.0;
         for (int p = p0; p < p1; ++p) {
             const float c = capacity[base + p], d = demand[base + p];
-            const double e0 = of_cost(c, demand_plus(d, 0.0), cp.slope, cp.zero_slope);
-            const double inc_h = incr_from(e0, of_cost(c, demand_plus(d, 0.5), cp.slope, cp.zero_slope), sc);
-            const double inc_1 = incr_from(e0, of_cost(c, demand_plus(d, 1.0), cp.slope, cp.zero_slope), sc);
+            double e0, eh, e1;
+            of_cost3(c, d, cp.slope, cp.zero_slope, e0, eh, e1);
+            const double inc_h = incr_from(e0, eh, sc);
+            const double inc_1 = incr_from(e0, e1, sc);
             // wcost: src/cost/full_update.cuh:44-46
             float w = (float)inc_1;
             if (p < len - 1) w = (float)((double)w + g.unit_wire * (double)elen[p]);

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gr_exp.hpp"
+
 namespace grotor {
 
 constexpr double kInf = 1e22;  // include/router_state.hpp:271
@@ -46,43 +48,39 @@ __host__ __device__ inline int track_of_cell(const DeviceGrid& g, int cell) {
     return g.track_begin[l] + r / len;
 }
 
-// Project exponential, binary32 -> binary32.  Cody-Waite reduction, Taylor
-// series to degree 10 in binary64 Horner form (no contraction: the whole
-// library is compiled with -ffp-contract=off), scaling by 2^n, one rounding
-// to binary32, results below FLT_MIN flushed to +0.  Stands in for CUDA's
-// __expf in src/cost/full_update.cuh:11-20.
-__device__ __forceinline__ float gr_expf(float xf) {
-    if (xf != xf) return xf;
-    if (xf > 89.0f) return __builtin_huge_valf();
-    if (xf < -88.0f) return 0.0f;
-    const double x = (double)xf;
-    const double n = floor(x * 0x1.71547652b82fep+0 + 0.5);
-    double r = x - n * 0x1.62e42fee00000p-1;
-    r = r - n * 0x1.a39ef35793c76p-33;
-    double p = 0x1.27e4fb7789f5cp-22;
-    p = p * r + 0x1.71de3a556c734p-19;
-    p = p * r + 0x1.a01a01a01a01ap-16;
-    p = p * r + 0x1.a01a01a01a01ap-13;
-    p = p * r + 0x1.6c16c16c16c17p-10;
-    p = p * r + 0x1.1111111111111p-7;
-    p = p * r + 0x1.5555555555555p-5;
-    p = p * r + 0x1.5555555555555p-3;
-    p = p * r + 0x1.0000000000000p-1;
-    p = p * r + 1.0;
-    p = p * r + 1.0;
-    const double v = ldexp(p, (int)n);
-    if (v < 0x1.0p-126) return 0.0f;
-    return (float)v;
-}
+// of_cost_scaled with of_cost_scale = 1 (src/cost/full_update.cuh:9-14) at
+// the three demands d, d + 0.5 and d + 1 of the increments (demand_plus),
+// each e = gr_expf(x) or 0.  The first exponential goes through the table
+// form and the other two step from it (x + 1/4 and x + 1/2 with capacity,
+// x + 3/4 and x + 3/2 without, at the default slopes); every value equals
+// the reference form's.
+__device__ __forceinline__ float demand_plus(float d, double inc) { return (float)((double)d + inc + 0.0); }
 
-// of_cost_scaled with of_cost_scale = 1 (src/cost/full_update.cuh:9-14).
-__device__ __forceinline__ double of_cost(float capacity, float demand, double slope, double zero_slope) {
-    if ((double)capacity > 0.001) {
-        const float diff = demand - capacity;
-        return (double)gr_expf((float)fmin(slope * (double)diff, 1.0 * slope * (double)diff));
+__device__ __forceinline__ void of_cost3(float capacity, float demand, double slope, double zero_slope, double& e0,
+                                         double& eh, double& e1) {
+    float x[3];
+    bool need[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float d = demand_plus(demand, 0.5 * k);
+        if ((double)capacity > 0.001) {
+            const float diff = d - capacity;
+            x[k] = (float)fmin(slope * (double)diff, 1.0 * slope * (double)diff);
+            need[k] = true;
+        } else {
+            x[k] = (float)fmin(zero_slope * (double)d, 1.0 * zero_slope * (double)d);
+            need[k] = d > 0.0f;
+        }
     }
-    if (demand > 0.0f) return (double)gr_expf((float)fmin(zero_slope * (double)demand, 1.0 * zero_slope * (double)demand));
-    return 0.0;
+    // the base exponential is taken even where e(d) = 0 (no capacity and no
+    // demand: x = 0), so that the other two can step from it
+    double base_v, v;
+    const float f0 = gr_expf_fast(x[0], &base_v);
+    e0 = need[0] ? (double)f0 : 0.0;
+    const float fh = gr_expf_step(x[1], x[0], base_v, &v);
+    eh = need[1] ? (double)fh : 0.0;
+    const float f1 = gr_expf_step(x[2], x[0], base_v, &v);
+    e1 = need[2] ? (double)f1 : 0.0;
 }
 
 // incremental_of_cost (src/cost/full_update.cuh:23-32) given the three
@@ -90,7 +88,5 @@ __device__ __forceinline__ double of_cost(float capacity, float demand, double s
 __device__ __forceinline__ double incr_from(double e_lo, double e_hi, double short_cost) {
     return fmin(1e12, (e_hi - e_lo) * short_cost);
 }
-
-__device__ __forceinline__ float demand_plus(float d, double inc) { return (float)((double)d + inc + 0.0); }
 
 }  // namespace grotor

@@ -333,3 +333,37 @@ def test_gpu_expf_matches_oracle(ga, ora):
         wo, vo, po = grid.refresh(demand)
         assert np.array_equal(_bits(w), _bits(wo)) and np.array_equal(_bits(v), _bits(vo))
     ctx.close()
+
+
+@pytest.mark.parametrize("slope,zero_slope", [(0.5, 1.5), (0.37, 2.2)])
+def test_gpu_refresh_wide_arguments_match_oracle(ga, ora, slope, zero_slope):
+    """The refresh's table and step exponentials over arguments across the
+    whole [-88, 89] range and beyond: capacities zero, fractional and up to
+    400, demands arbitrary binary32 values and exact halves; the default
+    slopes take the step path, the others (steps not 1/4, 1/2, 3/4, 3/2)
+    the table path for all three demands."""
+    _, t = topology(6, 0)
+    t = dict(t)
+    rng = np.random.default_rng(23)
+    cells = int(t["L"]) * int(t["X"]) * int(t["Y"])
+    cap = rng.uniform(0.0, 400.0, cells).astype(np.float32)
+    cap[rng.random(cells) < 0.25] = 0.0
+    ints = rng.random(cells) < 0.25
+    cap[ints] = np.round(cap[ints])
+    t["capacity"] = cap
+    print(f"refresh wide slope={slope} zero_slope={zero_slope}", flush=True)
+    ctx = ga.Context(t)
+    ctx.set_cost_params(slope=slope, zero_slope=zero_slope)
+    grid = ora.Grid(t)
+    for s in range(3):
+        demand = rng.uniform(0.0, 420.0, cells).astype(np.float32)
+        halves = rng.random(cells) < 0.5
+        demand[halves] = np.round(demand[halves] * 2.0) * 0.5
+        ctx.upload_demand(demand)
+        ctx.refresh(full=True)
+        w, v, p = ctx.costs()
+        wo, vo, po = grid.refresh(demand, slope=slope, zero_slope=zero_slope)
+        assert np.array_equal(_bits(w), _bits(wo)), s
+        assert np.array_equal(_bits(v), _bits(vo)), s
+        assert np.array_equal(_bits(p), _bits(po)), s
+    ctx.close()
