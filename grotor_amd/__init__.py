@@ -243,6 +243,34 @@ class Design:
                     nodes=take(p.nodes, 4 * int(node_off[-1])), edge_off=edge_off,
                     edges=take(p.edges, 2 * int(edge_off[-1])))
 
+    def nets(self):
+        """Per design net CSR arrays: pins (metal-layer cells), extra route
+        pairs and subnet ids (db::net, src/input.hpp:15-24)."""
+        count = ctypes.c_int()
+        ptrs = [ctypes.POINTER(ctypes.c_int)() for _ in range(6)]
+        _call("gr_design_nets", self._h, ctypes.byref(count), *[ctypes.byref(p) for p in ptrs])
+        n = count.value
+
+        def arr(p, k):
+            return np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0, np.int32)
+
+        pin_off, extra_off, sub_off = arr(ptrs[0], n + 1), arr(ptrs[2], n + 1), arr(ptrs[4], n + 1)
+        return dict(pin_off=pin_off, pins=arr(ptrs[1], int(pin_off[-1])), extra_off=extra_off,
+                    extra=arr(ptrs[3], int(extra_off[-1])), sub_off=sub_off, subs=arr(ptrs[5], int(sub_off[-1])))
+
+    def metal_grid(self):
+        """Metal-layer directions, capacities (float, as parsed) and overflow unit costs."""
+        m = ctypes.c_int()
+        _call("gr_design_metal_grid", self._h, ctypes.byref(m), None, None, None)
+        t = self.topology()
+        M = m.value
+        dirs = np.zeros(M, np.int32)
+        cap = np.zeros(M * int(t["X"]) * int(t["Y"]), np.float32)
+        ovf = np.zeros(M, np.float64)
+        _call("gr_design_metal_grid", self._h, ctypes.byref(m), _ptr(dirs, ctypes.c_int), _ptr(cap, ctypes.c_float),
+              _ptr(ovf, ctypes.c_double))
+        return dict(layers=M, dir=dirs, capacity=cap, overflow_cost=ovf)
+
     def write_output(self, routes, path, routes2=None, pin_off2=None, words_per_pin2=100):
         """.out solution; with routes2/pin_off2 the rerouted subnets print
         their phase-two words."""
@@ -454,6 +482,25 @@ class Context:
         _call("gr_lshape_tune", self._h, ctypes.c_int(len(batch_off) - 1), _ptr(batch_off, ctypes.c_int),
               _ptr(pw, ctypes.c_int), _ptr(ms, ctypes.c_double))
         return pw.tolist(), ms.reshape(2, 4).tolist()
+
+    def score(self, nets, layer_dir, capacity, overflow_cost, demand=False):
+        """Evaluator score of the resident routes (gr_score): dict of
+        wirelength, via, overflow and total cost, rejected nets, device ms,
+        and the 16-bit demand per metal-layer cell when asked."""
+        out = np.zeros(6, np.float64)
+        n = len(nets["pin_off"]) - 1
+        arrs = [_i32(nets[k]) for k in ("pin_off", "pins", "extra_off", "extra", "sub_off", "subs")]
+        dirs = _i32(layer_dir)
+        cap = np.ascontiguousarray(capacity, np.float64)
+        ovf = np.ascontiguousarray(overflow_cost, np.float64)
+        dem = np.zeros(len(cap), np.uint16) if demand else None
+        _call("gr_score", self._h, ctypes.c_int(n), *[_ptr(a, ctypes.c_int) for a in arrs], _ptr(dirs, ctypes.c_int),
+              _ptr(cap, ctypes.c_double), _ptr(ovf, ctypes.c_double), _ptr(out, ctypes.c_double),
+              _ptr(dem, ctypes.c_ushort))
+        r = dict(wirelength=out[0], via=out[1], overflow=out[2], total=out[3], rejected=int(out[4]), ms=out[5])
+        if demand:
+            r["demand"] = dem
+        return r
 
     def routes(self):
         r = np.empty(self.route_words, np.int32)

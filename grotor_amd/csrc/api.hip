@@ -1969,4 +1969,112 @@ int gr_batch_plan(gr_ctx* c, int n, const int* pt_off, const int* pts, const int
     return gr_batch_colors(c, limit, sparsity, color, num_batches);
 }
 
+int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int* extra_off, const int* extra,
+             const int* sub_off, const int* subs, const int* layer_dir, const double* capacity,
+             const double* overflow_cost, double* out, unsigned short* demand) {
+    return guarded([&] {
+        if (!c || nets < 0 || !pin_off || !extra_off || !sub_off || !layer_dir || !capacity || !overflow_cost || !out)
+            throw GrError(GR_E_ARG, "null argument");
+        if (!c->routes) throw GrError(GR_E_STATE, "routes not allocated (gr_ctx_set_routes)");
+        const int X = c->g.X, Y = c->g.Y, M = c->g.L + 1;
+        if (X > 65535 || Y > 65535 || M > kScoreLayers)
+            throw GrError(GR_E_ARG, "grid too large for the score's line packing and layer counters");
+        const long long XY = (long long)X * Y, cells = XY * M;
+        const int np = pin_off[nets], ne = extra_off[nets], ns = sub_off[nets];
+        if (np < 0 || ne < 0 || (ne & 1) || ns < 0 || (np && !pins) || (ne && !extra) || (ns && !subs))
+            throw GrError(GR_E_ARG, "invalid net line offsets");
+        for (int k = 0; k < nets; ++k)
+            if (pin_off[k + 1] < pin_off[k] || extra_off[k + 1] < extra_off[k] || sub_off[k + 1] < sub_off[k])
+                throw GrError(GR_E_ARG, "net offsets must be non-decreasing");
+        for (int i = 0; i < np; ++i)
+            if (pins[i] < 0 || pins[i] >= cells) throw GrError(GR_E_ARG, "pin cell outside the metal-layer grid");
+        for (int i = 0; i < ne; ++i)
+            if (extra[i] < 0 || extra[i] >= cells) throw GrError(GR_E_ARG, "extra route cell outside the grid");
+        const int nsub = int(c->pin_acc_num_h.size()) - 1;
+        for (int i = 0; i < ns; ++i)
+            if (subs[i] < 0 || subs[i] >= nsub) throw GrError(GR_E_ARG, "subnet id outside the route store");
+        for (int z = 0; z < M; ++z)
+            if (layer_dir[z] != 0 && layer_dir[z] != 1) throw GrError(GR_E_ARG, "layer direction must be 0 or 1");
+        const int blocks_per_layer = int((XY + kScoreCells - 1) / kScoreCells);
+        DeviceArena ar;
+        int* d_pin_off = ar.alloc<int>(nets + 1);
+        int* d_pins = ar.alloc<int>(np);
+        int* d_extra_off = ar.alloc<int>(nets + 1);
+        int* d_extra = ar.alloc<int>(ne);
+        int* d_sub_off = ar.alloc<int>(nets + 1);
+        int* d_subs = ar.alloc<int>(ns);
+        int* d_dir = ar.alloc<int>(M);
+        double* d_cap = ar.alloc<double>(cells);
+        int* d_wire = ar.alloc<int>(cells);
+        int* d_stacked = ar.alloc<int>(cells);
+        const int ncount = kStatStripes * kScoreLayers + 1;  // striped via counters, rejected nets
+        unsigned long long* d_counts = ar.alloc<unsigned long long>(ncount);
+        double* d_ploss = ar.alloc<double>(size_t(M) * blocks_per_layer);
+        unsigned long long* d_pwl = ar.alloc<unsigned long long>(size_t(M) * blocks_per_layer);
+        unsigned short* d_demand = demand ? ar.alloc<unsigned short>(cells) : nullptr;
+        hipStream_t st = c->stream;
+        upload(d_pin_off, pin_off, sizeof(int) * (nets + 1), st);
+        if (np) upload(d_pins, pins, sizeof(int) * np, st);
+        upload(d_extra_off, extra_off, sizeof(int) * (nets + 1), st);
+        if (ne) upload(d_extra, extra, sizeof(int) * ne, st);
+        upload(d_sub_off, sub_off, sizeof(int) * (nets + 1), st);
+        if (ns) upload(d_subs, subs, sizeof(int) * ns, st);
+        upload(d_dir, layer_dir, sizeof(int) * M, st);
+        upload(d_cap, capacity, sizeof(double) * cells, st);
+        ScoreArgs a;
+        a.nets = nets, a.X = X, a.Y = Y, a.M = M;
+        a.pin_off = d_pin_off, a.pins = d_pins, a.extra_off = d_extra_off, a.extra = d_extra;
+        a.sub_off = d_sub_off, a.subs = d_subs;
+        a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+        a.dir = d_dir, a.wire = d_wire, a.stacked = d_stacked, a.vias = d_counts, a.bad_nets = d_counts + ncount - 1;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        hip_check(hipEventRecord(e0, st), "hipEventRecord");
+        hipLaunchKernelGGL(fill_int_kernel, dim3(int(std::min<long long>(4096, (cells + 255) / 256))), dim3(256), 0, st,
+                           d_wire, 0, cells);
+        hipLaunchKernelGGL(fill_int_kernel, dim3(int(std::min<long long>(4096, (cells + 255) / 256))), dim3(256), 0, st,
+                           d_stacked, 0, cells);
+        hipLaunchKernelGGL(fill_int_kernel, dim3(16), dim3(256), 0, st, (int*)d_counts, 0, 2LL * ncount);
+        if (nets)
+            hipLaunchKernelGGL(score_nets_kernel, dim3(std::min(nets, std::max(1, c->cus) * 16)), dim3(kWave), 0, st, a);
+        hipLaunchKernelGGL(score_cells_kernel, dim3(blocks_per_layer, M), dim3(256), 0, st, a, d_cap, c->g.x_edge_len,
+                           c->g.y_edge_len, d_ploss, d_pwl, d_demand);
+        hip_check(hipEventRecord(e1, st), "hipEventRecord");
+        hip_check(hipGetLastError(), "score kernels");
+        std::vector<unsigned long long> counts(ncount), pwl(size_t(M) * blocks_per_layer);
+        std::vector<double> ploss(size_t(M) * blocks_per_layer);
+        download(counts.data(), d_counts, sizeof(unsigned long long) * ncount, st);
+        download(pwl.data(), d_pwl, sizeof(unsigned long long) * pwl.size(), st);
+        download(ploss.data(), d_ploss, sizeof(double) * ploss.size(), st);
+        if (demand) download(demand, d_demand, sizeof(unsigned short) * cells, st);
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        // resources/evaluator.cpp:575-623, every layer routable (CAP
+        // directions are 0 or 1)
+        double wl_cost = 0, via_cost = 0, overflow = 0;
+        for (int z = 0; z < M; ++z) {
+            unsigned long long wl = 0;
+            double layer_loss = 0;
+            for (int b = 0; b < blocks_per_layer; ++b) {
+                wl += pwl[size_t(z) * blocks_per_layer + b];
+                layer_loss += ploss[size_t(z) * blocks_per_layer + b];
+            }
+            unsigned long long vias = 0;
+            for (int t = 0; t < kStatStripes; ++t) vias += counts[size_t(t) * kScoreLayers + z];
+            overflow += layer_loss * overflow_cost[z];
+            via_cost += double(vias) * c->g.unit_via;
+            wl_cost += double(wl) * c->g.unit_wire;
+        }
+        out[0] = wl_cost;
+        out[1] = via_cost;
+        out[2] = overflow;
+        out[3] = overflow + via_cost + wl_cost;
+        out[4] = double(counts[ncount - 1]);
+        out[5] = double(ms);
+    });
+}
+
 }  // extern "C"

@@ -246,6 +246,26 @@ size_t soft_scratch_bytes(int L, long long nodes, int precision);
 void launch_soft_forward(const SoftArgs& a, int precision, hipStream_t stream);
 void launch_soft_backward(const SoftArgs& a, int precision, hipStream_t stream);
 
+// Evaluator-equivalent score (resources/evaluator.cpp:458-667)
+struct ScoreArgs {
+    int nets, X, Y, M;  // design nets, grid, metal layers (routing layers + 1)
+    const int *pin_off, *pins;      // per net: pins as metal-layer cells; metal-0 pins print a via to metal 1
+    const int *extra_off, *extra;   // per net: extra route endpoint pairs, metal-layer cells
+    const int *sub_off, *subs;      // per net: its subnets, whose route words are in rs
+    RouteStore rs;
+    const int* dir;                 // [M] layer directions as read by the evaluator
+    int* wire;                      // [M * X * Y] unit wire edges per cell
+    int* stacked;                   // [M * X * Y] stacked-via demand per cell
+    unsigned long long* vias;       // [kStatStripes * kScoreLayers] via layer-steps per stripe and layer
+    unsigned long long* bad_nets;   // nets with a line the evaluator rejects
+};
+constexpr int kScoreCells = 8192;  // cells per score_cells_kernel block
+constexpr int kScoreLines = 1024;  // lines of a net staged in LDS by score_nets_kernel
+constexpr int kScoreLayers = 32;   // metal layers the score supports (via counters per stripe)
+__global__ void score_nets_kernel(ScoreArgs a);
+__global__ void score_cells_kernel(ScoreArgs a, const double* capacity, const int* x_len, const int* y_len,
+                                   double* partial_loss, unsigned long long* partial_wl, unsigned short* demand_out);
+
 __global__ void fill_int_kernel(int* p, int value, long long n);
 
 }  // namespace grotor

@@ -1,0 +1,276 @@
+// Evaluator-equivalent score of a routing solution, computed from the
+// resident route words: NVR_DB::read_gr_solution and
+// update_stacked_via_counter (resources/evaluator.cpp:458-667).
+//
+// The evaluator reads the .out text net by net.  Per net it counts every
+// unit wire edge into wire_counter and every via layer-step into the layer's
+// via total, then adds a stacked-via demand on the via's own layer for each
+// distinct via location that none of the net's own wires covers (its flag
+// grid marks the net's wire cells, endpoints included, before the vias are
+// counted).  Demand per cell is 2 * wires + stacked vias in a 16-bit field;
+// the overflow loss is exp(0.5 * (demand - 2 * capacity) / 2) where the
+// capacity exceeds 0.001, exp(0.5 * 1.5 * demand) where it is zero and the
+// cell has demand.  Metal 0 never gets a direction in the evaluator
+// (:383-388): it is no routing layer, its wires are rejected and only its
+// vias are costed.
+//
+// score_nets_kernel: one wavefront per design net, its lines in the .out
+// order (metal-0 pin vias, extra routes, then every subnet's route pairs
+// shifted up one metal layer).  The stacking test of a via scans the net's
+// own lines (decoded once into LDS), so nets run in parallel with the
+// evaluator's sequential result.
+// score_cells_kernel: per cell the demand, loss and wirelength, summed per
+// layer in fixed-shape partial sums that the host adds in order.
+#include "gr_kernels.hpp"
+
+namespace grotor {
+
+namespace {
+
+struct Line {
+    int xl, yl, zl, xh, yh, zh;
+};
+
+__device__ __forceinline__ Line decode_line(int p0, int p1, int X, int Y) {
+    if (p0 > p1) {
+        const int t = p0;
+        p0 = p1;
+        p1 = t;
+    }
+    const int XY = X * Y;
+    return Line{p0 / Y % X, p0 % Y, p0 / XY, p1 / Y % X, p1 % Y, p1 / XY};
+}
+
+// Calls f(ord, p0, p1) for the lines of net k from lane `lane` with stride
+// `stride`, in .out order; ord numbers the lines (pin entries not on metal 0
+// keep their number without producing a line).
+template <class F>
+__device__ __forceinline__ void for_each_line(const ScoreArgs& a, int k, int lane, int stride, F&& f) {
+    const int XY = a.X * a.Y;
+    int ord = 0;
+    const int p0 = a.pin_off[k], p1 = a.pin_off[k + 1];
+    for (int j = p0 + lane; j < p1; j += stride) {
+        const int p = a.pins[j];
+        if (p < XY && f(ord + (j - p0), p, p + XY)) return;
+    }
+    ord += p1 - p0;
+    const int e0 = a.extra_off[k], e1 = a.extra_off[k + 1];
+    for (int j = e0 + 2 * lane; j + 1 < e1; j += 2 * stride)
+        if (f(ord + (j - e0) / 2, a.extra[j], a.extra[j + 1])) return;
+    ord += (e1 - e0) / 2;
+    for (int s = a.sub_off[k]; s < a.sub_off[k + 1]; ++s) {
+        const int id = a.subs[s];
+        const int* r = a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin;
+        const int pairs = (r[0] - 1) / 2;
+        for (int j = lane; j < pairs; j += stride)
+            if (f(ord + j, r[1 + 2 * j] + XY, r[2 + 2 * j] + XY)) return;
+        ord += pairs;
+    }
+}
+
+// Counts of one line (resources/evaluator.cpp:518-569, 637-667); covered(x,
+// y, z, ord) tells whether a wire of the net covers (x, y) on layer z or an
+// earlier via line of the net passes (x, y, z).
+template <class Covered>
+__device__ __forceinline__ void count_line(const ScoreArgs& a, const Line& ln, int ord, bool& bad,
+                                           unsigned long long* vias, Covered&& covered) {
+    const int X = a.X, Y = a.Y;
+    if (ln.zh != ln.zl) {
+        if (ln.xh != ln.xl || ln.yh != ln.yl) {
+            bad = true;  // a via that moves in x or y
+            return;
+        }
+        for (int z = ln.zl; z < ln.zh; ++z) {
+            atomicAdd(vias + z, 1ull);
+            if (covered(ln.xl, ln.yl, z, ord)) continue;
+            int* st = a.stacked + (size_t)z * X * Y;
+            if (a.dir[z] == 0) {
+                if (ln.xl > 0 && ln.xl < X - 1) {
+                    atomicAdd(st + (ln.xl - 1) * Y + ln.yl, 1);
+                    atomicAdd(st + ln.xl * Y + ln.yl, 1);
+                } else if (ln.xl > 0) {
+                    atomicAdd(st + (ln.xl - 1) * Y + ln.yl, 2);
+                } else if (ln.xl < X - 1) {
+                    atomicAdd(st + ln.xl * Y + ln.yl, 2);
+                }
+            } else {
+                if (ln.yl > 0 && ln.yl < Y - 1) {
+                    atomicAdd(st + ln.xl * Y + ln.yl - 1, 1);
+                    atomicAdd(st + ln.xl * Y + ln.yl, 1);
+                } else if (ln.yl > 0) {
+                    atomicAdd(st + ln.xl * Y + ln.yl - 1, 2);
+                } else if (ln.yl < Y - 1) {
+                    atomicAdd(st + ln.xl * Y + ln.yl, 2);
+                }
+            }
+        }
+        return;
+    }
+    if (ln.zl == 0) {
+        bad = true;  // metal 0 is no routing layer (:383-388, :556-559)
+        return;
+    }
+    int* wc = a.wire + (size_t)ln.zl * X * Y;
+    if (a.dir[ln.zl] == 0) {
+        if (!(ln.xh > ln.xl && ln.yh == ln.yl)) {
+            bad = true;
+            return;
+        }
+        for (int x = ln.xl; x < ln.xh; ++x) atomicAdd(wc + x * Y + ln.yl, 1);
+    } else {
+        if (!(ln.yh > ln.yl && ln.xh == ln.xl)) {
+            bad = true;
+            return;
+        }
+        for (int y = ln.yl; y < ln.yh; ++y) atomicAdd(wc + ln.xl * Y + y, 1);
+    }
+}
+
+// Whether line m (of the same net) makes the via step at (x, y, z) of line
+// ord not stacked.
+__device__ __forceinline__ bool covers(const ScoreArgs& a, const Line& m, int mord, int x, int y, int z, int ord) {
+    if (m.zl == m.zh) {
+        if (m.zl != z || z == 0) return false;
+        if (a.dir[z] == 0) return m.xh > m.xl && m.yh == m.yl && m.yl == y && m.xl <= x && x <= m.xh;
+        return m.yh > m.yl && m.xh == m.xl && m.xl == x && m.yl <= y && y <= m.yh;
+    }
+    return mord < ord && m.xl == m.xh && m.yl == m.yh && m.xl == x && m.yl == y && m.zl <= z && z < m.zh;
+}
+
+__device__ __forceinline__ int pack_xy(int x, int y) { return x | y << 16; }
+
+}  // namespace
+
+// One wavefront (workgroup) per design net.  A net of at most kScoreLines
+// lines is decoded once into LDS, packed as (xl | yl << 16, xh | yh << 16,
+// zl | zh << 8, or -1 for a metal-1 pin without a line), and every via scans
+// the LDS copy; larger nets scan their lines in global memory.  Via counts
+// per layer gather in LDS and leave once per workgroup, into stripe
+// blockIdx.x % kStatStripes of a.vias (kScoreLayers counters a stripe): one
+// global counter per layer would serialize millions of atomics.
+__global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
+    __shared__ int s_lo[kScoreLines], s_hi[kScoreLines], s_z[kScoreLines];
+    __shared__ unsigned long long s_vias[kScoreLayers];
+    const int lane = threadIdx.x;
+    const int X = a.X, Y = a.Y;
+    if (lane < kScoreLayers) s_vias[lane] = 0;
+    __syncthreads();
+    for (int k = blockIdx.x; k < a.nets; k += gridDim.x) {
+        // line count: pin entries, extra pairs, subnet pairs
+        int n = (a.pin_off[k + 1] - a.pin_off[k]) + (a.extra_off[k + 1] - a.extra_off[k]) / 2;
+        int subs_pairs = 0;
+        for (int s = a.sub_off[k] + lane; s < a.sub_off[k + 1]; s += kWave) {
+            const int* r = a.rs.routes + (size_t)a.rs.pin_acc_num[a.subs[s]] * a.rs.per_pin;
+            subs_pairs += max(0, (r[0] - 1) / 2);
+        }
+        for (int off = kWave / 2; off; off >>= 1) subs_pairs += __shfl_xor(subs_pairs, off, kWave);
+        n += subs_pairs;
+        bool bad = false;
+        if (n <= kScoreLines) {
+            for_each_line(a, k, lane, kWave, [&](int ord, int q0, int q1) {
+                const Line ln = decode_line(q0, q1, X, Y);
+                s_lo[ord] = pack_xy(ln.xl, ln.yl);
+                s_hi[ord] = pack_xy(ln.xh, ln.yh);
+                s_z[ord] = ln.zl | ln.zh << 8;
+                return false;
+            });
+            // metal-1 pins print no line
+            for (int j = a.pin_off[k] + lane; j < a.pin_off[k + 1]; j += kWave)
+                if (a.pins[j] >= X * Y) s_z[j - a.pin_off[k]] = -1;
+            __syncthreads();
+            for (int i = lane; i < n; i += kWave) {
+                if (s_z[i] < 0) continue;
+                const Line ln{s_lo[i] & 0xFFFF, s_lo[i] >> 16, s_z[i] & 0xFF,
+                              s_hi[i] & 0xFFFF, s_hi[i] >> 16, s_z[i] >> 8};
+                count_line(a, ln, i, bad, s_vias, [&](int x, int y, int z, int ord) {
+                    for (int m = 0; m < n; ++m) {
+                        if (s_z[m] < 0) continue;
+                        const Line l2{s_lo[m] & 0xFFFF, s_lo[m] >> 16, s_z[m] & 0xFF,
+                                      s_hi[m] & 0xFFFF, s_hi[m] >> 16, s_z[m] >> 8};
+                        if (covers(a, l2, m, x, y, z, ord)) return true;
+                    }
+                    return false;
+                });
+            }
+            __syncthreads();  // the LDS lines are reused by the next net
+        } else {
+            for_each_line(a, k, lane, kWave, [&](int ord, int q0, int q1) {
+                count_line(a, decode_line(q0, q1, X, Y), ord, bad, s_vias, [&](int x, int y, int z, int o) {
+                    bool hit = false;
+                    for_each_line(a, k, 0, 1, [&](int mord, int r0, int r1) {
+                        hit = covers(a, decode_line(r0, r1, X, Y), mord, x, y, z, o);
+                        return hit;
+                    });
+                    return hit;
+                });
+                return false;
+            });
+        }
+        if (__any(bad) && lane == 0) atomicAdd(a.bad_nets, 1ull);
+    }
+    __syncthreads();
+    if (lane < a.M && s_vias[lane])
+        atomicAdd(a.vias + (blockIdx.x % kStatStripes) * kScoreLayers + lane, s_vias[lane]);
+}
+
+// Block b of layer blockIdx.y covers cells [b * kScoreCells, (b + 1) *
+// kScoreCells) of the layer; its loss and wirelength sums go to
+// partial_loss / partial_wl[layer * gridDim.x + b].
+__global__ __launch_bounds__(256) void score_cells_kernel(ScoreArgs a, const double* capacity, const int* x_len,
+                                                          const int* y_len, double* partial_loss,
+                                                          unsigned long long* partial_wl, unsigned short* demand_out) {
+    __shared__ double s_loss[256];
+    __shared__ unsigned long long s_wl[256];
+    const int z = blockIdx.y;
+    const int XY = a.X * a.Y;
+    if (z == 0) {  // metal 0 is no routing layer: no demand, loss or wirelength
+        const int c0 = blockIdx.x * kScoreCells, c1 = min(XY, c0 + kScoreCells);
+        if (demand_out)
+            for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) demand_out[c] = 0;
+        if (threadIdx.x == 0) {
+            partial_loss[blockIdx.x] = 0.0;
+            partial_wl[blockIdx.x] = 0;
+        }
+        return;
+    }
+    const int dir = a.dir[z];
+    double loss = 0.0;
+    unsigned long long wl = 0;
+    const int c0 = blockIdx.x * kScoreCells;
+    const int c1 = min(XY, c0 + kScoreCells);
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+        const size_t cell = (size_t)z * XY + c;
+        const int w = a.wire[cell];
+        const unsigned short d = (unsigned short)(2 * w + a.stacked[cell]);  // NVR_Gcell::m_demand : 16
+        if (demand_out) demand_out[cell] = d;
+        const double cap = capacity[cell];
+        if (cap > 0.001) {
+            const double overflow = (double)d - 2 * cap;
+            loss += exp(overflow / 2 * 0.5);
+        } else if (cap >= 0 && d > 0) {
+            loss += exp(1.5 * (double)d * 0.5);
+        }
+        const int x = c / a.Y, y = c % a.Y;
+        if (dir == 0) {
+            if (x < a.X - 1) wl += (unsigned long long)(w * x_len[x]);
+        } else if (y < a.Y - 1) {
+            wl += (unsigned long long)(w * y_len[y]);
+        }
+    }
+    s_loss[threadIdx.x] = loss;
+    s_wl[threadIdx.x] = wl;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h; h >>= 1) {
+        if (threadIdx.x < h) {
+            s_loss[threadIdx.x] += s_loss[threadIdx.x + h];
+            s_wl[threadIdx.x] += s_wl[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial_loss[z * gridDim.x + blockIdx.x] = s_loss[0];
+        partial_wl[z * gridDim.x + blockIdx.x] = s_wl[0];
+    }
+}
+
+}  // namespace grotor

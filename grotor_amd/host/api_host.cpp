@@ -426,4 +426,19 @@ int gr_design_write_output(const gr_design* g, const int* routes, const char* ou
     });
 }
 
+int gr_design_metal_grid(const gr_design* g, int* metal_layers, int* layer_dir, float* capacity,
+                         double* overflow_cost) {
+    return guarded([&] {
+        if (!g || !metal_layers) throw GrError(GR_E_ARG, "null argument");
+        const Grid& grid = g->d.grid;
+        *metal_layers = grid.L;
+        const size_t cells = size_t(grid.L) * grid.X * grid.Y;
+        if (layer_dir)
+            for (int z = 0; z < grid.L; ++z) layer_dir[z] = grid.layers[z].dir;
+        if (capacity) std::copy(grid.capacity.begin(), grid.capacity.begin() + cells, capacity);
+        if (overflow_cost)
+            std::copy(grid.unit_length_short_costs.begin(), grid.unit_length_short_costs.end(), overflow_cost);
+    });
+}
+
 }  // extern "C"

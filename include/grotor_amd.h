@@ -99,6 +99,22 @@ int gr_timestamp_download(gr_ctx *ctx, int *timestamp);
 /* Statistics accumulated by commits (RoutingStatistics, include/router_state.hpp:17-20). */
 int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vias);
 
+/* Evaluator score of the context's route words, as the reference evaluator
+ * computes it from the .out text (NVR_DB::read_gr_solution and
+ * update_stacked_via_counter, resources/evaluator.cpp:458-667).  Per design
+ * net, in .out order: a via from metal 0 to metal 1 for each pin on metal 0
+ * (pins as metal-layer cells (z * X + x) * Y + y), the extra_routes pairs,
+ * then the route pairs of its subnets shifted up one metal layer.
+ * layer_dir, capacity (metal-layer cells, binary64 as the evaluator reads
+ * them) and overflow_cost cover the L + 1 metal layers.  out receives
+ * {wirelength cost, via cost, overflow cost, total cost, nets with a line the
+ * evaluator rejects, device ms}; demand, when not NULL, the evaluator's
+ * 16-bit demand per metal-layer cell.  The connectivity flood fill of
+ * check_connectivity is not part of this score. */
+int gr_score(gr_ctx *ctx, int nets, const int *pin_off, const int *pins, const int *extra_off, const int *extra,
+             const int *sub_off, const int *subs, const int *layer_dir, const double *capacity,
+             const double *overflow_cost, double *out, unsigned short *demand);
+
 /* Exact first-fit greedy batching of n nets given in priority order
  * (cudb::generate_batches_rsmt, src/batching/generate_batches_cpu.hpp:7-73;
  * timed wrapper generate_batches_cpu_timed, src/batching/generate_batches.hpp:14-25).
@@ -393,6 +409,12 @@ int gr_design_flt_plan(gr_design *design, const int *color, int num_batches, gr_
 int gr_flt_dag(int X, int Y, int L, int mode, int n, const int *positions, const int *adj_off, const int *adj,
                const float *xsum, const float *ysum, int *nodes, int *nnodes, int *edges, int *nedges,
                int *points, int *npoints);
+
+/* Metal-layer grid of a design as read from its CAP file (src/input.hpp:275-324):
+ * layer count, then, where not NULL, each layer's direction, the capacity per
+ * metal-layer cell (z * X + x) * Y + y and each layer's overflow unit cost. */
+int gr_design_metal_grid(const gr_design *design, int *metal_layers, int *layer_dir, float *capacity,
+                         double *overflow_cost);
 
 /* Per design net of a prepared design: selected pins (3D metal indices),
  * extra_routes pairs and routed subnet ids, as CSR arrays owned by the design

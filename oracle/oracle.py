@@ -313,6 +313,26 @@ def expf(x):
     return lib().ora_expf(float(x))
 
 
+def score(X, Y, M, nets, routes, pin_acc_num, per_pin, layer_dir, capacity, overflow_cost, x_len, y_len,
+          unit_wire, unit_via):
+    """Evaluator score of per-net lines (ora_score): dict of costs, rejected
+    nets and the 16-bit demand per metal-layer cell."""
+    out = np.zeros(5, np.float64)
+    dem = np.zeros(M * X * Y, np.uint16)
+    arrs = [_i32(nets[k]) for k in ("pin_off", "pins", "extra_off", "extra", "sub_off", "subs")]
+    routes = _i32(routes if routes is not None else np.zeros(1, np.int32))
+    acc = _i32(pin_acc_num if pin_acc_num is not None else np.zeros(1, np.int32))
+    dirs, xl, yl = _i32(layer_dir), _i32(x_len), _i32(y_len)
+    cap = np.ascontiguousarray(capacity, np.float64)
+    ovf = np.ascontiguousarray(overflow_cost, np.float64)
+    lib().ora_score(ctypes.c_int(X), ctypes.c_int(Y), ctypes.c_int(M), ctypes.c_int(len(arrs[0]) - 1),
+                    *[_p(a, ctypes.c_int) for a in arrs], _p(routes, ctypes.c_int), _p(acc, ctypes.c_int),
+                    ctypes.c_int(per_pin), _p(dirs, ctypes.c_int), _p(cap, ctypes.c_double),
+                    _p(ovf, ctypes.c_double), _p(xl, ctypes.c_int), _p(yl, ctypes.c_int), ctypes.c_double(unit_wire),
+                    ctypes.c_double(unit_via), _p(out, ctypes.c_double), _p(dem, ctypes.c_ushort))
+    return dict(wirelength=out[0], via=out[1], overflow=out[2], total=out[3], rejected=int(out[4]), demand=dem)
+
+
 def evaluator_path():
     return os.path.join(REF_DIR, "evaluator")
 

@@ -1,0 +1,188 @@
+"""Evaluator score (resources/evaluator.cpp:458-667): the oracle against the
+reference evaluator built in oracle/_ref on whole .out files, hand-built
+cases whose demand follows from the evaluator's rules, and (GPU) gr_score
+against the oracle on resident routes."""
+import gzip
+import os
+
+import numpy as np
+import pytest
+
+from tests import _golden as G
+from tests._instances import topology
+
+
+def read_cap(path):
+    """CAP text as the evaluator reads it: binary64 capacities per metal-layer
+    cell (z * X + x) * Y + y, directions, unit and overflow costs, edge
+    lengths (resources/evaluator.cpp:311-405)."""
+    op = gzip.open if path.endswith(".gz") else open
+    with op(path, "rt") as f:
+        tok = f.read().split()
+    it = iter(tok)
+    L, X, Y = int(next(it)), int(next(it)), int(next(it))
+    unit_wire, unit_via = float(next(it)), float(next(it))
+    ovf = np.array([float(next(it)) for _ in range(L)])
+    xl = np.array([int(next(it)) for _ in range(X - 1)], np.int32)
+    yl = np.array([int(next(it)) for _ in range(Y - 1)], np.int32)
+    dirs = np.zeros(L, np.int32)
+    cap = np.zeros((L, X, Y))
+    for z in range(L):
+        next(it)
+        dirs[z] = int(next(it))
+        next(it)
+        rows = np.array([float(next(it)) for _ in range(X * Y)]).reshape(Y, X)
+        cap[z] = rows.T
+    return dict(L=L, X=X, Y=Y, unit_wire=unit_wire, unit_via=unit_via, ovf=ovf, x_len=xl, y_len=yl, dir=dirs,
+                capacity=cap.reshape(-1))
+
+
+def read_out_lines(path, X, Y):
+    """Per net of a .out file its lines as metal-layer cell pairs, in file
+    order, as extra pairs (no pins, no subnets)."""
+    extra_off, extra = [0], []
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line or line == "(":
+                continue
+            if line == ")":
+                extra_off.append(len(extra))
+                continue
+            v = line.split()
+            if len(v) == 6:
+                xl, yl, zl, xh, yh, zh = map(int, v)
+                extra += [(zl * X + xl) * Y + yl, (zh * X + xh) * Y + yh]
+    n = len(extra_off) - 1
+    z = np.zeros(n + 1, np.int32)
+    return dict(pin_off=z, pins=np.zeros(0, np.int32), extra_off=np.array(extra_off, np.int32),
+                extra=np.array(extra, np.int32), sub_off=z, subs=np.zeros(0, np.int32))
+
+
+def oracle_score(ora, cap, nets, routes=None, acc=None, per_pin=12):
+    return ora.score(cap["X"], cap["Y"], cap["L"], nets, routes, acc, per_pin, cap["dir"], cap["capacity"], cap["ovf"],
+                     cap["x_len"], cap["y_len"], cap["unit_wire"], cap["unit_via"])
+
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_oracle_score_equals_reference_evaluator(ga, ora, name, tmp_path):
+    """The oracle pipeline's .out (L-shape, rip-up, FLT) scored by the oracle
+    and by the reference evaluator: every printed cost equal."""
+    if not os.path.exists(ora.evaluator_path()):
+        pytest.skip("reference evaluator not built (oracle/_ref)")
+    from tests.test_pipeline import oracle_pipeline
+    capf, netf = G.unpacked(name, "cap"), G.unpacked(name, "net")
+    out = str(tmp_path / "oracle.out")
+    oracle_pipeline(ga, ora, capf, netf, out)
+    cap = read_cap(capf)
+    mine = oracle_score(ora, cap, read_out_lines(out, cap["X"], cap["Y"]))
+    ref, res = ora.run_evaluator(capf, netf, out)
+    assert res.returncode == 0, res.stdout[-2000:]
+    for key in ("wirelength", "via", "overflow", "total"):
+        # the evaluator prints %.4lf
+        assert abs(mine[key] - ref[key]) <= 5e-5 + 1e-12 * abs(ref[key]), (key, mine[key], ref[key])
+    assert mine["rejected"] == 0
+
+
+def _tiny_cap(X=6, Y=5, L=3):
+    cap = np.full((L, X, Y), 4.0)
+    cap[0] = 0.0
+    return dict(L=L, X=X, Y=Y, unit_wire=0.5, unit_via=4.0, ovf=np.array([5.0, 6.0, 7.0]),
+                x_len=np.full(X - 1, 10, np.int32), y_len=np.full(Y - 1, 20, np.int32),
+                dir=np.array([0, 0, 1], np.int32), capacity=cap.reshape(-1))
+
+
+def _cell(cap, z, x, y):
+    return (z * cap["X"] + x) * cap["Y"] + y
+
+
+def _lines(cap, nets):
+    off, extra = [0], []
+    for lines in nets:
+        for (x0, y0, z0, x1, y1, z1) in lines:
+            extra += [_cell(cap, z0, x0, y0), _cell(cap, z1, x1, y1)]
+        off.append(len(extra))
+    z = np.zeros(len(nets) + 1, np.int32)
+    return dict(pin_off=z, pins=np.zeros(0, np.int32), extra_off=np.array(off, np.int32),
+                extra=np.array(extra, np.int32), sub_off=z, subs=np.zeros(0, np.int32))
+
+
+def test_stacked_vias_follow_the_evaluator_rules(ora):
+    """Hand-derived demand: a via under the net's own wire adds nothing, a
+    repeated via location counts once, a via of another net at a covered
+    cell counts, border vias add 2 to the one inner edge, and wires count
+    2 per unit edge."""
+    cap = _tiny_cap()
+    d = lambda r, z, x, y: int(r["demand"][_cell(cap, z, x, y)])  # noqa: E731
+    nets = [
+        # net 0: wire on layer 1 (horizontal) from x=1 to x=3 at y=2, via 1->2 at x=2 (under the wire)
+        [(1, 2, 1, 3, 2, 1), (2, 2, 1, 2, 2, 2)],
+        # net 1: via 1->2 at (2, 2) twice: counted once, on layer 1 at x-1 and x
+        [(2, 2, 1, 2, 2, 2), (2, 2, 1, 2, 2, 2)],
+        # net 2: via 1->2 at x = 0 (border: +2 on edge x=0) and at x = X-1 (+2 on edge X-2)
+        [(0, 4, 1, 0, 4, 2), (5, 4, 1, 5, 4, 2)],
+        # net 3: via 0->2 at (3, 0): stacked on layer 1; metal 0 is no routing
+        # layer for the evaluator (no demand there), but its via is costed
+        [(3, 0, 0, 3, 0, 2)],
+    ]
+    r = oracle_score(ora, cap, _lines(cap, nets))
+    assert r["rejected"] == 0
+    # wires of net 0: edges x=1, 2 on layer 1 -> 2 each; net 1's via adds 1 at x=1 and x=2
+    assert d(r, 1, 1, 2) == 2 + 1 and d(r, 1, 2, 2) == 2 + 1 and d(r, 1, 3, 2) == 0
+    assert d(r, 1, 0, 4) == 2 and d(r, 1, 4, 4) == 2
+    assert d(r, 0, 2, 0) == 0 and d(r, 0, 3, 0) == 0 and d(r, 1, 2, 0) == 1 and d(r, 1, 3, 0) == 1
+    assert int(r["demand"].astype(np.int64).sum()) == 2 * 2 + 2 + 2 + 2 + 2
+    # wirelength: 2 unit edges of length 10; vias: layer 1: 1 + 2 + 2 + 1, layer 0: 1
+    assert r["wirelength"] == pytest.approx(0.5 * 20)
+    assert r["via"] == pytest.approx(4.0 * (1 + 2 + 2 + 1 + 1))
+    # overflow: layers 1-2 (capacity 4) only
+    dem = r["demand"].reshape(3, cap["X"], cap["Y"]).astype(np.float64)
+    want = 0.0
+    for z in (1, 2):
+        want += cap["ovf"][z] * np.exp((dem[z] - 8.0) / 2 * 0.5).sum()
+    assert r["overflow"] == pytest.approx(want, rel=1e-12)
+
+
+def test_rejected_lines_and_demand_wrap(ora):
+    """A wire against its layer's direction, a wire on metal 0 and a via that
+    moves are rejected (nothing counted for them); demand is kept in 16 bits."""
+    cap = _tiny_cap()
+    nets = [[(1, 1, 2, 3, 1, 2)], [(1, 1, 1, 2, 2, 2)], [(0, 0, 1, 1, 0, 1)] * 40000, [(0, 3, 0, 2, 3, 0)]]
+    r = oracle_score(ora, cap, _lines(cap, nets))
+    assert r["rejected"] == 3
+    assert r["via"] == 0.0
+    # 40000 wires over one edge: 80000 mod 65536
+    assert int(r["demand"][_cell(cap, 1, 0, 0)]) == 80000 % 65536
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("config,seed", [(1, 0), (6, 4)])
+def test_gpu_score_matches_oracle(ga, ora, config, seed):
+    """gr_score on the resident routes of a GPU L-shape stage against the
+    oracle on the same routes: demand and rejected nets identical, wirelength
+    and via cost equal, overflow cost within 1e-10 relative: the device exp
+    and the order of each layer's sum differ from the evaluator's sequential
+    sum of up to millions of terms, whose own rounding error is of that
+    order."""
+    d, t = topology(config, seed)
+    print(f"score config={config} seed={seed}", flush=True)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    color, nb = ctx.batch_plan(*geom)
+    sched = ga.execution_schedule(t, color, nb)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
+    nets = d.nets()
+    mg = d.metal_grid()
+    cap = np.asarray(mg["capacity"], np.float64)
+    got = ctx.score(nets, mg["dir"], cap, mg["overflow_cost"], demand=True)
+    M, X, Y = mg["layers"], int(t["X"]), int(t["Y"])
+    want = ora.score(X, Y, M, nets, ctx.routes(), t["pin_acc_num"], 12, mg["dir"], cap, mg["overflow_cost"],
+                     t["x_edge_len"], t["y_edge_len"], t["unit_length_wire_cost"], t["unit_via_cost"])
+    assert np.array_equal(got["demand"], want["demand"])
+    assert got["rejected"] == want["rejected"] == 0
+    assert got["wirelength"] == want["wirelength"] and got["via"] == want["via"]
+    assert got["overflow"] == pytest.approx(want["overflow"], rel=1e-10)
+    assert got["ms"] > 0
+    ctx.close()
