@@ -1,6 +1,6 @@
 // Evaluator-equivalent score of a routing solution, computed from the
 // resident route words: NVR_DB::read_gr_solution and
-// update_stacked_via_counter (resources/evaluator.cpp:458-667).
+// update_stacked_via_counter (resources/evaluator.cpp:458-675).
 //
 // The evaluator reads the .out text net by net.  Per net it counts every
 // unit wire edge into wire_counter and every via layer-step into the layer's
@@ -68,7 +68,7 @@ __device__ __forceinline__ void for_each_line(const ScoreArgs& a, int k, int lan
     }
 }
 
-// Counts of one line (resources/evaluator.cpp:518-569, 637-667); covered(x,
+// Counts of one line (resources/evaluator.cpp:518-569, 635-675); covered(x,
 // y, z, ord) tells whether a wire of the net covers (x, y) on layer z or an
 // earlier via line of the net passes (x, y, z).
 template <class Covered>

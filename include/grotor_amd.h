@@ -101,7 +101,7 @@ int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vi
 
 /* Evaluator score of the context's route words, as the reference evaluator
  * computes it from the .out text (NVR_DB::read_gr_solution and
- * update_stacked_via_counter, resources/evaluator.cpp:458-667).  Per design
+ * update_stacked_via_counter, resources/evaluator.cpp:458-675).  Per design
  * net, in .out order: a via from metal 0 to metal 1 for each pin on metal 0
  * (pins as metal-layer cells (z * X + x) * Y + y), the extra_routes pairs,
  * then the route pairs of its subnets shifted up one metal layer.

@@ -1,6 +1,6 @@
 /*
  * Oracle of the evaluator score: NVR_DB::read_gr_solution and
- * update_stacked_via_counter (resources/evaluator.cpp:458-667), restated
+ * update_stacked_via_counter (resources/evaluator.cpp:458-675), restated
  * over line lists instead of .out text.  TEST INFRASTRUCTURE ONLY.
  *
  * Per net the lines come in .out order: a via from metal 0 to metal 1 for
@@ -84,7 +84,7 @@ static void take_line(line_sink *s, int p0, int p1) {
     }
 }
 
-/* resources/evaluator.cpp:637-667 */
+/* resources/evaluator.cpp:635-675 */
 static void stack_vias(line_sink *s) {
     for (int i = 0; i < s->nvia; ++i) {
         const int x = s->via_loc[3 * i], y = s->via_loc[3 * i + 1], z = s->via_loc[3 * i + 2];

@@ -1,4 +1,4 @@
-"""Evaluator score (resources/evaluator.cpp:458-667): the oracle against the
+"""Evaluator score (resources/evaluator.cpp:458-675): the oracle against the
 reference evaluator built in oracle/_ref on whole .out files, hand-built
 cases whose demand follows from the evaluator's rules, and (GPU) gr_score
 against the oracle on resident routes."""
