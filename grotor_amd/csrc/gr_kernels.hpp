@@ -246,7 +246,7 @@ size_t soft_scratch_bytes(int L, long long nodes, int precision);
 void launch_soft_forward(const SoftArgs& a, int precision, hipStream_t stream);
 void launch_soft_backward(const SoftArgs& a, int precision, hipStream_t stream);
 
-// Evaluator-equivalent score (resources/evaluator.cpp:458-667)
+// Evaluator-equivalent score (resources/evaluator.cpp:458-675)
 struct ScoreArgs {
     int nets, X, Y, M;  // design nets, grid, metal layers (routing layers + 1)
     const int *pin_off, *pins;      // per net: pins as metal-layer cells; metal-0 pins print a via to metal 1

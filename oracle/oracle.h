@@ -193,7 +193,7 @@ int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *
                   int *routes2, int remove_phase_one, int *stamp, double *total_wirelength, long long *total_vias,
                   int threads);
 
-/* Evaluator score (resources/evaluator.cpp:458-667) of per-net line lists:
+/* Evaluator score (resources/evaluator.cpp:458-675) of per-net line lists:
  * metal-0 pin vias, extra pairs, then subnet route pairs shifted up one metal
  * layer.  out: wirelength, via, overflow and total cost, rejected nets. */
 int ora_score(int X, int Y, int M, int nets, const int *pin_off, const int *pins, const int *extra_off,
