@@ -60,9 +60,10 @@ def parse():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the nets-per-wavefront autotuner (one net / DAG per wavefront), e.g. for counter runs")
-    ap.add_argument("--stage", choices=("lshape", "zshape", "soft", "flt"), default="lshape",
+    ap.add_argument("--stage", choices=("lshape", "zshape", "soft", "flt", "score"), default="lshape",
                     help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd; "
-                         "flt: the FLT rip-up stage after the L-shape stage")
+                         "flt: the FLT rip-up stage after the L-shape stage; score: the evaluator score of the "
+                         "L-shape stage's routes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
     return ap.parse_args()
@@ -125,6 +126,8 @@ def main():
         return bench_soft(args, ga, design, topo)
     if args.stage == "flt":
         return bench_flt(args, ga, design, topo)
+    if args.stage == "score":
+        return bench_score(args, ga, design, topo)
     ctx = ga.Context(topo)
     ctx.set_routes(topo["pin_acc_num"])
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
@@ -412,6 +415,55 @@ def bench_flt(args, ga, design, topo):
                    "parallelism": "single GPU", "kernel_breakdown": {"steps": 1, "classes": kernels}},
         "roofline": roofline, "cpu_baseline": cpu}), flush=True)
     ctx.close()
+
+
+def bench_score(args, ga, design, topo):
+    """Evaluator score (resources/evaluator.cpp:458-675) of the routes of one
+    GPU L-shape stage (untimed), resident on the device: a step is one
+    gr_score call over every design net, device time from its HIP events."""
+    import numpy as np
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"])
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+    color, nb = ctx.batch_plan(*geom)
+    sched = ga.execution_schedule(topo, color, nb)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
+    nets = design.nets()
+    mg = design.metal_grid()
+    cap = np.asarray(mg["capacity"], np.float64)
+    n = len(nets["pin_off"]) - 1
+    for i in range(args.warmup):
+        r = ctx.score(nets, mg["dir"], cap, mg["overflow_cost"])
+        print(f"[bench] score warmup {i}: {r['ms']:.3f} ms", file=sys.stderr, flush=True)
+    tot = 0.0
+    for i in range(args.steps):
+        r = ctx.score(nets, mg["dir"], cap, mg["overflow_cost"])
+        tot += r["ms"]
+        print(f"[bench] score step {i}: {r['ms']:.3f} ms", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as ora
+        c0 = time.perf_counter()
+        o = ora.score(int(topo["X"]), int(topo["Y"]), mg["layers"], nets, ctx.routes(), topo["pin_acc_num"], 12,
+                      mg["dir"], cap, mg["overflow_cost"], topo["x_edge_len"], topo["y_edge_len"],
+                      topo["unit_length_wire_cost"], topo["unit_via_cost"])
+        cs = time.perf_counter() - c0
+        same = bool(o["wirelength"] == r["wirelength"] and o["via"] == r["via"] and o["rejected"] == r["rejected"]
+                    and abs(o["overflow"] - r["overflow"]) <= 1e-10 * abs(o["overflow"]))
+        cpu = {"value": round(n / cs, 1), "unit": "nets/s", "cores": 1, "kind": "port",
+               "sample": f"whole workload: {n} design nets scored by the CPU oracle ({cs * 1e3:.1f} ms, "
+                         "route words already in host memory)", "costs_equal_to_gpu": same}
+    ms = tot / args.steps
+    print(json.dumps({
+        "metric": "scored nets/s (evaluator score of the L-shape stage routes)", "value": round(n / (ms / 1e3), 1),
+        "unit": "nets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64", "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} ({CONFIG_TEXT.get(args.config, '')}): evaluator score "
+                               f"of the L-shape stage routes, {n} design nets",
+                   "costs": {k: r[k] for k in ("wirelength", "via", "overflow", "total")}, "rejected": r["rejected"],
+                   "parallelism": "single GPU"},
+        "cpu_baseline": cpu}))
 
 
 def bench_zshape(args, ga, design, topo):
