@@ -1980,6 +1980,7 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
         if (X > 65535 || Y > 65535 || M > kScoreLayers)
             throw GrError(GR_E_ARG, "grid too large for the score's line packing and layer counters");
         const long long XY = (long long)X * Y, cells = XY * M;
+        if (cells >= (1LL << 31)) throw GrError(GR_E_ARG, "metal-layer grid too large for the score's cell keys");
         const int np = pin_off[nets], ne = extra_off[nets], ns = sub_off[nets];
         if (np < 0 || ne < 0 || (ne & 1) || ns < 0 || (np && !pins) || (ne && !extra) || (ns && !subs))
             throw GrError(GR_E_ARG, "invalid net line offsets");

@@ -142,14 +142,14 @@ __device__ __forceinline__ int pack_xy(int x, int y) { return x | y << 16; }
 }  // namespace
 
 // One wavefront (workgroup) per design net.  A net of at most kScoreLines
-// lines is decoded once into LDS, packed as (xl | yl << 16, xh | yh << 16,
-// zl | zh << 8, or -1 for a metal-1 pin without a line), and every via scans
-// the LDS copy; larger nets scan their lines in global memory.  Via counts
+// lines is decoded once into LDS, one int4 per line (xl | yl << 16,
+// xh | yh << 16, zl | zh << 8 or -1 for a metal-1 pin without a line), and
+// every via scans the LDS copy; larger nets scan their lines in global memory.  Via counts
 // per layer gather in LDS and leave once per workgroup, into stripe
 // blockIdx.x % kStatStripes of a.vias (kScoreLayers counters a stripe): one
 // global counter per layer would serialize millions of atomics.
 __global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
-    __shared__ int s_lo[kScoreLines], s_hi[kScoreLines], s_z[kScoreLines];
+    __shared__ int4 s_line[kScoreLines];  // (xl | yl << 16, xh | yh << 16, zl | zh << 8 or -1, 0)
     __shared__ unsigned long long s_vias[kScoreLayers];
     const int lane = threadIdx.x;
     const int X = a.X, Y = a.Y;
@@ -169,25 +169,40 @@ __global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
         if (n <= kScoreLines) {
             for_each_line(a, k, lane, kWave, [&](int ord, int q0, int q1) {
                 const Line ln = decode_line(q0, q1, X, Y);
-                s_lo[ord] = pack_xy(ln.xl, ln.yl);
-                s_hi[ord] = pack_xy(ln.xh, ln.yh);
-                s_z[ord] = ln.zl | ln.zh << 8;
+                s_line[ord] = make_int4(pack_xy(ln.xl, ln.yl), pack_xy(ln.xh, ln.yh), ln.zl | ln.zh << 8, 0);
                 return false;
             });
             // metal-1 pins print no line
             for (int j = a.pin_off[k] + lane; j < a.pin_off[k + 1]; j += kWave)
-                if (a.pins[j] >= X * Y) s_z[j - a.pin_off[k]] = -1;
+                if (a.pins[j] >= X * Y) s_line[j - a.pin_off[k]].z = -1;
             __syncthreads();
             for (int i = lane; i < n; i += kWave) {
-                if (s_z[i] < 0) continue;
-                const Line ln{s_lo[i] & 0xFFFF, s_lo[i] >> 16, s_z[i] & 0xFF,
-                              s_hi[i] & 0xFFFF, s_hi[i] >> 16, s_z[i] >> 8};
+                const int4 q = s_line[i];
+                if (q.z < 0) continue;
+                const Line ln{q.x & 0xFFFF, q.x >> 16, q.z & 0xFF, q.y & 0xFFFF, q.y >> 16, q.z >> 8};
                 count_line(a, ln, i, bad, s_vias, [&](int x, int y, int z, int ord) {
+                    // covers() on the packed lines: a wire of layer z through
+                    // (x, y), or an earlier via column at (x, y) spanning z
+                    const int xy = pack_xy(x, y);
+                    const bool hor = a.dir[z] == 0;
+#pragma unroll 4
                     for (int m = 0; m < n; ++m) {
-                        if (s_z[m] < 0) continue;
-                        const Line l2{s_lo[m] & 0xFFFF, s_lo[m] >> 16, s_z[m] & 0xFF,
-                                      s_hi[m] & 0xFFFF, s_hi[m] >> 16, s_z[m] >> 8};
-                        if (covers(a, l2, m, x, y, z, ord)) return true;
+                        const int4 w = s_line[m];
+                        if (w.z < 0) continue;
+                        const int zl = w.z & 0xFF, zh = w.z >> 8;
+                        if (zl == zh) {
+                            if (zl != z || z == 0) continue;
+                            if (hor) {
+                                if ((w.x >> 16) == y && (w.y >> 16) == y && (w.x & 0xFFFF) <= x &&
+                                    x <= (w.y & 0xFFFF) && (w.y & 0xFFFF) > (w.x & 0xFFFF))
+                                    return true;
+                            } else if ((w.x & 0xFFFF) == x && (w.y & 0xFFFF) == x && (w.x >> 16) <= y &&
+                                       y <= (w.y >> 16) && (w.y >> 16) > (w.x >> 16)) {
+                                return true;
+                            }
+                        } else if (m < ord && w.x == xy && w.y == xy && zl <= z && z < zh) {
+                            return true;
+                        }
                     }
                     return false;
                 });
