@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
-"""Regenerate the golden fixtures of tests/golden/ (run in the development
-container, where the reference tree is mounted; the GPU box never runs this).
+"""Regenerate the golden fixtures of tests/golden/ (run where the reference
+tree is mounted; GPU test runs only read the fixtures it wrote).
 
 Two instances, both with 10 metal layers because the reference reads only
 10-layer grids (src/input.hpp:292):
