@@ -1924,7 +1924,7 @@ int gr_batch_colors(gr_ctx* c, int limit, int sparsity, int* color, int* num_bat
                 closed_from.clear();
                 continue;
             }
-            check_device_error(c, "gr_batch_colors");
+            if (err) check_device_error(c, "gr_batch_colors");  // reads the word again, throws, clears it
             counts.assign(kMaxColors, 0);
             download(counts.data(), d_count, sizeof(int) * kMaxColors, st);
             while (!counts.empty() && !counts.back()) counts.pop_back();
