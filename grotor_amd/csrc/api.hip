@@ -172,7 +172,7 @@ struct gr_ctx {
     float *cxsum = nullptr, *cysum = nullptr;
     int per_wave[kTreeClasses] = {1, 1};  // L-shape DP nets per wavefront by tree size class (gr_lshape_tune)
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
-    int color_waves_per_cu = 16;
+    int color_waves_per_cu = 32;
     // kernel profiling (HIP events on the context stream)
     struct ProfRec {
         int kind;
