@@ -41,6 +41,12 @@ __device__ __forceinline__ Line decode_line(int p0, int p1, int X, int Y) {
     return Line{p0 / Y % X, p0 % Y, p0 / XY, p1 / Y % X, p1 % Y, p1 / XY};
 }
 
+// Route pairs of subnet id, at most what its route words can hold.
+__device__ __forceinline__ int route_pairs(const ScoreArgs& a, int id, const int* r) {
+    const int cap = (a.rs.pin_acc_num[id + 1] - a.rs.pin_acc_num[id]) * a.rs.per_pin;
+    return max(0, (min(r[0], cap) - 1) / 2);
+}
+
 // Calls f(ord, p0, p1) for the lines of net k from lane `lane` with stride
 // `stride`, in .out order; ord numbers the lines (pin entries not on metal 0
 // keep their number without producing a line).
@@ -61,7 +67,7 @@ __device__ __forceinline__ void for_each_line(const ScoreArgs& a, int k, int lan
     for (int s = a.sub_off[k]; s < a.sub_off[k + 1]; ++s) {
         const int id = a.subs[s];
         const int* r = a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin;
-        const int pairs = (r[0] - 1) / 2;
+        const int pairs = route_pairs(a, id, r);
         for (int j = lane; j < pairs; j += stride)
             if (f(ord + j, r[1 + 2 * j] + XY, r[2 + 2 * j] + XY)) return;
         ord += pairs;
@@ -160,8 +166,8 @@ __global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
         int n = (a.pin_off[k + 1] - a.pin_off[k]) + (a.extra_off[k + 1] - a.extra_off[k]) / 2;
         int subs_pairs = 0;
         for (int s = a.sub_off[k] + lane; s < a.sub_off[k + 1]; s += kWave) {
-            const int* r = a.rs.routes + (size_t)a.rs.pin_acc_num[a.subs[s]] * a.rs.per_pin;
-            subs_pairs += max(0, (r[0] - 1) / 2);
+            const int id = a.subs[s];
+            subs_pairs += route_pairs(a, id, a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin);
         }
         for (int off = kWave / 2; off; off >>= 1) subs_pairs += __shfl_xor(subs_pairs, off, kWave);
         n += subs_pairs;

@@ -139,7 +139,9 @@ int ora_score(int X, int Y, int M, int nets, const int *pin_off, const int *pins
         for (int j = extra_off[k]; j + 1 < extra_off[k + 1]; j += 2) take_line(&s, extra[j], extra[j + 1]);
         for (int t = sub_off[k]; t < sub_off[k + 1]; ++t) {
             const int *r = routes + (size_t)pin_acc_num[subs[t]] * per_pin;
-            for (int j = 1; j + 1 <= r[0] - 1; j += 2) take_line(&s, r[j] + (int)XY, r[j + 1] + (int)XY);
+            const int cap = (pin_acc_num[subs[t] + 1] - pin_acc_num[subs[t]]) * per_pin;
+            const int words = r[0] < cap ? r[0] : cap;
+            for (int j = 1; j + 1 <= words - 1; j += 2) take_line(&s, r[j] + (int)XY, r[j + 1] + (int)XY);
         }
         stack_vias(&s);
         bad_nets += s.bad;
