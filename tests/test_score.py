@@ -186,3 +186,52 @@ def test_gpu_score_matches_oracle(ga, ora, config, seed):
     assert got["overflow"] == pytest.approx(want["overflow"], rel=1e-10)
     assert got["ms"] > 0
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_score_hand_cases_match_oracle(ga, ora):
+    """gr_score on extra-route lines only (no subnets routed): stacking under
+    the net's own wire, repeated and border vias, metal-0 wires, moving vias
+    and wires against their layer's direction (rejected), 16-bit demand wrap,
+    and a net of 40,000 lines that exceeds the LDS line table and takes the
+    global-memory path; demand, costs and rejected nets against the oracle."""
+    d, t = topology(6, 0)
+    print("score hand cases", flush=True)
+    mg = d.metal_grid()
+    M, X, Y = mg["layers"], int(t["X"]), int(t["Y"])
+    dirs = [int(v) for v in mg["dir"]]
+    hz = next(z for z in range(1, M) if dirs[z] == 0)  # a horizontal routing layer
+    vz = next(z for z in range(1, M) if dirs[z] == 1)  # a vertical one
+    lo, hi = min(hz, vz), max(hz, vz)
+    cell = lambda z, x, y: (z * X + x) * Y + y  # noqa: E731
+    nets = [
+        [(1, 2, hz, 5, 2, hz), (3, 2, lo, 3, 2, hi)],      # via under the net's own wire on hz (when lo == hz)
+        [(3, 2, lo, 3, 2, hi), (3, 2, lo, 3, 2, hi)],      # the same via twice
+        [(0, 4, lo, 0, 4, hi), (X - 1, 4, lo, X - 1, 4, hi), (6, 0, lo, 6, 0, hi), (6, Y - 1, lo, 6, Y - 1, hi)],
+        [(2, 2, 0, 2, 2, hi)],                              # from metal 0
+        [(1, 1, 0, 4, 1, 0)],                               # wire on metal 0: rejected
+        [(1, 1, lo, 2, 2, hi)],                             # moving via: rejected
+        [(1, 1, vz, 4, 1, vz)],                             # along x on a vertical layer: rejected
+        [(7, 7, hz, 8, 7, hz)] * 40000,                     # 80,000 demand on one edge: wraps
+    ]
+    off, extra = [0], []
+    for lines in nets:
+        for (x0, y0, z0, x1, y1, z1) in lines:
+            extra += [cell(z0, x0, y0), cell(z1, x1, y1)]
+        off.append(len(extra))
+    z = np.zeros(len(nets) + 1, np.int32)
+    lines = dict(pin_off=z, pins=np.zeros(0, np.int32), extra_off=np.array(off, np.int32),
+                 extra=np.array(extra, np.int32), sub_off=z, subs=np.zeros(0, np.int32))
+    cap = np.asarray(mg["capacity"], np.float64)
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    got = ctx.score(lines, mg["dir"], cap, mg["overflow_cost"], demand=True)
+    want = ora.score(X, Y, M, lines, ctx.routes(), t["pin_acc_num"], 12, mg["dir"], cap, mg["overflow_cost"],
+                     t["x_edge_len"], t["y_edge_len"], t["unit_length_wire_cost"], t["unit_via_cost"])
+    assert want["rejected"] == 3
+    assert got["rejected"] == want["rejected"]
+    assert np.array_equal(got["demand"], want["demand"])
+    assert int(got["demand"][cell(hz, 7, 7)]) == 80000 % 65536
+    assert got["wirelength"] == want["wirelength"] and got["via"] == want["via"]
+    assert got["overflow"] == pytest.approx(want["overflow"], rel=1e-10)
+    ctx.close()
