@@ -121,6 +121,13 @@ void ora_zshape_reset(const ora_grid *g, int *map, const int *points, int count)
  * each net's DP optimum. */
 void ora_zshape_solve(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
                       const int *points, int count, int *raw, int *cost);
+/* Test probe of solve: also records where each edge's middle leg really
+ * ran; ideal = 1 traces the middle leg on that layer instead of the layer
+ * next to the last leg that the reference's traceback takes
+ * (src/routing/zshape.cuh:188,201,311-315); quirk[k] = 1 when the two
+ * differ on some edge of net k. */
+void ora_zshape_probe(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
+                      const int *points, int count, int *raw, int *cost, int ideal, int *quirk);
 /* convert_routes: raw words of net k (at raw_off[k]) into the route words
  * of net_ids[k] (at route_off, capacity route_cap); returns the last error
  * code (0 ok, 1 storage, 2 no route, 3 bad coordinate). */

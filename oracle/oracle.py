@@ -247,6 +247,26 @@ class Grid:
                                ctypes.c_int(count), _p(raw, ctypes.c_int), _p(cost, ctypes.c_int))
         return raw, cost[:count]
 
+    def zshape_probe(self, presum, vcost, records, count, raw_words, ideal):
+        """zshape_batch through the probe: (raw words, DP optimum, quirk flag
+        per net); ideal traces each middle leg on the layer it really ran on."""
+        n = max(self.X, self.Y)
+        plane = (self.L + 1) * n * n
+        zmap = np.zeros(6 * plane, np.int32)
+        zprev = np.zeros(16 * plane, np.int32)
+        raw = np.zeros(max(raw_words, 1), np.int32)
+        cost = np.zeros(max(count, 1), np.int32)
+        quirk = np.zeros(max(count, 1), np.int32)
+        records = _i32(records)
+        presum = np.ascontiguousarray(presum, np.float64)
+        vcost = np.ascontiguousarray(vcost, np.float32)
+        lib().ora_zshape_reset(ctypes.byref(self.c), _p(zmap, ctypes.c_int), _p(records, ctypes.c_int), ctypes.c_int(count))
+        lib().ora_zshape_probe(ctypes.byref(self.c), _p(presum, ctypes.c_double), _p(vcost, ctypes.c_float),
+                               _p(zmap, ctypes.c_int), _p(zprev, ctypes.c_int), _p(records, ctypes.c_int),
+                               ctypes.c_int(count), _p(raw, ctypes.c_int), _p(cost, ctypes.c_int),
+                               ctypes.c_int(1 if ideal else 0), _p(quirk, ctypes.c_int))
+        return raw, cost[:count], quirk[:count]
+
     def soft_costs(self, demand, slope=0.5, zero_slope=1.5, via_weight=1.0):
         """Smooth cost model: (wcost, vcost, dw_ds, dw_dz, dv_ds, dv_dz), FP64, IDX order."""
         demand = np.ascontiguousarray(demand, np.float32)

@@ -38,6 +38,13 @@ typedef struct {
     long long plane;     /* layers * n * n */
 } zctx;
 
+/* Test probe (ora_zshape_probe): when set, the last leg also records in
+ * prev planes 7-9 the layer cell where the middle leg really ran (the
+ * origin of the via-sweep chain at the second bend), and the traceback can
+ * take the middle leg from there instead of from the one recorded via step.
+ * Off in every stage run. */
+static int probe_origin = 0;
+
 static int zmin(int a, int b) { return a < b ? a : b; }
 static int zmax(int a, int b) { return a < b ? b : a; }
 
@@ -98,6 +105,13 @@ static void via_sweep(const zctx *z, int *map, int *prev, int x, int y) {
             pr[cur + P] = (int)last;
         }
     }
+}
+
+/* probe only: follow the via-sweep predecessors (plane 5) from a bend cell
+ * down to the layer cell whose value a leg wrote (its own predecessor) */
+static int sweep_origin(const int *pr5, int cell) {
+    while (pr5[cell] != cell) cell = pr5[cell];
+    return cell;
 }
 
 /* one child edge (from -> to) into child slot j of the node at `to`
@@ -191,6 +205,7 @@ static void zedge(const zctx *z, int *map, int *prev, int *prev2, int j, int fro
                         map[cur + j * P] = m4[last] + cost;
                         prev[cur + j * P] = p4[last];
                         prev2[cur + j * P] = prev[5 * P + last];
+                        if (probe_origin) prev[cur + (6 + j) * P] = sweep_origin(prev + 5 * P, prev[5 * P + last]);
                     }
                 }
             } else {
@@ -203,6 +218,7 @@ static void zedge(const zctx *z, int *map, int *prev, int *prev2, int j, int fro
                         map[cur + j * P] = m4[last] + cost;
                         prev[cur + j * P] = p4[last];
                         prev2[cur + j * P] = prev[5 * P + last];
+                        if (probe_origin) prev[cur + (6 + j) * P] = sweep_origin(prev + 5 * P, prev[5 * P + last]);
                     }
                 }
             }
@@ -274,7 +290,8 @@ static void zemit(int *routes, long long a, long long b) {
 
 /* zshape.cuh:290-399: root layer, then per node its via stack and the
  * leg pieces to each child, into raw words [start, length or -1] */
-static int ztrace(const zctx *z, int *map, int *prev, int *prev2, const int *pts, int node_cnt, int *routes) {
+static int ztrace(const zctx *z, int *map, int *prev, int *prev2, const int *pts, int node_cnt, int *routes,
+                  int ideal, int *quirk) {
     const long long P = z->plane, nn = (long long)z->n * z->n;
     const int L1 = z->layers, n = z->n;
     int min_total = ZINF;
@@ -310,7 +327,9 @@ static int ztrace(const zctx *z, int *map, int *prev, int *prev2, const int *pts
                 lasty = s;
             }
             if (lastx != x && lasty != y) {
-                const long long mid = prev2[(j + 1) * P + entry];
+                const long long rec_mid = prev2[(j + 1) * P + entry];
+                const long long mid = ideal ? prev[(7 + j) * P + entry] : rec_mid;
+                if (quirk && probe_origin && prev[(7 + j) * P + entry] != rec_mid) *quirk = 1;
                 const int mid_layer = (int)(mid / nn);
                 int midx = (int)(mid / n % n), midy = (int)(mid % n);
                 if (!((mid_layer & 1) ^ z->dirv)) {
@@ -361,8 +380,8 @@ void ora_zshape_reset(const ora_grid *g, int *map, const int *points, int count)
     }
 }
 
-void ora_zshape_solve(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
-                      const int *points, int count, int *raw, int *cost) {
+static void zsolve(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
+                   const int *points, int count, int *raw, int *cost, int ideal, int *quirk) {
     const int n = g->X > g->Y ? g->X : g->Y, layers = g->L + 1;
     zctx z = {g, presum, vcost, n, layers, g->DIR ^ 1, (long long)layers * n * n};
     int *prev2 = prev + 10 * z.plane;
@@ -381,9 +400,22 @@ void ora_zshape_solve(const ora_grid *g, const double *presum, const float *vcos
             }
             znode(&z, map, prev, pts + t);
         }
-        const int c = ztrace(&z, map, prev, prev2, pts, node_cnt, routes);
+        if (quirk) quirk[k] = 0;
+        const int c = ztrace(&z, map, prev, prev2, pts, node_cnt, routes, ideal, quirk ? quirk + k : NULL);
         if (cost) cost[k] = c;
     }
+}
+
+void ora_zshape_solve(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
+                      const int *points, int count, int *raw, int *cost) {
+    zsolve(g, presum, vcost, map, prev, points, count, raw, cost, 0, NULL);
+}
+
+void ora_zshape_probe(const ora_grid *g, const double *presum, const float *vcost, int *map, int *prev,
+                      const int *points, int count, int *raw, int *cost, int ideal, int *quirk) {
+    probe_origin = 1;
+    zsolve(g, presum, vcost, map, prev, points, count, raw, cost, ideal, quirk);
+    probe_origin = 0;
 }
 
 int ora_zshape_convert(const ora_grid *g, const int *raw, const int *raw_off, const int *net_ids, int count,
