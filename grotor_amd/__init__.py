@@ -560,6 +560,19 @@ class Context:
         return flags[:net_count]
 
 
+def batch_plan_host(X, Y, pt_off, pts, hs_off, hsegs, vs_off, vsegs, limit=1000000, sparsity=1):
+    """The same first-fit batches computed on the host (the reference's CPU
+    batching fallback): (color per net, batch count).  Needs no device."""
+    arrs = [_i32(a) for a in (pt_off, pts, hs_off, hsegs, vs_off, vsegs)]
+    n = len(arrs[0]) - 1
+    color = np.empty(max(n, 1), np.int32)
+    nb = ctypes.c_int()
+    _call("gr_batch_plan_host", ctypes.c_int(int(X)), ctypes.c_int(int(Y)), ctypes.c_int(n),
+          *[_ptr(a, ctypes.c_int) for a in arrs], ctypes.c_int(limit), ctypes.c_int(sparsity),
+          _ptr(color, ctypes.c_int), ctypes.byref(nb))
+    return color[:n], nb.value
+
+
 def batches_from_colors(color, nb):
     """Reference batch lists (vector<vector<int>>) from per-slot batch indices:
     batch b holds the priority slots with color b in priority order."""

@@ -110,6 +110,7 @@ struct gr_ctx {
     CostParams cp;
     // route storage
     int route_per_pin = kRoutePerPin;
+    int next_route_per_pin = kRoutePerPin;  // gr_ctx_set_route_stride, applied by gr_ctx_set_routes
     int net_count = 0;
     std::vector<int> pin_acc_num_h;
     int* pin_acc_num = nullptr;
@@ -194,6 +195,8 @@ struct gr_ctx {
     // while the stage topology, batch offsets and first stamp are unchanged
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the last synchronized batch-loop interval (L-shape, FLT or Z-shape run)
+    std::chrono::steady_clock::time_point last_t0{}, last_t1{};
     std::vector<int> graph_key;
     int stage_gen = 0;
 
@@ -386,6 +389,7 @@ constexpr size_t kLdsLimit = 64 * 1024;
 
 void ensure_gscratch(gr_ctx* c, size_t bytes) {
     if (bytes <= c->gscratch_bytes) return;
+    c->drop_graph();  // a captured batch loop holds the old scratch address
     c->mem.release(c->gscratch);
     c->gscratch = c->mem.alloc<double>((bytes + 7) / 8);
     c->gscratch_bytes = bytes;
@@ -627,6 +631,8 @@ int gr_ctx_destroy(gr_ctx* ctx) {
 int gr_ctx_set_routes(gr_ctx* c, int net_count, const int* pin_acc_num) {
     return guarded([&] {
         if (!c || net_count < 0 || !pin_acc_num) throw GrError(GR_E_ARG, "invalid route storage arguments");
+        c->drop_graph();  // a captured batch loop holds the old route storage addresses
+        c->route_per_pin = c->next_route_per_pin;
         c->mem.release(c->routes);
         c->mem.release(c->pin_acc_num);
         c->mem.release(c->of_flags);
@@ -645,7 +651,7 @@ int gr_ctx_set_route_stride(gr_ctx* c, int words_per_pin) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (words_per_pin < 4) throw GrError(GR_E_ARG, "route words per pin must be at least 4");
-        c->route_per_pin = words_per_pin;
+        c->next_route_per_pin = words_per_pin;  // the current storage keeps its stride
     });
 }
 
@@ -883,6 +889,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         }
         reserve_dp_scratch(c, plan);
         if (mode == 2 && c->dp_queue_cap < nbatch) {
+            c->drop_graph();
             c->mem.release(c->dp_queue);
             c->dp_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kDpQueues * kDpQueueStride);
             c->dp_queue_cap = nbatch;
@@ -938,8 +945,26 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         else enqueue();
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();
+        c->last_t0 = t0, c->last_t1 = t1;
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_lshape_run");
+    });
+}
+
+int gr_ctx_synchronize(gr_ctx* c) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    });
+}
+
+int gr_ctx_last_interval(gr_ctx* c, long long* start_ns, long long* end_ns) {
+    return guarded([&] {
+        if (!c || !start_ns || !end_ns) throw GrError(GR_E_ARG, "null argument");
+        using std::chrono::duration_cast;
+        using std::chrono::nanoseconds;
+        *start_ns = (long long)duration_cast<nanoseconds>(c->last_t0.time_since_epoch()).count();
+        *end_ns = (long long)duration_cast<nanoseconds>(c->last_t1.time_since_epoch()).count();
     });
 }
 
@@ -1224,6 +1249,7 @@ int gr_zshape_run(gr_ctx* c, int first_stamp, double* elapsed_ms) {
         }
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();
+        c->last_t0 = t0, c->last_t1 = t1;
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_zshape_run");
     });
@@ -1496,6 +1522,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed
         }
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();
+        c->last_t0 = t0, c->last_t1 = t1;
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_flt_run");
     });
@@ -1977,7 +2004,8 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
             throw GrError(GR_E_ARG, "null argument");
         if (!c->routes) throw GrError(GR_E_STATE, "routes not allocated (gr_ctx_set_routes)");
         const int X = c->g.X, Y = c->g.Y, M = c->g.L + 1;
-        if (X > 65535 || Y > 65535 || M > kScoreLayers)
+        // lines pack x | y << 16 into a signed int: coordinates must stay below 32768
+        if (X > 32768 || Y > 32768 || M > kScoreLayers)
             throw GrError(GR_E_ARG, "grid too large for the score's line packing and layer counters");
         const long long XY = (long long)X * Y, cells = XY * M;
         if (cells >= (1LL << 31)) throw GrError(GR_E_ARG, "metal-layer grid too large for the score's cell keys");

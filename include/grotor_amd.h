@@ -128,6 +128,15 @@ int gr_batch_plan(gr_ctx *ctx, int n, const int *pt_off, const int *pts,
                   const int *hs_off, const int *hsegs, const int *vs_off, const int *vsegs,
                   int limit, int sparsity, int *color, int *num_batches);
 
+/* The same batches computed on the host, without a device or context: the
+ * reference's CPU batching fallback (generate_batches_rsmt,
+ * src/batching/generate_batches_cpu.hpp:7-73, timed by
+ * generate_batches_cpu_timed, src/batching/generate_batches.hpp:14-25) for
+ * an X x Y grid.  Same arguments, errors and result as gr_batch_plan. */
+int gr_batch_plan_host(int X, int Y, int n, const int *pt_off, const int *pts,
+                       const int *hs_off, const int *hsegs, const int *vs_off, const int *vsegs,
+                       int limit, int sparsity, int *color, int *num_batches);
+
 /* Validate and upload the batching geometry of n nets (priority order) into
  * the context, where it stays until the next call: the reference's GPU
  * batching uploads it before its batch interval starts
@@ -173,6 +182,18 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
  * include/router_state.hpp:238-265). */
 int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,
                   int mode, double *elapsed_ms);
+
+/* Wait for all device work (the cudaDeviceSynchronize that opens the
+ * reference's timed intervals, src/batching/generate_batches.hpp:15,
+ * include/router_state.hpp:241-249). */
+int gr_ctx_synchronize(gr_ctx *ctx);
+
+/* Start and end of the last synchronized batch-loop interval (gr_lshape_run,
+ * gr_flt_run or gr_zshape_run) as std::chrono::steady_clock nanoseconds since
+ * its epoch, so a caller can print them against its own clock origin as the
+ * reference's DAG_TIMING line does (start_us / end_us,
+ * include/router_state.hpp:252-264).  Zero before the first run. */
+int gr_ctx_last_interval(gr_ctx *ctx, long long *start_ns, long long *end_ns);
 
 /* Nets per wavefront of the L-shape DP launches (1, 2, 4 or 8; each net gets
  * 64 / n lanes and its own tables) for the two tree size classes of a batch:

@@ -367,3 +367,46 @@ def test_gpu_refresh_wide_arguments_match_oracle(ga, ora, slope, zero_slope):
         assert np.array_equal(_bits(v), _bits(vo)), s
         assert np.array_equal(_bits(p), _bits(po)), s
     ctx.close()
+
+
+def test_graph_replay_after_storage_changes(ga, ora):
+    """A captured batch loop is dropped when the buffers it holds are
+    reallocated: replays after gr_ctx_set_routes, after a single-batch DP
+    over a wider range (global scratch growth) and after a stride change
+    (which waits for the next gr_ctx_set_routes) equal the oracle."""
+    _, t = topology(6, 5)
+    print("graph replay after storage changes config=6 seed=5", flush=True)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ora.batches(t["X"], t["Y"], *geom)
+    sched = ga.execution_schedule(t, color, nb)
+    routes_o, demand_o, ts_o, wl_o, vias_o = ora.Grid(t).lshape_stage(sched, t["pin_acc_num"], threads=8)
+
+    def check(ctx, tag):
+        assert np.array_equal(ctx.routes(), routes_o), tag
+        assert np.array_equal(_bits(ctx.demand()), _bits(demand_o)), tag
+        assert np.array_equal(ctx.timestamps(), ts_o), tag
+        assert ctx.stats() == (wl_o, vias_o), tag
+
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    for mode in (1, 2):
+        ctx.reset()
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
+        check(ctx, f"first mode {mode}")
+        ctx.set_routes(t["pin_acc_num"])  # new route storage
+        ctx.reset()
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
+        check(ctx, f"after set_routes mode {mode}")
+        ctx.reset()
+        ctx.refresh()
+        ctx.lshape_batch(0, int(sched["batch_off"][-1]), 1)  # every net at once: scratch may grow
+        ctx.reset()
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
+        check(ctx, f"after wide lshape_batch mode {mode}")
+    # a stride change applies to the next route storage only
+    ga._call("gr_ctx_set_route_stride", ctx._h, ga.ctypes.c_int(256))
+    ctx.reset()
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+    check(ctx, "stride pending")
+    ctx.close()
