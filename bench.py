@@ -1,25 +1,33 @@
 #!/usr/bin/env python3
 """Routing hot-path benchmark: routed nets/s (batching + DAG) on one MI355X.
 
-Workload (default): configuration 2 -- 512x512 grid, 9 metal layers (8 routing
-layers), capacities U{6..16} with 5% of routing cells blocked, 300k nets
-(60% 2-pin, 30% 3-5 pins, 8% clustered 20-120-pin nets), seed 0.  The
-front end (pin selection, trees, >12-pin split) runs once outside the timed
-region; a step is one L-shape stage over all routed subnets: GPU batching
-(exact first-fit greedy) followed by the batch loop (cost refresh, DP,
-commit).  As in the reference's metric (docs/amd.md:64-65,
-script/experiments/run_amd.py:98-108) a step's time is the batching call's
-wall time plus the synchronized DAG interval; host-side regrouping of the
-nets between the two is outside both, as it is in the reference.  The
-batching geometry is uploaded before the batching call, as the reference's
-GPU batching uploads it before its interval starts
-(src/batching/hiprt/grid_chunk.cuh:373-428, 519-524); its upload time is
-reported beside the metric.
+Workload (default): configuration 3 -- 1024x1024 grid, 10 metal layers (9
+routing layers), 1M synthetic nets with heavy-tailed pin counts (Zipf,
+s = 2.2, up to 200 pins), seed 0.  The front end (pin selection, trees,
+>12-pin split) runs once outside the timed region.  A step is the
+reference's rip-up pipeline (src/pipeline.hpp:14-157) over the instance:
+the L-shape stage over every routed subnet, then rip-up of the nets that
+cross more than 10 overflowed edges, and the FLT stage over them (their
+Steiner-topology routing DAGs).  As in the reference's metric
+(docs/amd.md:64-65, script/experiments/run_amd.py:98-108) a stage's time is
+its batching call's wall time plus its synchronized DAG interval; the FLT
+interval includes the upload of the stage's DAG records, as the reference's
+does (src/routing/flt.cuh:131-137).  Work between the intervals (host
+regrouping, rip-up classification, DAG construction, plan building) is
+outside both, as it is in the reference.  `value` is the routed nets of both
+stages per second of their summed intervals, in the persistent form.
+
+The same steps run in two forms on the same context, with identical routes
+and evaluator cost checked: plain (one launch per kernel per batch) and
+persistent (the batch loops captured as HIP graphs, the L-shape DP as one
+persistent launch per batch draining a device work queue).
 
 Prints one JSON line.  `roofline` reports the kernel class with the largest
-device time in the timed region, timed with HIP events on the context's
+device time in the persistent form, timed with HIP events on the context's
 stream; `cpu_baseline` times the CPU oracle (the restated reference path) on
-the same instance and checks that its routes equal the GPU's.
+the same instance, all host threads and one thread, and checks that its
+routes equal the GPU's; `config.soft_dp` carries the soft-DP forward + adjoint
+times on configuration 4 (512x512x9, 300k nets) for tau in {0.1, 1, 10}.
 """
 import argparse
 import json
@@ -38,6 +46,7 @@ CONFIG_TEXT = {
     1: "128x128x5, 20k nets",
     2: "512x512x9, 300k nets",
     3: "1024x1024x10, 1M nets",
+    4: "512x512x9, 300k nets (soft DP)",
     5: "2048x2048x10, 4M nets",
     6: "64x64x5, 2k nets",
 }
@@ -46,9 +55,10 @@ CONFIG_TEXT = {
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--config", type=int, default=None,
+                    help="workload configuration (default 3 for the pipeline, 2 for single-stage runs)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--net-scale", type=float, default=1.0)
     ap.add_argument("--mode", type=int, default=2, choices=(0, 1, 2),
@@ -60,10 +70,14 @@ def parse():
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 8))
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the nets-per-wavefront autotuner (one net / DAG per wavefront), e.g. for counter runs")
-    ap.add_argument("--stage", choices=("lshape", "zshape", "soft", "flt", "score"), default="lshape",
-                    help="lshape: the headline L-shape stage; zshape: the Z-shape stage; soft: soft-DP fwd+bwd; "
+    ap.add_argument("--stage", choices=("pipeline", "lshape", "zshape", "soft", "flt", "score"), default="pipeline",
+                    help="pipeline: L-shape stage + FLT rip-up stage, plain and persistent forms (the headline); "
+                         "lshape: the L-shape stage alone; zshape: the Z-shape stage; soft: soft-DP fwd+bwd; "
                          "flt: the FLT rip-up stage after the L-shape stage; score: the evaluator score of the "
                          "L-shape stage's routes")
+    ap.add_argument("--no-soft", action="store_true", help="pipeline: skip the configuration-4 soft-DP timings")
+    ap.add_argument("--cpu-sample", type=float, default=0.2,
+                    help="pipeline: fraction of the L-shape stage's nets (leading batches) in the one-thread CPU run")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (reference for HBM counters) and exit")
     return ap.parse_args()
@@ -102,6 +116,8 @@ def measured_traffic(kernel, workload):
 
 def main():
     args = parse()
+    if args.config is None:
+        args.config = 3 if args.stage == "pipeline" else 2
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
@@ -120,6 +136,8 @@ def main():
     print(f"[bench] instance config={args.config} subnets={topo['nets']} nodes={int(topo['node_off'][-1])} "
           f"prepare={prep_s:.2f}s device={ga.device_name()}", file=sys.stderr, flush=True)
 
+    if args.stage == "pipeline":
+        return bench_pipeline(args, ga, design, topo)
     if args.stage == "zshape":
         return bench_zshape(args, ga, design, topo)
     if args.stage == "soft":
@@ -277,6 +295,270 @@ def main():
     ctx.close()
 
 
+def host_info(threads):
+    """Host CPU model, sockets and thread settings for the CPU baseline."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "threads_used": threads, "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def bench_pipeline(args, ga, design, topo):
+    """The rip-up pipeline (L-shape stage, rip-up, FLT stage) in the plain
+    and persistent forms; see the module docstring."""
+    import numpy as np
+    nsub = len(topo["pin_acc_num"]) - 1
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+    ctx = ga.Context(topo)
+    ctx.set_routes(topo["pin_acc_num"])
+    forms = {"plain": {"lshape_mode": 0, "flt_mode": 0}, "persistent": {"lshape_mode": 2, "flt_mode": 1}}
+
+    # autotuners, once, before any step: nets per wavefront of the L-shape DP
+    # and DAGs per wavefront of the FLT DP
+    ctx.batch_geometry(*geom)
+    tcolor, tnb = ctx.batch_colors()
+    tsched = ga.execution_schedule(topo, tcolor, tnb)
+    ctx.lshape_upload(tsched["net_ids"], tsched["node_off"], tsched["nodes"], tsched["parents"])
+    l_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.lshape_tune(tsched["batch_off"])
+    ctx.reset()
+    ctx.lshape_run(tsched["batch_off"], first_stamp=1, mode=0)
+    flags = ctx.ripup(nsub)
+    _, xs, ys = ctx.congestion_view()
+    fs = design.flt_stage(flags, xs, ys, threads=args.threads)
+    fgeom = (fs["pt_off"], fs["pts"], fs["hs_off"], fs["hsegs"], fs["vs_off"], fs["vsegs"])
+    ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
+    ctx.batch_geometry(*fgeom)
+    fcolor, fnb = ctx.batch_colors(limit=300000)
+    ctx.flt_upload(design.flt_plan(fcolor, fnb))
+    f_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.flt_tune()
+    print(f"[bench] autotuners: L-shape nets per wavefront {l_tune[0]}, FLT DAGs per wavefront {f_tune[0]}; "
+          f"{fs['nets']} rerouted nets, {fs['nodes']} DAG nodes, {fs['edges']} DAG edges",
+          file=sys.stderr, flush=True)
+
+    def step(form, profile=False):
+        """One pipeline pass; returns its intervals (ms) and results."""
+        f = forms[form]
+        ctx.reset()
+        if profile:
+            ctx.profile_select(None)
+            ctx.profile_begin()
+        ctx.batch_geometry(*geom)
+        t0 = time.perf_counter()
+        color, nb = ctx.batch_colors()
+        lb = (time.perf_counter() - t0) * 1e3
+        sched = ga.execution_schedule(topo, color, nb)
+        ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+        ld = ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=f["lshape_mode"])
+        flg = ctx.ripup(nsub)
+        _, sx, sy = ctx.congestion_view()
+        st = design.flt_stage(flg, sx, sy, threads=args.threads)
+        assert st["nets"] == fs["nets"], "rip-up set changed between steps"
+        ctx.set_routes_phase2(st["pin_off2"], st["pins2"])
+        ctx.batch_geometry(*fgeom)
+        t0 = time.perf_counter()
+        fc, fb_n = ctx.batch_colors(limit=300000)
+        fb = (time.perf_counter() - t0) * 1e3
+        plan = design.flt_plan(fc, fb_n)
+        t0 = time.perf_counter()
+        ctx.flt_upload(plan)
+        fu = (time.perf_counter() - t0) * 1e3
+        fd = ctx.flt_run(first_stamp=nb + 1, remove_phase_one=True, mode=f["flt_mode"])
+        prof = ctx.profile_end() if profile else None
+        return dict(lb=lb, ld=ld, fb=fb, fu=fu, fd=fd, nb=nb, fnb=len(plan["batch_off"]) - 1, color=color, fc=fc,
+                    prof=prof)
+
+    results = {}
+    for form in ("plain", "persistent"):
+        for i in range(args.warmup):
+            r = step(form)
+            print(f"[bench] {form} warmup {i}: L-shape {r['lb']:.2f} + {r['ld']:.2f} ms, FLT {r['fb']:.2f} + "
+                  f"{r['fu']:.2f} + {r['fd']:.2f} ms", file=sys.stderr, flush=True)
+        acc = dict(lb=0.0, ld=0.0, fb=0.0, fu=0.0, fd=0.0)
+        for i in range(args.steps):
+            r = step(form)
+            for k in acc:
+                acc[k] += r[k]
+            print(f"[bench] {form} step {i}: L-shape {r['lb']:.2f} + {r['ld']:.2f} ms, FLT {r['fb']:.2f} + "
+                  f"{r['fu']:.2f} + {r['fd']:.2f} ms", file=sys.stderr, flush=True)
+        per = {k: v / args.steps for k, v in acc.items()}
+        wl, vias = ctx.stats()
+        nets = design.nets()
+        mg = design.metal_grid()
+        sc = ctx.score(nets, mg["dir"], np.asarray(mg["capacity"], np.float64), mg["overflow_cost"])
+        results[form] = dict(per=per, nb=r["nb"], fnb=r["fnb"], routes=ctx.routes(), routes2=ctx.routes2(),
+                             color=r["color"], fc=r["fc"], wl=wl, vias=vias,
+                             score={k: sc[k] for k in ("wirelength", "via", "overflow", "total")},
+                             rejected=sc["rejected"])
+
+    # per-kernel device time of the persistent form: one more step whose
+    # launches are issued directly (a replayed graph carries no events)
+    prof = step("persistent", profile=True)["prof"]
+    ln, fn = topo["nets"], fs["nets"]
+
+    def rate(per):
+        lms, fms = per["lb"] + per["ld"], per["fb"] + per["fu"] + per["fd"]
+        return dict(ms=lms + fms, lshape_ms=lms, flt_ms=fms, value=(ln + fn) / ((lms + fms) / 1e3),
+                    lshape_nets_per_s=ln / (lms / 1e3), flt_nets_per_s=fn / (fms / 1e3) if fn else None,
+                    batching_ms={"lshape": round(per["lb"], 4), "flt": round(per["fb"], 4)},
+                    dag_ms={"lshape": round(per["ld"], 4), "flt_upload": round(per["fu"], 4),
+                            "flt": round(per["fd"], 4)})
+
+    pl, pe = rate(results["plain"]["per"]), rate(results["persistent"]["per"])
+    a, b = results["plain"], results["persistent"]
+    identical = bool(np.array_equal(a["routes"], b["routes"]) and np.array_equal(a["routes2"], b["routes2"])
+                     and a["score"] == b["score"] and a["wl"] == b["wl"] and a["vias"] == b["vias"])
+    dom = max(range(len(prof["ms"])), key=lambda k: prof["ms"][k])
+    dom_ms, dom_launches = prof["ms"][dom], prof["launches"][dom]
+    achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": measured_traffic(KIND_NAMES[dom], f"config{args.config}-pipeline"),
+                "events": "one extra persistent-form step after the timed steps, its launches issued directly",
+                "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
+                "launches": int(dom_launches), "alg_bytes_per_launch": round(prof["bytes"][dom] / max(dom_launches, 1), 1)}
+    kernels = {KIND_NAMES[k]: {"ms": round(prof["ms"][k], 4), "launches": int(prof["launches"][k]),
+                               "GBs": round(prof["bytes"][k] / (prof["ms"][k] / 1e3) / 1e9, 2) if prof["ms"][k] else 0}
+               for k in range(len(prof["ms"])) if prof["launches"][k]}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_pipeline_baseline(args, ga, design, topo, fs, results["persistent"])
+    soft = None if args.no_soft else soft_timings(args, ga)
+    line = {
+        "metric": "routed nets/s (batching+DAG) at equal evaluator cost",
+        "value": round(pe["value"], 1),
+        "unit": "nets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(pe["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"configuration {args.config} rip-up pipeline ({CONFIG_TEXT.get(args.config, '')}): "
+                               f"L-shape stage over {ln} routed subnets, then the FLT stage over the {fn} rerouted "
+                               "nets; GPU batching + DAG loop per stage, persistent form",
+                   "routed_nets": {"lshape": ln, "flt": fn}, "batches": {"lshape": b["nb"], "flt": b["fnb"]},
+                   "dag_nodes_flt": fs["nodes"], "dag_edges_flt": fs["edges"],
+                   "persistent": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in pe.items()},
+                   "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in pl.items()},
+                   "persistent_over_plain": {"pipeline": round(pl["ms"] / pe["ms"], 4),
+                                             "lshape": round(pl["lshape_ms"] / pe["lshape_ms"], 4),
+                                             "flt": round(pl["flt_ms"] / pe["flt_ms"], 4)},
+                   "forms_identical": identical,
+                   "evaluator_cost": b["score"], "evaluator_rejected_nets": b["rejected"],
+                   "total_wirelength": b["wl"], "total_vias": b["vias"],
+                   "nets_per_wavefront": {"lshape": {"classes": ["<=16 nodes", "larger"], "chosen": l_tune[0],
+                                                     "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in l_tune[1]]},
+                                          "flt": {"classes": ["batches < 512 DAGs", ">= 512 DAGs"], "chosen": f_tune[0],
+                                                  "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in f_tune[1]]}},
+                   "soft_dp": soft, "parallelism": "single GPU",
+                   "kernel_breakdown": {"steps": 1, "form": "persistent", "classes": kernels}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+def cpu_pipeline_baseline(args, ga, design, topo, fs, gpu):
+    """The oracle on the same instance: both stages with the DAG loop on all
+    threads (batching is sequential), routes checked against the GPU's, and
+    the L-shape DAG loop on one thread over the leading batches."""
+    import numpy as np
+    from oracle import oracle as ora
+    geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
+    fgeom = (fs["pt_off"], fs["pts"], fs["hs_off"], fs["hsegs"], fs["vs_off"], fs["vsegs"])
+    g = ora.Grid(topo)
+    c0 = time.perf_counter()
+    color, nb = ora.batches(topo["X"], topo["Y"], *geom)
+    lb = time.perf_counter() - c0
+    sched = ga.execution_schedule(topo, color, nb)
+    c0 = time.perf_counter()
+    routes, demand, ts, wl, vias = g.lshape_stage(sched, topo["pin_acc_num"], threads=args.threads)
+    ld = time.perf_counter() - c0
+    c0 = time.perf_counter()
+    fcol, fnb = ora.batches(topo["X"], topo["Y"], *fgeom, limit=300000)
+    fb = time.perf_counter() - c0
+    plan = design.flt_plan(fcol, fnb)
+    c0 = time.perf_counter()
+    r2, _, _, wl2, vias2, rc, _ = g.flt_stage(plan, topo["pin_acc_num"], fs["pin_off2"], fs["pins2"], demand, ts,
+                                              routes, nb + 1, wl=wl, vias=vias, threads=args.threads)
+    fd = time.perf_counter() - c0
+    same = bool(rc == 0 and np.array_equal(color, gpu["color"]) and np.array_equal(fcol, gpu["fc"])
+                and np.array_equal(routes, gpu["routes"]) and np.array_equal(r2, gpu["routes2"])
+                and wl2 == gpu["wl"] and vias2 == gpu["vias"])
+    # one thread: the L-shape DAG loop over the leading batches holding about
+    # args.cpu_sample of the stage's nets
+    bo = sched["batch_off"]
+    k = int(np.searchsorted(bo, args.cpu_sample * bo[-1])) if args.cpu_sample > 0 else 1
+    k = max(1, min(len(bo) - 1, k))
+    sample = dict(sched)
+    sample["batch_off"] = bo[:k + 1]
+    c0 = time.perf_counter()
+    g.lshape_stage(sample, topo["pin_acc_num"], threads=1)
+    one = time.perf_counter() - c0
+    ln, fn = topo["nets"], fs["nets"]
+    total = lb + ld + fb + fd
+    print(f"[bench] cpu: L-shape batching {lb * 1e3:.1f} ms + DAG {ld * 1e3:.1f} ms, FLT batching {fb * 1e3:.1f} ms "
+          f"+ DAG {fd * 1e3:.1f} ms; one thread {int(bo[k])} nets in {one:.2f} s; identical={same}",
+          file=sys.stderr, flush=True)
+    return {"value": round((ln + fn) / total, 1), "unit": "nets/s", "cores": args.threads, "kind": "port",
+            "sample": f"whole workload: both stages ({ln} + {fn} nets); CPU oracle batching on 1 thread, DAG loops "
+                      f"with the nets of a batch over {args.threads} OpenMP threads",
+            "lshape": {"batching_ms": round(lb * 1e3, 3), "dag_ms": round(ld * 1e3, 3),
+                       "nets_per_s": round(ln / (lb + ld), 1)},
+            "flt": {"batching_ms": round(fb * 1e3, 3), "dag_ms": round(fd * 1e3, 3),
+                    "nets_per_s": round(fn / (fb + fd), 1) if fn else None},
+            "one_thread": {"sample": f"L-shape DAG loop over the first {k} of {len(bo) - 1} batches "
+                                     f"({int(bo[k])} nets)", "dag_ms": round(one * 1e3, 3),
+                           "nets_per_s": round(int(bo[k]) / one, 1)},
+            "host": host_info(args.threads), "routes_identical_to_gpu": same}
+
+
+def soft_timings(args, ga):
+    """Soft-DP forward + adjoint over configuration 4 (the 512x512x9,
+    300k-net instance) at the demand its L-shape stage leaves, FP32 tables,
+    tau in {0.1, 1, 10}; ms per pass over every routed subnet."""
+    d = ga.Design.generate(4, seed=args.seed)
+    d.prepare(args.threads)
+    t = d.topology()
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ctx.batch_plan(*geom)
+    sched = ga.execution_schedule(t, color, nb)
+    ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
+    ctx.lshape_upload(t["order"], t["node_off"], t["nodes"], t["parents"])
+    nets = t["nets"]
+    out = {"workload": f"configuration 4 ({nets} routed subnets, 512x512x9)", "precision": "f32 tables"}
+    for tau in (0.1, 1.0, 10.0):
+        ctx.soft_forward(0, nets, tau=tau, precision=32)
+        ctx.soft_backward()
+        f = b = 0.0
+        reps = 3
+        for _ in range(reps):
+            _, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
+            _, bm = ctx.soft_backward()
+            f += fm
+            b += bm
+        out[f"tau_{tau:g}"] = {"fwd_ms": round(f / reps, 3), "bwd_ms": round(b / reps, 3),
+                               "fwd_bwd_ms": round((f + b) / reps, 3)}
+        print(f"[bench] soft tau={tau}: fwd {f / reps:.2f} ms, bwd {b / reps:.2f} ms", file=sys.stderr, flush=True)
+    ctx.close()
+    return out
+
+
 def bench_flt(args, ga, design, topo):
     """FLT rip-up stage (src/routing/flt.cuh:75-182): after the L-shape
     stage, rip-up and the congestion view, GPU batching of the rerouted
@@ -320,6 +602,7 @@ def bench_flt(args, ga, design, topo):
 
     def step(profile_kinds=False):
         phase_one()
+        ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])  # a reset releases it
         ctx.batch_geometry(*fgeom)
         if profile_kinds is not False:
             ctx.profile_select(profile_kinds)

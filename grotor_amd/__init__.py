@@ -360,11 +360,12 @@ class Context:
               _ptr(np.ascontiguousarray(plan["edge_off"], np.int64), ctypes.c_longlong),
               _ptr(_i32(plan["edges"]), ctypes.c_int))
 
-    def flt_run(self, first_stamp=1, remove_phase_one=True):
-        """Run the uploaded FLT batches; returns the DAG interval in ms."""
+    def flt_run(self, first_stamp=1, remove_phase_one=True, mode=0):
+        """Run the uploaded FLT batches (mode 0 launches, 1 persistent form);
+        returns the DAG interval in ms."""
         ms = ctypes.c_double()
         _call("gr_flt_run", self._h, ctypes.c_int(first_stamp), ctypes.c_int(1 if remove_phase_one else 0),
-              ctypes.byref(ms))
+              ctypes.c_int(mode), ctypes.byref(ms))
         return ms.value
 
     def set_flt_per_wave(self, per_wave):

@@ -165,7 +165,12 @@ struct gr_ctx {
     FltArgs f{};
     int f_batches = 0;
     std::vector<int> f_batch_off;
+    std::vector<int> f_ids_h;  // subnet ids of the uploaded FLT stage, execution order
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
+    std::vector<int2> f_batch_max;  // per batch: child slots and nodes of its largest DAG
+    int* f_queue = nullptr;         // persistent FLT DP: kFltQueues counters per batch
+    int f_queue_cap = 0;
+    int lds_limit = 65536;          // LDS bytes a workgroup may use
     std::vector<double> f_edge_bytes, f_route_bytes;  // algorithmic bytes per batch launch
     int f_per_wave[2] = {1, 1};  // FLT DAGs per wavefront: batches of fewer / at least kFltWideBatch DAGs
     // congestion view (2D)
@@ -199,13 +204,29 @@ struct gr_ctx {
     std::chrono::steady_clock::time_point last_t0{}, last_t1{};
     std::vector<int> graph_key;
     int stage_gen = 0;
+    // FLT batch loop captured as a HIP graph (gr_flt_run mode 1), reused
+    // while the uploaded stage, its storage, first stamp and settings stay
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fgraph_exec = nullptr;
+    std::vector<int> fgraph_key;
+    int flt_gen = 0;
 
+    void drop_fgraph() {
+        if (fgraph_exec) (void)hipGraphExecDestroy(fgraph_exec);
+        if (fgraph) (void)hipGraphDestroy(fgraph);
+        fgraph_exec = nullptr;
+        fgraph = nullptr;
+        fgraph_key.clear();
+    }
+    // both captured loops: every buffer the L-shape loop holds is one the
+    // FLT loop may hold too
     void drop_graph() {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         graph_exec = nullptr;
         graph = nullptr;
         graph_key.clear();
+        drop_fgraph();
     }
 
     ~gr_ctx() {
@@ -379,9 +400,12 @@ BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots)
     for (int s = begin; s < begin + count; ++s) slots[at[tree_class(c->node_off_h[s + 1] - c->node_off_h[s])]++] = s;
     // tallest trees first inside a class: the nets with the longest level
     // chains start first and the launch's tail is short
+    // (slots enter in ascending order, so ordering ties by slot is the
+    // stable order, without the temporary buffer of a stable sort)
     for (int k = 0, lo = begin; k < kTreeClasses; lo += b.count[k++])
-        std::stable_sort(slots.begin() + lo, slots.begin() + lo + b.count[k],
-                         [&](int u, int v) { return c->height_h[u] > c->height_h[v]; });
+        std::sort(slots.begin() + lo, slots.begin() + lo + b.count[k], [&](int u, int v) {
+            return c->height_h[u] != c->height_h[v] ? c->height_h[u] > c->height_h[v] : u < v;
+        });
     return b;
 }
 
@@ -406,8 +430,15 @@ int per_wave_for(gr_ctx* c, int k, int nmax) {
     return p;
 }
 
+// Global DP tables of the trees that do not fit in LDS: the context's
+// grow-only scratch, or a caller-given buffer of `bytes`.
+struct DpTables {
+    double* base = nullptr;
+    size_t bytes = 0;
+};
+
 void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_override = 0,
-                         hipStream_t stream = nullptr) {
+                         hipStream_t stream = nullptr, DpTables tables = {}) {
     if (!b.count[k]) return;
     if (!stream) stream = c->stream;
     LshapeArgs a;
@@ -420,7 +451,8 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
     a.parents = c->parents;
     a.lvl = c->lvl;
     a.link = c->link;
-    a.gscratch = c->gscratch;
+    if (!tables.base) tables = {c->gscratch, c->gscratch_bytes};
+    a.gscratch = tables.base;
     a.error = c->error;
     a.queue = nullptr;
     a.cus = c->cus;
@@ -437,7 +469,7 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
         timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, stream); }, stream);
     } else {
         const size_t bytes = lshape_scratch_bytes(c->g.L, a.nmax);
-        if (((bytes + 7) / 8) * 8 * size_t(a.count) > c->gscratch_bytes) throw GrError(GR_E_STATE, "DP scratch not reserved");
+        if (((bytes + 7) / 8) * 8 * size_t(a.count) > tables.bytes) throw GrError(GR_E_STATE, "DP scratch not reserved");
         a.per_wave = 1;
         timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, a.count, false, stream); }, stream);
     }
@@ -446,15 +478,15 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
 
 // The two size classes of a batch are independent: the large trees (the
 // batch's critical path) run on the side stream beside the small ones.
-void launch_lshape(gr_ctx* c, const BatchLaunch& b) {
+void launch_lshape(gr_ctx* c, const BatchLaunch& b, DpTables tables = {}) {
     if (!b.count[0] || !b.count[1]) {
-        for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k);
+        for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k, 0, nullptr, tables);
         return;
     }
     hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
-    launch_lshape_class(c, b, 1, 0, c->side);
-    launch_lshape_class(c, b, 0, 0, c->stream);
+    launch_lshape_class(c, b, 1, 0, c->side, tables);
+    launch_lshape_class(c, b, 0, 0, c->stream, tables);
     hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
 }
@@ -468,8 +500,9 @@ BatchLaunch plan_batch_queue(gr_ctx* c, int begin, int count, std::vector<int>& 
         slots[s] = s;
         b.nmax[0] = std::max(b.nmax[0], c->node_off_h[s + 1] - c->node_off_h[s]);
     }
-    std::stable_sort(slots.begin() + begin, slots.begin() + begin + count,
-                     [&](int u, int v) { return c->height_h[u] > c->height_h[v]; });
+    std::sort(slots.begin() + begin, slots.begin() + begin + count, [&](int u, int v) {
+        return c->height_h[u] != c->height_h[v] ? c->height_h[u] > c->height_h[v] : u < v;
+    });
     return b;
 }
 
@@ -509,14 +542,19 @@ void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
 
 // Reserve the global DP scratch the batches will need (trees whose tables
 // do not fit in LDS) before anything is launched or captured.
+size_t dp_tables_needed(const gr_ctx* c, const BatchLaunch& b) {
+    size_t need = 0;
+    for (int k = 0; k < kTreeClasses; ++k) {
+        if (!b.count[k]) continue;
+        const size_t bytes = lshape_scratch_bytes(c->g.L, b.nmax[k]);
+        if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.count[k]));
+    }
+    return need;
+}
+
 void reserve_dp_scratch(gr_ctx* c, const std::vector<BatchLaunch>& plan) {
     size_t need = 0;
-    for (const BatchLaunch& b : plan)
-        for (int k = 0; k < kTreeClasses; ++k) {
-            if (!b.count[k]) continue;
-            const size_t bytes = lshape_scratch_bytes(c->g.L, b.nmax[k]);
-            if (bytes > kLdsLimit) need = std::max(need, ((bytes + 7) / 8) * 8 * size_t(b.count[k]));
-        }
+    for (const BatchLaunch& b : plan) need = std::max(need, dp_tables_needed(c, b));
     ensure_gscratch(c, need);
 }
 
@@ -534,6 +572,58 @@ void upload_desc(gr_ctx* c, const std::vector<int>& slots, int first, int last) 
         h = d;
     }
     if (changed) upload(c->desc + first, c->desc_h.data() + first, sizeof(int4) * size_t(last - first), c->stream);
+}
+
+// Launch arguments of FLT batch b of the uploaded stage.
+FltArgs flt_batch_args(gr_ctx* c, int b) {
+    const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
+    FltArgs a = c->f;
+    a.presum = c->presum;
+    a.vcost = c->vcost;
+    a.desc = c->f.desc + begin;
+    a.sizes = c->f.sizes + begin;
+    a.net_ids = c->f.net_ids + begin;
+    a.count = count;
+    a.node_base = c->f_base[b].x;
+    a.edge_base = c->f_base[b].y;
+    a.slot_base = c->f_base[b].z;
+    a.edges = c->f_base[b].w;
+    a.queue = nullptr;
+    a.lds_dag_bytes = a.lds_dag_slots = a.lds_dag_nodes = 0;
+    return a;
+}
+
+// The DP of batch b with `per_wave` DAGs per wavefront: each DAG's tables
+// in LDS when the batch's largest DAG fits (per_wave halved until the
+// workgroup's share fits), the persistent form draining `queue` when given;
+// otherwise the batch tables in global memory.
+void launch_flt_dp(gr_ctx* c, FltArgs a, int b, int per_wave, int* queue) {
+    const int2 mx = c->f_batch_max[b];
+    const size_t bytes = flt_lds_bytes(c->g.L, mx.x, mx.y);
+    int p = per_wave;
+    while (p > 1 && bytes * size_t(p) > size_t(c->lds_limit)) p /= 2;
+    a.per_wave = p;
+    if (bytes <= size_t(c->lds_limit)) {
+        a.lds_dag_bytes = int(bytes);
+        a.lds_dag_slots = mx.x;
+        a.lds_dag_nodes = mx.y;
+        a.queue = queue;
+        launch_flt_route(a, true, c->cus, c->stream);
+    } else {
+        a.queue = nullptr;
+        launch_flt_route(a, false, c->cus, c->stream);
+    }
+}
+
+// Route storage may have been replaced since the FLT upload: bind the
+// current buffers and check the stage's nets against them.
+void bind_flt_storage(gr_ctx* c) {
+    if (!c->routes2) throw GrError(GR_E_STATE, "no phase-two route storage");
+    for (int id : c->f_ids_h)
+        if (id >= c->net_count || c->pin_off2_h[id] < 0)
+            throw GrError(GR_E_STATE, "FLT stage net without phase-two route storage");
+    c->f.routes = RouteStore{c->routes2, c->pin_off2, c->route2_per_pin};
+    c->f.pin_acc_num = c->pin_acc_num;
 }
 
 void require_stage(gr_ctx* c, int begin, int count) {
@@ -573,6 +663,9 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
         c->cus = prop.multiProcessorCount;
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) == hipSuccess && lds > 0)
+            c->lds_limit = std::min(lds, 160 * 1024);
         hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming), "hipEventCreate");
@@ -633,6 +726,10 @@ int gr_ctx_set_routes(gr_ctx* c, int net_count, const int* pin_acc_num) {
         if (!c || net_count < 0 || !pin_acc_num) throw GrError(GR_E_ARG, "invalid route storage arguments");
         c->drop_graph();  // a captured batch loop holds the old route storage addresses
         c->route_per_pin = c->next_route_per_pin;
+        c->mem.release(c->routes2);  // phase-two offsets index the old subnets
+        c->mem.release(c->pin_off2);
+        c->pin_off2_h.clear();
+        c->route2_words = 0;
         c->mem.release(c->routes);
         c->mem.release(c->pin_acc_num);
         c->mem.release(c->of_flags);
@@ -794,6 +891,8 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
         c->parents = c->mem.alloc<int>(total);
         c->desc = c->mem.alloc<int4>(count);
         c->net_ids_h.assign(net_ids, net_ids + count);
+        c->slots_h.assign(size_t(count), 0);
+        c->desc_h.assign(size_t(count), int4{-1, -1, -1, -1});
         upload(c->net_ids, net_ids, sizeof(int) * count, c->stream);
         upload(c->node_off, node_off, sizeof(int) * (count + 1), c->stream);
         upload(c->nodes, nodes, sizeof(int) * total, c->stream);
@@ -1053,8 +1152,12 @@ int gr_ctx_reset(gr_ctx* c) {
         hip_check(hipMemsetAsync(c->stats, 0, sizeof(Stats) * kStatStripes, c->stream), "clear statistics");
         if (c->routes)
             hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
-        if (c->routes2)
-            hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
+        // a fresh context has no phase-two storage (it belongs to an FLT stage)
+        c->drop_fgraph();
+        c->mem.release(c->routes2);
+        c->mem.release(c->pin_off2);
+        c->pin_off2_h.clear();
+        c->route2_words = 0;
         std::vector<unsigned> ones(c->tracks, 1u);
         upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "gr_ctx_reset");
@@ -1266,6 +1369,7 @@ int gr_ctx_set_routes_phase2(gr_ctx* c, const int* pin_off2, long long pins2, in
             if (off < -1 || (off >= 0 && (long long)off + pins > pins2))
                 throw GrError(GR_E_ARG, "phase-two pin offset outside the storage");
         }
+        c->drop_fgraph();  // a captured FLT loop holds the old storage
         c->mem.release(c->routes2);
         c->mem.release(c->pin_off2);
         c->route2_per_pin = words_per_pin;
@@ -1398,6 +1502,16 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             max_slots = std::max(max_slots, slots_b);
             c->f_base[b] = make_int4(desc[k0].x, desc[k0].y, desc[k0].z, int(edges_b));
         }
+        c->f_batch_max.assign(nbatch, make_int2(0, 0));
+        for (int b = 0; b < nbatch; ++b)
+            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k)
+                c->f_batch_max[b] = make_int2(std::max(c->f_batch_max[b].x, sizes[k].z), std::max(c->f_batch_max[b].y, sizes[k].x));
+        if (c->f_queue_cap < nbatch) {
+            c->drop_graph();
+            c->mem.release(c->f_queue);
+            c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltQueues * kFltQueueStride);
+            c->f_queue_cap = nbatch;
+        }
         // Algorithmic bytes per launch, the FLT unit cost of one DAG: per
         // edge 12 * L * (len + 1) for the layer-change search (presum and via
         // cost gathers along the edge on every layer) and 8 for the edge
@@ -1476,50 +1590,76 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         f.routes = RouteStore{c->routes2, c->pin_off2, c->route2_per_pin};
         f.pin_acc_num = c->pin_acc_num;
         f.error = c->error;
+        c->drop_fgraph();  // the workspaces may have moved
+        ++c->flt_gen;
         c->f_batches = nbatch;
         c->f_batch_off.assign(batch_off, batch_off + nbatch + 1);
+        c->f_ids_h.assign(net_ids, net_ids + nets);
         hip_check(hipStreamSynchronize(c->stream), "FLT upload");
     });
 }
 
-int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, double* elapsed_ms) {
+int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, double* elapsed_ms) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
+        if (mode < 0 || mode > 1) throw GrError(GR_E_ARG, "FLT execution mode not available (0 = launches, 1 = persistent + graph)");
         if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
+        bind_flt_storage(c);
         const RouteStore rs2{c->routes2, c->pin_off2, c->route2_per_pin};
+        auto enqueue = [&] {
+            int stamp = first_stamp;
+            const long long qwords = (long long)c->f_batches * kFltQueues * kFltQueueStride;
+            if (mode == 1 && qwords)  // a kernel, not a memset: the loop may be captured
+                hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((qwords + 255) / 256)), dim3(256), 0, c->stream,
+                                   c->f_queue, 0, qwords);
+            for (int b = 0; b < c->f_batches; ++b) {
+                const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
+                const int* ids = c->f.net_ids + begin;
+                if (remove_phase_one) launch_commit(c, 0, count, stamp++, -1, ids);
+                const int s = stamp++;
+                launch_refresh(c, 0);
+                if (!count) continue;
+                const FltArgs a = flt_batch_args(c, b);
+                const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
+                timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] {
+                    if (threads)
+                        hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
+                });
+                int* queue = mode == 1 ? c->f_queue + size_t(b) * kFltQueues * kFltQueueStride : nullptr;
+                timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
+                    launch_flt_dp(c, a, b, c->f_per_wave[count >= kFltWideBatch], queue);
+                });
+                hip_check(hipGetLastError(), "FLT kernels");
+                launch_commit(c, 0, count, s, 1, ids, &rs2);
+            }
+        };
+        // a captured graph cannot carry profiling events: profiled runs of
+        // mode 1 issue the same launches directly
+        const bool graph = mode == 1 && !c->profiling;
+        if (graph) {
+            const std::vector<int> key{c->flt_gen, first_stamp, remove_phase_one ? 1 : 0, c->f_per_wave[0],
+                                       c->f_per_wave[1]};
+            if (!c->fgraph_exec || key != c->fgraph_key) {
+                c->drop_fgraph();
+                hip_check(hipStreamSynchronize(c->stream), "before graph capture");
+                hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    enqueue();
+                } catch (...) {
+                    hipGraph_t broken = nullptr;
+                    (void)hipStreamEndCapture(c->stream, &broken);
+                    if (broken) (void)hipGraphDestroy(broken);
+                    throw;
+                }
+                hip_check(hipStreamEndCapture(c->stream, &c->fgraph), "hipStreamEndCapture");
+                hip_check(hipGraphInstantiate(&c->fgraph_exec, c->fgraph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                c->fgraph_key = key;
+            }
+        }
         hip_check(hipDeviceSynchronize(), "before DAG interval");
         const auto t0 = std::chrono::steady_clock::now();
-        int stamp = first_stamp;
-        for (int b = 0; b < c->f_batches; ++b) {
-            const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
-            const int* ids = c->f.net_ids + begin;
-            if (remove_phase_one) launch_commit(c, 0, count, stamp++, -1, ids);
-            const int s = stamp++;
-            launch_refresh(c, 0);
-            if (!count) continue;
-            FltArgs a = c->f;
-            a.presum = c->presum;
-            a.vcost = c->vcost;
-            a.desc = c->f.desc + begin;
-            a.sizes = c->f.sizes + begin;
-            a.net_ids = ids;
-            a.count = count;
-            a.node_base = c->f_base[b].x;
-            a.edge_base = c->f_base[b].y;
-            a.slot_base = c->f_base[b].z;
-            a.edges = c->f_base[b].w;
-            const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
-            timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] {
-                if (threads)
-                    hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
-            });
-            a.per_wave = c->f_per_wave[count >= kFltWideBatch];
-            timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
-                launch_flt_route(a, (count + 4 * a.per_wave - 1) / (4 * a.per_wave), c->stream);
-            });
-            hip_check(hipGetLastError(), "FLT kernels");
-            launch_commit(c, 0, count, s, 1, ids, &rs2);
-        }
+        if (graph) hip_check(hipGraphLaunch(c->fgraph_exec, c->stream), "hipGraphLaunch");
+        else enqueue();
         hip_check(hipStreamSynchronize(c->stream), "DAG interval");
         const auto t1 = std::chrono::steady_clock::now();
         c->last_t0 = t0, c->last_t1 = t1;
@@ -1532,6 +1672,7 @@ int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
+        bind_flt_storage(c);
         launch_refresh(c, 0);
         const bool profiling = c->profiling;
         c->profiling = false;
@@ -1544,28 +1685,17 @@ int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
         for (int b = 0; b < c->f_batches; ++b) {
             const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
             if (!count) continue;
-            FltArgs a = c->f;
-            a.presum = c->presum;
-            a.vcost = c->vcost;
-            a.desc = c->f.desc + begin;
-            a.sizes = c->f.sizes + begin;
-            a.net_ids = c->f.net_ids + begin;
-            a.count = count;
-            a.node_base = c->f_base[b].x;
-            a.edge_base = c->f_base[b].y;
-            a.slot_base = c->f_base[b].z;
-            a.edges = c->f_base[b].w;
+            const FltArgs a = flt_batch_args(c, b);
             const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
             if (threads)
                 hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
             const int cls = count >= kFltWideBatch;
             seen[cls] = true;
             for (int ci = 0; ci < 4; ++ci) {
-                // the DP of this batch at the current costs; its words are
-                // rewritten when the stage runs
-                a.per_wave = candidates[ci];
+                // the DP of this batch at the current costs, in the launch
+                // form; its words are rewritten when the stage runs
                 hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
-                launch_flt_route(a, (count + 4 * a.per_wave - 1) / (4 * a.per_wave), c->stream);
+                launch_flt_dp(c, a, b, candidates[ci], nullptr);
                 hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
                 hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
                 float f = 0;
@@ -2055,6 +2185,7 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
         a.pin_off = d_pin_off, a.pins = d_pins, a.extra_off = d_extra_off, a.extra = d_extra;
         a.sub_off = d_sub_off, a.subs = d_subs;
         a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+        a.routes2 = c->routes2, a.pin_off2 = c->routes2 ? c->pin_off2 : nullptr, a.per_pin2 = c->route2_per_pin;
         a.dir = d_dir, a.wire = d_wire, a.stacked = d_stacked, a.vias = d_counts, a.bad_nets = d_counts + ncount - 1;
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");

@@ -61,21 +61,108 @@ __host__ __device__ __forceinline__ size_t change_slot(int e, int f, int t) {
     return ((size_t)e * 9 + f) * kFltChangeWidth + (t - (f & 1)) / 2;
 }
 
-// One DAG's tables, net-local.
+// One DAG's records and layer-change tables (global memory, read-only
+// during the DP), net-local.
 struct Dag {
     const int4* nodes;  // {coordinate, first edge, end edge, first slot * 8 + slots}
     const int4* edges;  // {parent, slot, parent coordinate, child coordinate}
-    double* costs;      // [node * kS + layer]
-    unsigned long long* packed;
-    unsigned long long* best;
-    int* layer;
     const double* cost_edges;  // [change_slot(edge, from, to)]
     const int* best_change;
 };
 
-// solve_flt_node (src/routing/flt_block.cuh:6-142), tables in global memory
-// (any number of child slots).
-__device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
+// Where a DAG's DP tables live.  Rows are child slots; a row holds the
+// layers 1..L.  GlobalTables: the batch tables in global memory (row stride
+// kS), values other lanes wrote read past L1 (l2_load).  LdsTables: the
+// DAG's rows in LDS (row stride L) with its nodes' via costs staged there
+// once; the slot keys, best paths and layer outputs never leave the
+// compute unit.  Both give the same values: the DP below is one code for
+// both.
+struct GlobalTables {
+    static constexpr bool kReload = false;  // keep slot keys in registers rather than reload them
+    double* costs;  // [node * kS + layer]; only the root's row is read
+    unsigned long long* packed;
+    unsigned long long* best;
+    int* layer;
+    const float* vcost;  // the grid's via costs
+    DeviceGrid g;
+    __device__ __forceinline__ unsigned long long key(int row, int li) const { return l2_load(packed + (size_t)row * kS + li); }
+    __device__ __forceinline__ void lower(int row, int li, unsigned long long v) const { atomicMin(packed + (size_t)row * kS + li, v); }
+    __device__ __forceinline__ unsigned long long path(int row, int li) const { return l2_load(best + (size_t)row * kS + li); }
+    __device__ __forceinline__ void set_path(int row, int li, unsigned long long v) const { best[(size_t)row * kS + li] = v; }
+    __device__ __forceinline__ int out(int v) const { return l2_load(layer + v); }
+    __device__ __forceinline__ void set_out(int v, int x) const { layer[v] = x; }
+    __device__ __forceinline__ double root_cost(int v, int li) const { return costs[(size_t)v * kS + li]; }
+    __device__ __forceinline__ void set_root_cost(int v, int li, double c) const { costs[(size_t)v * kS + li] = c; }
+    __device__ __forceinline__ float via(int /*v*/, int j, int x, int y) const { return vcost[cell_of(g, j, x, y)]; }
+    __device__ __forceinline__ void clear(const Dag&, int slots, int /*nodes*/, int lane, int W) const {
+        for (int i = lane; i < slots * kS; i += W) packed[i] = kEmpty;
+    }
+    __device__ __forceinline__ static void barrier() { depth_barrier(); }
+};
+
+template <int L>
+struct LdsTables {
+    static constexpr bool kReload = true;  // slot keys reloaded from LDS instead of held in registers
+    unsigned long long* packed;  // [row * L + li - 1]
+    unsigned* best;              // best paths, code * kS + layer (below 2^32: codes are node * 100 + choice)
+    int* layer;                  // [node]
+    double* rootc;               // [kS], the root's layer costs
+    float* vc;                   // [node * (L - 1) + j]: via cost of layer j at the node's cell
+    const float* vcost;
+    DeviceGrid g;
+    __device__ __forceinline__ unsigned long long key(int row, int li) const { return packed[row * L + li - 1]; }
+    __device__ __forceinline__ void lower(int row, int li, unsigned long long v) const { atomicMin(packed + row * L + li - 1, v); }
+    __device__ __forceinline__ unsigned long long path(int row, int li) const { return best[row * L + li - 1]; }
+    __device__ __forceinline__ void set_path(int row, int li, unsigned long long v) const { best[row * L + li - 1] = (unsigned)v; }
+    __device__ __forceinline__ int out(int v) const { return layer[v]; }
+    __device__ __forceinline__ void set_out(int v, int x) const { layer[v] = x; }
+    __device__ __forceinline__ double root_cost(int, int li) const { return rootc[li]; }
+    __device__ __forceinline__ void set_root_cost(int, int li, double c) const { rootc[li] = c; }
+    __device__ __forceinline__ float via(int v, int j, int, int) const { return vc[v * (L - 1) + j]; }
+    // child slots to the empty key, via costs of every node staged
+    __device__ __forceinline__ void clear(const Dag& d, int slots, int nodes, int lane, int W) const {
+        for (int i = lane; i < slots * L; i += W) packed[i] = kEmpty;
+        if (L > 1)
+            for (int i = lane; i < nodes * (L - 1); i += W) {
+                const int v = i / (L - 1), j = i % (L - 1);
+                const int c = d.nodes[v].x;
+                vc[i] = vcost[cell_of(g, j, c / g.Y % g.X, c % g.Y)];
+            }
+    }
+    // every LDS access of the wavefront has completed
+    __device__ __forceinline__ static void barrier() {
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+
+// Bytes of one DAG's LdsTables for a DAG of `slots` child slots and `nodes`
+// nodes on L layers, 16-byte aligned.
+__host__ __device__ inline size_t lds_tables_bytes(int L, int slots, int nodes) {
+    const size_t rows = (size_t)slots * L;
+    size_t b = rows * 8 + ((rows * 4 + 7) & ~size_t(7)) + (size_t)kS * 8 + (size_t)nodes * 4 +
+               (size_t)nodes * (L > 1 ? L - 1 : 0) * 4;
+    return (b + 15) & ~size_t(15);
+}
+
+template <int L>
+__device__ __forceinline__ LdsTables<L> lds_tables(char* base, int slots, int nodes, const FltArgs& a) {
+    LdsTables<L> t;
+    const size_t rows = (size_t)slots * L;
+    t.packed = reinterpret_cast<unsigned long long*>(base);
+    t.best = reinterpret_cast<unsigned*>(t.packed + rows);
+    t.rootc = reinterpret_cast<double*>(base + rows * 8 + ((rows * 4 + 7) & ~size_t(7)));
+    t.layer = reinterpret_cast<int*>(t.rootc + kS);
+    t.vc = reinterpret_cast<float*>(t.layer + nodes);
+    t.vcost = a.vcost;
+    t.g = a.g;
+    return t;
+}
+
+// solve_flt_node (src/routing/flt_block.cuh:6-142) for any number of child
+// slots, the node's layer costs in a private array.
+template <class Tb>
+__device__ void solve_node(const FltArgs& a, const Dag& d, const Tb& tb, int v) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
     const int4 r = d.nodes[v];
@@ -83,9 +170,8 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
     const int x = r.x / g.Y % g.X, y = r.x % g.Y;
     const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
     const int kids = r.w & 7;
-    unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
-    const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
-    double* cv = d.costs + (size_t)v * kS;
+    const int row0 = r.w >> 3;
+    double cv[kS];
     for (int li = 0; li < kS; ++li) cv[li] = kInf;
     double mins[6];
     unsigned long long paths[6];
@@ -93,9 +179,9 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
         for (int k = 0; k < kids; ++k) mins[k] = kInf;
         double via = 0.0;
         for (int li = low; li < L + 1; ++li) {
-            if (li > low) via += (double)a.vcost[cell_of(g, li - 2, x, y)];
+            if (li > low) via += (double)tb.via(v, li - 2, x, y);
             for (int k = 0; k < kids; ++k) {
-                const unsigned long long p = l2_load(pk + k * kS + li);
+                const unsigned long long p = tb.key(row0 + k, li);
                 const double c = unpack_cost(p);
                 if (c < mins[k]) {
                     mins[k] = c;
@@ -107,20 +193,22 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
                 for (int k = 0; k < kids; ++k) cost += mins[k];
                 if (cost < kInf && cost < cv[li]) {
                     cv[li] = cost;
-                    for (int k = 0; k < kids; ++k) bp[k * kS + li] = paths[k];
+                    for (int k = 0; k < kids; ++k) tb.set_path(row0 + k, li, paths[k]);
                 }
             }
         }
         for (int li = L - 1; li >= low; --li)
             if (cv[li + 1] < cv[li]) {
                 cv[li] = cv[li + 1];
-                for (int k = 0; k < kids; ++k) bp[k * kS + li] = bp[k * kS + li + 1];
+                for (int k = 0; k < kids; ++k) tb.set_path(row0 + k, li, tb.path(row0 + k, li + 1));
             }
     }
+    if (r.y == r.z)  // the root's trace reads its layer costs
+        for (int li = 0; li < kS; ++li) tb.set_root_cost(v, li, li <= L ? cv[li] : kInf);
     unsigned long long cache[kS];
     for (int e = r.y; e < r.z; ++e) {
         const int4 er = d.edges[e];
-        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
+        const int prow = (d.nodes[er.x].w >> 3) + er.y;
         const int px = er.z / g.Y % g.X, py = er.z % g.Y;
         const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
         for (int t = 1; t < L + 1; ++t) cache[t] = kEmpty;
@@ -151,7 +239,7 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, int v) {
             }
         }
         for (int t = 1; t < L + 1; ++t)
-            if (cache[t] < kEmpty) atomicMin(ppk + t, cache[t]);
+            if (cache[t] < kEmpty) tb.lower(prow, t, cache[t]);
     }
 }
 
@@ -171,13 +259,14 @@ constexpr int kFastKids = 4;
 // non-zero length selects one layer parity, so each lane walks only that
 // parity's layers (and to-layers); an edge between two nodes on the same
 // cell takes every layer.
-template <int L>
-__device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, int v, int x, int y, const int4 r, const double* cv) {
+template <int L, class Tb>
+__device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, const Tb& tb, int v, int x, int y,
+                                             const int4 r, const double* cv) {
     const DeviceGrid& g = a.g;
     constexpr int H = (L + 1) / 2;  // layers of one parity, at most
     for (int e = r.y; e < r.z; ++e) {
         const int4 er = d.edges[e];
-        unsigned long long* ppk = d.packed + (size_t)((d.nodes[er.x].w >> 3) + er.y) * kS;
+        const int prow = (d.nodes[er.x].w >> 3) + er.y;
         const int px = er.z / g.Y % g.X, py = er.z % g.Y;
         const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
         const bool far = abs(x - px) + abs(y - py) > 4;
@@ -192,7 +281,7 @@ __device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, int
                 }
                 const unsigned long long nv =
                     pack((float)(base + wire(a, layer - 1, minx, maxx, miny, maxy)), v * 100 + kStraight);
-                atomicMin(ppk + layer, nv);
+                tb.lower(prow, layer, nv);
             }
             continue;
         }
@@ -234,32 +323,35 @@ __device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, int
         }
 #pragma unroll
         for (int m = 0; m < H; ++m)
-            if (cache[m] < kEmpty && par + 2 * m <= L) atomicMin(ppk + par + 2 * m, cache[m]);
+            if (cache[m] < kEmpty && par + 2 * m <= L) tb.lower(prow, par + 2 * m, cache[m]);
     }
 }
 
-template <int L, int K>
-__device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, int v, const int4 r) {
+template <int L, int K, class Tb>
+__device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, const Tb& tb, int v, const int4 r) {
     const DeviceGrid& g = a.g;
     const int kids = r.w & 7;
     const int low_fixed = 1 + r.x / g.XY;
     const int x = r.x / g.Y % g.X, y = r.x % g.Y;
     const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
-    const unsigned long long* pk = d.packed + (size_t)(r.w >> 3) * kS;
+    const int row0 = r.w >> 3;
     // slot keys, loaded once: the costs for the pass below, the codes for
     // the best-path rows
-    unsigned long long pkv[K][L + 1];
+    constexpr int KK = Tb::kReload ? 1 : K;  // keys held in registers (LDS tables reload them)
+    constexpr int LK = Tb::kReload ? 1 : L + 1;
+    unsigned long long pkv[KK][LK];
     float pv[K][L + 1];  // slot costs (the packed key's float)
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
         for (int li = 1; li <= L; ++li) {
-            pkv[k][li] = k < kids ? l2_load(pk + k * kS + li) : 0ull;
-            pv[k][li] = k < kids ? (float)unpack_cost(pkv[k][li]) : 0.f;
+            const unsigned long long key = k < kids ? tb.key(row0 + k, li) : 0ull;
+            if constexpr (!Tb::kReload) pkv[k][li] = key;
+            pv[k][li] = k < kids ? (float)unpack_cost(key) : 0.f;
         }
     float vc[L];
 #pragma unroll
-    for (int j = 0; j + 1 < L; ++j) vc[j] = a.vcost[cell_of(g, j, x, y)];
+    for (int j = 0; j + 1 < L; ++j) vc[j] = tb.via(v, j, x, y);
     double cv[L + 1];
     unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
 #pragma unroll
@@ -306,7 +398,6 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
                 }
         }
     }
-    unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
 #pragma unroll
     for (int li = 1; li <= L; ++li) {
         if (!(cv[li] < kInf)) continue;
@@ -314,44 +405,48 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
         for (int k = 0; k < K; ++k) {
             if (k >= kids) continue;
             const int j = (at[li] >> (4 * k)) & 15;
-            unsigned long long key = 0;  // pkv[k][j], j a run-time layer
+            unsigned long long key = 0;  // slot k's key at layer j, j a run-time layer
+            if constexpr (Tb::kReload) {
+                key = tb.key(row0 + k, j);
+            } else {
 #pragma unroll
-            for (int lj = 1; lj <= L; ++lj)
-                if (lj == j) key = pkv[k][lj];
-            bp[k * kS + li] = (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j;
+                for (int lj = 1; lj <= L; ++lj)
+                    if (lj == j) key = pkv[k][lj];
+            }
+            tb.set_path(row0 + k, li, (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j);
         }
     }
     if (r.y == r.z) {  // the root's trace reads its layer costs
-        double* cg = d.costs + (size_t)v * kS;
-        cg[0] = kInf;
+        tb.set_root_cost(v, 0, kInf);
 #pragma unroll
-        for (int li = 1; li <= L; ++li) cg[li] = cv[li];
+        for (int li = 1; li <= L; ++li) tb.set_root_cost(v, li, cv[li]);
     }
-    push_parents<L>(a, d, v, x, y, r, cv);
+    push_parents<L>(a, d, tb, v, x, y, r, cv);
 }
 
 // trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
 // segments of the edge its chosen parent uses and its via stack, into w[].
-__device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
+template <class Tb>
+__device__ int trace_node(const FltArgs& a, const Dag& d, const Tb& tb, int v, int* w) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
     const int4 r = d.nodes[v];
     const int l = r.x / g.XY, cx = r.x / g.Y % g.X, cy = r.x % g.Y;
-    const unsigned long long* bp = d.best + (size_t)(r.w >> 3) * kS;
+    const int row0 = r.w >> 3;
     int n = 0;
     int lv;
     if (r.y == r.z) {
         int best = 0;
-        double bc = d.costs[(size_t)v * kS];
+        double bc = tb.root_cost(v, 0);
         for (int layer = 1; layer < L + 1; ++layer) {
-            const double c = d.costs[(size_t)v * kS + layer];
+            const double c = tb.root_cost(v, layer);
             if (c < bc) {
                 bc = c;
                 best = layer;
             }
         }
         lv = best;
-        d.layer[v] = lv;
+        tb.set_out(v, lv);
         if (!lv) {  // no finite cost at any layer: nothing to trace
             raise(a.error, kDevNoSolution);
             return 0;
@@ -361,12 +456,12 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
         lv = -1;
         for (int e = r.y; e < r.z; ++e) {
             const int4 er = d.edges[e];
-            const int pl = l2_load(d.layer + er.x);
+            const int pl = tb.out(er.x);
             if (pl <= 0) continue;
             par = er.x;
             par_edge = e;
             par_layer = pl;
-            const unsigned long long path = l2_load(d.best + (size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer);
+            const unsigned long long path = tb.path((d.nodes[par].w >> 3) + er.y, par_layer);
             const unsigned long long child = path / kS / 100, choice = (path / kS) % 100;
             if (child != (unsigned long long)v) {
                 lv = -1;
@@ -408,16 +503,16 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
             break;
         }
         if (par_layer == -1) {
-            d.layer[v] = -1;
+            tb.set_out(v, -1);
             return 0;
         }
         const int4 er = d.edges[par_edge];
-        const unsigned long long path = l2_load(d.best + (size_t)((d.nodes[par].w >> 3) + er.y) * kS + par_layer);
+        const unsigned long long path = tb.path((d.nodes[par].w >> 3) + er.y, par_layer);
         if (path / kS / 100 != (unsigned long long)v) {
-            d.layer[v] = -1;
+            tb.set_out(v, -1);
             return 0;
         }
-        d.layer[v] = lv;
+        tb.set_out(v, lv);
     }
     const int kids = r.w & 7;
     int minl = l + 1;
@@ -425,7 +520,7 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
     minl = min(minl, lv);
     maxl = max(maxl, lv);
     for (int k = 0; k < kids; ++k) {
-        const int lc = (int)(l2_load(bp + k * kS + lv) % kS);
+        const int lc = (int)(tb.path(row0 + k, lv) % kS);
         minl = min(minl, lc);
         maxl = max(maxl, lc);
     }
@@ -436,6 +531,79 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, int v, int* w) {
     return n;
 }
 
+// One DAG of the batch on W lanes (lane = this lane's index among them),
+// stepping with the other DAGs of its wavefront through `depths` (the
+// wavefront's largest depth count): clear the tables, solve the depths
+// bottom-up, trace them top-down and write the route words.
+template <int L, class Tb>
+__device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, int lane, int W, const Dag& d,
+                                          const Tb& tb, const int4 dn, const int* order, const int* doff, int depths) {
+    tb.clear(d, dn.z, dn.x, lane, W);
+    Tb::barrier();
+    for (int dep = depths - 1; dep >= 0; --dep) {
+        if (dep < dn.w) {
+            const int end = doff[dep + 1];
+            for (int i = doff[dep] + lane; i < end; i += W) {
+                const int v = order[i];
+                const int4 r = d.nodes[v];
+                if ((r.w & 7) <= 2)
+                    solve_node_fast<L, 2>(a, d, tb, v, r);
+                else if ((r.w & 7) <= kFastKids)
+                    solve_node_fast<L, kFastKids>(a, d, tb, v, r);
+                else
+                    solve_node(a, d, tb, v);
+            }
+        }
+        Tb::barrier();
+    }
+    const int id = live ? a.net_ids[k] : 0;
+    int* routes = a.routes.routes + (live ? (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin : 0);
+    const long long cap = live ? (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin : 0;
+    const int g_cells = a.g.L * a.g.XY;
+    int count = 1;  // the root's trace sets the word count to 1
+    for (int dep = 0; dep < depths; ++dep) {
+        const int first = dep < dn.w ? doff[dep] : 0, end = dep < dn.w ? doff[dep + 1] : 0;
+        int chunks = (end - first + W - 1) / W;
+        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+        for (int ch = 0; ch < chunks; ++ch) {
+            int w[8];
+            const int i = first + ch * W + lane;
+            int n = i < end ? trace_node(a, d, tb, order[i], w) : 0;
+            for (int j = 0; j < n; ++j)
+                if (w[j] < 0 || w[j] >= g_cells) {
+                    raise(a.error, kDevBadCoordinate);
+                    n = 0;
+                }
+            // inclusive prefix of the word counts within this DAG's lanes
+            int incl = n;
+            for (int s = 1; s < W; s <<= 1) {
+                const int o = __shfl_up(incl, s, W);
+                if (lane >= s) incl += o;
+            }
+            const int at = count + incl - n;
+            if (n) {
+                if (at + n - 1 >= cap) {
+                    raise(a.error, kDevRouteOverflow);
+                } else {
+                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
+                }
+            }
+            count += __shfl(incl, W - 1, W);
+        }
+        Tb::barrier();
+    }
+    if (live && lane == 0) routes[0] = count <= cap ? count : 1;
+}
+
+// The records of DAG k of the batch and its layer-change tables.
+__device__ __forceinline__ Dag dag_of(const FltArgs& a, const int4 ds) {
+    Dag d;
+    d.nodes = a.node_rec + ds.x;
+    d.edges = a.edge_rec + ds.y;
+    d.cost_edges = a.cost_edges + change_slot(ds.y - a.edge_base, 0, 0);
+    d.best_change = a.best_change + change_slot(ds.y - a.edge_base, 0, 0);
+    return d;
+}
 
 }  // namespace
 
@@ -484,10 +652,13 @@ __global__ void flt_edge_kernel(FltArgs a) {
     a.best_change[change_slot(e, f, to)] = pos;
 }
 
-// route_flt_block (src/routing/flt_block.cuh:334-355): a.per_wave DAGs per
-// wavefront (1, 2, 4 or 8; each gets 64 / per_wave lanes), four wavefronts
-// per workgroup.  The DAGs of a wavefront step through their depths
-// together, so a batch is uploaded with its DAGs in decreasing depth order.
+namespace {
+
+// route_flt_block (src/routing/flt_block.cuh:334-355), tables in global
+// memory (DAGs of any size): a.per_wave DAGs per wavefront (1, 2, 4 or 8;
+// each gets 64 / per_wave lanes), four wavefronts per workgroup.  The DAGs
+// of a wavefront step through their depths together, so a batch is
+// uploaded with its DAGs in decreasing depth order.
 template <int L>
 __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     const int P = a.per_wave, W = kWave / P;
@@ -499,85 +670,89 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     const bool live = k < a.count;
     const int4 ds = live ? a.desc[k] : make_int4(0, 0, 0, 0);  // stage-global first node, edge, slot, depth offset
     const int4 dn = live ? a.sizes[k] : make_int4(0, 0, 0, 0);  // nodes, edges, slots, depths
-    Dag d;
-    d.nodes = a.node_rec + ds.x;
-    d.edges = a.edge_rec + ds.y;
-    d.costs = a.costs + (size_t)(ds.x - a.node_base) * kS;
-    d.packed = a.packed + (size_t)(ds.z - a.slot_base) * kS;
-    d.best = a.best_path + (size_t)(ds.z - a.slot_base) * kS;
-    d.layer = a.layer_output + (ds.x - a.node_base);
-    d.cost_edges = a.cost_edges + change_slot(ds.y - a.edge_base, 0, 0);
-    d.best_change = a.best_change + change_slot(ds.y - a.edge_base, 0, 0);
-    const int* order = a.order + ds.x;
-    const int* doff = a.depth_off + ds.w;
+    const Dag d = dag_of(a, ds);
+    GlobalTables tb;
+    tb.costs = a.costs + (size_t)(ds.x - a.node_base) * kS;
+    tb.packed = a.packed + (size_t)(ds.z - a.slot_base) * kS;
+    tb.best = a.best_path + (size_t)(ds.z - a.slot_base) * kS;
+    tb.layer = a.layer_output + (ds.x - a.node_base);
+    tb.vcost = a.vcost;
+    tb.g = a.g;
     int depths = dn.w;  // the wavefront's deepest DAG
     for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
-
-    // initialize_batch (src/routing/flt_workspace.cuh:60-69) for this DAG:
-    // only the child slots need it here (a node's layer costs start at
-    // infinity in its solve, and every node's trace sets its layer output
-    // before any child reads it)
-    for (int i = lane; i < dn.z * kS; i += W) d.packed[i] = kEmpty;
-    depth_barrier();
-    for (int dep = depths - 1; dep >= 0; --dep) {
-        if (dep < dn.w) {
-            const int end = doff[dep + 1];
-            for (int i = doff[dep] + lane; i < end; i += W) {
-                const int v = order[i];
-                const int4 r = d.nodes[v];
-                if ((r.w & 7) <= 2)
-                    solve_node_fast<L, 2>(a, d, v, r);
-                else if ((r.w & 7) <= kFastKids)
-                    solve_node_fast<L, kFastKids>(a, d, v, r);
-                else
-                    solve_node(a, d, v);
-            }
-        }
-        depth_barrier();
-    }
-    const int id = live ? a.net_ids[k] : 0;
-    int* routes = a.routes.routes + (live ? (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin : 0);
-    const long long cap = live ? (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin : 0;
-    const int g_cells = a.g.L * a.g.XY;
-    int count = 1;  // the root's trace sets the word count to 1
-    for (int dep = 0; dep < depths; ++dep) {
-        const int first = dep < dn.w ? doff[dep] : 0, end = dep < dn.w ? doff[dep + 1] : 0;
-        int chunks = (end - first + W - 1) / W;
-        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
-        for (int ch = 0; ch < chunks; ++ch) {
-            int w[8];
-            const int i = first + ch * W + lane;
-            int n = i < end ? trace_node(a, d, order[i], w) : 0;
-            for (int j = 0; j < n; ++j)
-                if (w[j] < 0 || w[j] >= g_cells) {
-                    raise(a.error, kDevBadCoordinate);
-                    n = 0;
-                }
-            // inclusive prefix of the word counts within this DAG's lanes
-            int incl = n;
-            for (int s = 1; s < W; s <<= 1) {
-                const int o = __shfl_up(incl, s, W);
-                if (lane >= s) incl += o;
-            }
-            const int at = count + incl - n;
-            if (n) {
-                if (at + n - 1 >= cap) {
-                    raise(a.error, kDevRouteOverflow);
-                } else {
-                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
-                }
-            }
-            count += __shfl(incl, W - 1, W);
-        }
-        depth_barrier();
-    }
-    if (live && lane == 0) routes[0] = count <= cap ? count : 1;
+    route_dag<L>(a, k, live, lane, W, d, tb, dn, a.order + ds.x, a.depth_off + ds.w, depths);
 }
 
-void launch_flt_route(const FltArgs& a, int blocks, hipStream_t stream) {
+// The same DP with each DAG's tables in LDS: one wavefront per workgroup,
+// a.per_wave DAGs per wavefront, each with a.lds_dag_bytes of LDS (sized by
+// the batch's largest DAG).  Without a queue, workgroup w takes DAG group w;
+// with one (the persistent form), a grid no larger than what stays resident
+// drains the batch's DAG groups in order (deepest DAGs first) from
+// kFltQueues striped counters, zero at launch: workgroup w dequeues from
+// counter w % kFltQueues, which hands out groups q, q + kFltQueues, ...
+template <int L>
+__global__ __launch_bounds__(kWave) void flt_route_lds_kernel(FltArgs a) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ int s_group;
+    const int P = a.per_wave, W = kWave / P;
+    const int sub = threadIdx.x / W, lane = threadIdx.x % W;
+    const int q = blockIdx.x % kFltQueues;
+    int* counter = a.queue ? a.queue + q * kFltQueueStride : nullptr;
+    for (int round = 0;; ++round) {
+        int group;
+        if (counter) {
+            if (threadIdx.x == 0) s_group = atomicAdd(counter, 1) * kFltQueues + q;
+            __syncthreads();
+            group = s_group;
+            __syncthreads();  // every lane has read s_group before the next dequeue
+        } else {
+            if (round) break;
+            group = blockIdx.x;
+        }
+        const int first_dag = group * P;
+        if (first_dag >= a.count) break;
+        const int k = first_dag + sub;
+        const bool live = k < a.count;
+        const int4 ds = live ? a.desc[k] : make_int4(0, 0, 0, 0);
+        const int4 dn = live ? a.sizes[k] : make_int4(0, 0, 0, 0);
+        const Dag d = dag_of(a, ds);
+        const LdsTables<L> tb =
+            lds_tables<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_dag_slots, a.lds_dag_nodes, a);
+        int depths = dn.w;
+        for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
+        route_dag<L>(a, k, live, lane, W, d, tb, dn, a.order + ds.x, a.depth_off + ds.w, depths);
+        __syncthreads();  // the tables are reused by the next group
+    }
+}
+
+template <int L>
+void launch_for(const FltArgs& a, int mode_lds, int cus, hipStream_t stream) {
+    if (!mode_lds) {
+        const int blocks = (a.count + 4 * a.per_wave - 1) / (4 * a.per_wave);
+        hipLaunchKernelGGL(flt_route_kernel<L>, dim3(blocks), dim3(256), 0, stream, a);
+        return;
+    }
+    const size_t lds = (size_t)a.lds_dag_bytes * a.per_wave;
+    const int groups = (a.count + a.per_wave - 1) / a.per_wave;
+    int grid = groups;
+    if (a.queue) {
+        int resident = 0;  // workgroups per compute unit; any value keeps the queue correct
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, flt_route_lds_kernel<L>, kWave, lds) != hipSuccess ||
+            resident < 1)
+            resident = 1;
+        grid = std::max(1, std::min(groups, resident * cus));
+    }
+    hipLaunchKernelGGL(flt_route_lds_kernel<L>, dim3(grid), dim3(kWave), lds, stream, a);
+}
+
+}  // namespace
+
+size_t flt_lds_bytes(int L, int slots, int nodes) { return lds_tables_bytes(L, slots, nodes); }
+
+void launch_flt_route(const FltArgs& a, bool lds, int cus, hipStream_t stream) {
     switch (a.g.L) {
 #define GR_FLT_CASE(n) \
-    case n: hipLaunchKernelGGL(flt_route_kernel<n>, dim3(blocks), dim3(256), 0, stream, a); break;
+    case n: launch_for<n>(a, lds, cus, stream); break;
         GR_FLT_CASE(1)
         GR_FLT_CASE(2)
         GR_FLT_CASE(3)

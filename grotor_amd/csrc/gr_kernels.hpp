@@ -206,9 +206,19 @@ struct FltArgs {
     RouteStore routes;        // phase-two route words
     const int* pin_acc_num;   // pins per net: capacity = pins * routes.per_pin
     int* error;
+    // DP tables in LDS (flt_route_lds_kernel): bytes, child slots and nodes
+    // of one DAG's share, sized by the batch's largest DAG
+    int lds_dag_bytes, lds_dag_slots, lds_dag_nodes;
+    int* queue;  // persistent form: kFltQueues work-queue counters (nullptr: one workgroup per DAG group)
 };
+constexpr int kFltQueues = 32;       // persistent FLT DP: work-queue counters per batch
+constexpr int kFltQueueStride = 32;  // ints between counters (one 128-byte line)
 __global__ void flt_edge_kernel(FltArgs a);
-void launch_flt_route(const FltArgs& a, int blocks, hipStream_t stream);  // L = 1..9
+// LDS bytes of one DAG's DP tables (slots child slots, nodes nodes, L layers)
+size_t flt_lds_bytes(int L, int slots, int nodes);
+// The FLT DP of a batch (L = 1..9): with lds, each DAG's tables in LDS
+// (a.lds_dag_*, a.queue), else in the batch tables in global memory.
+void launch_flt_route(const FltArgs& a, bool lds, int cus, hipStream_t stream);
 
 // Congestion view (src/routing/congestion.cuh:60-112)
 __global__ void congestion_view_kernel(DeviceGrid g, const float* capacity, const float* demand,
@@ -253,6 +263,11 @@ struct ScoreArgs {
     const int *extra_off, *extra;   // per net: extra route endpoint pairs, metal-layer cells
     const int *sub_off, *subs;      // per net: its subnets, whose route words are in rs
     RouteStore rs;
+    // phase-two words (after an FLT stage): subnet id with pin_off2[id] >= 0
+    // has its words at routes2 + pin_off2[id] * per_pin2; nullptr: none
+    const int* routes2;
+    const int* pin_off2;
+    int per_pin2;
     const int* dir;                 // [M] layer directions as read by the evaluator
     int* wire;                      // [M * X * Y] unit wire edges per cell
     int* stacked;                   // [M * X * Y] stacked-via demand per cell

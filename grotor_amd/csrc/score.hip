@@ -41,10 +41,22 @@ __device__ __forceinline__ Line decode_line(int p0, int p1, int X, int Y) {
     return Line{p0 / Y % X, p0 % Y, p0 / XY, p1 / Y % X, p1 % Y, p1 / XY};
 }
 
-// Route pairs of subnet id, at most what its route words can hold.
-__device__ __forceinline__ int route_pairs(const ScoreArgs& a, int id, const int* r) {
-    const int cap = (a.rs.pin_acc_num[id + 1] - a.rs.pin_acc_num[id]) * a.rs.per_pin;
-    return max(0, (min(r[0], cap) - 1) / 2);
+// Route words of subnet id: its phase-two words when an FLT stage rerouted
+// it (src/output.hpp:98-106), else its phase-one words; *pairs receives
+// the route pairs, at most what those words can hold.
+__device__ __forceinline__ const int* subnet_words(const ScoreArgs& a, int id, int* pairs) {
+    const int pins = a.rs.pin_acc_num[id + 1] - a.rs.pin_acc_num[id];
+    const int* r;
+    int cap;
+    if (a.pin_off2 && a.pin_off2[id] >= 0) {
+        r = a.routes2 + (size_t)a.pin_off2[id] * a.per_pin2;
+        cap = pins * a.per_pin2;
+    } else {
+        r = a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin;
+        cap = pins * a.rs.per_pin;
+    }
+    *pairs = max(0, (min(r[0], cap) - 1) / 2);
+    return r;
 }
 
 // Calls f(ord, p0, p1) for the lines of net k from lane `lane` with stride
@@ -65,9 +77,8 @@ __device__ __forceinline__ void for_each_line(const ScoreArgs& a, int k, int lan
         if (f(ord + (j - e0) / 2, a.extra[j], a.extra[j + 1])) return;
     ord += (e1 - e0) / 2;
     for (int s = a.sub_off[k]; s < a.sub_off[k + 1]; ++s) {
-        const int id = a.subs[s];
-        const int* r = a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin;
-        const int pairs = route_pairs(a, id, r);
+        int pairs;
+        const int* r = subnet_words(a, a.subs[s], &pairs);
         for (int j = lane; j < pairs; j += stride)
             if (f(ord + j, r[1 + 2 * j] + XY, r[2 + 2 * j] + XY)) return;
         ord += pairs;
@@ -166,8 +177,9 @@ __global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
         int n = (a.pin_off[k + 1] - a.pin_off[k]) + (a.extra_off[k + 1] - a.extra_off[k]) / 2;
         int subs_pairs = 0;
         for (int s = a.sub_off[k] + lane; s < a.sub_off[k + 1]; s += kWave) {
-            const int id = a.subs[s];
-            subs_pairs += route_pairs(a, id, a.rs.routes + (size_t)a.rs.pin_acc_num[id] * a.rs.per_pin);
+            int pairs;
+            subnet_words(a, a.subs[s], &pairs);
+            subs_pairs += pairs;
         }
         for (int off = kWave / 2; off; off >>= 1) subs_pairs += __shfl_xor(subs_pairs, off, kWave);
         n += subs_pairs;

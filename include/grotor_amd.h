@@ -67,7 +67,8 @@ int gr_ctx_create(const gr_grid_desc *grid, gr_ctx **out);
 int gr_ctx_destroy(gr_ctx *ctx);
 
 /* Route storage: ROUTE_PER_PIN = 12 words per pin, net i at word
- * 12 * pin_acc_num[i] (include/router_state.hpp:31, src/preprocess.hpp:1699-1704). */
+ * 12 * pin_acc_num[i] (include/router_state.hpp:31, src/preprocess.hpp:1699-1704).
+ * Replaces the previous storage (cleared) and releases any phase-two storage. */
 int gr_ctx_set_routes(gr_ctx *ctx, int net_count, const int *pin_acc_num);
 
 /* Route words per pin for the next gr_ctx_set_routes: 12 by default
@@ -109,8 +110,11 @@ int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vi
  * them) and overflow_cost cover the L + 1 metal layers.  out receives
  * {wirelength cost, via cost, overflow cost, total cost, nets with a line the
  * evaluator rejects, device ms}; demand, when not NULL, the evaluator's
- * 16-bit demand per metal-layer cell.  The connectivity flood fill of
- * check_connectivity is not part of this score. */
+ * 16-bit demand per metal-layer cell.  Subnets that have phase-two storage
+ * (gr_ctx_set_routes_phase2, offset >= 0) are scored by their phase-two
+ * words, as the reference's output prints them (src/output.hpp:98-106).
+ * The connectivity flood fill of check_connectivity is not part of this
+ * score. */
 int gr_score(gr_ctx *ctx, int nets, const int *pin_off, const int *pins, const int *extra_off, const int *extra,
              const int *sub_off, const int *subs, const int *layer_dir, const double *capacity,
              const double *overflow_cost, double *out, unsigned short *demand);
@@ -210,8 +214,9 @@ int gr_ctx_set_nets_per_wave(gr_ctx *ctx, const int *per_wave);
 int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave, double *ms);
 
 /* Reset the routing state to that of a fresh context: demand, timestamps,
- * statistics and route words cleared, every track marked for refresh (the
- * DAG-throughput benchmark's reset, src/experiments/lshape_benchmark.cuh:84-97). */
+ * statistics and route words cleared, phase-two route storage released,
+ * every track marked for refresh (the DAG-throughput benchmark's reset,
+ * src/experiments/lshape_benchmark.cuh:84-97). */
 int gr_ctx_reset(gr_ctx *ctx);
 
 /* Kernel classes timed by gr_profile_*. */
@@ -302,8 +307,14 @@ int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_
  * (commit K = -1 at the next stamp); then at the next stamp the cost
  * refresh, precompute_cost_edge, the DAG DP and the commit of the
  * phase-two routes.  Stamps start at first_stamp and advance by one per
- * commit.  *elapsed_ms = the synchronized interval (DAG_TIMING). */
-int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, double *elapsed_ms);
+ * commit.  mode: 0 = one launch per step (the reference's loop),
+ *       1 = persistent form: the loop captured once into a HIP graph and
+ *           replayed (capture before the timed interval, reused while the
+ *           uploaded stage, its storage, first stamp and DAGs per wavefront
+ *           stay the same); results equal mode 0;
+ *       other values are rejected with GR_E_ARG.
+ * *elapsed_ms = the synchronized interval (DAG_TIMING). */
+int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, int mode, double *elapsed_ms);
 
 /* DAGs per wavefront of the FLT DP (1, 2, 4 or 8; each DAG gets 64 / n
  * lanes) for batches of fewer than 512 DAGs (per_wave[0]) and of at least

@@ -138,7 +138,7 @@ def test_cli_pipeline_equals_oracle_pipeline(ga, ora, name, tmp_path):
         assert open(mine).read() == open(ref).read()
         res_s = _run(["-cap", cap, "-net", net, "-out", mine, "--batching", backend, "--report-score", "1"] + extra)
         assert res_s.returncode == 0, res_s.stderr[-2000:]
-        score = [line for line in res_s.stdout.splitlines() if line.startswith("SCORE")][0]
+        score = [line for line in res_s.stdout.splitlines() if line.startswith("SCORE wirelength=")][0]
         assert f"wirelength={wl:.0f} vias={vias} rerouted={rerouted}" in score, score
         if rerouted == 0:
             continue
