@@ -457,6 +457,10 @@ class Context:
     def lshape_batch(self, begin, count, stamp):
         _call("gr_lshape_batch", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_int(stamp))
 
+    def lshape_batch_noalloc(self, begin, count, stamp):
+        """gr_lshape_batch without device allocation (library-owned scratch)."""
+        _call("gr_lshape_batch_noalloc", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_int(stamp))
+
     def commit_batch(self, begin, count, stamp, K=1):
         _call("gr_commit_batch", self._h, ctypes.c_int(begin), ctypes.c_int(count), ctypes.c_int(stamp),
               ctypes.c_int(K))

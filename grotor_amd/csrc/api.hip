@@ -8,6 +8,7 @@
 #include <chrono>
 #include <climits>
 #include <memory>
+#include <mutex>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -88,6 +89,33 @@ struct Workspace {
     }
 };
 
+// Scratch of the allocation-free DP entry point (gr_lshape_batch_noalloc):
+// the persistent DP's work-queue counters, then the global DP tables of trees
+// too large for LDS.  One copy per GPU, held until the process ends; host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the current device (the array's name is a host-side stand-in).
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr size_t kNoAllocQueueBytes = size_t(kDpQueues) * kDpQueueStride * sizeof(int);
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+double* noalloc_scratch_address() {
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw GrError(GR_E_HIP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    return g_scratch_ptr[dev];
+}
+
 }  // namespace
 
 struct gr_ctx {
@@ -130,6 +158,8 @@ struct gr_ctx {
     std::vector<int4> desc_h;
     double* gscratch = nullptr;
     size_t gscratch_bytes = 0;
+    double* noalloc_scratch = nullptr;    // this device's `scratch` (gr_lshape_batch_noalloc)
+    std::vector<int> noalloc_slots;       // sized at gr_lshape_upload
     // rip-up
     int* of_sum = nullptr;
     unsigned char* of_flags = nullptr;
@@ -663,6 +693,7 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
         c->cus = prop.multiProcessorCount;
+        c->noalloc_scratch = noalloc_scratch_address();  // with device 0 current
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) == hipSuccess && lds > 0)
             c->lds_limit = std::min(lds, 160 * 1024);
@@ -891,6 +922,7 @@ int gr_lshape_upload(gr_ctx* c, int count, const int* net_ids, const int* node_o
         c->parents = c->mem.alloc<int>(total);
         c->desc = c->mem.alloc<int4>(count);
         c->net_ids_h.assign(net_ids, net_ids + count);
+        c->noalloc_slots.assign(size_t(count), 0);
         c->slots_h.assign(size_t(count), 0);
         c->desc_h.assign(size_t(count), int4{-1, -1, -1, -1});
         upload(c->net_ids, net_ids, sizeof(int) * count, c->stream);
@@ -960,6 +992,45 @@ int gr_lshape_batch(gr_ctx* c, int begin, int count, int stamp) {
         launch_lshape(c, b);
         hip_check(hipStreamSynchronize(c->stream), "gr_lshape_batch");
         check_device_error(c, "gr_lshape_batch");
+    });
+}
+
+int gr_lshape_batch_noalloc(gr_ctx* c, int begin, int count, int stamp) {
+    (void)stamp;
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        require_stage(c, begin, count);
+        if (!c->noalloc_scratch) throw GrError(GR_E_STATE, "no scratch address for this device");
+        if (c->noalloc_slots.size() != size_t(c->stage_count) || c->desc_h.size() != size_t(c->stage_count))
+            throw GrError(GR_E_STATE, "stage topology not uploaded");
+        if (!count) return;
+        std::vector<int>& slots = c->noalloc_slots;
+        int* queue = reinterpret_cast<int*>(c->noalloc_scratch);
+        const DpTables tables{c->noalloc_scratch + kNoAllocQueueBytes / sizeof(double),
+                              kNoAllocScratchBytes - kNoAllocQueueBytes};
+        BatchLaunch b = plan_batch_queue(c, begin, count, slots);
+        const bool persistent = per_wave_for(c, 0, b.nmax[0]) > 0;
+        if (!persistent) {
+            // a tree too large for LDS: the size-class launches, its tables in scratch
+            b = plan_batch(c, begin, count, slots);
+            const size_t need = dp_tables_needed(c, b);
+            if (need > tables.bytes)
+                throw GrError(GR_E_CAPACITY, "batch needs " + std::to_string(need) + " bytes of DP tables; the "
+                              "allocation-free entry point holds " + std::to_string(tables.bytes));
+        }
+        std::copy(slots.begin() + begin, slots.begin() + begin + count, c->slots_h.begin() + begin);
+        upload_desc(c, slots, begin, begin + count);
+        if (persistent) {
+            // the counters keep the last call's values: zero them (a kernel, not a memset)
+            hipLaunchKernelGGL(fill_int_kernel, dim3((kDpQueues * kDpQueueStride + 255) / 256), dim3(256), 0, c->stream,
+                               queue, 0, (long long)kDpQueues * kDpQueueStride);
+            hip_check(hipGetLastError(), "clear work-queue counters");
+            launch_lshape_queue(c, b, queue);
+        } else {
+            launch_lshape(c, b, tables);
+        }
+        hip_check(hipStreamSynchronize(c->stream), "gr_lshape_batch_noalloc");
+        check_device_error(c, "gr_lshape_batch_noalloc");
     });
 }
 

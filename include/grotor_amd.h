@@ -165,6 +165,20 @@ int gr_lshape_upload(gr_ctx *ctx, int count, const int *net_ids, const int *node
  * routes, does not commit (Lshape_route_warp launch, src/routing/lshape.cuh:168). */
 int gr_lshape_batch(gr_ctx *ctx, int begin, int count, int stamp);
 
+/* gr_lshape_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  Same
+ * arguments and results as gr_lshape_batch (route words identical); the
+ * stage topology and route storage must already be in the context.  The DP
+ * runs in its persistent form with its work-queue counters, or, when a tree's
+ * tables do not fit in LDS, in the size-class launches with those tables,
+ * in one library-owned device array per GPU of 32 MiB (4 KiB of counters,
+ * the rest tables), held until the process ends.  A batch whose global DP
+ * tables exceed it fails with GR_E_CAPACITY.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (from several threads or contexts)
+ * race; calls on one context's stream do not. */
+int gr_lshape_batch_noalloc(gr_ctx *ctx, int begin, int count, int stamp);
+
 /* Commit the routes of stage slots [begin, begin + count): wires, then vias
  * (graph::commit_batch_demand, src/routing/demand.cuh:313-319). */
 int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
