@@ -7,8 +7,10 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <atomic>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -199,6 +201,8 @@ struct gr_ctx {
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
     std::vector<int2> f_batch_max;  // per batch: child slots and nodes of its largest DAG
     int* f_queue = nullptr;         // persistent FLT DP: kFltQueues counters per batch
+    char* f_stage = nullptr;        // pinned host staging of the FLT stage records
+    size_t f_stage_cap = 0;
     int f_queue_cap = 0;
     int lds_limit = 65536;          // LDS bytes a workgroup may use
     std::vector<double> f_edge_bytes, f_route_bytes;  // algorithmic bytes per batch launch
@@ -261,6 +265,7 @@ struct gr_ctx {
 
     ~gr_ctx() {
         drop_graph();
+        if (f_stage) (void)hipHostFree(f_stage);
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         if (join_ev) (void)hipEventDestroy(join_ev);
@@ -437,6 +442,39 @@ BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots)
             return c->height_h[u] != c->height_h[v] ? c->height_h[u] > c->height_h[v] : u < v;
         });
     return b;
+}
+
+// f(k) for k in [0, n) on up to 16 host threads (chunks of 256 items);
+// the first exception thrown by any item is rethrown once all have stopped.
+template <class F>
+void parallel_items(int n, F&& f) {
+    const int threads = std::max(1, std::min<int>(16, int(std::thread::hardware_concurrency())));
+    if (n < 4096 || threads == 1) {
+        for (int k = 0; k < n; ++k) f(k);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr error;
+    std::mutex m;
+    auto work = [&] {
+        try {
+            while (!failed.load(std::memory_order_relaxed)) {
+                const int k0 = next.fetch_add(256);
+                if (k0 >= n) return;
+                for (int k = k0; k < std::min(n, k0 + 256); ++k) f(k);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(m);
+            if (!error) error = std::current_exception();
+            failed = true;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
 }
 
 constexpr size_t kLdsLimit = 64 * 1024;
@@ -1497,22 +1535,23 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets out of order");
         const int nets = batch_off[nbatch];
         if (nets && !net_ids) throw GrError(GR_E_ARG, "null net ids");
+        for (int k = 0; k < nets; ++k)
+            if (node_off[k + 1] < node_off[k] || edge_off[k + 1] < edge_off[k])
+                throw GrError(GR_E_ARG, "FLT record offsets out of order");
         const long long total_nodes = node_off[nets], total_edges = edge_off[nets];
         if ((total_nodes && !nodes) || (total_edges && !edges)) throw GrError(GR_E_ARG, "null records");
         if (total_nodes > INT32_MAX / 16 || total_edges > INT32_MAX / 16) throw GrError(GR_E_ARG, "FLT stage too large");
-        std::vector<int4> nrec(static_cast<size_t>(total_nodes)), erec(static_cast<size_t>(total_edges)), desc(nets), sizes(nets);
-        std::vector<int> order(static_cast<size_t>(total_nodes)), doff;
-        std::vector<int> depth, first, count;
-        long long slots_total = 0;
         const int max_coord = (g.L + 1) * g.XY;
-        for (int k = 0; k < nets; ++k) {
+        const int L = g.L;
+
+        // pass 1, per net (threads): node records validated, depth and slot counts
+        std::vector<int> net_depths(nets), net_slots(nets);
+        parallel_items(nets, [&](int k) {
             const int id = net_ids[k];
             if (id < 0 || id >= c->net_count) throw GrError(GR_E_ARG, "net id outside route storage");
             if (c->pin_off2_h[id] < 0) throw GrError(GR_E_ARG, "net without phase-two route storage");
-            const long long n0 = node_off[k], e0 = edge_off[k];
-            const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
-            if (nn < 1 || ne < 0) throw GrError(GR_E_ARG, "FLT net without nodes");
-            depth.assign(size_t(nn), 0);
+            const long long n0 = node_off[k], nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - edge_off[k];
+            if (nn < 1) throw GrError(GR_E_ARG, "FLT net without nodes");
             int slot = 0, roots = 0, prev_end = 0, maxdepth = 0;
             for (long long i = 0; i < nn; ++i) {
                 const int* r = nodes + 4 * (n0 + i);
@@ -1524,114 +1563,154 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                     ++roots;
                     if (r[1] != 0) throw GrError(GR_E_ARG, "FLT node without parents below depth 0");
                 }
-                depth[i] = r[1];
                 maxdepth = std::max(maxdepth, r[1]);
-                nrec[n0 + i] = make_int4(r[0], prev_end, r[3], slot * 8 + r[2]);
                 slot += r[2];
                 prev_end = r[3];
             }
             if (prev_end != ne) throw GrError(GR_E_ARG, "FLT edge count differs from the last parent_end");
             if (roots != 1) throw GrError(GR_E_ARG, "FLT DAG needs exactly one root");
+            net_depths[k] = maxdepth + 1;
+            net_slots[k] = slot;
+        });
+        std::vector<long long> dbase(nets + 1, 0), sbase(nets + 1, 0);
+        for (int k = 0; k < nets; ++k) {
+            dbase[k + 1] = dbase[k] + net_depths[k] + 1;
+            sbase[k + 1] = sbase[k] + net_slots[k];
+        }
+        if (sbase[nets] > INT32_MAX / 16 || dbase[nets] > INT32_MAX) throw GrError(GR_E_ARG, "FLT stage too large");
+
+        // the device workspace layout, staged in pinned host memory so the
+        // whole stage crosses in one copy
+        const size_t ndoff = size_t(dbase[nets]);
+        auto layout = [&](auto& cv, int4*& nrec, int4*& erec, int*& order, int*& doff, int4*& desc, int4*& sizes,
+                          int*& ids) {
+            nrec = cv.template take<int4>(size_t(total_nodes));
+            erec = cv.template take<int4>(size_t(total_edges));
+            order = cv.template take<int>(size_t(total_nodes));
+            doff = cv.template take<int>(ndoff);
+            desc = cv.template take<int4>(nets);
+            sizes = cv.template take<int4>(nets);
+            ids = cv.template take<int>(nets);
+        };
+        Carver dry{nullptr};
+        {
+            int4 *p0, *p1, *p4, *p5;
+            int *p2, *p3, *p6;
+            layout(dry, p0, p1, p2, p3, p4, p5, p6);
+        }
+        const size_t stage_bytes = dry.off;
+        if (stage_bytes > c->f_stage_cap) {
+            if (c->f_stage) (void)hipHostFree(c->f_stage);
+            c->f_stage = nullptr;
+            c->f_stage_cap = 0;
+            hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->f_stage), stage_bytes + stage_bytes / 4), "hipHostMalloc");
+            c->f_stage_cap = stage_bytes + stage_bytes / 4;
+        }
+        int4 *nrec, *erec, *desc, *sizes;
+        int *order, *doff, *ids;
+        Carver host{c->f_stage};
+        layout(host, nrec, erec, order, doff, desc, sizes, ids);
+
+        // pass 2, per net (threads): records, edges validated, nodes by
+        // depth (stable, the reference's radix sort on (net, depth)), bytes
+        std::vector<double> net_eb(nets), net_rb(nets);
+        parallel_items(nets, [&](int k) {
+            const long long n0 = node_off[k], e0 = edge_off[k];
+            const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
+            int slot = 0, prev_end = 0;
+            for (long long i = 0; i < nn; ++i) {
+                const int* r = nodes + 4 * (n0 + i);
+                nrec[n0 + i] = make_int4(r[0], prev_end, r[3], slot * 8 + r[2]);
+                slot += r[2];
+                prev_end = r[3];
+            }
+            double eb = 0;
             for (long long i = 0; i < nn; ++i) {
                 const int4 nr = nrec[n0 + i];
+                const int di = nodes[4 * (n0 + i) + 1];
                 for (int e = nr.y; e < nr.z; ++e) {
                     const int* r = edges + 2 * (e0 + e);
-                    const int par = r[0], s = r[1] % 10;
+                    const int par = r[0], s2 = r[1] % 10;
                     if (par < 0 || par >= nn || r[1] / 10 != i) throw GrError(GR_E_ARG, "FLT edge record inconsistent");
-                    if (depth[par] >= depth[i]) throw GrError(GR_E_ARG, "FLT parent not shallower than its child");
-                    if (s >= (nrec[n0 + par].w & 7)) throw GrError(GR_E_ARG, "FLT edge slot outside its parent");
+                    if (nodes[4 * (n0 + par) + 1] >= di) throw GrError(GR_E_ARG, "FLT parent not shallower than its child");
+                    if (s2 >= nodes[4 * (n0 + par) + 2]) throw GrError(GR_E_ARG, "FLT edge slot outside its parent");
                     const int pc = nrec[n0 + par].x, cc = nr.x;
-                    if (pc / g.Y % g.X != cc / g.Y % g.X && pc % g.Y != cc % g.Y)
-                        throw GrError(GR_E_ARG, "FLT edge not straight");
-                    erec[e0 + e] = make_int4(par, s, pc, cc);
+                    const int pcx = pc / g.Y % g.X, pcy = pc % g.Y, ccx = cc / g.Y % g.X, ccy = cc % g.Y;
+                    if (pcx != ccx && pcy != ccy) throw GrError(GR_E_ARG, "FLT edge not straight");
+                    erec[e0 + e] = make_int4(par, s2, pc, cc);
+                    eb += 12.0 * L * (std::abs(pcx - ccx) + std::abs(pcy - ccy) + 1);
                 }
             }
-            // nodes by depth, stable (the reference's radix sort on (net, depth))
-            const int depths = maxdepth + 1;
-            count.assign(depths + 1, 0);
-            for (long long i = 0; i < nn; ++i) ++count[depth[i] + 1];
-            for (int d = 0; d < depths; ++d) count[d + 1] += count[d];
-            const int dbase = int(doff.size());
-            doff.insert(doff.end(), count.begin(), count.end());
-            first.assign(count.begin(), count.end() - 1);
-            for (long long i = 0; i < nn; ++i) order[n0 + first[depth[i]]++] = int(i);
-            desc[k] = make_int4(int(n0), int(e0), int(slots_total), dbase);
+            const int depths = net_depths[k];
+            int* dk = doff + dbase[k];
+            for (int d = 0; d <= depths; ++d) dk[d] = 0;
+            for (long long i = 0; i < nn; ++i) ++dk[nodes[4 * (n0 + i) + 1] + 1];
+            for (int d = 0; d < depths; ++d) dk[d + 1] += dk[d];
+            // dk[d] is the first position of depth d; place, then restore
+            for (long long i = 0; i < nn; ++i) order[n0 + dk[nodes[4 * (n0 + i) + 1]]++] = int(i);
+            for (int d = depths; d > 0; --d) dk[d] = dk[d - 1];
+            dk[0] = 0;
+            desc[k] = make_int4(int(n0), int(e0), int(sbase[k]), int(dbase[k]));
             sizes[k] = make_int4(int(nn), int(ne), slot, depths);
-            slots_total += slot;
-            if (slots_total > INT32_MAX / 16) throw GrError(GR_E_ARG, "FLT stage too large");
-        }
-        // per batch: stage-global bases and the table sizes
+            ids[k] = net_ids[k];
+            net_eb[k] = eb;
+            // the FLT unit cost of one DAG in the DP: per node 16 + 8 * L
+            // (record, layer costs), per edge 8 (its record)
+            net_rb[k] = nn * (16.0 + 8.0 * L) + 8.0 * ne;
+        });
+
+        // per batch: stage-global bases, the table sizes and the algorithmic
+        // bytes per launch (the layer-change search, 12 * L * (len + 1) per
+        // edge for the presum and via cost gathers along it; the per-edge
+        // change tables are scratch and not counted)
         long long max_nodes = 1, max_edges = 1, max_slots = 1;
         c->f_base.assign(nbatch, make_int4(0, 0, 0, 0));
+        c->f_batch_max.assign(nbatch, make_int2(0, 0));
+        c->f_edge_bytes.assign(nbatch, 0.0);
+        c->f_route_bytes.assign(nbatch, 0.0);
         for (int b = 0; b < nbatch; ++b) {
             const int k0 = batch_off[b], k1 = batch_off[b + 1];
             if (k0 == k1) continue;
             const long long nodes_b = node_off[k1] - node_off[k0], edges_b = edge_off[k1] - edge_off[k0];
-            const long long slots_b = (long long)desc[k1 - 1].z + sizes[k1 - 1].z - desc[k0].z;
+            const long long slots_b = sbase[k1] - sbase[k0];
             max_nodes = std::max(max_nodes, nodes_b);
             max_edges = std::max(max_edges, edges_b);
             max_slots = std::max(max_slots, slots_b);
             c->f_base[b] = make_int4(desc[k0].x, desc[k0].y, desc[k0].z, int(edges_b));
-        }
-        c->f_batch_max.assign(nbatch, make_int2(0, 0));
-        for (int b = 0; b < nbatch; ++b)
-            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k)
+            for (int k = k0; k < k1; ++k) {
                 c->f_batch_max[b] = make_int2(std::max(c->f_batch_max[b].x, sizes[k].z), std::max(c->f_batch_max[b].y, sizes[k].x));
+                c->f_edge_bytes[b] += net_eb[k];
+                c->f_route_bytes[b] += net_rb[k];
+            }
+        }
+        // the DAGs of a wavefront step through their depths together: order
+        // each batch by decreasing depth, then size (results do not depend
+        // on the order of nets inside a batch)
+        {
+            std::vector<int> perm;
+            std::vector<int4> dp, sp;
+            std::vector<int> ip;
+            for (int b = 0; b < nbatch; ++b) {
+                const int k0 = batch_off[b], k1 = batch_off[b + 1];
+                perm.resize(k1 - k0);
+                for (int i = 0; i < k1 - k0; ++i) perm[i] = k0 + i;
+                std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+                    return sizes[x].w != sizes[y].w ? sizes[x].w > sizes[y].w : sizes[x].x > sizes[y].x;
+                });
+                dp.resize(k1 - k0), sp.resize(k1 - k0), ip.resize(k1 - k0);
+                for (int i = 0; i < k1 - k0; ++i) dp[i] = desc[perm[i]], sp[i] = sizes[perm[i]], ip[i] = ids[perm[i]];
+                for (int i = 0; i < k1 - k0; ++i) desc[k0 + i] = dp[i], sizes[k0 + i] = sp[i], ids[k0 + i] = ip[i];
+            }
+        }
         if (c->f_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->f_queue);
             c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltQueues * kFltQueueStride);
             c->f_queue_cap = nbatch;
         }
-        // Algorithmic bytes per launch, the FLT unit cost of one DAG: per
-        // edge 12 * L * (len + 1) for the layer-change search (presum and via
-        // cost gathers along the edge on every layer) and 8 for the edge
-        // record in the DP; per node 16 + 8 * L (record, layer costs).  The
-        // per-edge change tables are scratch and not counted.
-        const int L = g.L;
-        c->f_edge_bytes.assign(nbatch, 0.0);
-        c->f_route_bytes.assign(nbatch, 0.0);
-        for (int b = 0; b < nbatch; ++b) {
-            double eb = 0, rb = 0;
-            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k) {
-                const int4 ds = desc[k], dn = sizes[k];
-                rb += dn.x * (16.0 + 8.0 * L) + 8.0 * dn.y;
-                for (int e = 0; e < dn.y; ++e) {
-                    const int4 er = erec[ds.y + e];
-                    const int len = std::abs(er.z / g.Y % g.X - er.w / g.Y % g.X) + std::abs(er.z % g.Y - er.w % g.Y);
-                    eb += 12.0 * L * (len + 1);
-                }
-            }
-            c->f_edge_bytes[b] = eb;
-            c->f_route_bytes[b] = rb;
-        }
-        // the DAGs of a wavefront step through their depths together: order
-        // each batch by decreasing depth, then size (results do not depend
-        // on the order of nets inside a batch)
-        std::vector<int> ids(net_ids, net_ids + nets);
-        for (int b = 0; b < nbatch; ++b) {
-            const int k0 = batch_off[b], k1 = batch_off[b + 1];
-            std::vector<int> perm(k1 - k0);
-            for (int i = 0; i < k1 - k0; ++i) perm[i] = k0 + i;
-            std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
-                return sizes[x].w != sizes[y].w ? sizes[x].w > sizes[y].w : sizes[x].x > sizes[y].x;
-            });
-            std::vector<int4> dp(k1 - k0), sp(k1 - k0);
-            std::vector<int> ip(k1 - k0);
-            for (int i = 0; i < k1 - k0; ++i) dp[i] = desc[perm[i]], sp[i] = sizes[perm[i]], ip[i] = net_ids[perm[i]];
-            for (int i = 0; i < k1 - k0; ++i) desc[k0 + i] = dp[i], sizes[k0 + i] = sp[i], ids[k0 + i] = ip[i];
-        }
         int4 *d_nrec, *d_erec, *d_desc, *d_sizes;
         int *d_order, *d_doff, *d_ids;
-        c->f_ws.carve([&](auto& cv) {
-            d_nrec = cv.template take<int4>(nrec.size());
-            d_erec = cv.template take<int4>(erec.size());
-            d_order = cv.template take<int>(order.size());
-            d_doff = cv.template take<int>(doff.size());
-            d_desc = cv.template take<int4>(nets);
-            d_sizes = cv.template take<int4>(nets);
-            d_ids = cv.template take<int>(nets);
-        });
+        c->f_ws.carve([&](auto& cv) { layout(cv, d_nrec, d_erec, d_order, d_doff, d_desc, d_sizes, d_ids); });
         FltArgs& f = c->f;
         c->f_tables.carve([&](auto& cv) {
             f.costs = cv.template take<double>(size_t(max_nodes) * kFltStride);
@@ -1641,15 +1720,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             f.cost_edges = cv.template take<double>(size_t(max_edges) * 9 * kFltChangeWidth);
             f.best_change = cv.template take<int>(size_t(max_edges) * 9 * kFltChangeWidth);
         });
-        if (!nrec.empty()) upload(d_nrec, nrec.data(), sizeof(int4) * nrec.size(), c->stream);
-        if (!erec.empty()) upload(d_erec, erec.data(), sizeof(int4) * erec.size(), c->stream);
-        if (!order.empty()) upload(d_order, order.data(), sizeof(int) * order.size(), c->stream);
-        if (!doff.empty()) upload(d_doff, doff.data(), sizeof(int) * doff.size(), c->stream);
-        if (nets) {
-            upload(d_desc, desc.data(), sizeof(int4) * nets, c->stream);
-            upload(d_sizes, sizes.data(), sizeof(int4) * nets, c->stream);
-            upload(d_ids, ids.data(), sizeof(int) * nets, c->stream);
-        }
+        if (stage_bytes) upload(d_nrec, c->f_stage, stage_bytes, c->stream);
         f.g = g;
         f.node_rec = d_nrec;
         f.edge_rec = d_erec;
