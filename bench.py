@@ -17,10 +17,12 @@ regrouping, rip-up classification, DAG construction, plan building) is
 outside both, as it is in the reference.  `value` is the routed nets of both
 stages per second of their summed intervals, in the persistent form.
 
-The same steps run in two forms on the same context, with identical routes
-and evaluator cost checked: plain (one launch per kernel per batch) and
-persistent (the batch loops captured as HIP graphs, the L-shape DP as one
-persistent launch per batch draining a device work queue).
+The same steps run in three forms on the same context, with identical
+routes and evaluator cost checked: plain (the reference's execution: one
+launch per kernel per batch, one net or DAG per wavefront, DP tables in
+global memory), launch (DP tables in LDS, autotuned nets per wavefront) and
+persistent (work-queue DP launches draining each batch's nets tallest
+first, tables in LDS, each stage's batch loop captured in a HIP graph).
 
 Prints one JSON line.  `roofline` reports the kernel class with the largest
 device time in the persistent form, timed with HIP events on the context's
@@ -62,8 +64,9 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--net-scale", type=float, default=1.0)
     ap.add_argument("--mode", type=int, default=2, choices=(0, 1, 2),
-                    help="0 plain launches, 1 the batch loop captured once in a HIP graph, "
-                         "2 persistent DP launches (one per batch, draining a work queue) in the captured loop")
+                    help="single-stage runs: 0 plain form (the reference's execution: one launch per batch, "
+                         "DP tables in global memory), 1 launch form (DP tables in LDS), 2 persistent form "
+                         "(work-queue DP launches, tables in LDS, the batch loop captured in a HIP graph)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-kernel HIP events in the timed steps (roofline from one extra profiled step)")
@@ -318,7 +321,8 @@ def bench_pipeline(args, ga, design, topo):
     geom = (topo["pt_off"], topo["pts"], topo["hs_off"], topo["hsegs"], topo["vs_off"], topo["vsegs"])
     ctx = ga.Context(topo)
     ctx.set_routes(topo["pin_acc_num"])
-    forms = {"plain": {"lshape_mode": 0, "flt_mode": 0}, "persistent": {"lshape_mode": 2, "flt_mode": 1}}
+    forms = {"plain": {"lshape_mode": 0, "flt_mode": 0}, "launch": {"lshape_mode": 1, "flt_mode": 1},
+             "persistent": {"lshape_mode": 2, "flt_mode": 2}}
 
     # autotuners, once, before any step: nets per wavefront of the L-shape DP
     # and DAGs per wavefront of the FLT DP
@@ -375,7 +379,7 @@ def bench_pipeline(args, ga, design, topo):
                     prof=prof)
 
     results = {}
-    for form in ("plain", "persistent"):
+    for form in ("plain", "launch", "persistent"):
         for i in range(args.warmup):
             r = step(form)
             print(f"[bench] {form} warmup {i}: L-shape {r['lb']:.2f} + {r['ld']:.2f} ms, FLT {r['fb']:.2f} + "
@@ -410,10 +414,11 @@ def bench_pipeline(args, ga, design, topo):
                     dag_ms={"lshape": round(per["ld"], 4), "flt_upload": round(per["fu"], 4),
                             "flt": round(per["fd"], 4)})
 
-    pl, pe = rate(results["plain"]["per"]), rate(results["persistent"]["per"])
-    a, b = results["plain"], results["persistent"]
-    identical = bool(np.array_equal(a["routes"], b["routes"]) and np.array_equal(a["routes2"], b["routes2"])
-                     and a["score"] == b["score"] and a["wl"] == b["wl"] and a["vias"] == b["vias"])
+    pl, la, pe = (rate(results[f]["per"]) for f in ("plain", "launch", "persistent"))
+    b = results["persistent"]
+    identical = all(np.array_equal(results[f]["routes"], b["routes"]) and np.array_equal(results[f]["routes2"], b["routes2"])
+                    and results[f]["score"] == b["score"] and results[f]["wl"] == b["wl"] and results[f]["vias"] == b["vias"]
+                    for f in ("plain", "launch"))
     dom = max(range(len(prof["ms"])), key=lambda k: prof["ms"][k])
     dom_ms, dom_launches = prof["ms"][dom], prof["launches"][dom]
     achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
@@ -451,9 +456,20 @@ def bench_pipeline(args, ga, design, topo):
                    "dag_nodes_flt": fs["nodes"], "dag_edges_flt": fs["edges"],
                    "persistent": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in pe.items()},
                    "plain": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in pl.items()},
+                   "launch": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in la.items()},
                    "persistent_over_plain": {"pipeline": round(pl["ms"] / pe["ms"], 4),
                                              "lshape": round(pl["lshape_ms"] / pe["lshape_ms"], 4),
                                              "flt": round(pl["flt_ms"] / pe["flt_ms"], 4)},
+                   "persistent_over_launch": {"pipeline": round(la["ms"] / pe["ms"], 4),
+                                              "lshape": round(la["lshape_ms"] / pe["lshape_ms"], 4),
+                                              "flt": round(la["flt_ms"] / pe["flt_ms"], 4)},
+                   "forms": {"plain": "the reference's execution: per batch one launch per kernel, one net or DAG "
+                                      "per wavefront, DP tables in global memory",
+                             "launch": "per batch one launch per kernel, DP tables in LDS, autotuned nets (DAGs) "
+                                       "per wavefront",
+                             "persistent": "per batch one work-queue DP launch draining the nets (DAGs) tallest "
+                                           "first, DP tables in LDS, autotuned nets (DAGs) per wavefront, each "
+                                           "stage's batch loop captured in a HIP graph"},
                    "forms_identical": identical,
                    "evaluator_cost": b["score"], "evaluator_rejected_nets": b["rejected"],
                    "total_wirelength": b["wl"], "total_vias": b["vias"],

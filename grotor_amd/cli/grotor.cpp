@@ -5,9 +5,9 @@
 //          [--sparsity N] [--lut-dir DIR] [--profile-stages 0|1] [--report-score 0|1]
 //          [--stop-after lshape|routing]
 // plus this build's own options:
-//          [--dag-mode 0|1|2]      batch loops: plain launches (0); L-shape captured
-//                                  graph (1) or persistent DP in the captured loop (2),
-//                                  FLT persistent form for 1 and 2 (default 2)
+//          [--dag-mode 0|1|2]      batch loops of both stages: plain form (0, the
+//                                  reference's execution), launch form with LDS
+//                                  tables (1), persistent form (2, the default)
 //          [--threads N]           host threads of the front end (default 8)
 // and the same stages (grotor::route, src/pipeline.hpp:14-157): L-shape stage
 // over every subnet, rip-up of the nets crossing more than 10 overflowed
@@ -287,7 +287,7 @@ int run(const Options& o) {
                              plan.edges),
                "FLT upload");
             double ms = 0;
-            ok(gr_flt_run(ctx, nb + 1, 1, o.dag_mode ? 1 : 0, &ms), "FLT stage");
+            ok(gr_flt_run(ctx, nb + 1, 1, o.dag_mode, &ms), "FLT stage");
             print_dag_timing(ctx, "detour", fs.nets, plan.batches);
             t0 = Clock::now();
             routes2.resize(100LL * fs.pins2 + 1);

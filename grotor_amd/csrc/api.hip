@@ -574,6 +574,50 @@ BatchLaunch plan_batch_queue(gr_ctx* c, int begin, int count, std::vector<int>& 
     return b;
 }
 
+// Plain form of one batch (the reference's execution: one net per
+// wavefront in batch order, every net's DP tables in global memory, one
+// launch): class 0 carries the batch, its node capacity the largest tree.
+BatchLaunch plan_batch_plain(gr_ctx* c, int begin, int count, std::vector<int>& slots) {
+    BatchLaunch b{begin, {count, 0}, {1, 1}};
+    for (int s = begin; s < begin + count; ++s) {
+        slots[s] = s;
+        b.nmax[0] = std::max(b.nmax[0], c->node_off_h[s + 1] - c->node_off_h[s]);
+    }
+    return b;
+}
+
+size_t plain_tables_needed(const gr_ctx* c, const BatchLaunch& b) {
+    return b.count[0] ? ((lshape_scratch_bytes(c->g.L, b.nmax[0]) + 7) / 8) * 8 * size_t(b.count[0]) : 0;
+}
+
+void launch_lshape_plain(gr_ctx* c, const BatchLaunch& b) {
+    if (!b.count[0]) return;
+    LshapeArgs a;
+    a.g = c->g;
+    a.vcost = c->vcost;
+    a.presum = c->presum;
+    a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+    a.desc = c->desc;
+    a.nodes = c->nodes;
+    a.parents = c->parents;
+    a.lvl = c->lvl;
+    a.link = c->link;
+    a.gscratch = c->gscratch;
+    a.error = c->error;
+    a.queue = nullptr;
+    a.cus = c->cus;
+    a.begin = b.begin;
+    a.count = b.count[0];
+    a.nmax = b.nmax[0];
+    a.per_wave = 1;
+    if (plain_tables_needed(c, b) > c->gscratch_bytes) throw GrError(GR_E_STATE, "DP scratch not reserved");
+    double alg = 0;
+    if (c->profiling)
+        for (int i = a.begin; i < a.begin + a.count; ++i) alg += c->slot_dp_bytes[c->slots_h[i]];
+    timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, a.count, false, c->stream); });
+    hip_check(hipGetLastError(), "lshape_dp plain kernel");
+}
+
 // The batch's nets drained by the persistent DP kernel from its queue
 // counters (zero beforehand); trees too large for LDS take the launch form.
 void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
@@ -1085,17 +1129,22 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
     return guarded([&] {
         if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
         if (mode < 0 || mode > 2)
-            throw GrError(GR_E_ARG, "execution mode not available (0 = launches, 1 = HIP graph, 2 = persistent + graph)");
+            throw GrError(GR_E_ARG, "execution mode not available (0 = plain, 1 = launches, 2 = persistent + graph)");
         require_stage(c, 0, batch_off[nbatch]);
         if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
         std::vector<int> slots(c->stage_count);
         std::vector<BatchLaunch> plan(nbatch);
+        size_t plain_need = 0;
         for (int b = 0; b < nbatch; ++b) {
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
-            plan[b] = mode == 2 ? plan_batch_queue(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots)
-                                : plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+            const int begin = batch_off[b], count = batch_off[b + 1] - batch_off[b];
+            plan[b] = mode == 2 ? plan_batch_queue(c, begin, count, slots)
+                      : mode == 1 ? plan_batch(c, begin, count, slots)
+                                  : plan_batch_plain(c, begin, count, slots);
+            if (mode == 0) plain_need = std::max(plain_need, plain_tables_needed(c, plan[b]));
         }
-        reserve_dp_scratch(c, plan);
+        if (mode == 0) ensure_gscratch(c, plain_need);
+        else reserve_dp_scratch(c, plan);
         if (mode == 2 && c->dp_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->dp_queue);
@@ -1109,13 +1158,14 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
             for (int b = 0; b < nbatch; ++b) {
                 launch_refresh(c, 0);
                 if (mode == 2) launch_lshape_queue(c, plan[b], c->dp_queue + size_t(b) * kDpQueues * kDpQueueStride);
-                else launch_lshape(c, plan[b]);
+                else if (mode == 1) launch_lshape(c, plan[b]);
+                else launch_lshape_plain(c, plan[b]);
                 launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
             }
         };
         // a captured graph cannot carry profiling events: profiled runs of
-        // modes 1 and 2 issue the same launches directly
-        const bool graph = mode >= 1 && !c->profiling;
+        // mode 2 issue the same launches directly
+        const bool graph = mode == 2 && !c->profiling;
         if (graph) {
             std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
@@ -1744,14 +1794,15 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
 int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, double* elapsed_ms) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
-        if (mode < 0 || mode > 1) throw GrError(GR_E_ARG, "FLT execution mode not available (0 = launches, 1 = persistent + graph)");
+        if (mode < 0 || mode > 2)
+            throw GrError(GR_E_ARG, "FLT execution mode not available (0 = plain, 1 = launches, 2 = persistent + graph)");
         if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
         bind_flt_storage(c);
         const RouteStore rs2{c->routes2, c->pin_off2, c->route2_per_pin};
         auto enqueue = [&] {
             int stamp = first_stamp;
             const long long qwords = (long long)c->f_batches * kFltQueues * kFltQueueStride;
-            if (mode == 1 && qwords)  // a kernel, not a memset: the loop may be captured
+            if (mode == 2 && qwords)  // a kernel, not a memset: the loop may be captured
                 hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((qwords + 255) / 256)), dim3(256), 0, c->stream,
                                    c->f_queue, 0, qwords);
             for (int b = 0; b < c->f_batches; ++b) {
@@ -1767,17 +1818,24 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, doubl
                     if (threads)
                         hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
                 });
-                int* queue = mode == 1 ? c->f_queue + size_t(b) * kFltQueues * kFltQueueStride : nullptr;
+                int* queue = mode == 2 ? c->f_queue + size_t(b) * kFltQueues * kFltQueueStride : nullptr;
                 timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
-                    launch_flt_dp(c, a, b, c->f_per_wave[count >= kFltWideBatch], queue);
+                    if (mode == 0) {
+                        // plain: one DAG per wavefront, the batch tables in global memory
+                        FltArgs p = a;
+                        p.per_wave = 1;
+                        launch_flt_route(p, false, c->cus, c->stream);
+                    } else {
+                        launch_flt_dp(c, a, b, c->f_per_wave[count >= kFltWideBatch], queue);
+                    }
                 });
                 hip_check(hipGetLastError(), "FLT kernels");
                 launch_commit(c, 0, count, s, 1, ids, &rs2);
             }
         };
         // a captured graph cannot carry profiling events: profiled runs of
-        // mode 1 issue the same launches directly
-        const bool graph = mode == 1 && !c->profiling;
+        // mode 2 issue the same launches directly
+        const bool graph = mode == 2 && !c->profiling;
         if (graph) {
             const std::vector<int> key{c->flt_gen, first_stamp, remove_phase_one ? 1 : 0, c->f_per_wave[0],
                                        c->f_per_wave[1]};

@@ -186,16 +186,20 @@ int gr_commit_batch(gr_ctx *ctx, int begin, int count, int stamp, int K);
 /* The whole L-shape batch loop (src/routing/lshape.cuh:148-180): for each
  * batch b, stamp = first_stamp + b, refresh costs, solve, commit.
  * batch_off has nbatch + 1 entries over the uploaded stage slots.
- * mode: 0 = one launch per step (the reference's loop),
- *       1 = the same launches captured once into a HIP graph and replayed
- *           (the capture is reused while the stage topology, batch offsets,
- *           first stamp, nets per wavefront and mode stay the same; capture
- *           happens before the timed interval),
- *       2 = persistent form, captured like mode 1: per batch one DP launch
- *           whose resident workgroups drain all of the batch's nets from a
- *           device work queue, tallest trees first (no reference
- *           counterpart; results equal modes 0 and 1),
- *       other values are rejected with GR_E_ARG.
+ * mode: 0 = plain form, the reference's execution (src/routing/lshape.cuh:148-180):
+ *           per batch one DP launch, one net per wavefront in batch order,
+ *           every net's DP tables in global memory;
+ *       1 = launch form: per batch the nets in two tree-size classes (the
+ *           large trees on a second stream), tables in LDS, nets per
+ *           wavefront from gr_lshape_tune;
+ *       2 = persistent form: per batch one DP launch whose resident
+ *           workgroups drain the batch's nets from a device work queue,
+ *           tallest trees first, tables in LDS; the whole loop captured once
+ *           into a HIP graph and replayed (the capture is reused while the
+ *           stage topology, batch offsets, first stamp and nets per wavefront
+ *           stay the same; capture happens before the timed interval);
+ *       other values are rejected with GR_E_ARG.  Results are identical in
+ *       every mode.
  * *elapsed_ms receives the synchronized interval (RoutingDagTimer,
  * include/router_state.hpp:238-265). */
 int gr_lshape_run(gr_ctx *ctx, int nbatch, const int *batch_off, int first_stamp,
@@ -321,12 +325,19 @@ int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_
  * (commit K = -1 at the next stamp); then at the next stamp the cost
  * refresh, precompute_cost_edge, the DAG DP and the commit of the
  * phase-two routes.  Stamps start at first_stamp and advance by one per
- * commit.  mode: 0 = one launch per step (the reference's loop),
- *       1 = persistent form: the loop captured once into a HIP graph and
- *           replayed (capture before the timed interval, reused while the
- *           uploaded stage, its storage, first stamp and DAGs per wavefront
- *           stay the same); results equal mode 0;
- *       other values are rejected with GR_E_ARG.
+ * commit.  mode: 0 = plain form, the reference's execution
+ *           (route_flt_block, src/routing/flt.cuh:158): per batch one DP
+ *           launch, one DAG per wavefront, the DP tables in global memory;
+ *       1 = launch form: each DAG's tables in LDS (DAGs whose tables do not
+ *           fit keep the global tables), DAGs per wavefront from gr_flt_tune;
+ *       2 = persistent form: as 1, with per batch one DP launch whose
+ *           resident workgroups drain the batch's DAGs from a device work
+ *           queue, deepest first, and the whole loop captured once into a
+ *           HIP graph and replayed (capture before the timed interval, reused
+ *           while the uploaded stage, its storage, first stamp and DAGs per
+ *           wavefront stay the same);
+ *       other values are rejected with GR_E_ARG.  Results are identical in
+ *       every mode.
  * *elapsed_ms = the synchronized interval (DAG_TIMING). */
 int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, int mode, double *elapsed_ms);
 

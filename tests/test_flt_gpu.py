@@ -24,8 +24,8 @@ def _gpu_phase_one(ga, I):
     return ctx
 
 
-@pytest.mark.parametrize("config,seed,scale,side,mode", [(3, 7, 0.01, 64, 0), (3, 7, 0.01, 64, 1), (1, 0, 0.5, 0, 0),
-                                                         (1, 0, 0.5, 0, 1)])
+@pytest.mark.parametrize("config,seed,scale,side,mode", [(3, 7, 0.01, 64, 0), (3, 7, 0.01, 64, 1), (3, 7, 0.01, 64, 2),
+                                                         (1, 0, 0.5, 0, 0), (1, 0, 0.5, 0, 2)])
 def test_flt_stage_matches_oracle(ga, ora, config, seed, scale, side, mode):
     I = flt_instance(ga, ora, config, seed, scale, side)
     t, fs, plan = I["t"], I["fs"], I["plan"]
@@ -43,7 +43,7 @@ def test_flt_stage_matches_oracle(ga, ora, config, seed, scale, side, mode):
     first = len(I["sched"]["batch_off"])
     ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
     ctx.flt_upload(plan)
-    if mode == 1:
+    if mode == 2:
         # new phase-two storage after the upload: the run binds the current buffers
         ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
     ms = ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=mode)
@@ -75,7 +75,7 @@ def test_flt_dags_per_wavefront_do_not_change_results(ga, ora, per_wave):
         assert all(p in (1, 2, 4, 8) for p in chosen) and max(max(r) for r in ms) > 0
     else:
         ctx.set_flt_per_wave(per_wave)
-    ctx.flt_run(first_stamp=first, remove_phase_one=True)
+    ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=1)
     r2, dem, ts, wl, vias, rc, _ = I["g"].flt_stage(plan, t["pin_acc_num"], fs["pin_off2"], fs["pins2"], I["demand"],
                                                      I["ts"], I["routes"], first, wl=I["wl"], vias=I["vias"])
     assert rc == 0
@@ -156,7 +156,7 @@ def test_flt_upload_rejects_malformed_dags(ga, ora):
 
 
 def test_flt_graph_mode_replays(ga, ora):
-    """Mode 1 replays the captured FLT loop: a second stage on the same
+    """Mode 2 replays the captured FLT loop: a second stage on the same
     context (phase one rebuilt, storage replaced) and a replay without
     changes equal the oracle."""
     I = flt_instance(ga, ora, 3, 7, 0.01, 64)
@@ -183,7 +183,7 @@ def test_flt_graph_mode_replays(ga, ora):
             ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
         if rep == 2:
             ctx.flt_upload(plan)
-        ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=1)
+        ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=2)
         assert np.array_equal(ctx.routes2(), r2), rep
         assert np.array_equal(_bits(ctx.demand()), _bits(dem)), rep
         assert np.array_equal(ctx.timestamps(), ts), rep
@@ -247,7 +247,7 @@ def _crafted_flt_plan(X, Y, L):
                 edges=np.array([r for _, ed in dags for r in ed], np.int32).reshape(-1))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_flt_wide_nodes_and_long_chains_match_oracle(ga, ora, mode):
     """Nodes with five and six child slots and a DAG too large for LDS,
     against the oracle, in both execution forms (each DAG's tables in LDS,

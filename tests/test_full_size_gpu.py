@@ -88,7 +88,7 @@ def test_full_size_batching_matches_oracle(ga, ora, full):
 
 
 @pytest.mark.timeout(300)
-@pytest.mark.parametrize("mode", [2, 0])
+@pytest.mark.parametrize("mode", [2, 1, 0])
 def test_full_size_lshape_stage_matches_oracle(ga, ora, full, mode):
     """The whole L-shape stage (refresh, DP, commit per batch) in the
     persistent captured form and in plain launches: route words, demand
@@ -144,7 +144,7 @@ def test_full_size_flt_stage_matches_oracle(ga, ora, full):
     assert nb == fnb and np.array_equal(color, fcol)
     ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
     ctx.flt_upload(plan)
-    ctx.flt_run(first_stamp=first, remove_phase_one=True)
+    ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=2)
     r2, dem, ts, wl, vias, rc, _ = g.flt_stage(plan, t["pin_acc_num"], fs["pin_off2"], fs["pins2"], demand_o, ts_o,
                                                routes_o, first, wl=wl_o, vias=vias_o, threads=THREADS)
     assert rc == 0

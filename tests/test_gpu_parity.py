@@ -128,7 +128,7 @@ def test_nets_per_wavefront_do_not_change_results(ga, ora, per_wave):
     ctx.set_routes(t["pin_acc_num"])
     ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
     ctx.set_nets_per_wave(per_wave)
-    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=0)
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
     assert np.array_equal(ctx.routes(), routes_o)
     assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
     assert np.array_equal(ctx.timestamps(), ts_o)
@@ -138,7 +138,7 @@ def test_nets_per_wavefront_do_not_change_results(ga, ora, per_wave):
         assert all(p in (1, 2, 4, 8) for p in chosen)
         assert any(v > 0 for row in ms for v in row)
         ctx.reset()
-        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
         assert np.array_equal(ctx.routes(), routes_o)
         assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
     with pytest.raises(ga.GrError):
@@ -147,7 +147,7 @@ def test_nets_per_wavefront_do_not_change_results(ga, ora, per_wave):
 
 
 def test_graph_mode_replays_match_oracle(ga, ora):
-    """Mode 1 (captured HIP graph): every replay, with resets and eager work
+    """Mode 2 (captured HIP graph): every replay, with resets and eager work
     between replays, equals the oracle; a changed schedule recaptures."""
     _, t = topology(6, 5)
     print("graph mode config=6 seed=5", flush=True)
@@ -161,7 +161,7 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     for rep in range(3):
         ctx.reset()
         ctx.batch_plan(*geom)  # eager device work between replays
-        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
         assert np.array_equal(ctx.routes(), routes_o), rep
         assert np.array_equal(_bits(ctx.demand()), _bits(demand_o)), rep
         assert np.array_equal(ctx.timestamps(), ts_o), rep
@@ -170,7 +170,7 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     # schedule: the slot permutation must be uploaded again, not assumed
     ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
     ctx.reset()
-    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
     assert np.array_equal(ctx.routes(), routes_o)
     assert np.array_equal(_bits(ctx.demand()), _bits(demand_o))
     # a different schedule (first half of the batches only)
@@ -178,7 +178,7 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     half["batch_off"] = sched["batch_off"][:len(sched["batch_off"]) // 2 + 1]
     routes_h, demand_h, _, _, _ = ora.Grid(t).lshape_stage(half, t["pin_acc_num"], threads=8)
     ctx.reset()
-    ctx.lshape_run(half["batch_off"], first_stamp=1, mode=1)
+    ctx.lshape_run(half["batch_off"], first_stamp=1, mode=2)
     assert np.array_equal(ctx.routes(), routes_h)
     assert np.array_equal(_bits(ctx.demand()), _bits(demand_h))
     with pytest.raises(ga.GrError):
@@ -201,7 +201,7 @@ def test_persistent_mode_matches_oracle(ga, ora, per_wave):
     ctx.set_routes(t["pin_acc_num"])
     ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
     ctx.set_nets_per_wave(per_wave)
-    for rep, mode in enumerate((2, 2, 1, 2)):
+    for rep, mode in enumerate((2, 2, 1, 0, 2)):
         ctx.reset()
         ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
         assert np.array_equal(ctx.routes(), routes_o), (rep, mode)
@@ -232,7 +232,7 @@ def _large_tree_stage(t, sizes, seed):
                 nodes=np.concatenate(nodes), parents=np.concatenate(parents))
 
 
-@pytest.mark.parametrize("mode", [0, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_trees_beyond_lds_match_oracle(ga, ora, mode):
     """Trees whose DP tables do not fit in LDS (300 and 240 nodes on four
     layers) take the global-scratch kernel, in the launch and persistent
@@ -390,7 +390,7 @@ def test_graph_replay_after_storage_changes(ga, ora):
     ctx = ga.Context(t)
     ctx.set_routes(t["pin_acc_num"])
     ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
-    for mode in (1, 2):
+    for mode in (2, 1):
         ctx.reset()
         ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
         check(ctx, f"first mode {mode}")
@@ -407,6 +407,6 @@ def test_graph_replay_after_storage_changes(ga, ora):
     # a stride change applies to the next route storage only
     ga._call("gr_ctx_set_route_stride", ctx._h, ga.ctypes.c_int(256))
     ctx.reset()
-    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=1)
+    ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
     check(ctx, "stride pending")
     ctx.close()
