@@ -593,7 +593,7 @@ def bench_flt(args, ga, design, topo):
         color, nb = ctx.batch_colors()
         sched = ga.execution_schedule(topo, color, nb)
         ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
-        ctx.lshape_run(sched["batch_off"], first_stamp=1)
+        ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
         return nb
 
     nb1 = phase_one()
@@ -630,7 +630,7 @@ def bench_flt(args, ga, design, topo):
         plan = design.flt_plan(color, nb)  # flt::Plans, outside the reference's DAG interval too
         ctx.flt_upload(plan)
         upload_ms = (time.perf_counter() - u0) * 1e3
-        dag_ms = ctx.flt_run(first_stamp=nb1 + 1, remove_phase_one=True)
+        dag_ms = ctx.flt_run(first_stamp=nb1 + 1, remove_phase_one=True, mode=args.mode)
         prof = ctx.profile_end() if profile_kinds is not False else None
         return batch_ms, dag_ms, upload_ms, plan, prof
 

@@ -199,7 +199,7 @@ struct gr_ctx {
     std::vector<int> f_batch_off;
     std::vector<int> f_ids_h;  // subnet ids of the uploaded FLT stage, execution order
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
-    std::vector<int2> f_batch_max;  // per batch: child slots and nodes of its largest DAG
+    std::vector<FltLdsShape> f_batch_max;  // per batch: the largest child slots, nodes, edges, depth width, depths
     int* f_queue = nullptr;         // persistent FLT DP: kFltQueues counters per batch
     char* f_stage = nullptr;        // pinned host staging of the FLT stage records
     size_t f_stage_cap = 0;
@@ -701,7 +701,8 @@ FltArgs flt_batch_args(gr_ctx* c, int b) {
     a.slot_base = c->f_base[b].z;
     a.edges = c->f_base[b].w;
     a.queue = nullptr;
-    a.lds_dag_bytes = a.lds_dag_slots = a.lds_dag_nodes = 0;
+    a.lds_dag_bytes = 0;
+    a.lds_shape = c->f_batch_max[b];
     return a;
 }
 
@@ -710,15 +711,15 @@ FltArgs flt_batch_args(gr_ctx* c, int b) {
 // workgroup's share fits), the persistent form draining `queue` when given;
 // otherwise the batch tables in global memory.
 void launch_flt_dp(gr_ctx* c, FltArgs a, int b, int per_wave, int* queue) {
-    const int2 mx = c->f_batch_max[b];
-    const size_t bytes = flt_lds_bytes(c->g.L, mx.x, mx.y);
+    const FltLdsShape& mx = c->f_batch_max[b];
+    const size_t bytes = flt_lds_bytes(c->g.L, mx.slots, mx.nodes, mx.edges, mx.dnodes, mx.depths);
+    const size_t avail = size_t(c->lds_limit) - 256;  // the kernel's static LDS stays beside the tables
     int p = per_wave;
-    while (p > 1 && bytes * size_t(p) > size_t(c->lds_limit)) p /= 2;
+    while (p > 1 && bytes * size_t(p) > avail) p /= 2;
     a.per_wave = p;
-    if (bytes <= size_t(c->lds_limit)) {
+    if (bytes <= avail) {
         a.lds_dag_bytes = int(bytes);
-        a.lds_dag_slots = mx.x;
-        a.lds_dag_nodes = mx.y;
+        a.lds_shape = mx;
         a.queue = queue;
         launch_flt_route(a, true, c->cus, c->stream);
     } else {
@@ -1632,21 +1633,25 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // the device workspace layout, staged in pinned host memory so the
         // whole stage crosses in one copy
         const size_t ndoff = size_t(dbase[nets]);
-        auto layout = [&](auto& cv, int4*& nrec, int4*& erec, int*& order, int*& doff, int4*& desc, int4*& sizes,
-                          int*& ids) {
-            nrec = cv.template take<int4>(size_t(total_nodes));
-            erec = cv.template take<int4>(size_t(total_edges));
-            order = cv.template take<int>(size_t(total_nodes));
-            doff = cv.template take<int>(ndoff);
-            desc = cv.template take<int4>(nets);
-            sizes = cv.template take<int4>(nets);
-            ids = cv.template take<int>(nets);
+        struct Stage {
+            int4 *nrec, *erec, *desc, *sizes;
+            int *order, *doff, *eorder, *edoff, *ids;
+        };
+        auto layout = [&](auto& cv, Stage& st) {
+            st.nrec = cv.template take<int4>(size_t(total_nodes));
+            st.erec = cv.template take<int4>(size_t(total_edges));
+            st.order = cv.template take<int>(size_t(total_nodes));
+            st.doff = cv.template take<int>(ndoff);
+            st.eorder = cv.template take<int>(size_t(total_edges));
+            st.edoff = cv.template take<int>(ndoff);
+            st.desc = cv.template take<int4>(nets);
+            st.sizes = cv.template take<int4>(nets);
+            st.ids = cv.template take<int>(nets);
         };
         Carver dry{nullptr};
         {
-            int4 *p0, *p1, *p4, *p5;
-            int *p2, *p3, *p6;
-            layout(dry, p0, p1, p2, p3, p4, p5, p6);
+            Stage unused{};
+            layout(dry, unused);
         }
         const size_t stage_bytes = dry.off;
         if (stage_bytes > c->f_stage_cap) {
@@ -1656,14 +1661,16 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->f_stage), stage_bytes + stage_bytes / 4), "hipHostMalloc");
             c->f_stage_cap = stage_bytes + stage_bytes / 4;
         }
-        int4 *nrec, *erec, *desc, *sizes;
-        int *order, *doff, *ids;
         Carver host{c->f_stage};
-        layout(host, nrec, erec, order, doff, desc, sizes, ids);
+        Stage hs{};
+        layout(host, hs);
+        int4 *nrec = hs.nrec, *erec = hs.erec, *desc = hs.desc, *sizes = hs.sizes;
+        int *order = hs.order, *doff = hs.doff, *ids = hs.ids;
 
         // pass 2, per net (threads): records, edges validated, nodes by
         // depth (stable, the reference's radix sort on (net, depth)), bytes
         std::vector<double> net_eb(nets), net_rb(nets);
+        std::vector<int> net_dnodes(nets);
         parallel_items(nets, [&](int k) {
             const long long n0 = node_off[k], e0 = edge_off[k];
             const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
@@ -1700,6 +1707,19 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             for (long long i = 0; i < nn; ++i) order[n0 + dk[nodes[4 * (n0 + i) + 1]]++] = int(i);
             for (int d = depths; d > 0; --d) dk[d] = dk[d - 1];
             dk[0] = 0;
+            // edges by the depth of their child, in node order within a depth
+            int* ek = hs.edoff + dbase[k];
+            int dn_max = 0, ne_at = 0;
+            for (int d = 0; d < depths; ++d) {
+                ek[d] = ne_at;
+                dn_max = std::max(dn_max, dk[d + 1] - dk[d]);
+                for (int q = dk[d]; q < dk[d + 1]; ++q) {
+                    const int4 nr = nrec[n0 + order[n0 + q]];
+                    for (int e = nr.y; e < nr.z; ++e) hs.eorder[e0 + ne_at++] = e;
+                }
+            }
+            ek[depths] = ne_at;
+            net_dnodes[k] = dn_max;
             desc[k] = make_int4(int(n0), int(e0), int(sbase[k]), int(dbase[k]));
             sizes[k] = make_int4(int(nn), int(ne), slot, depths);
             ids[k] = net_ids[k];
@@ -1715,7 +1735,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // change tables are scratch and not counted)
         long long max_nodes = 1, max_edges = 1, max_slots = 1;
         c->f_base.assign(nbatch, make_int4(0, 0, 0, 0));
-        c->f_batch_max.assign(nbatch, make_int2(0, 0));
+        c->f_batch_max.assign(nbatch, FltLdsShape{0, 0, 0, 0, 0});
         c->f_edge_bytes.assign(nbatch, 0.0);
         c->f_route_bytes.assign(nbatch, 0.0);
         for (int b = 0; b < nbatch; ++b) {
@@ -1728,7 +1748,12 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             max_slots = std::max(max_slots, slots_b);
             c->f_base[b] = make_int4(desc[k0].x, desc[k0].y, desc[k0].z, int(edges_b));
             for (int k = k0; k < k1; ++k) {
-                c->f_batch_max[b] = make_int2(std::max(c->f_batch_max[b].x, sizes[k].z), std::max(c->f_batch_max[b].y, sizes[k].x));
+                FltLdsShape& m = c->f_batch_max[b];
+                m.slots = std::max(m.slots, sizes[k].z);
+                m.nodes = std::max(m.nodes, sizes[k].x);
+                m.edges = std::max(m.edges, sizes[k].y);
+                m.dnodes = std::max(m.dnodes, net_dnodes[k]);
+                m.depths = std::max(m.depths, sizes[k].w);
                 c->f_edge_bytes[b] += net_eb[k];
                 c->f_route_bytes[b] += net_rb[k];
             }
@@ -1758,9 +1783,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltQueues * kFltQueueStride);
             c->f_queue_cap = nbatch;
         }
-        int4 *d_nrec, *d_erec, *d_desc, *d_sizes;
-        int *d_order, *d_doff, *d_ids;
-        c->f_ws.carve([&](auto& cv) { layout(cv, d_nrec, d_erec, d_order, d_doff, d_desc, d_sizes, d_ids); });
+        Stage dev{};
+        c->f_ws.carve([&](auto& cv) { layout(cv, dev); });
         FltArgs& f = c->f;
         c->f_tables.carve([&](auto& cv) {
             f.costs = cv.template take<double>(size_t(max_nodes) * kFltStride);
@@ -1770,15 +1794,17 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             f.cost_edges = cv.template take<double>(size_t(max_edges) * 9 * kFltChangeWidth);
             f.best_change = cv.template take<int>(size_t(max_edges) * 9 * kFltChangeWidth);
         });
-        if (stage_bytes) upload(d_nrec, c->f_stage, stage_bytes, c->stream);
+        if (stage_bytes) upload(dev.nrec, c->f_stage, stage_bytes, c->stream);
         f.g = g;
-        f.node_rec = d_nrec;
-        f.edge_rec = d_erec;
-        f.order = d_order;
-        f.depth_off = d_doff;
-        f.desc = d_desc;
-        f.sizes = d_sizes;
-        f.net_ids = d_ids;
+        f.node_rec = dev.nrec;
+        f.edge_rec = dev.erec;
+        f.order = dev.order;
+        f.depth_off = dev.doff;
+        f.eorder = dev.eorder;
+        f.edepth_off = dev.edoff;
+        f.desc = dev.desc;
+        f.sizes = dev.sizes;
+        f.net_ids = dev.ids;
         f.routes = RouteStore{c->routes2, c->pin_off2, c->route2_per_pin};
         f.pin_acc_num = c->pin_acc_num;
         f.error = c->error;

@@ -105,11 +105,18 @@ struct LdsTables {
     static constexpr bool kReload = true;  // slot keys reloaded from LDS instead of held in registers
     unsigned long long* packed;  // [row * L + li - 1]
     unsigned* best;              // best paths, code * kS + layer (below 2^32: codes are node * 100 + choice)
-    int* layer;                  // [node]
     double* rootc;               // [kS], the root's layer costs
+    int4* nrec;                  // the DAG's node records
+    int4* erec;                  // edge records {parent, slot, parent coordinate, child node}
+    int* eorder;                 // edges by the depth of their child (a.eorder)
+    int* order;                  // nodes by depth (a.order)
+    int* doff;                   // [depths + 1] offsets of the depths in order
+    int* edoff;                  // [depths + 1] offsets of the depths in eorder
+    double* cvs;                 // [position in depth * L + li - 1]: layer costs of a depth's solved nodes
+    unsigned* at;                // [position in depth * L + li - 1]: wide nodes' minimum slot positions
+    int* layer;                  // [node]
+    int* posof;                  // [node]: position in its depth
     float* vc;                   // [node * (L - 1) + j]: via cost of layer j at the node's cell
-    const float* vcost;
-    DeviceGrid g;
     __device__ __forceinline__ unsigned long long key(int row, int li) const { return packed[row * L + li - 1]; }
     __device__ __forceinline__ void lower(int row, int li, unsigned long long v) const { atomicMin(packed + row * L + li - 1, v); }
     __device__ __forceinline__ unsigned long long path(int row, int li) const { return best[row * L + li - 1]; }
@@ -119,16 +126,6 @@ struct LdsTables {
     __device__ __forceinline__ double root_cost(int, int li) const { return rootc[li]; }
     __device__ __forceinline__ void set_root_cost(int, int li, double c) const { rootc[li] = c; }
     __device__ __forceinline__ float via(int v, int j, int, int) const { return vc[v * (L - 1) + j]; }
-    // child slots to the empty key, via costs of every node staged
-    __device__ __forceinline__ void clear(const Dag& d, int slots, int nodes, int lane, int W) const {
-        for (int i = lane; i < slots * L; i += W) packed[i] = kEmpty;
-        if (L > 1)
-            for (int i = lane; i < nodes * (L - 1); i += W) {
-                const int v = i / (L - 1), j = i % (L - 1);
-                const int c = d.nodes[v].x;
-                vc[i] = vcost[cell_of(g, j, c / g.Y % g.X, c % g.Y)];
-            }
-    }
     // every LDS access of the wavefront has completed
     __device__ __forceinline__ static void barrier() {
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
@@ -136,33 +133,56 @@ struct LdsTables {
     }
 };
 
-// Bytes of one DAG's LdsTables for a DAG of `slots` child slots and `nodes`
-// nodes on L layers, 16-byte aligned.
-__host__ __device__ inline size_t lds_tables_bytes(int L, int slots, int nodes) {
-    const size_t rows = (size_t)slots * L;
-    size_t b = rows * 8 + ((rows * 4 + 7) & ~size_t(7)) + (size_t)kS * 8 + (size_t)nodes * 4 +
-               (size_t)nodes * (L > 1 ? L - 1 : 0) * 4;
-    return (b + 15) & ~size_t(15);
-}
+using LdsShape = FltLdsShape;
 
+// Lays out LdsTables over base (or only sizes them with base = nullptr);
+// returns the bytes, 16-byte aligned.
 template <int L>
-__device__ __forceinline__ LdsTables<L> lds_tables(char* base, int slots, int nodes, const FltArgs& a) {
-    LdsTables<L> t;
-    const size_t rows = (size_t)slots * L;
-    t.packed = reinterpret_cast<unsigned long long*>(base);
-    t.best = reinterpret_cast<unsigned*>(t.packed + rows);
-    t.rootc = reinterpret_cast<double*>(base + rows * 8 + ((rows * 4 + 7) & ~size_t(7)));
-    t.layer = reinterpret_cast<int*>(t.rootc + kS);
-    t.vc = reinterpret_cast<float*>(t.layer + nodes);
-    t.vcost = a.vcost;
-    t.g = a.g;
-    return t;
+__host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsTables<L>* t) {
+    size_t off = 0;
+    auto take = [&](size_t bytes, size_t align) {
+        off = (off + align - 1) & ~(align - 1);
+        char* p = base ? base + off : nullptr;
+        off += bytes;
+        return p;
+    };
+    char* packed = take((size_t)s.slots * L * 8, 16);
+    char* best = take((size_t)s.slots * L * 4, 8);
+    char* rootc = take((size_t)kS * 8, 8);
+    char* nrec = take((size_t)s.nodes * 16, 16);
+    char* erec = take((size_t)s.edges * 16, 16);
+    char* cvs = take((size_t)s.dnodes * L * 8, 8);
+    char* at = take((size_t)s.dnodes * L * 4, 4);
+    char* eorder = take((size_t)s.edges * 4, 4);
+    char* order = take((size_t)s.nodes * 4, 4);
+    char* doff = take((size_t)(s.depths + 1) * 4, 4);
+    char* edoff = take((size_t)(s.depths + 1) * 4, 4);
+    char* layer = take((size_t)s.nodes * 4, 4);
+    char* posof = take((size_t)s.nodes * 4, 4);
+    char* vc = take((size_t)s.nodes * (L > 1 ? L - 1 : 0) * 4, 4);
+    if (t) {
+        t->packed = reinterpret_cast<unsigned long long*>(packed);
+        t->best = reinterpret_cast<unsigned*>(best);
+        t->rootc = reinterpret_cast<double*>(rootc);
+        t->nrec = reinterpret_cast<int4*>(nrec);
+        t->erec = reinterpret_cast<int4*>(erec);
+        t->cvs = reinterpret_cast<double*>(cvs);
+        t->at = reinterpret_cast<unsigned*>(at);
+        t->eorder = reinterpret_cast<int*>(eorder);
+        t->order = reinterpret_cast<int*>(order);
+        t->doff = reinterpret_cast<int*>(doff);
+        t->edoff = reinterpret_cast<int*>(edoff);
+        t->layer = reinterpret_cast<int*>(layer);
+        t->posof = reinterpret_cast<int*>(posof);
+        t->vc = reinterpret_cast<float*>(vc);
+    }
+    return (off + 15) & ~size_t(15);
 }
 
 // solve_flt_node (src/routing/flt_block.cuh:6-142) for any number of child
 // slots, the node's layer costs in a private array.
-template <class Tb>
-__device__ void solve_node(const FltArgs& a, const Dag& d, const Tb& tb, int v) {
+template <bool kPush, class Tb>
+__device__ void solve_node(const FltArgs& a, const Dag& d, const Tb& tb, int v, int pos) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
     const int4 r = d.nodes[v];
@@ -205,6 +225,12 @@ __device__ void solve_node(const FltArgs& a, const Dag& d, const Tb& tb, int v) 
     }
     if (r.y == r.z)  // the root's trace reads its layer costs
         for (int li = 0; li < kS; ++li) tb.set_root_cost(v, li, li <= L ? cv[li] : kInf);
+    if constexpr (!kPush) {
+        // the depth's push items read the layer costs from LDS
+        tb.posof[v] = pos;
+        for (int li = 1; li <= L; ++li) tb.cvs[pos * L + li - 1] = cv[li];
+        return;
+    }
     unsigned long long cache[kS];
     for (int e = r.y; e < r.z; ++e) {
         const int4 er = d.edges[e];
@@ -327,8 +353,9 @@ __device__ __forceinline__ void push_parents(const FltArgs& a, const Dag& d, con
     }
 }
 
-template <int L, int K, class Tb>
-__device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, const Tb& tb, int v, const int4 r) {
+template <int L, int K, bool kPush, class Tb>
+__device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, const Tb& tb, int v, const int4 r,
+                                                int pos) {
     const DeviceGrid& g = a.g;
     const int kids = r.w & 7;
     const int low_fixed = 1 + r.x / g.XY;
@@ -356,10 +383,12 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
     unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
 #pragma unroll
     for (int li = 0; li <= L; ++li) cv[li] = kInf, at[li] = 0;
-    // passes over the stack's low end stay a loop; every layer index inside
-    // is a compile-time constant, so the tables stay in registers
-#pragma unroll 1
-    for (int low = 1; low <= min(L, low_fixed); ++low) {
+    // the passes over the stack's low end are unrolled too: every layer
+    // index is a compile-time constant (the tables stay in registers) and a
+    // pass walks only its own layers low..L
+#pragma unroll
+    for (int low = 1; low <= L; ++low) {
+        if (low > low_fixed) break;
         {
             double mins[K];
 #pragma unroll
@@ -421,7 +450,14 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
 #pragma unroll
         for (int li = 1; li <= L; ++li) tb.set_root_cost(v, li, cv[li]);
     }
-    push_parents<L>(a, d, tb, v, x, y, r, cv);
+    if constexpr (kPush) {
+        push_parents<L>(a, d, tb, v, x, y, r, cv);
+    } else {
+        // the depth's push items read the layer costs from LDS
+        tb.posof[v] = pos;
+#pragma unroll
+        for (int li = 1; li <= L; ++li) tb.cvs[pos * L + li - 1] = cv[li];
+    }
 }
 
 // trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
@@ -547,11 +583,11 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
                 const int v = order[i];
                 const int4 r = d.nodes[v];
                 if ((r.w & 7) <= 2)
-                    solve_node_fast<L, 2>(a, d, tb, v, r);
+                    solve_node_fast<L, 2, true>(a, d, tb, v, r, 0);
                 else if ((r.w & 7) <= kFastKids)
-                    solve_node_fast<L, kFastKids>(a, d, tb, v, r);
+                    solve_node_fast<L, kFastKids, true>(a, d, tb, v, r, 0);
                 else
-                    solve_node(a, d, tb, v);
+                    solve_node<true>(a, d, tb, v, 0);
             }
         }
         Tb::barrier();
@@ -591,6 +627,255 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
             count += __shfl(incl, W - 1, W);
         }
         Tb::barrier();
+    }
+    if (live && lane == 0) routes[0] = count <= cap ? count : 1;
+}
+
+// solve_flt_node (src/routing/flt_block.cuh:6-142) for nodes of up to six
+// child slots, kept off the registers of the common two-slot path: the
+// layer costs and, per layer, the slot positions of the running first
+// minima that produced them (4 bits per child slot, as in solve_node_fast)
+// live in the node's LDS rows (tb.cvs, tb.at); the best-path rows are
+// written once at the end from those positions.
+template <int L>
+__device__ __forceinline__ void solve_node_wide(const FltArgs& a, const Dag& d, const LdsTables<L>& tb, int v,
+                                                const int4 r, int pos) {
+    const DeviceGrid& g = a.g;
+    const int low_fixed = 1 + r.x / g.XY;
+    const int x = r.x / g.Y % g.X, y = r.x % g.Y;
+    const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
+    const int kids = r.w & 7;
+    const int row0 = r.w >> 3;
+    double* cv = tb.cvs + pos * L - 1;  // cv[li], li 1..L
+    unsigned* at = tb.at + pos * L - 1;
+    for (int li = 1; li <= L; ++li) cv[li] = kInf, at[li] = 0;
+    for (int low = 1; low <= min(L, low_fixed); ++low) {
+        double mins[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) mins[k] = kInf;
+        unsigned run = 0;
+        double via = 0.0;
+        for (int li = low; li <= L; ++li) {
+            if (li > low) via += (double)tb.via(v, li - 2, x, y);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (k < kids) {
+                    const double c = unpack_cost(tb.key(row0 + k, li));
+                    if (c < mins[k]) {
+                        mins[k] = c;
+                        run = (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
+                    }
+                }
+            if (li >= high_fixed) {
+                double cost = via;
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if (k < kids) cost += mins[k];
+                if (cost < kInf && cost < cv[li]) {
+                    cv[li] = cost;
+                    at[li] = run;
+                }
+            }
+        }
+        for (int li = L - 1; li >= low; --li)
+            if (cv[li + 1] < cv[li]) {
+                cv[li] = cv[li + 1];
+                at[li] = at[li + 1];
+            }
+    }
+    for (int li = 1; li <= L; ++li) {
+        if (!(cv[li] < kInf)) continue;
+        const unsigned pos_li = at[li];
+        for (int k = 0; k < kids; ++k) {
+            const int j = (pos_li >> (4 * k)) & 15;
+            const unsigned long long key = tb.key(row0 + k, j);
+            tb.set_path(row0 + k, li, (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j);
+        }
+    }
+    if (r.y == r.z) {  // the root's trace reads its layer costs
+        tb.set_root_cost(v, 0, kInf);
+        for (int li = 1; li <= L; ++li) tb.set_root_cost(v, li, cv[li]);
+    }
+    tb.posof[v] = pos;
+}
+
+// One push item of a depth: edge e of a node at that depth towards target
+// layer t of its parent's slot, with the additive costs of every from-layer
+// of the parity that runs along the edge (the second half of
+// solve_flt_node, :81-141, split by (edge, target) across lanes).  Its
+// global loads (prefix sums, layer-change costs) depend only on the edge, so
+// they are issued before the depth's nodes are solved.
+template <int L>
+struct PushItem {
+    static constexpr int H = (L + 1) / 2;
+    int t = 0;           // target layer, 0: no item
+    int v = 0, row = 0;  // child node, parent slot row
+    int par = 1;         // first from-layer of the usable parity
+    bool zero = false;   // zero-length edge: straight on every layer
+    bool far = false;    // long enough to change layers (> 4)
+    double add[H];       // per from-layer par + 2i: wire cost (its layer == t) or layer-change cost
+};
+
+template <int L>
+__device__ __forceinline__ PushItem<L> load_item(const FltArgs& a, const Dag& d, const LdsTables<L>& tb, int e0,
+                                                 int q) {
+    constexpr int H = PushItem<L>::H;
+    const DeviceGrid& g = a.g;
+    PushItem<L> it;
+    const int e = tb.eorder[e0 + q / L], mi = q % L;
+    const int4 er = tb.erec[e];  // {parent, slot, parent coordinate, child node}
+    const int v = er.w;
+    const int cc = tb.nrec[v].x;
+    const int x = cc / g.Y % g.X, y = cc % g.Y, px = er.z / g.Y % g.X, py = er.z % g.Y;
+    const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+    it.v = v;
+    it.row = (tb.nrec[er.x].w >> 3) + er.y;
+    it.zero = x == px && y == py;
+    it.far = abs(x - px) + abs(y - py) > 4;
+    if (it.zero) {
+        it.t = mi + 1;
+        it.add[0] = wire(a, mi, minx, maxx, miny, maxy);
+        return it;
+    }
+    // a layer runs along y when ((layer & 1) ^ DIR) == 0 (:101-102)
+    it.par = (x == px) == (g.DIR == 1) ? 1 : 2;
+    const int t = it.par + 2 * mi;
+    if (t > L) return it;  // it.t stays 0
+    it.t = t;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const int j = it.par + 2 * i;
+        if (j > L) continue;
+        if (j == t) it.add[i] = wire(a, j - 1, minx, maxx, miny, maxy);
+        else if (it.far) it.add[i] = d.cost_edges[change_slot(e, j - 1, t - 1)];
+    }
+    return it;
+}
+
+template <int L>
+__device__ __forceinline__ void finish_item(const FltArgs& a, const LdsTables<L>& tb, const PushItem<L>& it) {
+    constexpr int H = PushItem<L>::H;
+    if (!it.t) return;
+    const double* cv = tb.cvs + tb.posof[it.v] * L - 1;  // cv[layer], layer 1..L
+    if (it.zero) {
+        const double base = cv[it.t];
+        if (!(base < kInf)) {  // the reference asserts here
+            raise(a.error, kDevNoSolution);
+            return;
+        }
+        tb.lower(it.row, it.t, pack((float)(base + it.add[0]), it.v * 100 + kStraight));
+        return;
+    }
+    unsigned long long best = kEmpty;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const int j = it.par + 2 * i;
+        if (j > L) continue;
+        const double base = cv[j];
+        if (!(base < kInf)) {  // the reference asserts here
+            raise(a.error, kDevNoSolution);
+            continue;
+        }
+        double cost2;
+        int choice = kStraight;
+        if (j == it.t) {
+            cost2 = base + it.add[i];
+        } else if (it.far) {
+            if (!(it.add[i] < kInf)) continue;
+            cost2 = base + it.add[i];
+            choice = (j - 1) * 9 + (it.t - 1);
+        } else {
+            continue;
+        }
+        const unsigned long long nv = pack((float)cost2, it.v * 100 + choice);
+        if (nv < best) best = nv;
+    }
+    if (best < kEmpty) tb.lower(it.row, it.t, best);
+}
+
+// One DAG on W lanes with its tables and records in LDS.  Per depth: the
+// push items' global loads are issued first, the depth's nodes are solved
+// into LDS, then the items lower their parents' slot keys; every other
+// access of the depth loop stays in LDS.
+template <int L>
+__device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live, int lane, int W, const Dag& dg,
+                                              const LdsTables<L>& tb, const int4 ds, const int4 dn, int depths) {
+    // stage the DAG: child slots to the empty key, records, orders, via costs
+    for (int i = lane; i < dn.z * L; i += W) tb.packed[i] = kEmpty;
+    for (int v = lane; v < dn.x; v += W) {
+        const int4 r = dg.nodes[v];
+        tb.nrec[v] = r;
+        tb.order[v] = a.order[ds.x + v];
+        const int x = r.x / a.g.Y % a.g.X, y = r.x % a.g.Y;
+        for (int j = 0; j + 1 < L; ++j) tb.vc[v * (L - 1) + j] = a.vcost[cell_of(a.g, j, x, y)];
+        for (int e = r.y; e < r.z; ++e) {
+            const int4 er = dg.edges[e];
+            tb.erec[e] = make_int4(er.x, er.y, er.z, v);
+        }
+    }
+    for (int i = lane; i < dn.y; i += W) tb.eorder[i] = a.eorder[ds.y + i];
+    for (int i = lane; i <= dn.w; i += W) {
+        tb.doff[i] = a.depth_off[ds.w + i];
+        tb.edoff[i] = a.edepth_off[ds.w + i];
+    }
+    LdsTables<L>::barrier();
+    Dag d = dg;
+    d.nodes = tb.nrec;
+    d.edges = tb.erec;  // {parent, slot, parent coordinate, child node}: the trace reads the first three
+    for (int dep = depths - 1; dep >= 0; --dep) {
+        const bool here = dep < dn.w;
+        const int e0 = here ? tb.edoff[dep] : 0, items = here ? (tb.edoff[dep + 1] - e0) * L : 0;
+        PushItem<L> first;
+        if (lane < items) first = load_item<L>(a, d, tb, e0, lane);
+        if (here) {
+            const int n0 = tb.doff[dep], end = tb.doff[dep + 1];
+            for (int i = n0 + lane; i < end; i += W) {
+                const int v = tb.order[i];
+                const int4 r = tb.nrec[v];
+                if ((r.w & 7) <= 2)
+                    solve_node_fast<L, 2, false>(a, d, tb, v, r, i - n0);
+                else
+                    solve_node_wide<L>(a, d, tb, v, r, i - n0);
+            }
+        }
+        LdsTables<L>::barrier();
+        for (int q = lane; q < items; q += W) finish_item<L>(a, tb, q == lane ? first : load_item<L>(a, d, tb, e0, q));
+        LdsTables<L>::barrier();
+    }
+    const int id = live ? a.net_ids[k] : 0;
+    int* routes = a.routes.routes + (live ? (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin : 0);
+    const long long cap = live ? (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin : 0;
+    const int g_cells = a.g.L * a.g.XY;
+    int count = 1;  // the root's trace sets the word count to 1
+    for (int dep = 0; dep < depths; ++dep) {
+        const int first = dep < dn.w ? tb.doff[dep] : 0, end = dep < dn.w ? tb.doff[dep + 1] : 0;
+        int chunks = (end - first + W - 1) / W;
+        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+        for (int ch = 0; ch < chunks; ++ch) {
+            int w[8];
+            const int i = first + ch * W + lane;
+            int n = i < end ? trace_node(a, d, tb, tb.order[i], w) : 0;
+            for (int j = 0; j < n; ++j)
+                if (w[j] < 0 || w[j] >= g_cells) {
+                    raise(a.error, kDevBadCoordinate);
+                    n = 0;
+                }
+            int incl = n;
+            for (int s = 1; s < W; s <<= 1) {
+                const int o = __shfl_up(incl, s, W);
+                if (lane >= s) incl += o;
+            }
+            const int at = count + incl - n;
+            if (n) {
+                if (at + n - 1 >= cap) {
+                    raise(a.error, kDevRouteOverflow);
+                } else {
+                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
+                }
+            }
+            count += __shfl(incl, W - 1, W);
+        }
+        LdsTables<L>::barrier();
     }
     if (live && lane == 0) routes[0] = count <= cap ? count : 1;
 }
@@ -716,11 +1001,11 @@ __global__ __launch_bounds__(kWave) void flt_route_lds_kernel(FltArgs a) {
         const int4 ds = live ? a.desc[k] : make_int4(0, 0, 0, 0);
         const int4 dn = live ? a.sizes[k] : make_int4(0, 0, 0, 0);
         const Dag d = dag_of(a, ds);
-        const LdsTables<L> tb =
-            lds_tables<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_dag_slots, a.lds_dag_nodes, a);
+        LdsTables<L> tb;
+        lds_layout<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_shape, &tb);
         int depths = dn.w;
         for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
-        route_dag<L>(a, k, live, lane, W, d, tb, dn, a.order + ds.x, a.depth_off + ds.w, depths);
+        route_dag_lds<L>(a, k, live, lane, W, d, tb, ds, dn, depths);
         __syncthreads();  // the tables are reused by the next group
     }
 }
@@ -747,7 +1032,24 @@ void launch_for(const FltArgs& a, int mode_lds, int cus, hipStream_t stream) {
 
 }  // namespace
 
-size_t flt_lds_bytes(int L, int slots, int nodes) { return lds_tables_bytes(L, slots, nodes); }
+size_t flt_lds_bytes(int L, int slots, int nodes, int edges, int dnodes, int depths) {
+    const LdsShape shape{slots, nodes, edges, dnodes, depths};
+    switch (L) {
+#define GR_FLT_BYTES(n) \
+    case n: return lds_layout<n>(nullptr, shape, nullptr);
+        GR_FLT_BYTES(1)
+        GR_FLT_BYTES(2)
+        GR_FLT_BYTES(3)
+        GR_FLT_BYTES(4)
+        GR_FLT_BYTES(5)
+        GR_FLT_BYTES(6)
+        GR_FLT_BYTES(7)
+        GR_FLT_BYTES(8)
+        GR_FLT_BYTES(9)
+#undef GR_FLT_BYTES
+        default: return 0;
+    }
+}
 
 void launch_flt_route(const FltArgs& a, bool lds, int cus, hipStream_t stream) {
     switch (a.g.L) {

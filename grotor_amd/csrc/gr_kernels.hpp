@@ -180,6 +180,10 @@ __global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteSto
 constexpr int kFltStride = 10;  // layer index 1..L of the FLT tables (L <= 9), 0 unused
 constexpr int kFltChangeWidth = 5;  // same-parity to-layers per (edge, from-layer)
 constexpr int kFltWideBatch = 512;  // FLT batches of at least this many DAGs form the second tuning class
+// One DAG's share of LDS in flt_route_lds_kernel, sized by the batch's largest DAG.
+struct FltLdsShape {
+    int slots, nodes, edges, dnodes, depths;  // child slots, nodes, edges, nodes of its widest depth, depths
+};
 struct FltArgs {
     DeviceGrid g;
     const double* presum;
@@ -206,16 +210,20 @@ struct FltArgs {
     RouteStore routes;        // phase-two route words
     const int* pin_acc_num;   // pins per net: capacity = pins * routes.per_pin
     int* error;
-    // DP tables in LDS (flt_route_lds_kernel): bytes, child slots and nodes
-    // of one DAG's share, sized by the batch's largest DAG
-    int lds_dag_bytes, lds_dag_slots, lds_dag_nodes;
+    const int* eorder;      // per net: its edges by the depth of their child node (node order within a depth)
+    const int* edepth_off;  // per net: depths + 1 offsets into its eorder (at the depth_off positions)
+    // DP tables in LDS (flt_route_lds_kernel): bytes and shape of one DAG's
+    // share, sized by the batch's largest DAG
+    int lds_dag_bytes;
+    FltLdsShape lds_shape;
     int* queue;  // persistent form: kFltQueues work-queue counters (nullptr: one workgroup per DAG group)
 };
 constexpr int kFltQueues = 32;       // persistent FLT DP: work-queue counters per batch
 constexpr int kFltQueueStride = 32;  // ints between counters (one 128-byte line)
 __global__ void flt_edge_kernel(FltArgs a);
-// LDS bytes of one DAG's DP tables (slots child slots, nodes nodes, L layers)
-size_t flt_lds_bytes(int L, int slots, int nodes);
+// LDS bytes of one DAG's share of flt_route_lds_kernel (child slots, nodes,
+// edges, nodes of its widest depth, depths; L layers)
+size_t flt_lds_bytes(int L, int slots, int nodes, int edges, int dnodes, int depths);
 // The FLT DP of a batch (L = 1..9): with lds, each DAG's tables in LDS
 // (a.lds_dag_*, a.queue), else in the batch tables in global memory.
 void launch_flt_route(const FltArgs& a, bool lds, int cus, hipStream_t stream);
