@@ -379,52 +379,86 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
     float vc[L];
 #pragma unroll
     for (int j = 0; j + 1 < L; ++j) vc[j] = tb.via(v, j, x, y);
-    double cv[L + 1];
-    unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
+    // The reference's passes over the stack's low end (low = 1..lf, each an
+    // upward sweep with strict updates, then a downward sweep copying only
+    // strictly smaller costs) leave at every layer li the first minimum, in
+    // the order (low, then li' >= li), of the costs C(low, li') of the via
+    // stacks [low, li']: a pass never touches layers below its low, keeps an
+    // earlier pass's cost on a tie and copies down only what is strictly
+    // smaller.  The C(low, li') are independent of each other, so they are
+    // computed side by side (each pass's prefix sum of via costs and running
+    // first minima of the slot costs in their own registers) rather than pass
+    // after pass; the selection then keeps, per low, the first minimum over
+    // li' >= li and takes the first minimum over the lows.  Every cost is the
+    // same sum in the same order as in the reference's passes.
+    const int lf = min(L, low_fixed);
+    double C[L][L];    // C[low - 1][li - 1], kInf where the stack has no finite cost
+    unsigned R[L][L];  // slot positions of C's running minima, 4 bits per child slot
+    {
+        double V[L];   // per low: via costs of layers low+1..li, summed upward
+        float M[L][K];  // per low: running first minima of the slot costs over low..li
+        unsigned P[L];  // their positions
 #pragma unroll
-    for (int li = 0; li <= L; ++li) cv[li] = kInf, at[li] = 0;
-    // the passes over the stack's low end are unrolled too: every layer
-    // index is a compile-time constant (the tables stay in registers) and a
-    // pass walks only its own layers low..L
+        for (int li = 1; li <= L; ++li) {
 #pragma unroll
-    for (int low = 1; low <= L; ++low) {
-        if (low > low_fixed) break;
-        {
-            double mins[K];
-#pragma unroll
-            for (int k = 0; k < K; ++k) mins[k] = kInf;
-            unsigned run = 0;
-            double via = 0.0;
-#pragma unroll
-            for (int li = 1; li <= L; ++li) {
-                if (li < low) continue;
-                if (li > low) via += (double)vc[li - 2];
-#pragma unroll
-                for (int k = 0; k < K; ++k)
-                    if (k < kids) {
-                        const double c = (double)pv[k][li];
-                        if (c < mins[k]) {
-                            mins[k] = c;
-                            run = (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
-                        }
-                    }
-                if (li >= high_fixed) {
-                    double cost = via;
+            for (int low = 1; low <= li; ++low) {
+                const int i = low - 1;
+                if (li == low) {
+                    V[i] = 0.0;
+                    P[i] = 0;
 #pragma unroll
                     for (int k = 0; k < K; ++k)
-                        if (k < kids) cost += mins[k];
-                    if (cost < kInf && cost < cv[li]) {
-                        cv[li] = cost;
-                        at[li] = run;
-                    }
+                        if (k < kids) {
+                            M[i][k] = pv[k][li];
+                            P[i] |= (unsigned)li << (4 * k);
+                        }
+                } else {
+                    V[i] += (double)vc[li - 2];
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        if (k < kids && pv[k][li] < M[i][k]) {
+                            M[i][k] = pv[k][li];
+                            P[i] = (P[i] & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
+                        }
+                }
+                double cost = V[i];
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+                    if (k < kids) cost += (double)M[i][k];
+                C[i][li - 1] = low <= lf && li >= high_fixed && cost < kInf ? cost : kInf;
+                R[i][li - 1] = P[i];
+            }
+        }
+    }
+    double cv[L + 1];
+    unsigned at[L + 1];  // per layer: the slot positions of its cost, 4 bits per child slot
+    {
+        double S[L];     // per low: first minimum of C(low, li') over li' >= li
+        unsigned SR[L];
+#pragma unroll
+        for (int i = 0; i < L; ++i) S[i] = kInf, SR[i] = 0;
+        cv[0] = kInf, at[0] = 0;
+#pragma unroll
+        for (int li = L; li >= 1; --li) {
+#pragma unroll
+            for (int low = 1; low <= li; ++low) {
+                const int i = low - 1;
+                const double c = C[i][li - 1];
+                if (c < kInf && c <= S[i]) {  // going down, a tie goes to the lower li'
+                    S[i] = c;
+                    SR[i] = R[i][li - 1];
                 }
             }
+            double best = kInf;
+            unsigned pos = 0;
 #pragma unroll
-            for (int li = L - 1; li >= 1; --li)
-                if (li >= low && cv[li + 1] < cv[li]) {
-                    cv[li] = cv[li + 1];
-                    at[li] = at[li + 1];
+            for (int low = 1; low <= li; ++low)
+                if (S[low - 1] < best) {  // a tie goes to the lower low
+                    best = S[low - 1];
+                    pos = SR[low - 1];
                 }
+            cv[li] = best;
+            at[li] = pos;
         }
     }
 #pragma unroll
