@@ -557,20 +557,25 @@ def soft_timings(args, ga):
     ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=2)
     ctx.lshape_upload(t["order"], t["node_off"], t["nodes"], t["parents"])
     nets = t["nets"]
-    out = {"workload": f"configuration 4 ({nets} routed subnets, 512x512x9)", "precision": "f32 tables"}
-    for tau in (0.1, 1.0, 10.0):
-        ctx.soft_forward(0, nets, tau=tau, precision=32)
-        ctx.soft_backward()
-        f = b = 0.0
-        reps = 3
-        for _ in range(reps):
-            _, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
-            _, bm = ctx.soft_backward()
-            f += fm
-            b += bm
-        out[f"tau_{tau:g}"] = {"fwd_ms": round(f / reps, 3), "bwd_ms": round(b / reps, 3),
-                               "fwd_bwd_ms": round((f + b) / reps, 3)}
-        print(f"[bench] soft tau={tau}: fwd {f / reps:.2f} ms, bwd {b / reps:.2f} ms", file=sys.stderr, flush=True)
+    out = {"workload": f"configuration 4 ({nets} routed subnets, 512x512x9)", "precision": "f32 tables",
+           "edges": "L-shape DP edges (the tau_* entries) and the Z-shape route family (zshape)"}
+    for kind in (0, 1):
+        ctx.set_soft_kind(kind)
+        dest = out if kind == 0 else out.setdefault("zshape", {})
+        for tau in (0.1, 1.0, 10.0):
+            ctx.soft_forward(0, nets, tau=tau, precision=32)
+            ctx.soft_backward()
+            f = b = 0.0
+            reps = 3
+            for _ in range(reps):
+                _, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
+                _, bm = ctx.soft_backward()
+                f += fm
+                b += bm
+            dest[f"tau_{tau:g}"] = {"fwd_ms": round(f / reps, 3), "bwd_ms": round(b / reps, 3),
+                                    "fwd_bwd_ms": round((f + b) / reps, 3)}
+            print(f"[bench] soft {'zshape' if kind else 'lshape'} tau={tau}: fwd {f / reps:.2f} ms, "
+                  f"bwd {b / reps:.2f} ms", file=sys.stderr, flush=True)
     ctx.close()
     return out
 
@@ -841,28 +846,31 @@ def bench_soft(args, ga, design, topo):
     ctx.lshape_run(sched["batch_off"], first_stamp=1)
     ctx.lshape_upload(topo["order"], topo["node_off"], topo["nodes"], topo["parents"])
     nets = topo["nets"]
-    per_tau = {}
-    for tau in (0.1, 1.0, 10.0):
-        for _ in range(args.warmup):
-            ctx.soft_forward(0, nets, tau=tau, precision=32)
-            ctx.soft_backward()
-        f = b = 0.0
-        for _ in range(args.steps):
-            total, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
-            gp, bm = ctx.soft_backward()
-            f += fm
-            b += bm
-        per_tau[str(tau)] = {"fwd_ms": round(f / args.steps, 3), "bwd_ms": round(b / args.steps, 3),
-                             "total": total, "grad_params": [float(x) for x in gp]}
-        print(f"[bench] soft tau={tau}: fwd {f / args.steps:.2f} ms, bwd {b / args.steps:.2f} ms",
-              file=sys.stderr, flush=True)
+    per_tau, zshape = {}, {}
+    for kind, dest in ((1, zshape), (0, per_tau)):
+        ctx.set_soft_kind(kind)
+        for tau in (0.1, 1.0, 10.0):
+            for _ in range(args.warmup):
+                ctx.soft_forward(0, nets, tau=tau, precision=32)
+                ctx.soft_backward()
+            f = b = 0.0
+            for _ in range(args.steps):
+                total, _, fm = ctx.soft_forward(0, nets, tau=tau, precision=32)
+                gp, bm = ctx.soft_backward()
+                f += fm
+                b += bm
+            dest[str(tau)] = {"fwd_ms": round(f / args.steps, 3), "bwd_ms": round(b / args.steps, 3),
+                              "total": total, "grad_params": [float(x) for x in gp]}
+            print(f"[bench] soft {'zshape' if kind else 'lshape'} tau={tau}: fwd {f / args.steps:.2f} ms, "
+                  f"bwd {b / args.steps:.2f} ms", file=sys.stderr, flush=True)
     ms = sum(v["fwd_ms"] + v["bwd_ms"] for v in per_tau.values()) / len(per_tau)
     print(json.dumps({
         "metric": "soft-DP fwd+bwd ms per stage", "value": round(ms, 3), "unit": "ms", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"configuration {args.config} soft L-shape DP over {nets} routed subnets at the "
-                               f"L-shape stage's final demand", "per_tau": per_tau}}), flush=True)
+                               f"L-shape stage's final demand", "per_tau": per_tau,
+                   "zshape_per_tau": zshape}}), flush=True)
     ctx.close()
 
 

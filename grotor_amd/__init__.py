@@ -258,6 +258,19 @@ class Design:
         return dict(pin_off=pin_off, pins=arr(ptrs[1], int(pin_off[-1])), extra_off=extra_off,
                     extra=arr(ptrs[3], int(extra_off[-1])), sub_off=sub_off, subs=arr(ptrs[5], int(sub_off[-1])))
 
+    def access(self):
+        """Per design net its pins' access points as read from the NET file:
+        dict(pin_off, access_off, access) (metal-layer cells)."""
+        count = ctypes.c_int()
+        ptrs = [ctypes.POINTER(ctypes.c_int)() for _ in range(3)]
+        _call("gr_design_access", self._h, ctypes.byref(count), *[ctypes.byref(p) for p in ptrs])
+        n = count.value
+        pin_off = np.ctypeslib.as_array(ptrs[0], shape=(n + 1,)).copy()
+        acc_off = np.ctypeslib.as_array(ptrs[1], shape=(int(pin_off[-1]) + 1,)).copy()
+        acc = (np.ctypeslib.as_array(ptrs[2], shape=(int(acc_off[-1]),)).copy() if acc_off[-1]
+               else np.zeros(0, np.int32))
+        return dict(pin_off=pin_off, access_off=acc_off, access=acc)
+
     def metal_grid(self):
         """Metal-layer directions, capacities (float, as parsed) and overflow unit costs."""
         m = ctypes.c_int()
@@ -324,6 +337,8 @@ class Context:
         _call("gr_ctx_set_routes", self._h, ctypes.c_int(len(pin_acc_num) - 1), _ptr(pin_acc_num, ctypes.c_int))
         self.route_words = words_per_pin * int(pin_acc_num[-1]) + 1
         self.words_per_pin = words_per_pin
+        self.nets = len(pin_acc_num) - 1
+        self.words_per_pin2 = 0
 
     def set_cost_params(self, slope=0.5, zero_slope=1.5, via_weight=1.0, integer_wire=False):
         _call("gr_ctx_set_cost_params", self._h, ctypes.c_double(slope), ctypes.c_double(zero_slope),
@@ -336,6 +351,19 @@ class Context:
               ctypes.c_int(words_per_pin))
         self.route2_words = words_per_pin * int(pins2) + 1
         self.words_per_pin2 = words_per_pin
+
+    def reserve_routes_phase2(self, net_ids, words_per_pin=100):
+        """Phase-two storage for a further rip-up stage: appends words for
+        the nets in net_ids that have none, keeps the others' words; returns
+        (pin_off2 per net, pins2)."""
+        ids = _i32(net_ids)
+        off = np.empty(max(self.nets, 1), np.int32)
+        pins = ctypes.c_longlong()
+        _call("gr_ctx_reserve_routes_phase2", self._h, ctypes.c_int(len(ids)), _ptr(ids, ctypes.c_int),
+              ctypes.c_int(words_per_pin), _ptr(off, ctypes.c_int), ctypes.byref(pins))
+        self.words_per_pin2 = max(self.words_per_pin2, words_per_pin)
+        self.route2_words = self.words_per_pin2 * pins.value + 1
+        return off[:self.nets], pins.value
 
     def routes2(self):
         r = np.empty(self.route2_words, np.int32)
@@ -360,12 +388,15 @@ class Context:
               _ptr(np.ascontiguousarray(plan["edge_off"], np.int64), ctypes.c_longlong),
               _ptr(_i32(plan["edges"]), ctypes.c_int))
 
-    def flt_run(self, first_stamp=1, remove_phase_one=True, mode=0):
-        """Run the uploaded FLT batches (mode 0 launches, 1 persistent form);
-        returns the DAG interval in ms."""
+    def flt_run(self, first_stamp=1, remove_phase_one=True, mode=0, remove=None):
+        """Run the uploaded FLT batches (mode 0 plain, 1 launch, 2 persistent
+        form); remove (0 none, 1 phase-one routes, 2 current routes) defaults
+        to remove_phase_one; returns the DAG interval in ms."""
+        if remove is None:
+            remove = 1 if remove_phase_one else 0
         ms = ctypes.c_double()
-        _call("gr_flt_run", self._h, ctypes.c_int(first_stamp), ctypes.c_int(1 if remove_phase_one else 0),
-              ctypes.c_int(mode), ctypes.byref(ms))
+        _call("gr_flt_run", self._h, ctypes.c_int(first_stamp), ctypes.c_int(remove), ctypes.c_int(mode),
+              ctypes.byref(ms))
         return ms.value
 
     def set_flt_per_wave(self, per_wave):
@@ -507,6 +538,20 @@ class Context:
             r["demand"] = dem
         return r
 
+    def connectivity(self, nets, layer_dir, access):
+        """The evaluator's completion test of every net (gr_check_connectivity):
+        (status per net: 0 complete, 1 open, 2 not connected; counts of open,
+        not connected and incomplete nets)."""
+        n = len(nets["pin_off"]) - 1
+        arrs = [_i32(nets[k]) for k in ("pin_off", "pins", "extra_off", "extra", "sub_off", "subs")]
+        acc = [_i32(access[k]) for k in ("pin_off", "access_off", "access")]
+        status = np.zeros(max(n, 1), np.uint8)
+        counts = np.zeros(3, np.int64)
+        _call("gr_check_connectivity", self._h, ctypes.c_int(n), *[_ptr(a, ctypes.c_int) for a in arrs],
+              _ptr(_i32(layer_dir), ctypes.c_int), *[_ptr(a, ctypes.c_int) for a in acc],
+              _ptr(status, ctypes.c_ubyte), _ptr(counts, ctypes.c_longlong))
+        return status[:n], dict(open=int(counts[0]), disconnected=int(counts[1]), incomplete=int(counts[2]))
+
     def routes(self):
         r = np.empty(self.route_words, np.int32)
         _call("gr_routes_download", self._h, _ptr(r, ctypes.c_int), ctypes.c_longlong(self.route_words))
@@ -544,6 +589,26 @@ class Context:
               ctypes.c_double(zero_slope), ctypes.c_double(via_weight), ctypes.c_double(tau), ctypes.c_int(precision),
               ctypes.byref(total), _ptr(vals, ctypes.c_double), ctypes.byref(ms))
         return total.value, (vals[:count] if per_net else None), ms.value
+
+    def soft_forward_costs(self, begin, count, wcost, vcost, tau=1.0, precision=64, per_net=False):
+        """Soft DP over stage slots [begin, begin + count) at given wire and via
+        costs (IDX order): (total, per-net values or None, ms)."""
+        total, ms = ctypes.c_double(), ctypes.c_double()
+        vals = np.empty(max(count, 1), np.float64) if per_net else None
+        w = np.ascontiguousarray(wcost, np.float64)
+        v = np.ascontiguousarray(vcost, np.float64)
+        if w.size != self.cells or v.size != self.cells:
+            raise ValueError("cost arrays must hold one value per routing cell")
+        self.soft_generation += 1
+        _call("gr_soft_forward_costs", self._h, ctypes.c_int(begin), ctypes.c_int(count), _ptr(w, ctypes.c_double),
+              _ptr(v, ctypes.c_double), ctypes.c_double(tau), ctypes.c_int(precision), ctypes.byref(total),
+              _ptr(vals, ctypes.c_double), ctypes.byref(ms))
+        return total.value, (vals[:count] if per_net else None), ms.value
+
+    def set_soft_kind(self, kind):
+        """Edges of the soft DP: 0 the L-shape DP's, 1 the Z-shape route family."""
+        _call("gr_ctx_set_soft_kind", self._h, ctypes.c_int(kind))
+        self.soft_generation += 1
 
     def soft_backward(self):
         """Adjoint of the last soft_forward: (d/d(slope, zero_slope, via_weight), ms)."""

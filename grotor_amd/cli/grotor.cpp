@@ -9,6 +9,11 @@
 //                                  reference's execution), launch form with LDS
 //                                  tables (1), persistent form (2, the default)
 //          [--threads N]           host threads of the front end (default 8)
+//          [--flt-stages N]        rip-up -> FLT stages (default 1, as the
+//                                  reference); stage k > 1 reroutes the nets
+//                                  still over the threshold, removing each
+//                                  net's current route, and logs its records
+//                                  as flt<k> / detour<k>
 // and the same stages (grotor::route, src/pipeline.hpp:14-157): L-shape stage
 // over every subnet, rip-up of the nets crossing more than 10 overflowed
 // edges, congestion view, FLT DAGs, the FLT stage, .out output.
@@ -30,6 +35,7 @@
 // Timestamps are microseconds of std::chrono::steady_clock since the start
 // of the program (the reference's batch_clock_origin,
 // include/router_state.hpp:235).
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -88,6 +94,7 @@ Schedule lshape_schedule(const gr_topology& t, const std::vector<int>& color, in
 struct Options {
     std::string cap, net, out;
     int sparsity = 1, dag_mode = 2, threads = 8;
+    int flt_stages = 1;  // rip-up -> FLT stages (the reference runs one)
     bool gpu_batching = false, validate = false;
     bool profile_stages = false, report_score = true, stop_after_lshape = false;
 };
@@ -109,7 +116,7 @@ Options parse(int argc, char** argv) {
     const char* usage =
         "Usage: grotor -cap FILE -net FILE -out FILE [--batching cpu|hiprt] [--validate-batches 0|1] [--sparsity N] "
         "[--lut-dir DIR] [--profile-stages 0|1] [--report-score 0|1] [--stop-after lshape|routing] "
-        "[--dag-mode 0|1|2] [--threads N]";
+        "[--dag-mode 0|1|2] [--threads N] [--flt-stages N]";
     if (argc < 7 || argc % 2 != 1) throw std::invalid_argument(usage);
     Options o;
     for (int i = 1; i < argc; i += 2) {
@@ -144,6 +151,7 @@ Options parse(int argc, char** argv) {
             if (used != val.size() || o.sparsity < 1) throw std::invalid_argument("Sparsity must be a positive integer");
         } else if (opt == "--dag-mode") o.dag_mode = int_option(opt, val, 0, 2);
         else if (opt == "--threads") o.threads = int_option(opt, val, 1, 256);
+        else if (opt == "--flt-stages") o.flt_stages = int_option(opt, val, 1, 16);
         else throw std::invalid_argument("Unknown option: " + opt);
     }
     if (o.cap.empty() || o.net.empty() || o.out.empty()) throw std::invalid_argument("-cap, -net and -out are required");
@@ -259,26 +267,43 @@ int run(const Options& o) {
     double download_ms = ms_since(t0);
     double flt_ms = 0, class_ms = 0;
     int rerouted = 0;
-    std::vector<int> routes2;
-    const int* pin_off2 = nullptr;
+    std::vector<int> routes2, pin_off2;
+    long long pins2 = 0;
     if (!o.stop_after_lshape) {
-        // rip-up classification and the congestion view (src/pipeline.hpp:86-108)
-        t0 = Clock::now();
+        // rip-up classification, the congestion view and the FLT stage
+        // (src/pipeline.hpp:86-137); with --flt-stages N > 1 the stage
+        // repeats on the nets still over the rip-up threshold, each net's
+        // current route removed and phase-two storage appended for nets
+        // without it (the reroute storage of src/pipeline_cugr2.hpp:15-62)
         std::vector<unsigned char> flags(std::max(t.subnets, 1));
         std::vector<float> xsum(size_t(t.X) * t.Y), ysum(size_t(t.X) * t.Y);
-        ok(gr_congestion_view(ctx, nullptr, xsum.data(), ysum.data()), "congestion view");
-        ok(gr_ripup(ctx, 0, flags.data()), "rip-up");
-        class_ms = ms_since(t0);
-        const auto t_flt = Clock::now();
-        gr_flt_stage fs{};
-        ok(gr_design_flt_stage(d, flags.data(), xsum.data(), ysum.data(), o.threads, &fs), "FLT topology");
-        printf("TOPOLOGY_TIMING stage=flt elapsed_ms=%.6f geometry_fused=0\n", ms_since(t_flt));
-        rerouted = fs.nets;
-        if (fs.nets) {
-            ok(gr_ctx_set_routes_phase2(ctx, fs.pin_off2, fs.pins2, 100), "phase-two route storage");
+        int stamp = nb + 1;  // the L-shape batches took stamps 1..nb
+        for (int stage = 1; stage <= o.flt_stages; ++stage) {
+            t0 = Clock::now();
+            ok(gr_congestion_view(ctx, nullptr, xsum.data(), ysum.data()), "congestion view");
+            ok(gr_ripup(ctx, 0, flags.data()), "rip-up");
+            class_ms += ms_since(t0);
+            const auto t_flt = Clock::now();
+            const std::string suffix = stage == 1 ? "" : std::to_string(stage);
+            gr_flt_stage fs{};
+            ok(gr_design_flt_stage(d, flags.data(), xsum.data(), ysum.data(), o.threads, &fs), "FLT topology");
+            printf("TOPOLOGY_TIMING stage=flt%s elapsed_ms=%.6f geometry_fused=0\n", suffix.c_str(), ms_since(t_flt));
+            if (stage > 1) printf("RIPUP_STAGE index=%d nets=%d\n", stage, fs.nets);
+            if (!fs.nets) break;
+            if (stage == 1) {
+                ok(gr_ctx_set_routes_phase2(ctx, fs.pin_off2, fs.pins2, 100), "phase-two route storage");
+                pin_off2.assign(fs.pin_off2, fs.pin_off2 + t.subnets);
+                pins2 = fs.pins2;
+            } else {
+                std::vector<int> ids(fs.order, fs.order + fs.nets);
+                std::sort(ids.begin(), ids.end());
+                ok(gr_ctx_reserve_routes_phase2(ctx, int(ids.size()), ids.data(), 100, pin_off2.data(), &pins2),
+                   "phase-two route storage");
+                printf("REROUTE_STORAGE pins=%lld stride=100 bytes=%lld\n", pins2, (100 * pins2 + 1) * 4);
+            }
             std::vector<int> fcolor;
             const Geometry fg{fs.nets, fs.pt_off, fs.pts, fs.hs_off, fs.hsegs, fs.vs_off, fs.vsegs};
-            const int fnb = batch_stage(o, ctx, t.X, t.Y, fg, 300000, "flt", fcolor);
+            const int fnb = batch_stage(o, ctx, t.X, t.Y, fg, 300000, ("flt" + suffix).c_str(), fcolor);
             const auto t_prep = Clock::now();
             gr_flt_plan plan{};
             ok(gr_design_flt_plan(d, fcolor.data(), fnb, &plan), "FLT plan");
@@ -287,25 +312,43 @@ int run(const Options& o) {
                              plan.edges),
                "FLT upload");
             double ms = 0;
-            ok(gr_flt_run(ctx, nb + 1, 1, o.dag_mode, &ms), "FLT stage");
-            print_dag_timing(ctx, "detour", fs.nets, plan.batches);
+            ok(gr_flt_run(ctx, stamp, stage == 1 ? 1 : 2, o.dag_mode, &ms), "FLT stage");
+            stamp += 2 * plan.batches;  // a removal and a commit per batch
+            print_dag_timing(ctx, ("detour" + suffix).c_str(), fs.nets, plan.batches);
+            flt_ms += ms_since(t_flt);
+        }
+        for (int off : pin_off2) rerouted += off >= 0;
+        if (!pin_off2.empty()) {
             t0 = Clock::now();
-            routes2.resize(100LL * fs.pins2 + 1);
+            routes2.resize(100LL * pins2 + 1);
             ok(gr_routes2_download(ctx, routes2.data(), (long long)routes2.size()), "download phase-two routes");
             download_ms += ms_since(t0);
-            pin_off2 = fs.pin_off2;
         }
-        flt_ms = ms_since(t_flt);
     }
     if (o.report_score) {
         double wl = 0;
         long long vias = 0;
         ok(gr_stats_download(ctx, &wl, &vias), "statistics");
         printf("SCORE wirelength=%.0f vias=%lld rerouted=%d\n", wl, vias, rerouted);
+        // the evaluator's completion test of every net (resources/evaluator.cpp:527-538, 677-749)
+        int nn = 0, na = 0, metal = 0;
+        const int *pin_off = nullptr, *pins = nullptr, *extra_off = nullptr, *extra = nullptr, *sub_off = nullptr,
+                  *subs = nullptr, *apin = nullptr, *aoff = nullptr, *acc = nullptr;
+        ok(gr_design_nets(d, &nn, &pin_off, &pins, &extra_off, &extra, &sub_off, &subs), "design nets");
+        ok(gr_design_access(d, &na, &apin, &aoff, &acc), "access points");
+        ok(gr_design_metal_grid(d, &metal, nullptr, nullptr, nullptr), "metal grid");
+        std::vector<int> dirs(std::max(metal, 1));
+        ok(gr_design_metal_grid(d, &metal, dirs.data(), nullptr, nullptr), "metal grid");
+        long long counts[3] = {0, 0, 0};
+        ok(gr_check_connectivity(ctx, nn, pin_off, pins, extra_off, extra, sub_off, subs, dirs.data(), apin, aoff,
+                                 acc, nullptr, counts),
+           "connectivity");
+        printf("SCORE_CONNECTIVITY open=%lld not_connected=%lld incomplete=%lld\n", counts[0], counts[1], counts[2]);
     }
     t0 = Clock::now();
-    if (pin_off2)
-        ok(gr_design_write_output2(d, routes.data(), routes2.data(), pin_off2, 100, o.out.c_str()), "write output");
+    if (!pin_off2.empty())
+        ok(gr_design_write_output2(d, routes.data(), routes2.data(), pin_off2.data(), 100, o.out.c_str()),
+           "write output");
     else
         ok(gr_design_write_output(d, routes.data(), o.out.c_str()), "write output");
     const double output_ms = ms_since(t0);

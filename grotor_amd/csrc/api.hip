@@ -176,6 +176,8 @@ struct gr_ctx {
     SoftCostArgs soft_cost{};
     SoftArgs soft{};
     int soft_precision = 0;
+    int soft_kind = 0;        // 0: L-shape edges, 1: the Z-shape route family
+    bool soft_given = false;  // the last forward ran on caller-given costs
     bool soft_ready = false;  // a forward pass whose tables are still held
     double* soft_params = nullptr;
     // Z-shape stage: plan on the device, shared DP map, raw words
@@ -193,6 +195,10 @@ struct gr_ctx {
     int* routes2 = nullptr;
     long long route2_words = 0;
     std::vector<int> pin_off2_h;
+    // nets whose current route is their phase-two words (an FLT stage has
+    // routed them): rip-up reads those words, a later stage removes them
+    unsigned char* live2 = nullptr;
+    std::vector<unsigned char> live2_h;
     Workspace f_ws, f_tables;
     FltArgs f{};
     int f_batches = 0;
@@ -261,6 +267,15 @@ struct gr_ctx {
         graph = nullptr;
         graph_key.clear();
         drop_fgraph();
+    }
+    void release_phase2() {
+        drop_fgraph();  // a captured FLT loop holds the phase-two addresses
+        mem.release(routes2);
+        mem.release(pin_off2);
+        mem.release(live2);
+        pin_off2_h.clear();
+        live2_h.clear();
+        route2_words = 0;
     }
 
     ~gr_ctx() {
@@ -386,15 +401,21 @@ CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
     a.count = count;
     a.stamp = stamp;
     a.K = K;
+    a.rs2 = {c->routes2, c->pin_off2, c->route2_per_pin};
+    a.live2 = nullptr;
     return a;
 }
 
+// Commit (K = 1) or removal (K = -1) of stage slots [begin, begin + count):
+// routes from `rs` (phase-one storage when null), or, with `current`, each
+// net's current route (phase-two words for nets marked live).
 void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int* net_ids = nullptr,
-                   const RouteStore* rs = nullptr) {
+                   const RouteStore* rs = nullptr, bool current = false) {
     if (!count) return;
     CommitArgs a = commit_args(c, begin, count, stamp, K);
     if (net_ids) a.net_ids = net_ids;
     if (rs) a.rs = *rs;
+    if (current) a.live2 = c->live2;
     const int blocks = std::min((count + 3) / 4, 4096);
     timed(c, GR_K_COMMIT_WIRES, 0.0,
           [&] { hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
@@ -739,10 +760,144 @@ void bind_flt_storage(gr_ctx* c) {
     c->f.pin_acc_num = c->pin_acc_num;
 }
 
+// The per-net line inputs of the evaluator score and connectivity check,
+// checked and uploaded into `ar`: every index the kernels derive from them
+// is validated here.
+ScoreArgs net_lines(gr_ctx* c, DeviceArena& ar, int nets, const int* pin_off, const int* pins, const int* extra_off,
+                    const int* extra, const int* sub_off, const int* subs, const int* layer_dir) {
+    if (nets < 0 || !pin_off || !extra_off || !sub_off || !layer_dir) throw GrError(GR_E_ARG, "null argument");
+    if (!c->routes) throw GrError(GR_E_STATE, "routes not allocated (gr_ctx_set_routes)");
+    const int X = c->g.X, Y = c->g.Y, M = c->g.L + 1;
+    // lines pack x | y << 16 into a signed int: coordinates must stay below 32768
+    if (X > 32768 || Y > 32768 || M > kScoreLayers)
+        throw GrError(GR_E_ARG, "grid too large for the score's line packing and layer counters");
+    const long long XY = (long long)X * Y, cells = XY * M;
+    if (cells >= (1LL << 31)) throw GrError(GR_E_ARG, "metal-layer grid too large for the score's cell keys");
+    const int np = pin_off[nets], ne = extra_off[nets], ns = sub_off[nets];
+    if (np < 0 || ne < 0 || (ne & 1) || ns < 0 || (np && !pins) || (ne && !extra) || (ns && !subs))
+        throw GrError(GR_E_ARG, "invalid net line offsets");
+    for (int k = 0; k < nets; ++k)
+        if (pin_off[k + 1] < pin_off[k] || extra_off[k + 1] < extra_off[k] || sub_off[k + 1] < sub_off[k])
+            throw GrError(GR_E_ARG, "net offsets must be non-decreasing");
+    for (int i = 0; i < np; ++i)
+        if (pins[i] < 0 || pins[i] >= cells) throw GrError(GR_E_ARG, "pin cell outside the metal-layer grid");
+    for (int i = 0; i < ne; ++i)
+        if (extra[i] < 0 || extra[i] >= cells) throw GrError(GR_E_ARG, "extra route cell outside the grid");
+    const int nsub = int(c->pin_acc_num_h.size()) - 1;
+    for (int i = 0; i < ns; ++i)
+        if (subs[i] < 0 || subs[i] >= nsub) throw GrError(GR_E_ARG, "subnet id outside the route store");
+    for (int z = 0; z < M; ++z)
+        if (layer_dir[z] != 0 && layer_dir[z] != 1) throw GrError(GR_E_ARG, "layer direction must be 0 or 1");
+    ScoreArgs a{};
+    a.nets = nets, a.X = X, a.Y = Y, a.M = M;
+    int* d_pin_off = ar.alloc<int>(nets + 1);
+    int* d_pins = ar.alloc<int>(np);
+    int* d_extra_off = ar.alloc<int>(nets + 1);
+    int* d_extra = ar.alloc<int>(ne);
+    int* d_sub_off = ar.alloc<int>(nets + 1);
+    int* d_subs = ar.alloc<int>(ns);
+    int* d_dir = ar.alloc<int>(M);
+    hipStream_t st = c->stream;
+    upload(d_pin_off, pin_off, sizeof(int) * (nets + 1), st);
+    if (np) upload(d_pins, pins, sizeof(int) * np, st);
+    upload(d_extra_off, extra_off, sizeof(int) * (nets + 1), st);
+    if (ne) upload(d_extra, extra, sizeof(int) * ne, st);
+    upload(d_sub_off, sub_off, sizeof(int) * (nets + 1), st);
+    if (ns) upload(d_subs, subs, sizeof(int) * ns, st);
+    upload(d_dir, layer_dir, sizeof(int) * M, st);
+    a.pin_off = d_pin_off, a.pins = d_pins, a.extra_off = d_extra_off, a.extra = d_extra;
+    a.sub_off = d_sub_off, a.subs = d_subs, a.dir = d_dir;
+    a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
+    a.routes2 = c->routes2, a.pin_off2 = c->routes2 ? c->pin_off2 : nullptr, a.per_pin2 = c->route2_per_pin;
+    return a;
+}
+
 void require_stage(gr_ctx* c, int begin, int count) {
     if (!c->net_ids) throw GrError(GR_E_STATE, "no L-shape stage topology uploaded");
     if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
     if (begin < 0 || count < 0 || begin + count > c->stage_count) throw GrError(GR_E_ARG, "slot range outside the stage");
+}
+
+// Soft DP forward over slots [begin, begin + count): the smooth cost model
+// at (slope, zero_slope, via_weight), or, with wcost and vcost (IDX order,
+// host), those costs.
+void soft_forward(gr_ctx* c, int begin, int count, double slope, double zero_slope, double via_weight,
+                  const double* wcost, const double* vcost, double tau, int precision, double* total, double* per_net,
+                  double* elapsed_ms) {
+    if (!c || !total) throw GrError(GR_E_ARG, "null argument");
+    if (precision != 32 && precision != 64) throw GrError(GR_E_ARG, "precision must be 32 or 64");
+    if (!(tau > 0.0)) throw GrError(GR_E_ARG, "tau must be positive");
+    if (c->g.L > kSoftLayers) throw GrError(GR_E_ARG, "the soft DP supports at most 10 routing layers");
+    require_stage(c, begin, count);
+    c->soft_ready = false;
+    const size_t cells = c->cells;
+    SoftCostArgs& sc = c->soft_cost;
+    SoftArgs& a = c->soft;
+    c->soft_ws.carve([&](auto& cv) {
+        sc.wcost = cv.template take<double>(cells);
+        sc.presum = cv.template take<double>(cells);
+        sc.vcost = cv.template take<double>(cells);
+        sc.dw_ds = cv.template take<double>(cells);
+        sc.dw_dz = cv.template take<double>(cells);
+        sc.dv_ds = cv.template take<double>(cells);
+        sc.dv_dz = cv.template take<double>(cells);
+        a.gw = cv.template take<double>(cells);
+        a.gv = cv.template take<double>(cells);
+        c->soft_params = cv.template take<double>(size_t(kStatStripes) * kCounterStride);
+        a.value = cv.template take<double>(std::max(count, 1));
+    });
+    const long long nodes = (long long)c->node_off_h[begin + count] - c->node_off_h[begin];
+    void* tables = nullptr;
+    c->soft_tables.carve(
+        [&](auto& cv) { tables = cv.template take<char>(soft_scratch_bytes(c->g.L, nodes, precision)); });
+    sc.g = c->g;
+    sc.capacity = c->capacity;
+    sc.demand = c->demand;
+    sc.slope = slope;
+    sc.zero_slope = zero_slope;
+    sc.via_weight = via_weight;
+    const bool given = wcost != nullptr;
+    if (given) {
+        std::vector<double> h;
+        to_track_major(c->g, wcost, h);
+        upload(sc.wcost, h.data(), sizeof(double) * cells, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "soft cost upload");
+        to_track_major(c->g, vcost, h);
+        upload(sc.vcost, h.data(), sizeof(double) * cells, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "soft cost upload");
+    }
+    a.g = c->g;
+    a.presum = sc.presum;
+    a.vcost = sc.vcost;
+    a.node_off = c->node_off;
+    a.nodes = c->nodes;
+    a.parents = c->parents;
+    a.begin = begin;
+    a.count = count;
+    a.tau = tau;
+    a.scratch = tables;
+    hipEvent_t e0 = take_event(c), e1 = take_event(c);
+    hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+    if (given)
+        hipLaunchKernelGGL(soft_prefix_kernel, dim3(c->tracks), dim3(256), 0, c->stream, c->g, sc.wcost, sc.presum);
+    else
+        hipLaunchKernelGGL(soft_costs_kernel, dim3(c->tracks), dim3(256), 0, c->stream, sc);
+    if (count) launch_soft_forward(a, precision, c->soft_kind, c->cus, c->stream);
+    hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+    hip_check(hipGetLastError(), "soft forward kernels");
+    std::vector<double> v(std::max(count, 1));
+    download(v.data(), a.value, sizeof(double) * count, c->stream);
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (elapsed_ms) *elapsed_ms = ms;
+    c->next_event -= 2;
+    double sum = 0.0;
+    for (int k = 0; k < count; ++k) sum += v[k];
+    *total = sum;
+    if (per_net) std::memcpy(per_net, v.data(), sizeof(double) * count);
+    c->soft_precision = precision;
+    c->soft_given = given;
+    c->soft_ready = true;
 }
 
 }  // namespace
@@ -840,10 +995,7 @@ int gr_ctx_set_routes(gr_ctx* c, int net_count, const int* pin_acc_num) {
         if (!c || net_count < 0 || !pin_acc_num) throw GrError(GR_E_ARG, "invalid route storage arguments");
         c->drop_graph();  // a captured batch loop holds the old route storage addresses
         c->route_per_pin = c->next_route_per_pin;
-        c->mem.release(c->routes2);  // phase-two offsets index the old subnets
-        c->mem.release(c->pin_off2);
-        c->pin_off2_h.clear();
-        c->route2_words = 0;
+        c->release_phase2();  // phase-two offsets index the old subnets
         c->mem.release(c->routes);
         c->mem.release(c->pin_acc_num);
         c->mem.release(c->of_flags);
@@ -1313,11 +1465,7 @@ int gr_ctx_reset(gr_ctx* c) {
         if (c->routes)
             hip_check(hipMemsetAsync(c->routes, 0, sizeof(int) * size_t(c->route_words), c->stream), "clear routes");
         // a fresh context has no phase-two storage (it belongs to an FLT stage)
-        c->drop_fgraph();
-        c->mem.release(c->routes2);
-        c->mem.release(c->pin_off2);
-        c->pin_off2_h.clear();
-        c->route2_words = 0;
+        c->release_phase2();
         std::vector<unsigned> ones(c->tracks, 1u);
         upload(c->track_dirty, ones.data(), sizeof(unsigned) * c->tracks, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "gr_ctx_reset");
@@ -1399,7 +1547,9 @@ int gr_ripup(gr_ctx* c, int threshold, unsigned char* is_of_net) {
         hipLaunchKernelGGL(overflow_prefix_kernel, dim3(c->tracks), dim3(256), 0, c->stream, a);
         if (c->net_count)
             hipLaunchKernelGGL(overflow_nets_kernel, dim3((c->net_count + 255) / 256), dim3(256), 0, c->stream, c->g,
-                               c->of_sum, RouteStore{c->routes, c->pin_acc_num, c->route_per_pin}, c->net_count, c->of_flags);
+                               c->of_sum, RouteStore{c->routes, c->pin_acc_num, c->route_per_pin},
+                               RouteStore{c->routes2, c->pin_off2, c->route2_per_pin}, c->live2, c->net_count,
+                               c->of_flags);
         hip_check(hipGetLastError(), "ripup kernels");
         download(is_of_net, c->of_flags, size_t(c->net_count), c->stream);
     });
@@ -1529,17 +1679,82 @@ int gr_ctx_set_routes_phase2(gr_ctx* c, const int* pin_off2, long long pins2, in
             if (off < -1 || (off >= 0 && (long long)off + pins > pins2))
                 throw GrError(GR_E_ARG, "phase-two pin offset outside the storage");
         }
-        c->drop_fgraph();  // a captured FLT loop holds the old storage
-        c->mem.release(c->routes2);
-        c->mem.release(c->pin_off2);
+        c->release_phase2();
         c->route2_per_pin = words_per_pin;
         c->pin_off2_h.assign(pin_off2, pin_off2 + c->net_count);
+        c->live2_h.assign(c->net_count, 0);
         c->route2_words = (long long)words_per_pin * pins2 + 1;
         c->pin_off2 = c->mem.alloc<int>(std::max(c->net_count, 1));
         if (c->net_count) upload(c->pin_off2, pin_off2, sizeof(int) * c->net_count, c->stream);
+        c->live2 = c->mem.alloc<unsigned char>(std::max(c->net_count, 1));
+        hip_check(hipMemsetAsync(c->live2, 0, std::max(c->net_count, 1), c->stream), "clear live flags");
         c->routes2 = c->mem.alloc<int>(size_t(c->route2_words));
         hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
         hip_check(hipStreamSynchronize(c->stream), "phase-two route storage");
+    });
+}
+
+int gr_ctx_reserve_routes_phase2(gr_ctx* c, int count, const int* net_ids, int words_per_pin, int* pin_off2,
+                                 long long* pins2) {
+    return guarded([&] {
+        if (!c || count < 0 || (count && !net_ids)) throw GrError(GR_E_ARG, "invalid phase-two reservation arguments");
+        if (!c->routes) throw GrError(GR_E_STATE, "no route storage");
+        if (words_per_pin < 8) throw GrError(GR_E_ARG, "phase-two route words per pin must be at least 8");
+        const int n = c->net_count;
+        const bool had = c->routes2 != nullptr;
+        const int old_stride = had ? c->route2_per_pin : words_per_pin;
+        const int stride = std::max(old_stride, words_per_pin);
+        const long long old_pins = had ? (c->route2_words - 1) / old_stride : 0;
+        std::vector<int> off = had ? c->pin_off2_h : std::vector<int>(size_t(n), -1);
+        long long pins = old_pins;
+        for (int k = 0; k < count; ++k) {
+            const int id = net_ids[k];
+            if (id < 0 || id >= n) throw GrError(GR_E_ARG, "net id outside route storage");
+            if (off[id] >= 0) continue;
+            off[id] = int(pins);
+            pins += c->pin_acc_num_h[id + 1] - c->pin_acc_num_h[id];
+            if (pins > INT32_MAX) throw GrError(GR_E_ARG, "too many phase-two pins");
+        }
+        if (!had || pins != old_pins || stride != old_stride) {
+            // new storage at the wider stride; kept nets keep their offsets
+            // in pins, their words (and live flags) carried over
+            std::vector<int> kept;
+            if (had)
+                for (int id = 0; id < n; ++id)
+                    if (c->pin_off2_h[id] >= 0) kept.push_back(id);
+            const long long words = (long long)stride * pins + 1;
+            int* routes = c->mem.alloc<int>(size_t(words));
+            int* offsets = c->mem.alloc<int>(std::max(n, 1));
+            hip_check(hipMemsetAsync(routes, 0, sizeof(int) * size_t(words), c->stream), "clear routes");
+            if (n) upload(offsets, off.data(), sizeof(int) * n, c->stream);
+            if (had && stride == old_stride) {
+                hip_check(hipMemcpyAsync(routes, c->routes2, sizeof(int) * size_t(c->route2_words),
+                                         hipMemcpyDeviceToDevice, c->stream),
+                          "carry phase-two routes");
+            } else if (!kept.empty()) {
+                int* ids = c->mem.alloc<int>(kept.size());
+                upload(ids, kept.data(), sizeof(int) * kept.size(), c->stream);
+                hipLaunchKernelGGL(restride_routes_kernel, dim3(unsigned(kept.size())), dim3(256), 0, c->stream,
+                                   c->routes2, routes, ids, offsets, c->pin_acc_num, old_stride, stride);
+                hip_check(hipGetLastError(), "restride kernel");
+                hip_check(hipStreamSynchronize(c->stream), "restride phase-two routes");
+                c->mem.release(ids);
+            }
+            hip_check(hipStreamSynchronize(c->stream), "phase-two route storage");
+            std::vector<unsigned char> live = had ? c->live2_h : std::vector<unsigned char>(size_t(n), 0);
+            c->release_phase2();
+            c->routes2 = routes;
+            c->pin_off2 = offsets;
+            c->route2_per_pin = stride;
+            c->route2_words = words;
+            c->pin_off2_h = std::move(off);
+            c->live2_h = std::move(live);
+            c->live2 = c->mem.alloc<unsigned char>(std::max(n, 1));
+            if (n) upload(c->live2, c->live2_h.data(), size_t(n), c->stream);
+            hip_check(hipStreamSynchronize(c->stream), "phase-two live flags");
+        }
+        if (pin_off2 && n) std::copy(c->pin_off2_h.begin(), c->pin_off2_h.end(), pin_off2);
+        if (pins2) *pins2 = (c->route2_words - 1) / c->route2_per_pin;
     });
 }
 
@@ -1817,11 +2032,13 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
     });
 }
 
-int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, double* elapsed_ms) {
+int gr_flt_run(gr_ctx* c, int first_stamp, int remove, int mode, double* elapsed_ms) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (mode < 0 || mode > 2)
             throw GrError(GR_E_ARG, "FLT execution mode not available (0 = plain, 1 = launches, 2 = persistent + graph)");
+        if (remove < 0 || remove > 2)
+            throw GrError(GR_E_ARG, "FLT removal must be 0 (none), 1 (phase-one routes) or 2 (current routes)");
         if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
         bind_flt_storage(c);
         const RouteStore rs2{c->routes2, c->pin_off2, c->route2_per_pin};
@@ -1834,7 +2051,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, doubl
             for (int b = 0; b < c->f_batches; ++b) {
                 const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
                 const int* ids = c->f.net_ids + begin;
-                if (remove_phase_one) launch_commit(c, 0, count, stamp++, -1, ids);
+                if (remove) launch_commit(c, 0, count, stamp++, -1, ids, nullptr, remove == 2);
                 const int s = stamp++;
                 launch_refresh(c, 0);
                 if (!count) continue;
@@ -1863,8 +2080,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, doubl
         // mode 2 issue the same launches directly
         const bool graph = mode == 2 && !c->profiling;
         if (graph) {
-            const std::vector<int> key{c->flt_gen, first_stamp, remove_phase_one ? 1 : 0, c->f_per_wave[0],
-                                       c->f_per_wave[1]};
+            const std::vector<int> key{c->flt_gen, first_stamp, remove, c->f_per_wave[0], c->f_per_wave[1]};
             if (!c->fgraph_exec || key != c->fgraph_key) {
                 c->drop_fgraph();
                 hip_check(hipStreamSynchronize(c->stream), "before graph capture");
@@ -1891,6 +2107,13 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove_phase_one, int mode, doubl
         c->last_t0 = t0, c->last_t1 = t1;
         if (elapsed_ms) *elapsed_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         check_device_error(c, "gr_flt_run");
+        // the stage's nets now hold their phase-two routes
+        bool changed = false;
+        for (int id : c->f_ids_h) changed |= !c->live2_h[id], c->live2_h[id] = 1;
+        if (changed) {
+            upload(c->live2, c->live2_h.data(), size_t(c->net_count), c->stream);
+            hip_check(hipStreamSynchronize(c->stream), "phase-two live flags");
+        }
     });
 }
 
@@ -1899,6 +2122,10 @@ int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
         if (!c) throw GrError(GR_E_ARG, "null context");
         if (!c->f.node_rec && c->f_batches) throw GrError(GR_E_STATE, "no FLT stage uploaded");
         bind_flt_storage(c);
+        int* saved = c->mem.alloc<int>(size_t(c->route2_words));
+        hip_check(hipMemcpyAsync(saved, c->routes2, sizeof(int) * size_t(c->route2_words), hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "save phase-two routes");
         launch_refresh(c, 0);
         const bool profiling = c->profiling;
         c->profiling = false;
@@ -1930,9 +2157,13 @@ int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
             }
         }
         hip_check(hipGetLastError(), "FLT tuning launches");
-        // the tuning runs wrote phase-two words: start the stage from clear storage
-        hip_check(hipMemsetAsync(c->routes2, 0, sizeof(int) * size_t(c->route2_words), c->stream), "clear routes");
+        // the tuning runs wrote phase-two words: put back what the stage
+        // starts from (earlier stages' routes included)
+        hip_check(hipMemcpyAsync(c->routes2, saved, sizeof(int) * size_t(c->route2_words), hipMemcpyDeviceToDevice,
+                                 c->stream),
+                  "restore phase-two routes");
         hip_check(hipStreamSynchronize(c->stream), "FLT tuning");
+        c->mem.release(saved);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         c->profiling = profiling;
@@ -1963,65 +2194,25 @@ int gr_ctx_set_flt_per_wave(gr_ctx* c, const int* per_wave) {
 int gr_soft_forward(gr_ctx* c, int begin, int count, double slope, double zero_slope, double via_weight, double tau,
                     int precision, double* total, double* per_net, double* elapsed_ms) {
     return guarded([&] {
-        if (!c || !total) throw GrError(GR_E_ARG, "null argument");
-        if (precision != 32 && precision != 64) throw GrError(GR_E_ARG, "precision must be 32 or 64");
-        if (!(tau > 0.0)) throw GrError(GR_E_ARG, "tau must be positive");
-        require_stage(c, begin, count);
+        soft_forward(c, begin, count, slope, zero_slope, via_weight, nullptr, nullptr, tau, precision, total, per_net,
+                     elapsed_ms);
+    });
+}
+
+int gr_soft_forward_costs(gr_ctx* c, int begin, int count, const double* wcost, const double* vcost, double tau,
+                          int precision, double* total, double* per_net, double* elapsed_ms) {
+    return guarded([&] {
+        if (!wcost || !vcost) throw GrError(GR_E_ARG, "null cost arrays");
+        soft_forward(c, begin, count, 0.0, 0.0, 0.0, wcost, vcost, tau, precision, total, per_net, elapsed_ms);
+    });
+}
+
+int gr_ctx_set_soft_kind(gr_ctx* c, int kind) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (kind != 0 && kind != 1) throw GrError(GR_E_ARG, "soft DP kind must be 0 (L-shape) or 1 (Z-shape)");
+        c->soft_kind = kind;
         c->soft_ready = false;
-        const size_t cells = c->cells;
-        SoftCostArgs& sc = c->soft_cost;
-        SoftArgs& a = c->soft;
-        c->soft_ws.carve([&](auto& cv) {
-            sc.wcost = cv.template take<double>(cells);
-            sc.presum = cv.template take<double>(cells);
-            sc.vcost = cv.template take<double>(cells);
-            sc.dw_ds = cv.template take<double>(cells);
-            sc.dw_dz = cv.template take<double>(cells);
-            sc.dv_ds = cv.template take<double>(cells);
-            sc.dv_dz = cv.template take<double>(cells);
-            a.gw = cv.template take<double>(cells);
-            a.gv = cv.template take<double>(cells);
-            c->soft_params = cv.template take<double>(size_t(kStatStripes) * kCounterStride);
-            a.value = cv.template take<double>(std::max(count, 1));
-        });
-        const long long nodes = (long long)c->node_off_h[begin + count] - c->node_off_h[begin];
-        void* tables = nullptr;
-        c->soft_tables.carve(
-            [&](auto& cv) { tables = cv.template take<char>(soft_scratch_bytes(c->g.L, nodes, precision)); });
-        sc.g = c->g;
-        sc.capacity = c->capacity;
-        sc.demand = c->demand;
-        sc.slope = slope;
-        sc.zero_slope = zero_slope;
-        sc.via_weight = via_weight;
-        a.g = c->g;
-        a.presum = sc.presum;
-        a.vcost = sc.vcost;
-        a.node_off = c->node_off;
-        a.nodes = c->nodes;
-        a.parents = c->parents;
-        a.begin = begin;
-        a.count = count;
-        a.tau = tau;
-        a.scratch = tables;
-        hipEvent_t e0 = take_event(c), e1 = take_event(c);
-        hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
-        hipLaunchKernelGGL(soft_costs_kernel, dim3(c->tracks), dim3(256), 0, c->stream, sc);
-        if (count) launch_soft_forward(a, precision, c->stream);
-        hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
-        hip_check(hipGetLastError(), "soft forward kernels");
-        std::vector<double> v(std::max(count, 1));
-        download(v.data(), a.value, sizeof(double) * count, c->stream);
-        float ms = 0.f;
-        hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        if (elapsed_ms) *elapsed_ms = ms;
-        c->next_event -= 2;
-        double sum = 0.0;
-        for (int k = 0; k < count; ++k) sum += v[k];
-        *total = sum;
-        if (per_net) std::memcpy(per_net, v.data(), sizeof(double) * count);
-        c->soft_precision = precision;
-        c->soft_ready = true;
     });
 }
 
@@ -2037,10 +2228,11 @@ int gr_soft_backward(gr_ctx* c, double* grad_params, double* elapsed_ms) {
                   "clear parameter gradients");
         hipEvent_t e0 = take_event(c), e1 = take_event(c);
         hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
-        if (a.count) launch_soft_backward(a, c->soft_precision, c->stream);
+        if (a.count) launch_soft_backward(a, c->soft_precision, c->soft_kind, c->cus, c->stream);
         const int blocks = int(std::min<size_t>(1024, (cells + 255) / 256));
-        hipLaunchKernelGGL(soft_params_kernel, dim3(blocks), dim3(256), 0, c->stream, c->soft_cost, a.gw, a.gv,
-                           (long long)cells, c->soft_params);
+        if (!c->soft_given)  // the cost-model parameters exist only for model costs
+            hipLaunchKernelGGL(soft_params_kernel, dim3(blocks), dim3(256), 0, c->stream, c->soft_cost, a.gw, a.gv,
+                               (long long)cells, c->soft_params);
         hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
         hip_check(hipGetLastError(), "soft backward kernels");
         std::vector<double> part(size_t(kStatStripes) * kCounterStride);
@@ -2356,39 +2548,12 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
              const int* sub_off, const int* subs, const int* layer_dir, const double* capacity,
              const double* overflow_cost, double* out, unsigned short* demand) {
     return guarded([&] {
-        if (!c || nets < 0 || !pin_off || !extra_off || !sub_off || !layer_dir || !capacity || !overflow_cost || !out)
-            throw GrError(GR_E_ARG, "null argument");
-        if (!c->routes) throw GrError(GR_E_STATE, "routes not allocated (gr_ctx_set_routes)");
-        const int X = c->g.X, Y = c->g.Y, M = c->g.L + 1;
-        // lines pack x | y << 16 into a signed int: coordinates must stay below 32768
-        if (X > 32768 || Y > 32768 || M > kScoreLayers)
-            throw GrError(GR_E_ARG, "grid too large for the score's line packing and layer counters");
-        const long long XY = (long long)X * Y, cells = XY * M;
-        if (cells >= (1LL << 31)) throw GrError(GR_E_ARG, "metal-layer grid too large for the score's cell keys");
-        const int np = pin_off[nets], ne = extra_off[nets], ns = sub_off[nets];
-        if (np < 0 || ne < 0 || (ne & 1) || ns < 0 || (np && !pins) || (ne && !extra) || (ns && !subs))
-            throw GrError(GR_E_ARG, "invalid net line offsets");
-        for (int k = 0; k < nets; ++k)
-            if (pin_off[k + 1] < pin_off[k] || extra_off[k + 1] < extra_off[k] || sub_off[k + 1] < sub_off[k])
-                throw GrError(GR_E_ARG, "net offsets must be non-decreasing");
-        for (int i = 0; i < np; ++i)
-            if (pins[i] < 0 || pins[i] >= cells) throw GrError(GR_E_ARG, "pin cell outside the metal-layer grid");
-        for (int i = 0; i < ne; ++i)
-            if (extra[i] < 0 || extra[i] >= cells) throw GrError(GR_E_ARG, "extra route cell outside the grid");
-        const int nsub = int(c->pin_acc_num_h.size()) - 1;
-        for (int i = 0; i < ns; ++i)
-            if (subs[i] < 0 || subs[i] >= nsub) throw GrError(GR_E_ARG, "subnet id outside the route store");
-        for (int z = 0; z < M; ++z)
-            if (layer_dir[z] != 0 && layer_dir[z] != 1) throw GrError(GR_E_ARG, "layer direction must be 0 or 1");
-        const int blocks_per_layer = int((XY + kScoreCells - 1) / kScoreCells);
+        if (!c || !capacity || !overflow_cost || !out) throw GrError(GR_E_ARG, "null argument");
         DeviceArena ar;
-        int* d_pin_off = ar.alloc<int>(nets + 1);
-        int* d_pins = ar.alloc<int>(np);
-        int* d_extra_off = ar.alloc<int>(nets + 1);
-        int* d_extra = ar.alloc<int>(ne);
-        int* d_sub_off = ar.alloc<int>(nets + 1);
-        int* d_subs = ar.alloc<int>(ns);
-        int* d_dir = ar.alloc<int>(M);
+        ScoreArgs a = net_lines(c, ar, nets, pin_off, pins, extra_off, extra, sub_off, subs, layer_dir);
+        const int X = c->g.X, Y = c->g.Y, M = c->g.L + 1;
+        const long long XY = (long long)X * Y, cells = XY * M;
+        const int blocks_per_layer = int((XY + kScoreCells - 1) / kScoreCells);
         double* d_cap = ar.alloc<double>(cells);
         int* d_wire = ar.alloc<int>(cells);
         int* d_stacked = ar.alloc<int>(cells);
@@ -2398,21 +2563,8 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
         unsigned long long* d_pwl = ar.alloc<unsigned long long>(size_t(M) * blocks_per_layer);
         unsigned short* d_demand = demand ? ar.alloc<unsigned short>(cells) : nullptr;
         hipStream_t st = c->stream;
-        upload(d_pin_off, pin_off, sizeof(int) * (nets + 1), st);
-        if (np) upload(d_pins, pins, sizeof(int) * np, st);
-        upload(d_extra_off, extra_off, sizeof(int) * (nets + 1), st);
-        if (ne) upload(d_extra, extra, sizeof(int) * ne, st);
-        upload(d_sub_off, sub_off, sizeof(int) * (nets + 1), st);
-        if (ns) upload(d_subs, subs, sizeof(int) * ns, st);
-        upload(d_dir, layer_dir, sizeof(int) * M, st);
         upload(d_cap, capacity, sizeof(double) * cells, st);
-        ScoreArgs a;
-        a.nets = nets, a.X = X, a.Y = Y, a.M = M;
-        a.pin_off = d_pin_off, a.pins = d_pins, a.extra_off = d_extra_off, a.extra = d_extra;
-        a.sub_off = d_sub_off, a.subs = d_subs;
-        a.rs = {c->routes, c->pin_acc_num, c->route_per_pin};
-        a.routes2 = c->routes2, a.pin_off2 = c->routes2 ? c->pin_off2 : nullptr, a.per_pin2 = c->route2_per_pin;
-        a.dir = d_dir, a.wire = d_wire, a.stacked = d_stacked, a.vias = d_counts, a.bad_nets = d_counts + ncount - 1;
+        a.wire = d_wire, a.stacked = d_stacked, a.vias = d_counts, a.bad_nets = d_counts + ncount - 1;
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
@@ -2460,6 +2612,70 @@ int gr_score(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int
         out[3] = overflow + via_cost + wl_cost;
         out[4] = double(counts[ncount - 1]);
         out[5] = double(ms);
+    });
+}
+
+int gr_check_connectivity(gr_ctx* c, int nets, const int* pin_off, const int* pins, const int* extra_off,
+                          const int* extra, const int* sub_off, const int* subs, const int* layer_dir,
+                          const int* access_pin_off, const int* access_off, const int* access, unsigned char* status,
+                          long long* counts) {
+    return guarded([&] {
+        if (!c || !access_pin_off || !access_off || !counts) throw GrError(GR_E_ARG, "null argument");
+        DeviceArena ar;
+        ScoreArgs a = net_lines(c, ar, nets, pin_off, pins, extra_off, extra, sub_off, subs, layer_dir);
+        const long long cells = (long long)c->g.X * c->g.Y * (c->g.L + 1);
+        if (access_pin_off[0] != 0) throw GrError(GR_E_ARG, "pin offsets must start at 0");
+        for (int k = 0; k < nets; ++k)
+            if (access_pin_off[k + 1] < access_pin_off[k]) throw GrError(GR_E_ARG, "pin offsets must be non-decreasing");
+        const int npins = access_pin_off[nets];
+        if (access_off[0] != 0) throw GrError(GR_E_ARG, "access offsets must start at 0");
+        for (int p = 0; p < npins; ++p)
+            if (access_off[p + 1] < access_off[p]) throw GrError(GR_E_ARG, "access offsets must be non-decreasing");
+        const int nacc = access_off[npins];
+        if (nacc && !access) throw GrError(GR_E_ARG, "null access points");
+        for (int t = 0; t < nacc; ++t)
+            if (access[t] < 0 || access[t] >= cells) throw GrError(GR_E_ARG, "access point outside the metal-layer grid");
+        // staging of the nets whose lines can exceed kScoreLines: at most
+        // their pin entries, extra pairs and the pairs their subnets' route
+        // words can hold
+        std::vector<long long> big_off(size_t(std::max(nets, 1)), -1);
+        long long big = 0;
+        for (int k = 0; k < nets; ++k) {
+            long long bound = (pin_off[k + 1] - pin_off[k]) + (extra_off[k + 1] - extra_off[k]) / 2;
+            for (int t = sub_off[k]; t < sub_off[k + 1]; ++t) {
+                const int id = subs[t];
+                const long long np = c->pin_acc_num_h[id + 1] - c->pin_acc_num_h[id];
+                const bool two = c->routes2 && c->pin_off2_h[id] >= 0;
+                bound += std::max(0LL, (np * (two ? c->route2_per_pin : c->route_per_pin) - 1) / 2);
+            }
+            if (bound > kScoreLines) big_off[k] = big, big += bound;
+        }
+        ConnArgs q{};
+        int* d_apin = ar.alloc<int>(nets + 1);
+        int* d_aoff = ar.alloc<int>(npins + 1);
+        int* d_acc = ar.alloc<int>(std::max(nacc, 1));
+        long long* d_big_off = ar.alloc<long long>(big_off.size());
+        q.big_lines = ar.alloc<int4>(size_t(std::max(big, 1LL)));
+        q.big_par = ar.alloc<int>(size_t(std::max(big, 1LL)));
+        q.status = ar.alloc<unsigned char>(std::max(nets, 1));
+        q.counts = ar.alloc<unsigned long long>(2);
+        hipStream_t st = c->stream;
+        upload(d_apin, access_pin_off, sizeof(int) * (nets + 1), st);
+        upload(d_aoff, access_off, sizeof(int) * (npins + 1), st);
+        if (nacc) upload(d_acc, access, sizeof(int) * nacc, st);
+        upload(d_big_off, big_off.data(), sizeof(long long) * big_off.size(), st);
+        hip_check(hipMemsetAsync(q.counts, 0, 2 * sizeof(unsigned long long), st), "clear counts");
+        q.apin_off = d_apin, q.acc_off = d_aoff, q.acc = d_acc, q.big_off = d_big_off;
+        if (nets)
+            hipLaunchKernelGGL(connectivity_kernel, dim3(std::min(nets, std::max(1, c->cus) * 16)), dim3(kWave), 0, st,
+                               a, q);
+        hip_check(hipGetLastError(), "connectivity kernel");
+        unsigned long long n2[2];
+        download(n2, q.counts, sizeof(n2), st);
+        if (status && nets) download(status, q.status, size_t(nets), st);
+        counts[0] = (long long)n2[0];
+        counts[1] = (long long)n2[1];
+        counts[2] = counts[0] + counts[1];
     });
 }
 

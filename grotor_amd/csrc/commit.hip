@@ -18,6 +18,14 @@ __device__ __forceinline__ const int* route_of(const RouteStore& rs, int id) {
     return rs.routes + (size_t)rs.pin_acc_num[id] * rs.per_pin;
 }
 
+// The route a net holds now: its phase-two words once a rip-up stage has
+// routed it there (the ripup_flag choice of commit_wire_demand,
+// src/routing/demand.cuh:148-155), else its phase-one words.
+__device__ __forceinline__ const int* current_route(const RouteStore& rs, const RouteStore& rs2,
+                                                    const unsigned char* live2, int id) {
+    return live2 && live2[id] ? route_of(rs2, id) : route_of(rs, id);
+}
+
 __device__ __forceinline__ void add_demand(const CommitArgs& a, int cell, float value) {
     atomicAdd(a.demand + cell, value);
     a.track_dirty[track_of_cell(a.g, cell)] = 1u;
@@ -42,7 +50,7 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
     unsigned char* flags = s_flags[wib];  // bit 0: along y, bit 1: one track
     for (int s = wave; s < a.count; s += waves) {
         const int slot = a.slots ? a.slots[a.begin + s] : a.begin + s;
-        const int* r = route_of(a.rs, a.net_ids[slot]);
+        const int* r = current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]);
         const int nseg = (r[0] - 1) / 2;
         double length = 0.0;
         long long edges = 0;
@@ -114,7 +122,7 @@ __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
     const DeviceGrid& g = a.g;
     for (int s = wave; s < a.count; s += waves) {
         const int slot = a.slots ? a.slots[a.begin + s] : a.begin + s;
-        const int* r = route_of(a.rs, a.net_ids[slot]);
+        const int* r = current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]);
         const int end = r[0];
         long long vias = 0;
         for (int i = 1 + 2 * lane; i < end; i += 2 * kWave) {
@@ -183,10 +191,11 @@ __global__ __launch_bounds__(256) void overflow_prefix_kernel(RipupArgs a) {
 }
 
 __global__ __launch_bounds__(256) void overflow_nets_kernel(DeviceGrid g, const int* of_sum, RouteStore rs,
-                                                            int net_count, unsigned char* is_of_net) {
+                                                            RouteStore rs2, const unsigned char* live2, int net_count,
+                                                            unsigned char* is_of_net) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= net_count) return;
-    const int* r = route_of(rs, id);
+    const int* r = current_route(rs, rs2, live2, id);
     int of = 0;
     unsigned char flag = 0;
     for (int i = 1; i < r[0]; i += 2) {
@@ -200,6 +209,19 @@ __global__ __launch_bounds__(256) void overflow_nets_kernel(DeviceGrid g, const 
         }
     }
     is_of_net[id] = flag;
+}
+
+// Phase-two storage re-laid at a wider stride: one workgroup copies one
+// kept net's words (its pins at old_per_pin words each), offsets in pins
+// unchanged.
+__global__ __launch_bounds__(256) void restride_routes_kernel(const int* src, int* dst, const int* ids,
+                                                              const int* pin_off2, const int* pin_acc_num,
+                                                              int old_per_pin, int per_pin) {
+    const int id = ids[blockIdx.x];
+    const long long n = (long long)(pin_acc_num[id + 1] - pin_acc_num[id]) * old_per_pin;
+    const int* s = src + (size_t)pin_off2[id] * old_per_pin;
+    int* d = dst + (size_t)pin_off2[id] * per_pin;
+    for (long long i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
 }
 
 __global__ void fill_int_kernel(int* p, int value, long long n) {

@@ -57,6 +57,8 @@ struct CommitArgs {
     const int* net_ids;  // stage slots -> net id
     const int* slots;    // optional permutation of stage slots (nullptr = identity)
     int begin, count, stamp, K;
+    RouteStore rs2;              // phase-two words, read for nets marked in live2
+    const unsigned char* live2;  // nullptr: every net reads rs
 };
 __global__ void commit_wires_kernel(CommitArgs a);
 __global__ void commit_vias_kernel(CommitArgs a);
@@ -98,8 +100,8 @@ struct RipupArgs {
     int threshold;
 };
 __global__ void overflow_prefix_kernel(RipupArgs a);
-__global__ void overflow_nets_kernel(DeviceGrid g, const int* of_sum, RouteStore rs, int net_count,
-                                     unsigned char* is_of_net);
+__global__ void overflow_nets_kernel(DeviceGrid g, const int* of_sum, RouteStore rs, RouteStore rs2,
+                                     const unsigned char* live2, int net_count, unsigned char* is_of_net);
 
 // Batching
 struct BatchGeom {
@@ -260,9 +262,12 @@ struct SoftArgs {
 };
 __global__ void soft_costs_kernel(SoftCostArgs a);
 __global__ void soft_params_kernel(SoftCostArgs c, const double* gw, const double* gv, long long cells, double* out);
+__global__ void soft_prefix_kernel(DeviceGrid g, const double* wcost, double* presum);
+constexpr int kSoftLayers = 10;  // routing layers the soft DP supports
 size_t soft_scratch_bytes(int L, long long nodes, int precision);
-void launch_soft_forward(const SoftArgs& a, int precision, hipStream_t stream);
-void launch_soft_backward(const SoftArgs& a, int precision, hipStream_t stream);
+// kind 0: L-shape edges, 1: the Z-shape route family
+void launch_soft_forward(const SoftArgs& a, int precision, int kind, int cus, hipStream_t stream);
+void launch_soft_backward(const SoftArgs& a, int precision, int kind, int cus, hipStream_t stream);
 
 // Evaluator-equivalent score (resources/evaluator.cpp:458-675)
 struct ScoreArgs {
@@ -286,9 +291,25 @@ constexpr int kScoreCells = 8192;  // cells per score_cells_kernel block
 constexpr int kScoreLines = 1024;  // lines of a net staged in LDS by score_nets_kernel
 constexpr int kScoreLayers = 32;   // metal layers the score supports (via counters per stripe)
 __global__ void score_nets_kernel(ScoreArgs a);
+
+// The evaluator's completion test per net (resources/evaluator.cpp:527-538,
+// check_connectivity :677-749) over the same lines as the score.
+struct ConnArgs {
+    const int* apin_off;       // [nets + 1] per net: its pins
+    const int* acc_off;        // [pins + 1] per pin: its access points
+    const int* acc;            // access points as metal-layer cells
+    const long long* big_off;  // per net: offset of its staging in big_lines / big_par, -1: LDS
+    int4* big_lines;
+    int* big_par;
+    unsigned char* status;        // per net: 0 complete, 1 open (a rejected line), 2 not connected
+    unsigned long long* counts;   // [2] open, not connected
+};
+__global__ void connectivity_kernel(ScoreArgs a, ConnArgs c);
 __global__ void score_cells_kernel(ScoreArgs a, const double* capacity, const int* x_len, const int* y_len,
                                    double* partial_loss, unsigned long long* partial_wl, unsigned short* demand_out);
 
 __global__ void fill_int_kernel(int* p, int value, long long n);
+__global__ void restride_routes_kernel(const int* src, int* dst, const int* ids, const int* pin_off2,
+                                       const int* pin_acc_num, int old_per_pin, int per_pin);
 
 }  // namespace grotor

@@ -246,6 +246,178 @@ __global__ __launch_bounds__(kWave) void score_nets_kernel(ScoreArgs a) {
         atomicAdd(a.vias + (blockIdx.x % kStatStripes) * kScoreLayers + lane, s_vias[lane]);
 }
 
+namespace {
+
+// A staged line as a box of metal-layer cells: the cells the evaluator
+// marks for it (a wire's cells, both ends included; a via column from zl to
+// zh).  Lines of an open net are not boxes; entries without a line (metal-1
+// pins) have z = -1.
+struct Box {
+    int x0, y0, z0, x1, y1, z1;
+};
+
+__device__ __forceinline__ Box box_of(int4 q) {
+    return Box{q.x & 0xFFFF, q.x >> 16, q.z & 0xFF, q.y & 0xFFFF, q.y >> 16, q.z >> 8};
+}
+
+// The evaluator rejects the line (resources/evaluator.cpp:518-569): a via
+// that moves, a wire on metal 0 or across its layer's direction, an empty
+// wire.
+__device__ __forceinline__ bool rejected(const Box& b, const int* dir) {
+    if (b.z0 != b.z1) return b.x0 != b.x1 || b.y0 != b.y1;
+    if (b.z0 == 0) return true;
+    if (dir[b.z0] == 0) return !(b.x1 > b.x0 && b.y1 == b.y0);
+    return !(b.y1 > b.y0 && b.x1 == b.x0);
+}
+
+__device__ __forceinline__ bool meet(int a0, int a1, int b0, int b1, int gap) {
+    return a0 <= b1 + gap && b0 <= a1 + gap;
+}
+
+// Whether the flood fill of check_connectivity passes between a cell of A
+// and a cell of B: a shared cell, the same (x, y) on adjacent layers, or
+// neighbours along a common routing layer's direction (metal 0 has none).
+__device__ __forceinline__ bool touch(const Box& A, const Box& B, const int* dir) {
+    const bool xi = meet(A.x0, A.x1, B.x0, B.x1, 0), yi = meet(A.y0, A.y1, B.y0, B.y1, 0);
+    if (xi && yi && meet(A.z0, A.z1, B.z0, B.z1, 1)) return true;
+    const bool x1 = meet(A.x0, A.x1, B.x0, B.x1, 1), y1 = meet(A.y0, A.y1, B.y0, B.y1, 1);
+    if (!(yi && x1) && !(xi && y1)) return false;
+    for (int z = max(max(A.z0, B.z0), 1); z <= min(A.z1, B.z1); ++z)
+        if (dir[z] == 0 ? yi && x1 : xi && y1) return true;
+    return false;
+}
+
+__device__ __forceinline__ bool holds(const Box& b, int x, int y, int z) {
+    return b.x0 <= x && x <= b.x1 && b.y0 <= y && y <= b.y1 && b.z0 <= z && z <= b.z1;
+}
+
+__device__ __forceinline__ int uf_load(int* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// Root of i with path halving; roots only ever link to smaller indices.
+__device__ __forceinline__ int uf_find(int* par, int i) {
+    for (;;) {
+        const int p = uf_load(par + i);
+        if (p == i) return i;
+        const int g = uf_load(par + p);
+        if (g != p) __hip_atomic_store(par + i, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        i = p;
+    }
+}
+
+__device__ __forceinline__ void uf_union(int* par, int a, int b) {
+    for (;;) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        if (atomicCAS(par + a, a, b) == a) return;
+    }
+}
+
+}  // namespace
+
+// One wavefront per design net.  Its lines are staged as boxes (in LDS, or
+// for nets that may exceed kScoreLines lines in the global staging the host
+// sized), lines whose cells the flood fill connects are united (union-find,
+// lanes over line pairs), the components holding a marked access point of
+// the first pin are flagged, and every other pin needs an access point in a
+// flagged component: the evaluator's traced cells are exactly the marked
+// cells of those components.
+__global__ __launch_bounds__(kWave) void connectivity_kernel(ScoreArgs a, ConnArgs c) {
+    __shared__ int4 s_line[kScoreLines];
+    __shared__ int s_par[kScoreLines];
+    __shared__ unsigned long long s_counts[2];
+    const int lane = threadIdx.x;
+    const int X = a.X, Y = a.Y, XY = X * Y;
+    if (lane < 2) s_counts[lane] = 0;
+    __syncthreads();
+    for (int k = blockIdx.x; k < a.nets; k += gridDim.x) {
+        int n = (a.pin_off[k + 1] - a.pin_off[k]) + (a.extra_off[k + 1] - a.extra_off[k]) / 2;
+        int subs_pairs = 0;
+        for (int s = a.sub_off[k] + lane; s < a.sub_off[k + 1]; s += kWave) {
+            int pairs;
+            subnet_words(a, a.subs[s], &pairs);
+            subs_pairs += pairs;
+        }
+        for (int off = kWave / 2; off; off >>= 1) subs_pairs += __shfl_xor(subs_pairs, off, kWave);
+        n += subs_pairs;
+        const bool big = n > kScoreLines;
+        // the host sizes every net that can exceed kScoreLines lines
+        int4* L = big ? c.big_lines + c.big_off[k] : s_line;
+        int* par = big ? c.big_par + c.big_off[k] : s_par;
+        for_each_line(a, k, lane, kWave, [&](int ord, int q0, int q1) {
+            const Line ln = decode_line(q0, q1, X, Y);
+            L[ord] = make_int4(pack_xy(ln.xl, ln.yl), pack_xy(ln.xh, ln.yh), ln.zl | ln.zh << 8, 0);
+            return false;
+        });
+        for (int j = a.pin_off[k] + lane; j < a.pin_off[k + 1]; j += kWave)
+            if (a.pins[j] >= XY) L[j - a.pin_off[k]].z = -1;  // metal-1 pins print no line
+        __syncthreads();
+        bool bad = false;
+        for (int i = lane; i < n; i += kWave) {
+            const int4 q = L[i];
+            par[i] = i;
+            if (q.z >= 0) bad |= rejected(box_of(q), a.dir);
+        }
+        unsigned char st = 0;
+        if (__any(bad)) {
+            st = 1;  // has_connectivity_violation: counted open, never flood filled (:529-530)
+        } else {
+            __syncthreads();
+            for (int i = lane; i < n; i += kWave) {
+                const int4 qi = L[i];
+                if (qi.z < 0) continue;
+                const Box bi = box_of(qi);
+                for (int j = 0; j < i; ++j) {
+                    const int4 qj = L[j];
+                    if (qj.z >= 0 && touch(bi, box_of(qj), a.dir)) uf_union(par, i, j);
+                }
+            }
+            __syncthreads();
+            for (int i = lane; i < n; i += kWave) par[i] = uf_find(par, i);
+            __syncthreads();
+            // components holding a marked access point of the first pin
+            const int p0 = c.apin_off[k], p1 = c.apin_off[k + 1];
+            if (p1 > p0) {
+                for (int t = c.acc_off[p0]; t < c.acc_off[p0 + 1]; ++t) {
+                    const int cell = c.acc[t];
+                    const int z = cell / XY, x = cell / Y % X, y = cell % Y;
+                    for (int i = lane; i < n; i += kWave) {
+                        const int4 q = L[i];
+                        if (q.z >= 0 && holds(box_of(q), x, y, z)) L[par[i]].w = 1;
+                    }
+                }
+                __syncthreads();
+                bool open = false;
+                for (int p = p0 + 1 + lane; p < p1; p += kWave) {
+                    bool ok = false;
+                    for (int t = c.acc_off[p]; t < c.acc_off[p + 1] && !ok; ++t) {
+                        const int cell = c.acc[t];
+                        const int z = cell / XY, x = cell / Y % X, y = cell % Y;
+                        for (int i = 0; i < n && !ok; ++i) {
+                            const int4 q = L[i];
+                            ok = q.z >= 0 && L[par[i]].w && holds(box_of(q), x, y, z);
+                        }
+                    }
+                    open |= !ok;
+                }
+                if (__any(open)) st = 2;
+            }
+        }
+        if (lane == 0) {
+            c.status[k] = st;
+            if (st) atomicAdd(&s_counts[st - 1], 1ull);
+        }
+        __syncthreads();  // the staging is reused by the next net
+    }
+    __syncthreads();
+    if (lane < 2 && s_counts[lane]) atomicAdd(c.counts + lane, s_counts[lane]);
+}
+
 // Block b of layer blockIdx.y covers cells [b * kScoreCells, (b + 1) *
 // kScoreCells) of the layer; its loss and wirelength sums go to
 // partial_loss / partial_wl[layer * gridDim.x + b].

@@ -29,6 +29,7 @@ struct gr_design {
     std::vector<float> capacity;  // routing layers, IDX order
     std::vector<double> short_costs;
     std::vector<int> net_pin_off, net_pins, net_extra_off, net_extra, net_sub_off, net_subs;
+    std::vector<int> acc_pin_off, acc_off, acc;
     // Z-shape stage
     std::vector<int> zorder, zpt_off, zpts, zhs_off, zhsegs, zvs_off, zvsegs;
     ZshapePlan zplan;
@@ -343,6 +344,25 @@ int gr_design_nets(gr_design* g, int* count, const int** pin_off, const int** pi
         if (extra) *extra = g->net_extra.data();
         if (subnet_off) *subnet_off = g->net_sub_off.data();
         if (subnets) *subnets = g->net_subs.data();
+    });
+}
+
+int gr_design_access(gr_design* g, int* count, const int** pin_off, const int** access_off, const int** access) {
+    return guarded([&] {
+        if (!g || !count) throw GrError(GR_E_ARG, "null argument");
+        g->acc_pin_off.assign(1, 0), g->acc_off.assign(1, 0);
+        g->acc.clear();
+        for (const auto& n : g->d.nets) {
+            for (const auto& pin : n.access_points) {
+                g->acc.insert(g->acc.end(), pin.begin(), pin.end());
+                g->acc_off.push_back(int(g->acc.size()));
+            }
+            g->acc_pin_off.push_back(int(g->acc_off.size()) - 1);
+        }
+        *count = int(g->d.nets.size());
+        if (pin_off) *pin_off = g->acc_pin_off.data();
+        if (access_off) *access_off = g->acc_off.data();
+        if (access) *access = g->acc.data();
     });
 }
 

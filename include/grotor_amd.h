@@ -113,11 +113,26 @@ int gr_stats_download(gr_ctx *ctx, double *total_wirelength, long long *total_vi
  * 16-bit demand per metal-layer cell.  Subnets that have phase-two storage
  * (gr_ctx_set_routes_phase2, offset >= 0) are scored by their phase-two
  * words, as the reference's output prints them (src/output.hpp:98-106).
- * The connectivity flood fill of check_connectivity is not part of this
- * score. */
+ * The connectivity flood fill is gr_check_connectivity. */
 int gr_score(gr_ctx *ctx, int nets, const int *pin_off, const int *pins, const int *extra_off, const int *extra,
              const int *sub_off, const int *subs, const int *layer_dir, const double *capacity,
              const double *overflow_cost, double *out, unsigned short *demand);
+
+/* The evaluator's completion test of every net over the same lines as
+ * gr_score (NVR_DB::read_gr_solution :527-538 with check_connectivity,
+ * resources/evaluator.cpp:677-749): the cells its lines mark, flood filled
+ * from the first pin's marked access points along each routing layer's
+ * direction and between adjacent layers, must reach an access point of
+ * every other pin.  access_pin_off[nets+1]: each net's pins in NET-file
+ * order; access_off[pins+1]: each pin's access points in access, as
+ * metal-layer cells (z * X + x) * Y + y.  status[k] (may be NULL): 0
+ * complete, 1 open (a line the evaluator rejects, counted in its "open
+ * nets"), 2 not connected (the evaluator prints the net and stops).
+ * counts: {open, not connected, incomplete = open + not connected}. */
+int gr_check_connectivity(gr_ctx *ctx, int nets, const int *pin_off, const int *pins, const int *extra_off,
+                          const int *extra, const int *sub_off, const int *subs, const int *layer_dir,
+                          const int *access_pin_off, const int *access_off, const int *access, unsigned char *status,
+                          long long *counts);
 
 /* Exact first-fit greedy batching of n nets given in priority order
  * (cudb::generate_batches_rsmt, src/batching/generate_batches_cpu.hpp:7-73;
@@ -269,7 +284,10 @@ int gr_routes_download(gr_ctx *ctx, int *routes, long long words);
 
 /* Rip-up classification over all nets of the route storage
  * (graph::ripup, src/routing/congestion.cuh:114-136): is_of_net[i] = 1 if net
- * i's route crosses more than 10 edges with capacity + threshold <= demand. */
+ * i's route crosses more than 10 edges with capacity + threshold <= demand.
+ * A net an FLT stage has routed is classified by its phase-two route, as
+ * mark_overflow_nets reads routes_phase2 for nets with a rip-up flag
+ * (:38-45). */
 int gr_ripup(gr_ctx *ctx, int threshold, unsigned char *is_of_net);
 
 /* ---- Z-shape (GGR) stage on the device (src/pipeline_ggr.hpp:88-127) ----
@@ -296,6 +314,16 @@ int gr_zshape_run(gr_ctx *ctx, int first_stamp, double *elapsed_ms);
  * words_per_pin, capacity words_per_pin * its pin count.  Cleared to zero. */
 int gr_ctx_set_routes_phase2(gr_ctx *ctx, const int *pin_off2, long long pins2, int words_per_pin);
 
+/* Phase-two storage for a further rip-up stage (reserve_reroute_storage,
+ * src/pipeline_cugr2.hpp:15-62): nets of net_ids without phase-two words get
+ * them appended in the given order; nets that have them keep their offsets
+ * and their words (and whether an FLT stage routed them); the stride
+ * becomes max(current, words_per_pin).  Creates the storage if there is
+ * none.  pin_off2 (may be NULL) receives every net's offset in pins (-1:
+ * none), *pins2 (may be NULL) the pin total. */
+int gr_ctx_reserve_routes_phase2(gr_ctx *ctx, int count, const int *net_ids, int words_per_pin, int *pin_off2,
+                                 long long *pins2);
+
 /* Copy `words` phase-two route words (from word 0) to the host
  * (RouterState::download_routes(true), include/router_state.hpp:215-222). */
 int gr_routes2_download(gr_ctx *ctx, int *routes, long long words);
@@ -321,8 +349,11 @@ int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_
                   const int *edges);
 
 /* Run the uploaded FLT batches (the loop of src/routing/flt.cuh:141-168):
- * per batch, if remove_phase_one, the nets' phase-one routes are removed
- * (commit K = -1 at the next stamp); then at the next stamp the cost
+ * per batch, the nets' routes are removed (commit K = -1 at the next stamp)
+ * when `remove` is 1 (their phase-one routes: remove_phase_one) or 2 (each
+ * net's current route: its phase-two words if an earlier FLT stage routed
+ * it, else its phase-one words; a repeated rip-up stage); then at the next
+ * stamp the cost
  * refresh, precompute_cost_edge, the DAG DP and the commit of the
  * phase-two routes.  Stamps start at first_stamp and advance by one per
  * commit.  mode: 0 = plain form, the reference's execution
@@ -339,7 +370,7 @@ int gr_flt_upload(gr_ctx *ctx, int nbatch, const int *batch_off, const int *net_
  *       other values are rejected with GR_E_ARG.  Results are identical in
  *       every mode.
  * *elapsed_ms = the synchronized interval (DAG_TIMING). */
-int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove_phase_one, int mode, double *elapsed_ms);
+int gr_flt_run(gr_ctx *ctx, int first_stamp, int remove, int mode, double *elapsed_ms);
 
 /* DAGs per wavefront of the FLT DP (1, 2, 4 or 8; each DAG gets 64 / n
  * lanes) for batches of fewer than 512 DAGs (per_wave[0]) and of at least
@@ -355,13 +386,14 @@ int gr_ctx_set_flt_per_wave(gr_ctx *ctx, const int *per_wave);
  * cleared afterwards.  No reference counterpart. */
 int gr_flt_tune(gr_ctx *ctx, int *per_wave, double *ms);
 
-/* ---- soft (log-sum-exp) L-shape DP ----
+/* ---- soft (log-sum-exp) DP over L-shape or Z-shape edges ----
  * No counterpart in the reference: a differentiable relaxation of the
- * L-shape DP above (every minimum becomes smin_tau(x) = -tau log sum
- * exp(-x/tau)) over the smooth form of the cost model
- * (src/cost/full_update.cuh:9-83 with exact exp), at the context's current
- * demand.  Its gradient is the expected wire and via usage of every cell,
- * and the derivative in the cost-model parameters. */
+ * L-shape DP above, or of the Z-shape route family (gr_ctx_set_soft_kind),
+ * in which every minimum becomes smin_tau(x) = -tau log sum exp(-x/tau),
+ * over the smooth form of the cost model (src/cost/full_update.cuh:9-83
+ * with exact exp) at the context's current demand, or over given costs.
+ * Its gradient is the expected wire and via usage of every cell, and the
+ * derivative in the cost-model parameters. */
 
 /* Soft DP of the uploaded stage slots [begin, begin + count) with overflow
  * slopes (slope, zero_slope; 0.5 and 1.5 in the reference), via weight
@@ -373,8 +405,23 @@ int gr_soft_forward(gr_ctx *ctx, int begin, int count, double slope, double zero
                     double via_weight, double tau, int precision, double *total,
                     double *per_net, double *elapsed_ms);
 
+/* The same soft DP at caller-given costs: wcost and vcost per routing cell
+ * (IDX order, host memory); the wire prefix sums are taken on the device.
+ * F is then a function of these arrays, its gradient gr_soft_edge_grads. */
+int gr_soft_forward_costs(gr_ctx *ctx, int begin, int count, const double *wcost, const double *vcost,
+                          double tau, int precision, double *total, double *per_net, double *elapsed_ms);
+
+/* Edges of the soft DP for the following forward passes: 0 (default) the
+ * L-shape DP's (straight, or two legs with the corner vias strictly between
+ * their layers); 1 the route family of the Z-shape DP
+ * (src/routing/zshape.cuh:62-288): straight, both L-shapes and the
+ * Z-shapes with the middle leg strictly inside the box, a full via stack
+ * at each bend.  At most 10 routing layers. */
+int gr_ctx_set_soft_kind(gr_ctx *ctx, int kind);
+
 /* Adjoint of the last gr_soft_forward: grad_params[0..2] = d total /
- * d(slope, zero_slope, via_weight); per-cell gradients are kept for
+ * d(slope, zero_slope, via_weight) (zero after gr_soft_forward_costs, whose
+ * costs have no parameters); per-cell gradients are kept for
  * gr_soft_edge_grads.  *elapsed_ms: device time. */
 int gr_soft_backward(gr_ctx *ctx, double *grad_params, double *elapsed_ms);
 
@@ -478,6 +525,14 @@ int gr_design_metal_grid(const gr_design *design, int *metal_layers, int *layer_
  * (db::net::pins / extra_routes / subnets, src/input.hpp:15-24). */
 int gr_design_nets(gr_design *design, int *count, const int **pin_off, const int **pins,
                    const int **extra_off, const int **extra, const int **subnet_off, const int **subnets);
+
+/* Access points of every design net's pins as read from the NET file
+ * (NVR_DB::read_nets, resources/evaluator.cpp:407-441; src/input.hpp:385-419):
+ * per net its pins in file order (pin_off[count+1]), per pin its access
+ * points (access_off[pins+1]) as metal-layer cells (z * X + x) * Y + y;
+ * arrays owned by the design. */
+int gr_design_access(gr_design *design, int *count, const int **pin_off, const int **access_off,
+                     const int **access);
 
 /* Batching geometry of one tree (net::construct_rsmt, src/preprocess.hpp:94-126):
  * positions are 2D cells x*Y + y, adjacency in CSR.  Capacities in, counts

@@ -616,8 +616,8 @@ int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *
                   float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
                   const long long *node_off, const int *node_rec, const long long *edge_off, const int *edge_rec,
                   const long long *route_off1, int *routes1, const long long *route_off2, const int *route_cap2,
-                  int *routes2, int remove_phase_one, int *stamp, double *total_wirelength, long long *total_vias,
-                  int threads) {
+                  int *routes2, int remove, const unsigned char *live2, int *stamp, double *total_wirelength,
+                  long long *total_vias, int threads) {
     int maxcount = 1;
     for (int b = 0; b < nbatch; ++b)
         if (batch_off[b + 1] - batch_off[b] > maxcount) maxcount = batch_off[b + 1] - batch_off[b];
@@ -629,8 +629,15 @@ int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *
     int status = 0;
     for (int b = 0; b < nbatch; ++b) {
         const int begin = batch_off[b], count = batch_off[b + 1] - begin;
-        if (remove_phase_one) {
-            for (int k = 0; k < count; ++k) rof[k] = routes1 + route_off1[net_ids[begin + k]];
+        if (remove) {
+            /* remove 1: the phase-one routes (route_prepared_topology's
+             * remove_phase_one, src/routing/flt.cuh:145-152); remove 2: each
+             * net's current route, its phase-two words where live2 marks them
+             * (a later rip-up stage of the same nets) */
+            for (int k = 0; k < count; ++k) {
+                const int id = net_ids[begin + k];
+                rof[k] = remove == 2 && live2 && live2[id] ? routes2 + route_off2[id] : routes1 + route_off1[id];
+            }
             const int s1 = ++*stamp;
             ora_commit_batch(g, demand, timestamp, count, rof, s1, -1, total_wirelength, total_vias, dirty);
         }

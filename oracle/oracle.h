@@ -143,7 +143,7 @@ int ora_zshape_stage(const ora_grid *g, const float *capacity, float *demand, in
                      const long long *route_off, const int *route_cap, int *routes, int *stamp,
                      double *total_wirelength, long long *total_vias, int threads);
 
-/* ---- soft (log-sum-exp) L-shape DP, FP64 (soft.c) ---- */
+/* ---- soft (log-sum-exp) DP over L-shape or Z-shape edges, FP64 (soft.c) ---- */
 
 /* Smooth cost model: wcost, vcost per routing cell (IDX order) and their
  * partial derivatives in slope and zero_slope (any output may be NULL
@@ -153,7 +153,11 @@ void ora_soft_costs(const ora_grid *g, const float *capacity, const float *deman
                     double *dv_dz);
 
 /* Soft DP value F of one net at temperature tau; with gw and gv non-NULL
- * adds dF/dwcost and dF/dvcost (IDX order) into them. */
+ * adds dF/dwcost and dF/dvcost (IDX order) into them.  kind 0: the L-shape
+ * DP's edges; kind 1: the Z-shape route family (straight, both L-shapes,
+ * Z-shapes with the middle leg strictly inside the box). */
+double ora_soft_net(const ora_grid *g, const double *wcost, const double *vcost, int n, const int *nodes,
+                    const int *parents, double tau, int kind, double *gw, double *gv);
 double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double *vcost, int n, const int *nodes,
                            const int *parents, double tau, double *gw, double *gv);
 
@@ -163,6 +167,9 @@ double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double 
 double ora_soft_stage(const ora_grid *g, const float *capacity, const float *demand, double slope, double zero_slope,
                       double via_weight, double tau, int count, const int *node_off, const int *nodes,
                       const int *parents, double *gw, double *gv, double *gparams);
+double ora_soft_stage_kind(const ora_grid *g, const float *capacity, const float *demand, double slope,
+                           double zero_slope, double via_weight, double tau, int kind, int count, const int *node_off,
+                           const int *nodes, const int *parents, double *gw, double *gv, double *gparams);
 
 /* Congestion view (src/routing/congestion.cuh:60-112), X*Y cells in x*Y+y
  * order: worst demand - capacity over the layers, its exclusive prefix
@@ -191,14 +198,16 @@ int ora_flt_batch(const ora_grid *g, const double *presum, const float *vcost, i
                   const long long *route_off, const int *route_cap, int *routes, int *layer_output);
 
 /* One FLT stage (src/routing/flt.cuh:141-168): per batch, optionally remove
- * the nets' phase-one routes (K = -1 at ++*stamp), then at ++*stamp refresh,
- * route and commit the phase-two routes.  Returns the last batch error. */
+ * the nets' routes (K = -1 at ++*stamp; remove 1: phase-one routes, remove 2:
+ * phase-two routes for nets with live2[net] set, phase-one for the others),
+ * then at ++*stamp refresh, route and commit the phase-two routes.  Returns
+ * the last batch error. */
 int ora_flt_stage(const ora_grid *g, const float *capacity, float *demand, int *timestamp, float *wcost,
                   float *vcost, double *presum, int nbatch, const int *batch_off, const int *net_ids,
                   const long long *node_off, const int *node_rec, const long long *edge_off, const int *edge_rec,
                   const long long *route_off1, int *routes1, const long long *route_off2, const int *route_cap2,
-                  int *routes2, int remove_phase_one, int *stamp, double *total_wirelength, long long *total_vias,
-                  int threads);
+                  int *routes2, int remove, const unsigned char *live2, int *stamp, double *total_wirelength,
+                  long long *total_vias, int threads);
 
 /* Evaluator score (resources/evaluator.cpp:458-675) of per-net line lists:
  * metal-0 pin vias, extra pairs, then subnet route pairs shifted up one metal
@@ -208,6 +217,17 @@ int ora_score(int X, int Y, int M, int nets, const int *pin_off, const int *pins
               int per_pin, const int *layer_dir, const double *capacity, const double *overflow_cost,
               const int *x_len, const int *y_len, double unit_wire, double unit_via, double *out,
               unsigned short *demand);
+
+/* The evaluator's completion test per net (resources/evaluator.cpp:527-538,
+ * check_connectivity :677-749) over the same lines as ora_score; subnets with
+ * pin_off2[id] >= 0 (routes2 may be NULL) take their phase-two words.  Pins
+ * of net k: apin_off[k]..apin_off[k+1]; access points of pin p:
+ * acc[acc_off[p]..acc_off[p+1]] as metal-layer cells.  status[k]: 0 complete,
+ * 1 a line the evaluator rejects (open), 2 not connected. */
+int ora_connectivity(int X, int Y, int M, int nets, const int *pin_off, const int *pins, const int *extra_off,
+                     const int *extra, const int *sub_off, const int *subs, const int *routes, const int *pin_acc_num,
+                     int per_pin, const int *routes2, const int *pin_off2, int per_pin2, const int *layer_dir,
+                     const int *apin_off, const int *acc_off, const int *acc, unsigned char *status);
 
 #ifdef __cplusplus
 }

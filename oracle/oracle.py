@@ -125,14 +125,25 @@ class Grid:
         del nsub
         return routes, demand, ts, wl.value, vias.value
 
-    def ripup(self, demand, routes, pin_acc_num, threshold=0):
+    def ripup(self, demand, routes, pin_acc_num, threshold=0, routes2=None, pin_off2=None, live2=None,
+              words_per_pin=100):
+        """Rip-up flags (src/routing/congestion.cuh:114-136).  A net marked in
+        live2 is classified by its phase-two route (routes2 at
+        words_per_pin * pin_off2[net]), as mark_overflow_nets reads
+        routes_phase2 for nets with a rip-up flag (:38-45)."""
         pin_acc_num = _i32(pin_acc_num)
         n = len(pin_acc_num) - 1
         routes = _i32(routes)
         base = routes.ctypes.data
+        if live2 is not None:
+            routes2 = _i32(routes2)
+            base2 = routes2.ctypes.data
         ptrs = (ctypes.POINTER(ctypes.c_int) * max(n, 1))()
         for i in range(n):
-            ptrs[i] = ctypes.cast(base + 4 * ROUTE_PER_PIN * int(pin_acc_num[i]), _i32p)
+            if live2 is not None and live2[i]:
+                ptrs[i] = ctypes.cast(base2 + 4 * words_per_pin * int(pin_off2[i]), _i32p)
+            else:
+                ptrs[i] = ctypes.cast(base + 4 * ROUTE_PER_PIN * int(pin_acc_num[i]), _i32p)
         flags = np.zeros(max(n, 1), np.uint8)
         demand = np.ascontiguousarray(demand, np.float32)
         lib().ora_ripup(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
@@ -172,16 +183,25 @@ class Grid:
         return routes, layer[:nodes_b], rc
 
     def flt_stage(self, plan, pin_acc_num, pin_off2, pins2, demand, ts, routes1, first_stamp, remove_phase_one=True,
-                  words_per_pin=100, wl=0.0, vias=0, threads=1):
+                  words_per_pin=100, wl=0.0, vias=0, threads=1, routes2=None, live2=None):
         """FLT stage continuing from a phase-one state (demand, timestamps,
         phase-one routes at 12 words per pin, updated in place copies);
-        returns (routes2, demand, ts, wl, vias, status, last stamp)."""
+        returns (routes2, demand, ts, wl, vias, status, last stamp).  With
+        routes2 (phase-two words of earlier stages, laid out by pin_off2) and
+        live2 (nets whose current route is there), the stage continues a
+        multi-stage rip-up: each net's current route is removed."""
         pin_acc_num, pin_off2 = _i32(pin_acc_num), _i32(pin_off2)
         route_off1 = (ROUTE_PER_PIN * pin_acc_num[:-1].astype(np.int64)).astype(np.int64)
         route_off2 = (words_per_pin * pin_off2.astype(np.int64)).astype(np.int64)
         route_cap2 = (words_per_pin * np.diff(pin_acc_num)).astype(np.int32)
         routes1 = _i32(routes1).copy()
-        routes2 = np.zeros(words_per_pin * int(pins2) + 1, np.int32)
+        if routes2 is None:
+            routes2 = np.zeros(words_per_pin * int(pins2) + 1, np.int32)
+        else:
+            routes2 = _i32(routes2).copy()
+            assert len(routes2) == words_per_pin * int(pins2) + 1
+        remove = (2 if live2 is not None else 1) if remove_phase_one else 0
+        live = np.ascontiguousarray(live2 if live2 is not None else np.zeros(1), np.uint8)
         demand = np.ascontiguousarray(demand, np.float32).copy()
         ts = _i32(ts).copy()
         w = np.empty(self.cells, np.float32)
@@ -197,7 +217,7 @@ class Grid:
             _p(np.ascontiguousarray(plan["node_off"], np.int64), ctypes.c_longlong), _p(_i32(plan["nodes"]), ctypes.c_int),
             _p(np.ascontiguousarray(plan["edge_off"], np.int64), ctypes.c_longlong), _p(_i32(plan["edges"]), ctypes.c_int),
             _p(route_off1, ctypes.c_longlong), _p(routes1, ctypes.c_int), _p(route_off2, ctypes.c_longlong),
-            _p(route_cap2, ctypes.c_int), _p(routes2, ctypes.c_int), ctypes.c_int(1 if remove_phase_one else 0),
+            _p(route_cap2, ctypes.c_int), _p(routes2, ctypes.c_int), ctypes.c_int(remove), _p(live, ctypes.c_ubyte),
             ctypes.byref(stamp), ctypes.byref(wlc), ctypes.byref(vc), ctypes.c_int(threads))
         return routes2, demand, ts, wlc.value, vc.value, rc, stamp.value
 
@@ -276,31 +296,34 @@ class Grid:
                              *[_p(a, ctypes.c_double) for a in out])
         return out
 
-    def soft_net(self, wcost, vcost, nodes, parents, tau, grads=False):
-        """Soft DP value of one net; with grads also (dF/dwcost, dF/dvcost)."""
+    def soft_net(self, wcost, vcost, nodes, parents, tau, grads=False, kind=0):
+        """Soft DP value of one net (kind 0: L-shape edges, 1: Z-shape route
+        family); with grads also (dF/dwcost, dF/dvcost)."""
         wcost = np.ascontiguousarray(wcost, np.float64)
         vcost = np.ascontiguousarray(vcost, np.float64)
         nodes, parents = _i32(nodes), _i32(parents)
         gw = np.zeros(self.cells, np.float64) if grads else None
         gv = np.zeros(self.cells, np.float64) if grads else None
-        lib().ora_soft_lshape_net.restype = ctypes.c_double
-        F = lib().ora_soft_lshape_net(ctypes.byref(self.c), _p(wcost, ctypes.c_double), _p(vcost, ctypes.c_double),
-                                      ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
-                                      ctypes.c_double(tau), _p(gw, ctypes.c_double), _p(gv, ctypes.c_double))
+        lib().ora_soft_net.restype = ctypes.c_double
+        F = lib().ora_soft_net(ctypes.byref(self.c), _p(wcost, ctypes.c_double), _p(vcost, ctypes.c_double),
+                               ctypes.c_int(len(nodes)), _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int),
+                               ctypes.c_double(tau), ctypes.c_int(kind), _p(gw, ctypes.c_double),
+                               _p(gv, ctypes.c_double))
         return (F, gw, gv) if grads else F
 
     def soft_stage(self, demand, node_off, nodes, parents, tau, slope=0.5, zero_slope=1.5, via_weight=1.0,
-                   grads=False):
+                   grads=False, kind=0):
         """Sum of soft DP values over the nets; with grads also (gw, gv, [dslope, dzero_slope, dvia_weight])."""
         demand = np.ascontiguousarray(demand, np.float32)
         node_off, nodes, parents = _i32(node_off), _i32(nodes), _i32(parents)
         gw = np.zeros(self.cells, np.float64) if grads else None
         gv = np.zeros(self.cells, np.float64) if grads else None
         gp = np.zeros(3, np.float64) if grads else None
-        lib().ora_soft_stage.restype = ctypes.c_double
-        F = lib().ora_soft_stage(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
+        lib().ora_soft_stage_kind.restype = ctypes.c_double
+        F = lib().ora_soft_stage_kind(ctypes.byref(self.c), _p(self.capacity, ctypes.c_float), _p(demand, ctypes.c_float),
                                  ctypes.c_double(slope), ctypes.c_double(zero_slope), ctypes.c_double(via_weight),
-                                 ctypes.c_double(tau), ctypes.c_int(len(node_off) - 1), _p(node_off, ctypes.c_int),
+                                 ctypes.c_double(tau), ctypes.c_int(kind), ctypes.c_int(len(node_off) - 1),
+                                 _p(node_off, ctypes.c_int),
                                  _p(nodes, ctypes.c_int), _p(parents, ctypes.c_int), _p(gw, ctypes.c_double),
                                  _p(gv, ctypes.c_double), _p(gp, ctypes.c_double))
         return (F, gw, gv, gp) if grads else F
@@ -351,6 +374,27 @@ def score(X, Y, M, nets, routes, pin_acc_num, per_pin, layer_dir, capacity, over
                     _p(ovf, ctypes.c_double), _p(xl, ctypes.c_int), _p(yl, ctypes.c_int), ctypes.c_double(unit_wire),
                     ctypes.c_double(unit_via), _p(out, ctypes.c_double), _p(dem, ctypes.c_ushort))
     return dict(wirelength=out[0], via=out[1], overflow=out[2], total=out[3], rejected=int(out[4]), demand=dem)
+
+
+def connectivity(X, Y, M, nets, routes, pin_acc_num, per_pin, layer_dir, access, routes2=None, pin_off2=None,
+                 per_pin2=100):
+    """The evaluator's completion test per net (ora_connectivity): status per
+    net, 0 complete, 1 open (a rejected line), 2 not connected."""
+    arrs = [_i32(nets[k]) for k in ("pin_off", "pins", "extra_off", "extra", "sub_off", "subs")]
+    n = len(arrs[0]) - 1
+    routes = _i32(routes if routes is not None else np.zeros(1, np.int32))
+    acc = _i32(pin_acc_num if pin_acc_num is not None else np.zeros(1, np.int32))
+    r2 = _i32(routes2) if routes2 is not None else None
+    o2 = _i32(pin_off2) if pin_off2 is not None else None
+    ap = [_i32(access[k]) for k in ("pin_off", "access_off", "access")]
+    status = np.zeros(max(n, 1), np.uint8)
+    lib().ora_connectivity(ctypes.c_int(X), ctypes.c_int(Y), ctypes.c_int(M), ctypes.c_int(n),
+                           *[_p(a, ctypes.c_int) for a in arrs], _p(routes, ctypes.c_int), _p(acc, ctypes.c_int),
+                           ctypes.c_int(per_pin), _p(r2, ctypes.c_int) if r2 is not None else None,
+                           _p(o2, ctypes.c_int) if o2 is not None else None, ctypes.c_int(per_pin2),
+                           _p(_i32(layer_dir), ctypes.c_int), *[_p(a, ctypes.c_int) for a in ap],
+                           _p(status, ctypes.c_ubyte))
+    return status[:n]
 
 
 def evaluator_path():

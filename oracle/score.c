@@ -9,9 +9,15 @@
  * the x-major per-layer loss sum follow the evaluator statement by
  * statement, metal 0 included: the evaluator never sets its direction, so it
  * is no routing layer (its wires are rejected, it has no overflow or
- * wirelength, its vias are costed).  The connectivity flood fill
- * (check_connectivity) is not restated.  Parity: pinned against the reference evaluator binary built in
- * oracle/_ref on whole .out files (tests/test_score.py).
+ * wirelength, its vias are costed).  Parity: pinned against the reference
+ * evaluator binary built in oracle/_ref on whole .out files
+ * (tests/test_score.py).
+ *
+ * ora_connectivity restates the evaluator's per-net completion test
+ * (resources/evaluator.cpp:527-538 with check_connectivity, :677-749): the
+ * same flag grid marks, then a flood fill from the first pin's access
+ * points.  Parity: pinned against the same evaluator binary on solutions
+ * with and without planted opens (tests/test_score.py).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -185,5 +191,86 @@ int ora_score(int X, int Y, int M, int nets, const int *pin_off, const int *pins
     out[3] = overflow_cost_total + via_cost + wl_cost;
     out[4] = (double)bad_nets;
     free(s.flag), free(s.wire), free(s.stacked), free(s.vias), free(s.via_loc);
+    return 0;
+}
+
+/* resources/evaluator.cpp:677-749: flood fill over the cells marked with
+ * `mark`, from the first pin's marked access points, along each routing
+ * layer's direction (metal 0 has none) and up and down; every other pin
+ * needs a traced access point. */
+static int connected(line_sink *s, int mark, int traced, int pin0, int npins, const int *acc_off, const int *acc,
+                     int *stack) {
+    const int XY = s->X * s->Y;
+    int top = 0;
+    for (int a = acc_off[pin0]; a < acc_off[pin0 + 1]; ++a) {
+        const int c = acc[a];
+        if (s->flag[c] == mark) {
+            s->flag[c] = traced;
+            stack[top++] = c;
+        }
+    }
+    while (top) {
+        const int c = stack[--top];
+        const int z = c / XY, x = c / s->Y % s->X, y = c % s->Y;
+        int nb[4], n = 0;
+        if (z != 0 && s->dir[z] == 0) {
+            if (x > 0) nb[n++] = c - s->Y;
+            if (x < s->X - 1) nb[n++] = c + s->Y;
+        } else if (z != 0) {
+            if (y > 0) nb[n++] = c - 1;
+            if (y < s->Y - 1) nb[n++] = c + 1;
+        }
+        if (z > 0) nb[n++] = c - XY;
+        if (z < s->M - 1) nb[n++] = c + XY;
+        for (int i = 0; i < n; ++i)
+            if (s->flag[nb[i]] == mark) {
+                s->flag[nb[i]] = traced;
+                stack[top++] = nb[i];
+            }
+    }
+    for (int p = 1; p < npins; ++p) {
+        int ok = 0;
+        for (int a = acc_off[pin0 + p]; a < acc_off[pin0 + p + 1] && !ok; ++a) ok = s->flag[acc[a]] == traced;
+        if (!ok) return 0;
+    }
+    return 1;
+}
+
+int ora_connectivity(int X, int Y, int M, int nets, const int *pin_off, const int *pins, const int *extra_off,
+                     const int *extra, const int *sub_off, const int *subs, const int *routes, const int *pin_acc_num,
+                     int per_pin, const int *routes2, const int *pin_off2, int per_pin2, const int *layer_dir,
+                     const int *apin_off, const int *acc_off, const int *acc, unsigned char *status) {
+    const size_t XY = (size_t)X * Y, cells = XY * (size_t)M;
+    line_sink s;
+    memset(&s, 0, sizeof(s));
+    s.X = X, s.Y = Y, s.M = M, s.dir = layer_dir;
+    s.flag = (int *)malloc(sizeof(int) * cells);
+    s.wire = (int *)calloc(cells, sizeof(int));
+    s.stacked = (int *)calloc(cells, sizeof(int));
+    s.vias = (unsigned long long *)calloc((size_t)M, sizeof(unsigned long long));
+    int *stack = (int *)malloc(sizeof(int) * cells); /* every cell is pushed at most once per net */
+    for (size_t i = 0; i < cells; ++i) s.flag[i] = -1;
+    for (int k = 0; k < nets; ++k) {
+        s.net = k, s.nvia = 0, s.bad = 0;
+        for (int j = pin_off[k]; j < pin_off[k + 1]; ++j)
+            if ((size_t)pins[j] < XY) take_line(&s, pins[j], pins[j] + (int)XY);
+        for (int j = extra_off[k]; j + 1 < extra_off[k + 1]; j += 2) take_line(&s, extra[j], extra[j + 1]);
+        for (int t = sub_off[k]; t < sub_off[k + 1]; ++t) {
+            const int id = subs[t], np = pin_acc_num[id + 1] - pin_acc_num[id];
+            const int two = routes2 && pin_off2 && pin_off2[id] >= 0;
+            const int *r = two ? routes2 + (size_t)pin_off2[id] * per_pin2 : routes + (size_t)pin_acc_num[id] * per_pin;
+            const int cap = np * (two ? per_pin2 : per_pin);
+            const int words = r[0] < cap ? r[0] : cap;
+            for (int j = 1; j + 1 <= words - 1; j += 2) take_line(&s, r[j] + (int)XY, r[j + 1] + (int)XY);
+        }
+        stack_vias(&s);
+        if (s.bad) status[k] = 1; /* has_connectivity_violation: counted open (:529-530) */
+        else if (apin_off[k + 1] > apin_off[k] &&
+                 !connected(&s, k, k + nets, apin_off[k], apin_off[k + 1] - apin_off[k], acc_off, acc, stack))
+            status[k] = 2;
+        else
+            status[k] = 0;
+    }
+    free(s.flag), free(s.wire), free(s.stacked), free(s.vias), free(s.via_loc), free(stack);
     return 0;
 }

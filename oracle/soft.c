@@ -143,8 +143,126 @@ typedef struct {
     int x, y, px, py, straight;
 } edge_geom;
 
+/* ---- Z-shape family (kind 1) ----
+ * The route family the Z-shape DP prices per tree edge
+ * (src/routing/zshape.cuh:62-288): from child (x, y) to parent (px, py) a
+ * straight wire on one layer of the edge's direction when they share a
+ * column or row (a column when they share both); otherwise the L-shape
+ * through (x, py) (first leg vertical) or through (px, y) (first leg
+ * horizontal), and the Z-shapes whose middle leg crosses the box on a row
+ * strictly between y and py or a column strictly between x and px; each leg
+ * on one layer of its direction, a via stack at each bend costing
+ * vcost(k) for k from the lower layer up to below the upper one.  The edge
+ * leaves the child on its first leg's layer s and arrives on its last leg's
+ * layer e.  visit() calls back every candidate (s, e, cost); the adjoint
+ * variant scatters a weight to the candidate's wire and via cells. */
+typedef struct {
+    const ora_grid *g;
+    const double *wcost, *vcost;
+    double *gw, *gv; /* NULL in the forward pass */
+    double w;        /* the adjoint weight of the current candidate */
+} zsoft;
+
+static double zleg(zsoft *z, int l, int along_y, int fixed, int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    if (z->gw) leg_grad(z->g, z->gw, l, along_y, fixed, lo, hi, z->w);
+    return leg(z->g, z->wcost, l, along_y, fixed, lo, hi);
+}
+
+static double zstack(zsoft *z, int x, int y, int a, int b) {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    double s = 0.0;
+    for (int k = lo; k < hi; ++k) {
+        s += z->vcost[IDX3(z->g, k, x, y)];
+        if (z->gv) z->gv[IDX3(z->g, k, x, y)] += z->w;
+    }
+    return s;
+}
+
+/* One pass over the candidates of an edge.  Forward (z->gw == NULL): the
+ * online soft minimum per arriving layer into mn[e], sm[e].  Adjoint: each
+ * candidate's weight gdiag[e] * exp(-(best[s] + cost - diag[e]) / tau) goes
+ * to gbest[s] and to its cells (the cost is evaluated once to get the
+ * weight, then again with the weight set to scatter it). */
+static void zcandidate(zsoft *z, int s, int e, double cost, const double *best, double tau, double *mn, double *sm,
+                       const double *diag, const double *gdiag, double *gbest, int *rescatter) {
+    const double b = best[s];
+    if (b >= ORA_BIG) return;
+    const double x = b + cost;
+    if (!z->gw) {
+        if (x < mn[e]) {
+            sm[e] = (mn[e] >= ORA_BIG ? 0.0 : sm[e] * exp(-(mn[e] - x) / tau)) + 1.0;
+            mn[e] = x;
+        } else {
+            sm[e] += exp(-(x - mn[e]) / tau);
+        }
+        return;
+    }
+    const double w = gdiag[e] * weight(x, diag[e], tau);
+    gbest[s] += w;
+    z->w = w;
+    *rescatter = w != 0.0;
+}
+
+#define ZCAND(S, E, EXPR)                                                                \
+    do {                                                                                 \
+        double *gw_ = z->gw, *gv_ = z->gv;                                               \
+        int again = 0;                                                                   \
+        z->gw = NULL, z->gv = NULL;                                                      \
+        const double c_ = (EXPR);                                                        \
+        z->gw = gw_, z->gv = gv_;                                                        \
+        zcandidate(z, S, E, c_, best, tau, mn, sm, diag, gdiag, gbest, &again);          \
+        if (again) (void)(EXPR);                                                         \
+    } while (0)
+
+static void zedge(zsoft *z, int x, int y, int px, int py, const double *best, double tau, double *mn, double *sm,
+                  const double *diag, const double *gdiag, double *gbest) {
+    const ora_grid *g = z->g;
+    const int L = g->L;
+    const int minx = x < px ? x : px, maxx = x < px ? px : x, miny = y < py ? y : py, maxy = y < py ? py : y;
+    if (x == px) {
+        for (int a = 0; a < L; ++a)
+            if (layer_dir(g, a)) ZCAND(a, a, zleg(z, a, 1, x, miny, maxy));
+        return;
+    }
+    if (y == py) {
+        for (int a = 0; a < L; ++a)
+            if (!layer_dir(g, a)) ZCAND(a, a, zleg(z, a, 0, y, minx, maxx));
+        return;
+    }
+    for (int a = 0; a < L; ++a)
+        for (int h = 0; h < L; ++h) {
+            if (layer_dir(g, a) != 1 || layer_dir(g, h) != 0) continue;
+            /* vertical then horizontal through (x, py) */
+            ZCAND(a, h, zleg(z, a, 1, x, y, py) + zstack(z, x, py, a, h) + zleg(z, h, 0, py, x, px));
+            /* horizontal then vertical through (px, y): first layer h, last a */
+            ZCAND(h, a, zleg(z, h, 0, y, x, px) + zstack(z, px, y, h, a) + zleg(z, a, 1, px, y, py));
+        }
+    for (int b = miny + 1; b < maxy; ++b) /* vertical, horizontal on row b, vertical */
+        for (int a = 0; a < L; ++a)
+            for (int h = 0; h < L; ++h)
+                for (int v = 0; v < L; ++v) {
+                    if (layer_dir(g, a) != 1 || layer_dir(g, h) != 0 || layer_dir(g, v) != 1) continue;
+                    ZCAND(a, v, zleg(z, a, 1, x, y, b) + zstack(z, x, b, a, h) + zleg(z, h, 0, b, x, px) +
+                                    zstack(z, px, b, h, v) + zleg(z, v, 1, px, b, py));
+                }
+    for (int b = minx + 1; b < maxx; ++b) /* horizontal, vertical on column b, horizontal */
+        for (int a = 0; a < L; ++a)
+            for (int v = 0; v < L; ++v)
+                for (int h = 0; h < L; ++h) {
+                    if (layer_dir(g, a) != 0 || layer_dir(g, v) != 1 || layer_dir(g, h) != 0) continue;
+                    ZCAND(a, h, zleg(z, a, 0, y, x, b) + zstack(z, b, y, a, v) + zleg(z, v, 1, b, y, py) +
+                                    zstack(z, b, py, v, h) + zleg(z, h, 0, py, b, px));
+                }
+}
+
 double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double *vcost, int n, const int *nodes,
                            const int *parents, double tau, double *gw, double *gv) {
+    return ora_soft_net(g, wcost, vcost, n, nodes, parents, tau, 0, gw, gv);
+}
+
+double ora_soft_net(const ora_grid *g, const double *wcost, const double *vcost, int n, const int *nodes,
+                    const int *parents, double tau, int kind, double *gw, double *gv) {
     const int L = g->L, X = g->X, Y = g->Y, XY = X * Y, T = L * (L + 1) / 2;
     double *D = (double *)calloc((size_t)n * T, sizeof(double));
     double *best = (double *)calloc((size_t)n * L, sizeof(double));
@@ -191,7 +309,14 @@ double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double 
             W1[(size_t)i * L + c] = w1;
             W2[(size_t)i * L + c] = w2;
         }
-        for (int pl = 0; pl < L; ++pl) {
+        if (kind == 1) {
+            double mn[32], sm[32];
+            for (int e = 0; e < L; ++e) mn[e] = ORA_INF, sm[e] = 0.0;
+            zsoft z = {g, wcost, vcost, NULL, NULL, 0.0};
+            zedge(&z, x, y, px, py, best + (size_t)i * L, tau, mn, sm, NULL, NULL, NULL);
+            for (int e = 0; e < L; ++e) diag[(size_t)i * L + e] = mn[e] >= ORA_BIG ? ORA_INF : mn[e] - tau * log(sm[e]);
+        }
+        for (int pl = 0; pl < L && kind == 0; ++pl) {
             const int d = layer_dir(g, pl);
             double val = ORA_INF;
             if (straight) {
@@ -241,7 +366,11 @@ double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double 
                 }
             const int x = e.x, y = e.y, px = e.px, py = e.py;
             const int minx = x < px ? x : px, maxx = x < px ? px : x, miny = y < py ? y : py, maxy = y < py ? py : y;
-            for (int pl = 0; pl < L; ++pl) {
+            if (kind == 1) {
+                zsoft z = {g, wcost, vcost, gw, gv, 0.0};
+                zedge(&z, x, y, px, py, best + (size_t)c * L, tau, NULL, NULL, dg, gdiag, gbest);
+            }
+            for (int pl = 0; pl < L && kind == 0; ++pl) {
                 if (gdiag[pl] == 0.0) continue;
                 const int d = layer_dir(g, pl);
                 if (e.straight) {
@@ -301,6 +430,13 @@ double ora_soft_lshape_net(const ora_grid *g, const double *wcost, const double 
 double ora_soft_stage(const ora_grid *g, const float *capacity, const float *demand, double slope, double zero_slope,
                       double via_weight, double tau, int count, const int *node_off, const int *nodes,
                       const int *parents, double *gw, double *gv, double *gparams) {
+    return ora_soft_stage_kind(g, capacity, demand, slope, zero_slope, via_weight, tau, 0, count, node_off, nodes,
+                               parents, gw, gv, gparams);
+}
+
+double ora_soft_stage_kind(const ora_grid *g, const float *capacity, const float *demand, double slope,
+                           double zero_slope, double via_weight, double tau, int kind, int count, const int *node_off,
+                           const int *nodes, const int *parents, double *gw, double *gv, double *gparams) {
     const size_t cells = (size_t)g->L * g->X * g->Y;
     double *w = (double *)malloc(sizeof(double) * cells), *v = (double *)malloc(sizeof(double) * cells);
     double *ws = (double *)malloc(sizeof(double) * cells), *wz = (double *)malloc(sizeof(double) * cells);
@@ -314,8 +450,8 @@ double ora_soft_stage(const ora_grid *g, const float *capacity, const float *dem
     double F = 0.0;
     for (int k = 0; k < count; ++k) {
         const int b = node_off[k];
-        F += ora_soft_lshape_net(g, w, v, node_off[k + 1] - b, nodes + b, parents + b, tau, grads ? gw : NULL,
-                                 grads ? gv : NULL);
+        F += ora_soft_net(g, w, v, node_off[k + 1] - b, nodes + b, parents + b, tau, kind, grads ? gw : NULL,
+                          grads ? gv : NULL);
     }
     if (grads && gparams) {
         double s = 0.0, z = 0.0, u = 0.0;

@@ -275,3 +275,45 @@ def test_flt_wide_nodes_and_long_chains_match_oracle(ga, ora, mode):
     assert np.array_equal(ctx.timestamps(), ts)
     assert ctx.stats() == (wl, vias)
     ctx.close()
+
+
+def test_reserve_phase_two_keeps_routed_words_at_a_wider_stride(ga, ora):
+    """gr_ctx_reserve_routes_phase2 after an FLT stage: nets routed by the
+    stage keep their offsets and words (copied to the wider stride), new
+    nets are appended in the order given, with empty words; reserving nets
+    that already have words changes nothing; the tuner leaves earlier
+    stages' words in place."""
+    I = flt_instance(ga, ora, 3, 7, 0.01, 64)
+    t, fs, plan = I["t"], I["fs"], I["plan"]
+    print("reserve phase-two storage after an FLT stage", flush=True)
+    pin_acc = np.asarray(t["pin_acc_num"], np.int64)
+    ctx = _gpu_phase_one(ga, I)
+    first = len(I["sched"]["batch_off"])
+    ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
+    ctx.flt_upload(plan)
+    ctx.flt_run(first_stamp=first, remove_phase_one=True, mode=1)
+    before = ctx.routes2()
+    ctx.flt_tune()
+    assert np.array_equal(ctx.routes2(), before)
+    off0 = np.asarray(fs["pin_off2"], np.int64)
+    kept = np.flatnonzero(off0 >= 0)
+    fresh = np.flatnonzero(off0 < 0)[::-3][:40]  # appended in this (decreasing) order
+    ids = np.concatenate([fresh, kept[:5]])
+    off, pins2 = ctx.reserve_routes_phase2(ids, words_per_pin=128)
+    assert np.array_equal(off[kept], off0[kept])
+    expect, p = off0.copy(), int(fs["pins2"])
+    for i in fresh:
+        expect[i] = p
+        p += int(pin_acc[i + 1] - pin_acc[i])
+    assert np.array_equal(off, expect) and pins2 == p
+    after = ctx.routes2()
+    assert len(after) == 128 * p + 1
+    for i in kept:
+        n = int(pin_acc[i + 1] - pin_acc[i]) * 100
+        assert np.array_equal(after[128 * off[i]:128 * off[i] + n], before[100 * off0[i]:100 * off0[i] + n]), i
+    for i in fresh:
+        assert after[128 * off[i]] == 0
+    off2, pins3 = ctx.reserve_routes_phase2(kept[:3], words_per_pin=100)
+    assert np.array_equal(off2, off) and pins3 == p
+    assert np.array_equal(ctx.routes2(), after)
+    ctx.close()

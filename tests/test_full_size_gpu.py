@@ -183,3 +183,53 @@ def test_full_grid_zshape_stage_matches_oracle(ga, ora):
     assert np.array_equal(ctx.timestamps(), ts_o)
     assert ctx.stats() == (wl_o, vias_o)
     ctx.close()
+
+
+@pytest.mark.timeout(600)
+def test_full_size_repeated_ripup_stages_match_oracle(ga, ora, full):
+    """Three rip-up -> FLT stages (configuration 5's pipeline), at
+    configurations 3 and 5: per stage the rip-up flags (phase-two routes of
+    nets an earlier stage rerouted), congestion view, FLT batches, appended
+    phase-two storage and the stage itself (each net's current route
+    removed) equal the oracle's; the stages run in the persistent, launch
+    and plain forms in turn."""
+    from tests import _ripup as R
+    if full["cfg"] == 2:
+        pytest.skip("configuration 2 is the Z-shape workload")
+    t, g, d, s = full["t"], full["g"], full["d"], full["sched"]
+    print(f"full repeated rip-up stages configuration {full['cfg']}", flush=True)
+    routes_o, demand_o, ts_o, wl_o, vias_o = _oracle_stage(full)
+    nets = len(t["pin_acc_num"]) - 1
+    ctx = ga.Context(t)
+    ctx.set_routes(t["pin_acc_num"])
+    ctx.lshape_upload(s["net_ids"], s["node_off"], s["nodes"], s["parents"])
+    ctx.lshape_run(s["batch_off"], first_stamp=1, mode=2)
+    done = []
+
+    def on_stage(o):
+        k, fs = o["stage"], o["fs"]
+        print(f"full repeated rip-up configuration {full['cfg']} stage {k}: {fs['nets']} nets", flush=True)
+        flags = ctx.ripup(nets)
+        assert np.array_equal(flags, o["flags"]), k
+        cong, xs, ys = ctx.congestion_view()
+        assert np.array_equal(cong, o["cong"])
+        assert np.array_equal(_bits(xs), _bits(o["xs"])) and np.array_equal(_bits(ys), _bits(o["ys"]))
+        color, nb = ctx.batch_plan(*_geom(fs), limit=300000)
+        assert nb == o["nb"] and np.array_equal(color, o["color"])
+        if k == 1:
+            ctx.set_routes_phase2(fs["pin_off2"], fs["pins2"])
+        else:
+            off, pins2 = ctx.reserve_routes_phase2(np.flatnonzero(flags))
+            assert np.array_equal(off, o["off"]) and pins2 == o["pins2"]
+        ctx.flt_upload(o["plan"])
+        ctx.flt_run(first_stamp=o["first"], remove=1 if k == 1 else 2, mode={1: 2, 2: 1, 3: 0}[k])
+        assert np.array_equal(ctx.routes2(), o["r2"]), k
+        assert np.array_equal(_bits(ctx.demand()), _bits(o["demand"])), k
+        assert np.array_equal(ctx.timestamps(), o["ts"]), k
+        assert ctx.stats() == (o["wl"], o["vias"]), k
+        done.append(k)
+
+    st = R.oracle_stages(ora, d, t, g, full["nb"], routes_o, demand_o, ts_o, wl_o, vias_o, 3, threads=THREADS,
+                         on_stage=on_stage)
+    assert done == [1, 2, 3] and min(st["nets"]) > 0
+    ctx.close()

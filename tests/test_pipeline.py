@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from tests import _golden as G
+from tests import _ripup as R
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CLI = os.path.join(ROOT, "grotor_amd", "grotor")
@@ -46,9 +47,12 @@ def test_cli_rejects_bad_arguments(args, message):
     assert res.stderr.startswith("GRotor error:") and message in res.stderr, res.stderr
 
 
-def oracle_pipeline(ga, ora, cap, net, out):
+def oracle_pipeline(ga, ora, cap, net, out, stages=1, counts=None):
     """The pipeline of src/pipeline.hpp on the CPU oracle: L-shape stage,
-    congestion view, rip-up, FLT DAGs and the FLT stage, then .out."""
+    congestion view, rip-up, FLT DAGs and the FLT stage, then .out; with
+    stages > 1 the rip-up -> FLT stage repeats (tests/_ripup.py).  Returns
+    (wirelength, vias, nets holding phase-two routes); counts, when a list,
+    receives each stage's nets."""
     d = ga.Design.read(cap, net)
     d.prepare(8)
     t = d.topology()
@@ -57,20 +61,14 @@ def oracle_pipeline(ga, ora, cap, net, out):
     color, nb = ora.batches(t["X"], t["Y"], *geom)
     sched = ga.execution_schedule(t, color, nb)
     routes, demand, ts, wl, vias = g.lshape_stage(sched, t["pin_acc_num"], threads=4)
-    cong, xs, ys = g.congestion_view(demand)
-    flags = g.ripup(demand, routes, t["pin_acc_num"])
-    fs = d.flt_stage(flags, xs, ys)
-    if fs["nets"] == 0:
+    st = R.oracle_stages(ora, d, t, g, nb, routes, demand, ts, wl, vias, stages)
+    if counts is not None:
+        counts.extend(st["nets"])
+    if st["off"] is None:
         d.write_output(routes, out)
-        return wl, vias, 0
-    fgeom = (fs["pt_off"], fs["pts"], fs["hs_off"], fs["hsegs"], fs["vs_off"], fs["vsegs"])
-    fcol, fnb = ora.batches(t["X"], t["Y"], *fgeom, limit=300000)
-    plan = d.flt_plan(fcol, fnb)
-    r2, _, _, wl2, vias2, rc, _ = g.flt_stage(plan, t["pin_acc_num"], fs["pin_off2"], fs["pins2"], demand, ts, routes,
-                                              nb + 1, wl=wl, vias=vias)
-    assert rc == 0
-    d.write_output(routes, out, r2, fs["pin_off2"])
-    return wl2, vias2, fs["nets"]
+        return st["wl"], st["vias"], 0
+    d.write_output(routes, out, st["r2"], st["off"])
+    return st["wl"], st["vias"], int(np.count_nonzero(st["off"] >= 0))
 
 
 BATCH_RE = re.compile(r"^BATCH_TIMING backend=(\w+) start_us=(\d+) end_us=(\d+) elapsed_ms=([\d.]+) nets=(\d+) "
@@ -140,6 +138,7 @@ def test_cli_pipeline_equals_oracle_pipeline(ga, ora, name, tmp_path):
         assert res_s.returncode == 0, res_s.stderr[-2000:]
         score = [line for line in res_s.stdout.splitlines() if line.startswith("SCORE wirelength=")][0]
         assert f"wirelength={wl:.0f} vias={vias} rerouted={rerouted}" in score, score
+        assert "SCORE_CONNECTIVITY open=0 not_connected=0 incomplete=0" in res_s.stdout
         if rerouted == 0:
             continue
         res = _run(["-cap", cap, "-net", net, "-out", mine, "--batching", backend, "--profile-stages", "1",
@@ -147,3 +146,78 @@ def test_cli_pipeline_equals_oracle_pipeline(ga, ora, name, tmp_path):
         assert res.returncode == 0, res.stderr[-2000:]
         rows = check_run_log(res.stdout, backend, "--validate-batches" in extra)
         assert rows[1]["nets"] == rerouted
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", G.NAMES)
+def test_cli_repeated_ripup_stages_equal_oracle(ga, ora, name, tmp_path):
+    """--flt-stages 3: the later stages reroute the nets whose current
+    (phase-two) routes still cross more than 10 overflowed edges; the .out,
+    statistics and per-stage records equal the oracle's three-stage
+    pipeline."""
+    cap, net = G.unpacked(name, "cap"), G.unpacked(name, "net")
+    ref = str(tmp_path / "oracle3.out")
+    print(f"pipeline {name} --flt-stages 3", flush=True)
+    counts = []
+    wl, vias, rerouted = oracle_pipeline(ga, ora, cap, net, ref, stages=3, counts=counts)
+    if name == "syn10":
+        assert counts[1] > 0  # the instance reroutes nets again in stage 2
+    mine = str(tmp_path / "gpu3.out")
+    res = _run(["-cap", cap, "-net", net, "-out", mine, "--batching", "hiprt", "--flt-stages", "3"])
+    assert res.returncode == 0, res.stderr[-2000:]
+    assert open(mine).read() == open(ref).read()
+    score = [line for line in res.stdout.splitlines() if line.startswith("SCORE wirelength=")][0]
+    assert f"wirelength={wl:.0f} vias={vias} rerouted={rerouted}" in score, score
+    assert "SCORE_CONNECTIVITY open=0 not_connected=0 incomplete=0" in res.stdout
+    dags = DAG_RE.findall(res.stdout)
+    expect = ["lshape"] + ["detour" + ("" if k == 0 else str(k + 1)) for k, n in enumerate(counts) if n]
+    assert [x[0] for x in dags] == expect, dags
+    assert [int(x[4]) for x in dags[1:]] == [n for n in counts if n]
+
+
+def test_oracle_repeated_stages_keep_statistics_of_current_routes(ga, ora):
+    """After every rip-up stage the oracle's wirelength and via statistics
+    equal those of the routes the nets hold now (phase-two words for nets an
+    FLT stage routed): a removal that read a net's stale phase-one route
+    after its phase-two route replaced it would leave both off."""
+    cap, net = G.unpacked("syn10", "cap"), G.unpacked("syn10", "net")
+    d = ga.Design.read(cap, net)
+    d.prepare(4)
+    t = d.topology()
+    g = ora.Grid(t)
+    geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
+    color, nb = ora.batches(t["X"], t["Y"], *geom)
+    sched = ga.execution_schedule(t, color, nb)
+    routes, demand, ts, wl, vias = g.lshape_stage(sched, t["pin_acc_num"], threads=4)
+    assert R.current_route_totals(t, routes, dict(live=None)) == (wl, vias)
+    seen = []
+
+    def check(s):
+        live = np.zeros(len(s["off"]), np.uint8)
+        live[s["off"] >= 0] = 1  # every net with phase-two words has been routed there by now
+        totals = R.current_route_totals(t, routes, dict(live=live, off=s["off"], r2=s["r2"]))
+        assert totals == (s["wl"], s["vias"]), (s["stage"], totals, s["wl"], s["vias"])
+        seen.append(s["stage"])
+
+    st = R.oracle_stages(ora, d, t, g, nb, routes, demand, ts, wl, vias, 3, on_stage=check)
+    assert seen == [1, 2, 3] and st["nets"][1] > 0
+    # the rip-up of stage 2 read phase-two routes: the nets it flags are
+    # classified by them, not by their phase-one routes
+    flags_now = g.ripup(st["demand"], routes, t["pin_acc_num"], routes2=st["r2"], pin_off2=st["off"], live2=st["live"])
+    flags_stale = g.ripup(st["demand"], routes, t["pin_acc_num"])
+    assert not np.array_equal(flags_now, flags_stale)
+
+    # the same run with the stale removal planted in the oracle's stage
+    # (phase-one routes removed again in stage 2) breaks the invariant
+    class StaleRemoval:
+        def __getattr__(self, name):
+            return getattr(g, name)
+
+        def flt_stage(self, *args, **kw):
+            kw["live2"] = None
+            return g.flt_stage(*args, **kw)
+
+    seen.clear()
+    with pytest.raises(AssertionError, match=r"^\(2, "):
+        R.oracle_stages(ora, d, t, StaleRemoval(), nb, routes, demand, ts, wl, vias, 2, on_stage=check)
+    assert seen == [1]

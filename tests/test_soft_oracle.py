@@ -34,14 +34,15 @@ def _fd_tol(F, h):
     return 8 * np.finfo(np.float64).eps * max(1.0, abs(F)) / h
 
 
+@pytest.mark.parametrize("kind", [0, 1])
 @pytest.mark.parametrize("tau", [0.1, 1.0, 10.0])
-def test_soft_adjoint_matches_finite_differences(ora, tau):
+def test_soft_adjoint_matches_finite_differences(ora, tau, kind):
     _, t = topology(6, 0)
     g, w, v = _costs(ora, t, 1)
     rng = np.random.default_rng(7)
     checked = {"w": 0, "v": 0}
     for nodes, parents in _nets(t, 8, min_nodes=3):
-        F, gw, gv = g.soft_net(w, v, nodes, parents, tau, grads=True)
+        F, gw, gv = g.soft_net(w, v, nodes, parents, tau, grads=True, kind=kind)
         for name, arr, grad in (("w", w, gw), ("v", v, gv)):
             used = np.flatnonzero(np.abs(grad) > 1e-3)
             checked[name] += min(6, len(used))
@@ -49,23 +50,23 @@ def test_soft_adjoint_matches_finite_differences(ora, tau):
                 h = 1e-5 * max(1.0, abs(arr[cell]))
                 a = arr.copy()
                 a[cell] += h
-                fp = g.soft_net(a if arr is w else w, a if arr is v else v, nodes, parents, tau)
+                fp = g.soft_net(a if arr is w else w, a if arr is v else v, nodes, parents, tau, kind=kind)
                 a[cell] -= 2 * h
-                fm = g.soft_net(a if arr is w else w, a if arr is v else v, nodes, parents, tau)
+                fm = g.soft_net(a if arr is w else w, a if arr is v else v, nodes, parents, tau, kind=kind)
                 fd = (fp - fm) / (2 * h)
                 assert abs(fd - grad[cell]) <= 1e-6 * max(1.0, abs(fd)) + _fd_tol(F, h), (cell, fd, grad[cell])
     assert checked["w"] >= 20 and checked["v"] >= 10, checked
 
 
-def test_soft_gradients_are_expected_usage(ora):
+@pytest.mark.parametrize("kind", [0, 1])
+def test_soft_gradients_are_expected_usage(ora, kind):
     """dF/dcost is a probability-weighted count: nonnegative, and on a
-    two-pin net the wire gradients along the chosen track sum to the
-    expected number of unit edges (at most the Manhattan length per layer
-    pair)."""
+    two-pin net the wire gradients sum to the expected number of unit edges,
+    the Manhattan length (every route of both families is monotone)."""
     _, t = topology(6, 1)
     g, w, v = _costs(ora, t, 2)
     for nodes, parents in _nets(t, 20, min_nodes=2, max_nodes=2):
-        F, gw, gv = g.soft_net(w, v, nodes, parents, 1.0, grads=True)
+        F, gw, gv = g.soft_net(w, v, nodes, parents, 1.0, grads=True, kind=kind)
         assert gw.min() >= -1e-12 and gv.min() >= -1e-12
         a, b = int(nodes[0]), int(nodes[1])
         Y, X = t["Y"], t["X"]
@@ -202,3 +203,41 @@ def test_soft_adjoint_matches_torch_autograd(ora):
         assert abs(Ft.item() - F) <= 1e-9 * max(1.0, abs(F))
         assert np.allclose(wt.grad.numpy(), gw, rtol=1e-9, atol=1e-12)
         assert np.allclose(vt.grad.numpy(), gv, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_soft_zshape_tends_to_enumerated_optimum(ora, L):
+    """The Z-shape family (kind 1) at tau -> 0 against the exhaustive
+    minimum over the Z-shape route space on tiny boxes (the enumeration of
+    test_zshape_oracle.py: every node interval, every edge shape, bend and
+    leg layer): with integer wire steps and via costs of at least 1 its
+    integer costs are the real ones, so the soft value lies within
+    tau * log(number of alternatives) below the optimum."""
+    from tests.test_zshape_oracle import _brute_force, _random_tree, _tiny_instance
+    rng = np.random.default_rng(40 + L)
+    done = 0
+    for trial in range(30):
+        X, Y = int(rng.integers(3, 6)), int(rng.integers(3, 6))
+        DIR = int(rng.integers(0, 2))
+        grid, presum, vcost = _tiny_instance(rng, X, Y, L, DIR)
+        vcost = np.maximum(vcost, 1).astype(np.float32)
+        tree = _random_tree(rng, X, Y, L, int(rng.integers(2, 5)))
+        want = _brute_force(grid, presum, vcost, tree)
+        # wcost[p] is the step from track position p to p + 1
+        ps = presum.reshape(L, X, Y)
+        w = np.zeros((L, X, Y))
+        for l in range(L):
+            if (l & 1) ^ DIR:  # along y
+                w[l, :, :-1] = ps[l, :, 1:] - ps[l, :, :-1]
+            else:
+                w[l, :-1, :] = ps[l, 1:, :] - ps[l, :-1, :]
+        t = dict(grid, nets=0)
+        g = ora.Grid(t)
+        nodes = [(L if pin is None else pin - 1) * X * Y + x * Y + y for x, y, pin, _, _ in tree]
+        parents = [max(p, 0) for _, _, _, p, _ in tree]
+        for tau in (1e-2, 1e-3):
+            soft = g.soft_net(w.reshape(-1), vcost.astype(np.float64), nodes, parents, tau, kind=1)
+            alternatives = len(tree) * np.log(L * L) + (len(tree) - 1) * np.log(L ** 3 * (X + Y + 2))
+            assert soft <= want + 1e-9 and want - soft <= tau * alternatives + 1e-9, (trial, tau, soft, want)
+        done += 1
+    assert done == 30
