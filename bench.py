@@ -465,11 +465,12 @@ def bench_pipeline(args, ga, design, topo):
                                               "flt": round(la["flt_ms"] / pe["flt_ms"], 4)},
                    "forms": {"plain": "the reference's execution: per batch one launch per kernel, one net or DAG "
                                       "per wavefront, DP tables in global memory",
-                             "launch": "per batch one launch per kernel, DP tables in LDS, autotuned nets (DAGs) "
-                                       "per wavefront",
-                             "persistent": "per batch one work-queue DP launch draining the nets (DAGs) tallest "
-                                           "first, DP tables in LDS, autotuned nets (DAGs) per wavefront, each "
-                                           "stage's batch loop captured in a HIP graph"},
+                             "launch": "per batch one launch per kernel (L-shape DP: one per tree-size class, "
+                                       "on two streams), DP tables in LDS, autotuned nets (DAGs) per wavefront",
+                             "persistent": "per batch (L-shape: per tree-size class, on two streams) one "
+                                           "work-queue DP launch draining the nets (DAGs) tallest first, DP tables "
+                                           "in LDS, autotuned nets (DAGs) per wavefront, each stage's batch loop "
+                                           "captured in a HIP graph"},
                    "forms_identical": identical,
                    "evaluator_cost": b["score"], "evaluator_rejected_nets": b["rejected"],
                    "total_wirelength": b["wl"], "total_vias": b["vias"],

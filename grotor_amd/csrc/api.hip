@@ -639,13 +639,15 @@ void launch_lshape_plain(gr_ctx* c, const BatchLaunch& b) {
     hip_check(hipGetLastError(), "lshape_dp plain kernel");
 }
 
-// The batch's nets drained by the persistent DP kernel from its queue
-// counters (zero beforehand); trees too large for LDS take the launch form.
-void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
-    if (!b.count[0]) return;
-    const int p = per_wave_for(c, 0, b.nmax[0]);
+// Class k of the batch drained by the persistent DP kernel from its queue
+// counters (zero beforehand), on `stream`; trees too large for LDS take the
+// launch form.
+void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue, int k = 0, hipStream_t stream = nullptr) {
+    if (!b.count[k]) return;
+    if (!stream) stream = c->stream;
+    const int p = per_wave_for(c, k, b.nmax[k]);
     if (p <= 0) {
-        launch_lshape_class(c, b, 0);
+        launch_lshape_class(c, b, k, 0, stream);
         return;
     }
     LshapeArgs a;
@@ -663,14 +665,34 @@ void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
     a.queue = queue;
     a.cus = c->cus;
     a.begin = b.begin;
-    a.count = b.count[0];
-    a.nmax = b.nmax[0];
+    for (int j = 0; j < k; ++j) a.begin += b.count[j];
+    a.count = b.count[k];
+    a.nmax = b.nmax[k];
     a.per_wave = p;
     double alg = 0;
     if (c->profiling)
         for (int i = a.begin; i < a.begin + a.count; ++i) alg += c->slot_dp_bytes[c->slots_h[i]];
-    timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, c->stream); });
+    timed(c, GR_K_LSHAPE, alg, [&] { launch_lshape_dp(a, (a.count + p - 1) / p, true, stream); }, stream);
     hip_check(hipGetLastError(), "lshape_dp queue kernel");
+}
+
+// Persistent form of one batch, by size class (plan_batch): each class
+// drained from its own queue (kDpQueues counters at queue + k * kDpQueues *
+// kDpQueueStride), so the many small trees keep a small LDS footprint and a
+// high occupancy; the large trees (the batch's critical path) run on the
+// side stream beside them.
+void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
+    const size_t qs = size_t(kDpQueues) * kDpQueueStride;
+    if (!b.count[0] || !b.count[1]) {
+        for (int k = 0; k < kTreeClasses; ++k) launch_lshape_queue(c, b, queue + k * qs, k, c->stream);
+        return;
+    }
+    hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
+    launch_lshape_queue(c, b, queue + qs, 1, c->side);
+    launch_lshape_queue(c, b, queue, 0, c->stream);
+    hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
 }
 
 // Reserve the global DP scratch the batches will need (trees whose tables
@@ -1291,26 +1313,25 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         for (int b = 0; b < nbatch; ++b) {
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
             const int begin = batch_off[b], count = batch_off[b + 1] - batch_off[b];
-            plan[b] = mode == 2 ? plan_batch_queue(c, begin, count, slots)
-                      : mode == 1 ? plan_batch(c, begin, count, slots)
-                                  : plan_batch_plain(c, begin, count, slots);
+            plan[b] = mode == 0 ? plan_batch_plain(c, begin, count, slots) : plan_batch(c, begin, count, slots);
             if (mode == 0) plain_need = std::max(plain_need, plain_tables_needed(c, plan[b]));
         }
         if (mode == 0) ensure_gscratch(c, plain_need);
         else reserve_dp_scratch(c, plan);
+        const long long qwords = (long long)kTreeClasses * kDpQueues * kDpQueueStride;  // per batch
         if (mode == 2 && c->dp_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->dp_queue);
-            c->dp_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kDpQueues * kDpQueueStride);
+            c->dp_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * qwords);
             c->dp_queue_cap = nbatch;
         }
         auto enqueue = [&] {
             if (mode == 2 && nbatch)  // a kernel, not a memset: the loop may be captured
-                hipLaunchKernelGGL(fill_int_kernel, dim3((nbatch * kDpQueues * kDpQueueStride + 255) / 256), dim3(256), 0,
-                                   c->stream, c->dp_queue, 0, (long long)nbatch * kDpQueues * kDpQueueStride);
+                hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((nbatch * qwords + 255) / 256)), dim3(256), 0,
+                                   c->stream, c->dp_queue, 0, nbatch * qwords);
             for (int b = 0; b < nbatch; ++b) {
                 launch_refresh(c, 0);
-                if (mode == 2) launch_lshape_queue(c, plan[b], c->dp_queue + size_t(b) * kDpQueues * kDpQueueStride);
+                if (mode == 2) launch_lshape_persistent(c, plan[b], c->dp_queue + size_t(b) * qwords);
                 else if (mode == 1) launch_lshape(c, plan[b]);
                 else launch_lshape_plain(c, plan[b]);
                 launch_commit(c, batch_off[b], batch_off[b + 1] - batch_off[b], first_stamp + b, 1);
