@@ -568,7 +568,7 @@ void launch_lshape_class(gr_ctx* c, const BatchLaunch& b, int k, int per_wave_ov
 // The two size classes of a batch are independent: the large trees (the
 // batch's critical path) run on the side stream beside the small ones.
 void launch_lshape(gr_ctx* c, const BatchLaunch& b, DpTables tables = {}) {
-    if (!b.count[0] || !b.count[1]) {
+    if (!b.count[0] || !b.count[1] || c->profiling) {  // profiled runs time each launch alone
         for (int k = 0; k < kTreeClasses; ++k) launch_lshape_class(c, b, k, 0, nullptr, tables);
         return;
     }
@@ -683,7 +683,8 @@ void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue, int k = 0,
 // side stream beside them.
 void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     const size_t qs = size_t(kDpQueues) * kDpQueueStride;
-    if (!b.count[0] || !b.count[1]) {
+    // profiled runs time each launch alone: both classes on the main stream
+    if (!b.count[0] || !b.count[1] || c->profiling) {
         for (int k = 0; k < kTreeClasses; ++k) launch_lshape_queue(c, b, queue + k * qs, k, c->stream);
         return;
     }
