@@ -127,6 +127,10 @@ struct gr_ctx {
     // batch, which runs beside the small-tree launch; fork/join events
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // third stream (default priority) for the persistent form's smallest
+    // trees, beside the other two classes
+    hipStream_t side2 = nullptr;
+    hipEvent_t join2_ev = nullptr;
     int cus = 0;
     DeviceGrid g{};
     int tracks = 0, maxlen = 0;
@@ -285,6 +289,8 @@ struct gr_ctx {
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         if (join_ev) (void)hipEventDestroy(join_ev);
         if (side) (void)hipStreamDestroy(side);
+        if (join2_ev) (void)hipEventDestroy(join2_ev);
+        if (side2) (void)hipStreamDestroy(side2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -292,6 +298,7 @@ struct gr_ctx {
 namespace {
 
 constexpr int kSmallTree = 16;
+constexpr int kTinyTree = 4;
 constexpr int kMaxTreeNodes = 32767;  // node indices and heights are packed in 16 bits
 
 hipEvent_t take_event(gr_ctx* c) {
@@ -435,6 +442,11 @@ struct BatchLaunch {
     int begin;
     int count[kTreeClasses];
     int nmax[kTreeClasses];
+    // plan_batch: the last `tiny` slots of class 0 are its trees of at most
+    // kTinyTree nodes (the persistent form drains them from a queue of their
+    // own, with tables sized for them)
+    int tiny = 0;
+    int tiny_nmax = 1;
 };
 
 int tree_class(int n) {
@@ -455,13 +467,21 @@ BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots)
     for (int k = 1; k < kTreeClasses; ++k) at[k] = at[k - 1] + b.count[k - 1];
     for (int s = begin; s < begin + count; ++s) slots[at[tree_class(c->node_off_h[s + 1] - c->node_off_h[s])]++] = s;
     // tallest trees first inside a class: the nets with the longest level
-    // chains start first and the launch's tail is short
+    // chains start first and the launch's tail is short; class 0's trees of
+    // at most kTinyTree nodes after its others
     // (slots enter in ascending order, so ordering ties by slot is the
     // stable order, without the temporary buffer of a stable sort)
+    auto tiny = [&](int u) { return c->node_off_h[u + 1] - c->node_off_h[u] <= kTinyTree; };
     for (int k = 0, lo = begin; k < kTreeClasses; lo += b.count[k++])
         std::sort(slots.begin() + lo, slots.begin() + lo + b.count[k], [&](int u, int v) {
+            if (k == 0 && tiny(u) != tiny(v)) return tiny(v);
             return c->height_h[u] != c->height_h[v] ? c->height_h[u] > c->height_h[v] : u < v;
         });
+    for (int j = begin; j < begin + b.count[0]; ++j)
+        if (tiny(slots[j])) {
+            ++b.tiny;
+            b.tiny_nmax = std::max(b.tiny_nmax, c->node_off_h[slots[j] + 1] - c->node_off_h[slots[j]]);
+        }
     return b;
 }
 
@@ -639,15 +659,15 @@ void launch_lshape_plain(gr_ctx* c, const BatchLaunch& b) {
     hip_check(hipGetLastError(), "lshape_dp plain kernel");
 }
 
-// Class k of the batch drained by the persistent DP kernel from its queue
-// counters (zero beforehand), on `stream`; trees too large for LDS take the
-// launch form.
-void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue, int k = 0, hipStream_t stream = nullptr) {
-    if (!b.count[k]) return;
-    if (!stream) stream = c->stream;
-    const int p = per_wave_for(c, k, b.nmax[k]);
-    if (p <= 0) {
-        launch_lshape_class(c, b, k, 0, stream);
+// Slots [begin, begin + count) of class k (trees of at most nmax nodes)
+// drained by the persistent DP kernel from its queue counters (zero
+// beforehand), on `stream`; trees too large for LDS take the launch form.
+void launch_lshape_queue_range(gr_ctx* c, int begin, int count, int nmax, int k, int* queue, hipStream_t stream) {
+    if (!count) return;
+    const int p = per_wave_for(c, k, nmax);
+    if (p <= 0) {  // tables beyond LDS: the global-table launch over the range
+        const BatchLaunch one{begin, {count, 0}, {nmax, 1}};
+        launch_lshape_class(c, one, 0, 0, stream);
         return;
     }
     LshapeArgs a;
@@ -664,10 +684,9 @@ void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue, int k = 0,
     a.error = c->error;
     a.queue = queue;
     a.cus = c->cus;
-    a.begin = b.begin;
-    for (int j = 0; j < k; ++j) a.begin += b.count[j];
-    a.count = b.count[k];
-    a.nmax = b.nmax[k];
+    a.begin = begin;
+    a.count = count;
+    a.nmax = nmax;
     a.per_wave = p;
     double alg = 0;
     if (c->profiling)
@@ -676,24 +695,46 @@ void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue, int k = 0,
     hip_check(hipGetLastError(), "lshape_dp queue kernel");
 }
 
-// Persistent form of one batch, by size class (plan_batch): each class
-// drained from its own queue (kDpQueues counters at queue + k * kDpQueues *
-// kDpQueueStride), so the many small trees keep a small LDS footprint and a
-// high occupancy; the large trees (the batch's critical path) run on the
-// side stream beside them.
+// Class 0 of the batch (or all of it) drained from one queue on the main
+// stream: the allocation-free entry point's form.
+void launch_lshape_queue(gr_ctx* c, const BatchLaunch& b, int* queue) {
+    launch_lshape_queue_range(c, b.begin, b.count[0], b.nmax[0], 0, queue, c->stream);
+}
+
+constexpr int kPersistentGroups = 3;  // queues per batch in the persistent form
+
+// Persistent form of one batch (plan_batch), in three groups with a queue
+// each (kDpQueues counters at queue + g * kDpQueues * kDpQueueStride): the
+// trees of at most kTinyTree nodes (most nets), the rest of the small
+// class, the large trees.  Each group's tables are sized for its own
+// largest tree, so the many tiny trees keep a small LDS footprint and a
+// high occupancy; the groups run on three streams, side by side.
 void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     const size_t qs = size_t(kDpQueues) * kDpQueueStride;
-    // profiled runs time each launch alone: both classes on the main stream
-    if (!b.count[0] || !b.count[1] || c->profiling) {
-        for (int k = 0; k < kTreeClasses; ++k) launch_lshape_queue(c, b, queue + k * qs, k, c->stream);
+    const int mid = b.count[0] - b.tiny;
+    struct Group {
+        int begin, count, nmax, k;
+    } grp[kPersistentGroups] = {{b.begin + mid, b.tiny, b.tiny_nmax, 0},
+                                {b.begin, mid, b.nmax[0], 0},
+                                {b.begin + b.count[0], b.count[1], b.nmax[1], 1}};
+    int live = 0;
+    for (const Group& g : grp) live += g.count > 0;
+    // profiled runs time each launch alone: every group on the main stream
+    if (live < 2 || c->profiling) {
+        for (int j = 0; j < kPersistentGroups; ++j)
+            launch_lshape_queue_range(c, grp[j].begin, grp[j].count, grp[j].nmax, grp[j].k, queue + j * qs, c->stream);
         return;
     }
     hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
-    launch_lshape_queue(c, b, queue + qs, 1, c->side);
-    launch_lshape_queue(c, b, queue, 0, c->stream);
+    hip_check(hipStreamWaitEvent(c->side2, c->fork_ev, 0), "hipStreamWaitEvent");
+    launch_lshape_queue_range(c, grp[2].begin, grp[2].count, grp[2].nmax, 1, queue + 2 * qs, c->side);
+    launch_lshape_queue_range(c, grp[1].begin, grp[1].count, grp[1].nmax, 0, queue + qs, c->side2);
+    launch_lshape_queue_range(c, grp[0].begin, grp[0].count, grp[0].nmax, 0, queue, c->stream);
     hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
+    hip_check(hipEventRecord(c->join2_ev, c->side2), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
+    hip_check(hipStreamWaitEvent(c->stream, c->join2_ev, 0), "hipStreamWaitEvent");
 }
 
 // Reserve the global DP scratch the batches will need (trees whose tables
@@ -962,6 +1003,8 @@ int gr_ctx_create(const gr_grid_desc* grid, gr_ctx** out) {
         hip_check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming), "hipEventCreate");
         hip_check(hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreateWithFlags(&c->join2_ev, hipEventDisableTiming), "hipEventCreate");
         DeviceGrid& g = c->g;
         g.L = grid->L, g.X = grid->X, g.Y = grid->Y, g.DIR = grid->DIR & 1, g.XY = grid->X * grid->Y;
         g.unit_wire = grid->unit_length_wire_cost;
@@ -1319,7 +1362,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         }
         if (mode == 0) ensure_gscratch(c, plain_need);
         else reserve_dp_scratch(c, plan);
-        const long long qwords = (long long)kTreeClasses * kDpQueues * kDpQueueStride;  // per batch
+        const long long qwords = (long long)kPersistentGroups * kDpQueues * kDpQueueStride;  // per batch
         if (mode == 2 && c->dp_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->dp_queue);
