@@ -331,6 +331,7 @@ def bench_pipeline(args, ga, design, topo):
     tsched = ga.execution_schedule(topo, tcolor, tnb)
     ctx.lshape_upload(tsched["net_ids"], tsched["node_off"], tsched["nodes"], tsched["parents"])
     l_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.lshape_tune(tsched["batch_off"])
+    g_tune = (4, [-1.0] * 4) if args.no_tune else ctx.lshape_tune_groups(tsched["batch_off"])
     ctx.reset()
     ctx.lshape_run(tsched["batch_off"], first_stamp=1, mode=0)
     flags = ctx.ripup(nsub)
@@ -342,7 +343,8 @@ def bench_pipeline(args, ga, design, topo):
     fcolor, fnb = ctx.batch_colors(limit=300000)
     ctx.flt_upload(design.flt_plan(fcolor, fnb))
     f_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.flt_tune()
-    print(f"[bench] autotuners: L-shape nets per wavefront {l_tune[0]}, FLT DAGs per wavefront {f_tune[0]}; "
+    print(f"[bench] autotuners: L-shape nets per wavefront {l_tune[0]}, persistent tiny-tree group <= {g_tune[0]} "
+          f"nodes {[round(v, 3) for v in g_tune[1]]}, FLT DAGs per wavefront {f_tune[0]}; "
           f"{fs['nets']} rerouted nets, {fs['nodes']} DAG nodes, {fs['edges']} DAG edges",
           file=sys.stderr, flush=True)
 
@@ -477,7 +479,10 @@ def bench_pipeline(args, ga, design, topo):
                    "nets_per_wavefront": {"lshape": {"classes": ["<=16 nodes", "larger"], "chosen": l_tune[0],
                                                      "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in l_tune[1]]},
                                           "flt": {"classes": ["batches < 512 DAGs", ">= 512 DAGs"], "chosen": f_tune[0],
-                                                  "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in f_tune[1]]}},
+                                                  "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in f_tune[1]]},
+                                          "lshape_persistent_groups": {
+                                              "tiny_tree_bound": g_tune[0],
+                                              "dp_ms_bound_1_2_4_8": [round(v, 3) for v in g_tune[1]]}},
                    "soft_dp": soft, "parallelism": "single GPU",
                    "kernel_breakdown": {"steps": 1, "form": "persistent", "classes": kernels}},
         "roofline": roofline,

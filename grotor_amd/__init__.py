@@ -519,6 +519,19 @@ class Context:
               _ptr(pw, ctypes.c_int), _ptr(ms, ctypes.c_double))
         return pw.tolist(), ms.reshape(2, 4).tolist()
 
+    def lshape_tune_groups(self, batch_off):
+        """Autotune the persistent form's group shape (the tiny-tree bound):
+        (chosen bound, ms for bounds 1, 2, 4, 8)."""
+        batch_off = _i32(batch_off)
+        chosen = ctypes.c_int()
+        ms = np.zeros(4, np.float64)
+        _call("gr_lshape_tune_groups", self._h, ctypes.c_int(len(batch_off) - 1), _ptr(batch_off, ctypes.c_int),
+              ctypes.byref(chosen), _ptr(ms, ctypes.c_double))
+        return chosen.value, ms.tolist()
+
+    def set_tiny_tree(self, nodes):
+        _call("gr_ctx_set_tiny_tree", self._h, ctypes.c_int(nodes))
+
     def score(self, nets, layer_dir, capacity, overflow_cost, demand=False):
         """Evaluator score of the resident routes (gr_score): dict of
         wirelength, via, overflow and total cost, rejected nets, device ms,

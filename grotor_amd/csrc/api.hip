@@ -221,6 +221,7 @@ struct gr_ctx {
     unsigned char* cview = nullptr;
     float *cxsum = nullptr, *cysum = nullptr;
     int per_wave[kTreeClasses] = {1, 1};  // L-shape DP nets per wavefront by tree size class (gr_lshape_tune)
+    int tiny_bound = 4;  // persistent form: trees of at most this many nodes form its first group (gr_lshape_tune_groups)
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 32;
     // kernel profiling (HIP events on the context stream)
@@ -298,7 +299,6 @@ struct gr_ctx {
 namespace {
 
 constexpr int kSmallTree = 16;
-constexpr int kTinyTree = 4;
 constexpr int kMaxTreeNodes = 32767;  // node indices and heights are packed in 16 bits
 
 hipEvent_t take_event(gr_ctx* c) {
@@ -443,7 +443,7 @@ struct BatchLaunch {
     int count[kTreeClasses];
     int nmax[kTreeClasses];
     // plan_batch: the last `tiny` slots of class 0 are its trees of at most
-    // kTinyTree nodes (the persistent form drains them from a queue of their
+    // tiny_bound nodes (the persistent form drains them from a queue of their
     // own, with tables sized for them)
     int tiny = 0;
     int tiny_nmax = 1;
@@ -468,10 +468,10 @@ BatchLaunch plan_batch(gr_ctx* c, int begin, int count, std::vector<int>& slots)
     for (int s = begin; s < begin + count; ++s) slots[at[tree_class(c->node_off_h[s + 1] - c->node_off_h[s])]++] = s;
     // tallest trees first inside a class: the nets with the longest level
     // chains start first and the launch's tail is short; class 0's trees of
-    // at most kTinyTree nodes after its others
+    // at most c->tiny_bound nodes after its others
     // (slots enter in ascending order, so ordering ties by slot is the
     // stable order, without the temporary buffer of a stable sort)
-    auto tiny = [&](int u) { return c->node_off_h[u + 1] - c->node_off_h[u] <= kTinyTree; };
+    auto tiny = [&](int u) { return c->node_off_h[u + 1] - c->node_off_h[u] <= c->tiny_bound; };
     for (int k = 0, lo = begin; k < kTreeClasses; lo += b.count[k++])
         std::sort(slots.begin() + lo, slots.begin() + lo + b.count[k], [&](int u, int v) {
             if (k == 0 && tiny(u) != tiny(v)) return tiny(v);
@@ -705,7 +705,7 @@ constexpr int kPersistentGroups = 3;  // queues per batch in the persistent form
 
 // Persistent form of one batch (plan_batch), in three groups with a queue
 // each (kDpQueues counters at queue + g * kDpQueues * kDpQueueStride): the
-// trees of at most kTinyTree nodes (most nets), the rest of the small
+// trees of at most tiny_bound nodes (most nets), the rest of the small
 // class, the large trees.  Each group's tables are sized for its own
 // largest tree, so the many tiny trees keep a small LDS footprint and a
 // high occupancy; the groups run on three streams, side by side.
@@ -1385,7 +1385,7 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         // mode 2 issue the same launches directly
         const bool graph = mode == 2 && !c->profiling;
         if (graph) {
-            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode};
+            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode, c->tiny_bound};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
             // the graph reads the descriptors when it runs
             c->slots_h = slots;
@@ -1511,6 +1511,70 @@ int gr_lshape_tune(gr_ctx* c, int nbatch, const int* batch_off, int* per_wave, d
         c->profiling = profiling;
         c->drop_graph();
         check_device_error(c, "gr_lshape_tune");
+    });
+}
+
+int gr_lshape_tune_groups(gr_ctx* c, int nbatch, const int* batch_off, int* tiny_nodes, double* ms) {
+    return guarded([&] {
+        if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
+        require_stage(c, 0, batch_off[nbatch]);
+        if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
+        for (int b = 0; b < nbatch; ++b)
+            if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
+        const long long qwords = (long long)kPersistentGroups * kDpQueues * kDpQueueStride;
+        int* queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * qwords);
+        const bool profiling = c->profiling;
+        c->profiling = false;
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        const int candidates[4] = {1, 2, 4, 8};  // 1: no group of its own (every routed tree has 2 nodes or more)
+        const int keep = c->tiny_bound;
+        double best = 1e30;
+        int choice = keep;
+        std::vector<int> slots(c->stage_count);
+        launch_refresh(c, 0);
+        for (int ci = 0; ci < 4; ++ci) {
+            c->tiny_bound = candidates[ci];
+            std::vector<BatchLaunch> plan(nbatch);
+            for (int b = 0; b < nbatch; ++b) plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+            reserve_dp_scratch(c, plan);
+            c->slots_h = slots;
+            upload_desc(c, slots, 0, c->stage_count);
+            // every batch's persistent DP at the stage's current costs, the
+            // groups on their streams; the routes it writes are rewritten by
+            // the stage
+            double t = 1e30;
+            for (int rep = 0; rep < 2; ++rep) {
+                hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((nbatch * qwords + 255) / 256)), dim3(256), 0,
+                                   c->stream, queue, 0, nbatch * qwords);
+                hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+                for (int b = 0; b < nbatch; ++b) launch_lshape_persistent(c, plan[b], queue + size_t(b) * qwords);
+                hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                float f = 0;
+                hip_check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+                t = std::min(t, double(f));
+            }
+            if (ms) ms[ci] = t;
+            if (t < best) best = t, choice = candidates[ci];
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->mem.release(queue);
+        c->tiny_bound = choice;
+        c->profiling = profiling;
+        c->drop_graph();
+        if (tiny_nodes) *tiny_nodes = choice;
+        check_device_error(c, "gr_lshape_tune_groups");
+    });
+}
+
+int gr_ctx_set_tiny_tree(gr_ctx* c, int nodes) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (nodes < 1 || nodes > kSmallTree) throw GrError(GR_E_ARG, "tiny-tree bound must lie in [1, 16]");
+        c->tiny_bound = nodes;
     });
 }
 

@@ -246,6 +246,18 @@ int gr_ctx_set_nets_per_wave(gr_ctx *ctx, const int *per_wave);
  * writes are rewritten by the next stage run.  No reference counterpart. */
 int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave, double *ms);
 
+/* Group shapes of the persistent form (gr_lshape_run mode 2; no reference
+ * counterpart): per batch the trees of at most `tiny` nodes are drained
+ * from a work queue of their own with DP tables sized for them, beside the
+ * rest of the small class and the large trees.  Times the persistent DP of
+ * the uploaded batches for tiny = 1 (no group of its own), 2, 4 and 8 and
+ * keeps the fastest; *tiny_nodes receives it, ms[4] (may be NULL) the
+ * times.  Routes are the same for every choice; route words are
+ * overwritten, so call it before the stage run.  gr_ctx_set_tiny_tree sets
+ * the bound by hand (1..16, default 4). */
+int gr_lshape_tune_groups(gr_ctx *ctx, int nbatch, const int *batch_off, int *tiny_nodes, double *ms);
+int gr_ctx_set_tiny_tree(gr_ctx *ctx, int nodes);
+
 /* Reset the routing state to that of a fresh context: demand, timestamps,
  * statistics and route words cleared, phase-two route storage released,
  * every track marked for refresh (the DAG-throughput benchmark's reset,
