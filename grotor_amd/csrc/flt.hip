@@ -113,7 +113,6 @@ struct LdsTables {
     int* doff;                   // [depths + 1] offsets of the depths in order
     int* edoff;                  // [depths + 1] offsets of the depths in eorder
     double* cvs;                 // [position in depth * L + li - 1]: layer costs of a depth's solved nodes
-    unsigned* at;                // [position in depth * L + li - 1]: wide nodes' minimum slot positions
     int* layer;                  // [node]
     int* posof;                  // [node]: position in its depth
     float* vc;                   // [node * (L - 1) + j]: via cost of layer j at the node's cell
@@ -152,7 +151,6 @@ __host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsT
     char* nrec = take((size_t)s.nodes * 16, 16);
     char* erec = take((size_t)s.edges * 16, 16);
     char* cvs = take((size_t)s.dnodes * L * 8, 8);
-    char* at = take((size_t)s.dnodes * L * 4, 4);
     char* eorder = take((size_t)s.edges * 4, 4);
     char* order = take((size_t)s.nodes * 4, 4);
     char* doff = take((size_t)(s.depths + 1) * 4, 4);
@@ -167,7 +165,6 @@ __host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsT
         t->nrec = reinterpret_cast<int4*>(nrec);
         t->erec = reinterpret_cast<int4*>(erec);
         t->cvs = reinterpret_cast<double*>(cvs);
-        t->at = reinterpret_cast<unsigned*>(at);
         t->eorder = reinterpret_cast<int*>(eorder);
         t->order = reinterpret_cast<int*>(order);
         t->doff = reinterpret_cast<int*>(doff);
@@ -665,72 +662,90 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
     if (live && lane == 0) routes[0] = count <= cap ? count : 1;
 }
 
-// solve_flt_node (src/routing/flt_block.cuh:6-142) for nodes of up to six
-// child slots, kept off the registers of the common two-slot path: the
-// layer costs and, per layer, the slot positions of the running first
-// minima that produced them (4 bits per child slot, as in solve_node_fast)
-// live in the node's LDS rows (tb.cvs, tb.at); the best-path rows are
-// written once at the end from those positions.
+// solve_flt_node (src/routing/flt_block.cuh:6-142) for nodes of three to six
+// child slots, by the reference's passes themselves: per low end of the via
+// stack an upward sweep (running first minima of each slot's costs, strict
+// updates of the layer costs) and a downward sweep copying strictly smaller
+// costs.  The slot costs, layer costs and their slot positions (4 bits per
+// child slot) stay in registers; every loop over layers is unrolled, the
+// run-time low end masking the layers below it.  The best-path rows are
+// written once at the end from the positions.
 template <int L>
 __device__ __forceinline__ void solve_node_wide(const FltArgs& a, const Dag& d, const LdsTables<L>& tb, int v,
                                                 const int4 r, int pos) {
+    constexpr int K = 6;
     const DeviceGrid& g = a.g;
     const int low_fixed = 1 + r.x / g.XY;
     const int x = r.x / g.Y % g.X, y = r.x % g.Y;
     const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
     const int kids = r.w & 7;
     const int row0 = r.w >> 3;
-    double* cv = tb.cvs + pos * L - 1;  // cv[li], li 1..L
-    unsigned* at = tb.at + pos * L - 1;
-    for (int li = 1; li <= L; ++li) cv[li] = kInf, at[li] = 0;
-    for (int low = 1; low <= min(L, low_fixed); ++low) {
-        double mins[6];
+    float pv[K][L];  // slot costs (the packed keys' floats), layer li at [li - 1]
 #pragma unroll
-        for (int k = 0; k < 6; ++k) mins[k] = kInf;
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int li = 1; li <= L; ++li) pv[k][li - 1] = k < kids ? (float)unpack_cost(tb.key(row0 + k, li)) : 0.f;
+    float vc[L];
+#pragma unroll
+    for (int j = 0; j + 1 < L; ++j) vc[j] = tb.via(v, j, x, y);
+    double cv[L];    // layer li at [li - 1]
+    unsigned at[L];  // slot positions of cv, 4 bits per child slot
+#pragma unroll
+    for (int i = 0; i < L; ++i) cv[i] = kInf, at[i] = 0;
+    const int lf = min(L, low_fixed);
+    for (int low = 1; low <= lf; ++low) {
+        float mins[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) mins[k] = (float)kInf;  // an empty key (1.01e22) never lowers it
         unsigned run = 0;
         double via = 0.0;
-        for (int li = low; li <= L; ++li) {
-            if (li > low) via += (double)tb.via(v, li - 2, x, y);
 #pragma unroll
-            for (int k = 0; k < 6; ++k)
-                if (k < kids) {
-                    const double c = unpack_cost(tb.key(row0 + k, li));
-                    if (c < mins[k]) {
-                        mins[k] = c;
-                        run = (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
-                    }
+        for (int li = 1; li <= L; ++li) {
+            if (li < low) continue;
+            if (li > low) via += (double)vc[li - 2];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (k < kids && pv[k][li - 1] < mins[k]) {
+                    mins[k] = pv[k][li - 1];
+                    run = (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k));
                 }
             if (li >= high_fixed) {
                 double cost = via;
 #pragma unroll
-                for (int k = 0; k < 6; ++k)
-                    if (k < kids) cost += mins[k];
-                if (cost < kInf && cost < cv[li]) {
-                    cv[li] = cost;
-                    at[li] = run;
+                for (int k = 0; k < K; ++k)
+                    if (k < kids) cost += (double)mins[k];
+                if (cost < kInf && cost < cv[li - 1]) {
+                    cv[li - 1] = cost;
+                    at[li - 1] = run;
                 }
             }
         }
-        for (int li = L - 1; li >= low; --li)
-            if (cv[li + 1] < cv[li]) {
-                cv[li] = cv[li + 1];
-                at[li] = at[li + 1];
+#pragma unroll
+        for (int li = L - 1; li >= 1; --li)
+            if (li >= low && cv[li] < cv[li - 1]) {
+                cv[li - 1] = cv[li];
+                at[li - 1] = at[li];
             }
     }
+#pragma unroll
     for (int li = 1; li <= L; ++li) {
-        if (!(cv[li] < kInf)) continue;
-        const unsigned pos_li = at[li];
-        for (int k = 0; k < kids; ++k) {
-            const int j = (pos_li >> (4 * k)) & 15;
+        if (!(cv[li - 1] < kInf)) continue;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k >= kids) continue;
+            const int j = (at[li - 1] >> (4 * k)) & 15;
             const unsigned long long key = tb.key(row0 + k, j);
             tb.set_path(row0 + k, li, (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j);
         }
     }
     if (r.y == r.z) {  // the root's trace reads its layer costs
         tb.set_root_cost(v, 0, kInf);
-        for (int li = 1; li <= L; ++li) tb.set_root_cost(v, li, cv[li]);
+#pragma unroll
+        for (int li = 1; li <= L; ++li) tb.set_root_cost(v, li, cv[li - 1]);
     }
     tb.posof[v] = pos;
+#pragma unroll
+    for (int li = 1; li <= L; ++li) tb.cvs[pos * L + li - 1] = cv[li - 1];
 }
 
 // One push item of a depth: edge e of a node at that depth towards target
