@@ -532,6 +532,11 @@ class Context:
     def set_tiny_tree(self, nodes):
         _call("gr_ctx_set_tiny_tree", self._h, ctypes.c_int(nodes))
 
+    def set_flt_split(self, nbytes):
+        """LDS-share bound (bytes) between the FLT DP's two DAG classes, for
+        the next flt_upload (0: one class)."""
+        _call("gr_ctx_set_flt_split", self._h, ctypes.c_int(nbytes))
+
     def score(self, nets, layer_dir, capacity, overflow_cost, demand=False):
         """Evaluator score of the resident routes (gr_score): dict of
         wirelength, via, overflow and total cost, rejected nets, device ms,

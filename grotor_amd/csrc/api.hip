@@ -210,7 +210,12 @@ struct gr_ctx {
     std::vector<int> f_ids_h;  // subnet ids of the uploaded FLT stage, execution order
     std::vector<int4> f_base;  // per batch: stage-global first node, edge, slot; edges
     std::vector<FltLdsShape> f_batch_max;  // per batch: the largest child slots, nodes, edges, depth width, depths
-    int* f_queue = nullptr;         // persistent FLT DP: kFltQueues counters per batch
+    // per batch: DAGs whose LDS share exceeds f_lds_split bytes first (f_large
+    // of them), then the rest, each class with its own largest shape
+    std::vector<int> f_large;
+    std::vector<FltLdsShape> f_class_max;  // [batch * kFltClasses + class], class 0 the large DAGs
+    int f_lds_split = 20 * 1024;
+    int* f_queue = nullptr;         // persistent FLT DP: kFltClasses x kFltQueues counters per batch
     char* f_stage = nullptr;        // pinned host staging of the FLT stage records
     size_t f_stage_cap = 0;
     int f_queue_cap = 0;
@@ -791,12 +796,17 @@ FltArgs flt_batch_args(gr_ctx* c, int b) {
     return a;
 }
 
-// The DP of batch b with `per_wave` DAGs per wavefront: each DAG's tables
-// in LDS when the batch's largest DAG fits (per_wave halved until the
-// workgroup's share fits), the persistent form draining `queue` when given;
-// otherwise the batch tables in global memory.
-void launch_flt_dp(gr_ctx* c, FltArgs a, int b, int per_wave, int* queue) {
-    const FltLdsShape& mx = c->f_batch_max[b];
+// One DAG class of batch b: DAGs [off, off + count) of the batch with
+// `per_wave` DAGs per wavefront, each DAG's tables in LDS when the class's
+// largest DAG fits (per_wave halved until the workgroup's share fits), the
+// persistent form draining `queue` when given; otherwise the batch tables in
+// global memory.
+void launch_flt_class(gr_ctx* c, FltArgs a, const FltLdsShape& mx, int off, int count, int per_wave, int* queue,
+                      hipStream_t stream) {
+    a.desc += off;
+    a.sizes += off;
+    a.net_ids += off;
+    a.count = count;
     const size_t bytes = flt_lds_bytes(c->g.L, mx.slots, mx.nodes, mx.edges, mx.dnodes, mx.depths);
     const size_t avail = size_t(c->lds_limit) - 256;  // the kernel's static LDS stays beside the tables
     int p = per_wave;
@@ -806,11 +816,32 @@ void launch_flt_dp(gr_ctx* c, FltArgs a, int b, int per_wave, int* queue) {
         a.lds_dag_bytes = int(bytes);
         a.lds_shape = mx;
         a.queue = queue;
-        launch_flt_route(a, true, c->cus, c->stream);
+        launch_flt_route(a, true, c->cus, stream);
     } else {
         a.queue = nullptr;
-        launch_flt_route(a, false, c->cus, c->stream);
+        launch_flt_route(a, false, c->cus, stream);
     }
+}
+
+// The DP of batch b: its large-DAG class on the side stream beside the
+// rest (one stream when profiling, so each launch is timed alone), each
+// class with LDS shares sized by its own largest DAG.
+void launch_flt_dp(gr_ctx* c, const FltArgs& a, int b, int per_wave, int* queue) {
+    const int count = c->f_batch_off[b + 1] - c->f_batch_off[b], large = c->f_large[b];
+    const FltLdsShape* mx = &c->f_class_max[size_t(b) * kFltClasses];
+    const size_t qs = size_t(kFltQueues) * kFltQueueStride;
+    int* q1 = queue ? queue + qs : nullptr;
+    if (!large || large == count || c->profiling) {
+        if (large) launch_flt_class(c, a, mx[0], 0, large, per_wave, queue, c->stream);
+        if (count > large) launch_flt_class(c, a, mx[1], large, count - large, per_wave, q1, c->stream);
+        return;
+    }
+    hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
+    launch_flt_class(c, a, mx[0], 0, large, per_wave, queue, c->side);
+    launch_flt_class(c, a, mx[1], large, count - large, per_wave, q1, c->stream);
+    hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
+    hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
 }
 
 // Route storage may have been replaced since the FLT upload: bind the
@@ -1578,6 +1609,14 @@ int gr_ctx_set_tiny_tree(gr_ctx* c, int nodes) {
     });
 }
 
+int gr_ctx_set_flt_split(gr_ctx* c, int bytes) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (bytes < 0) throw GrError(GR_E_ARG, "FLT class split must be at least 0 bytes");
+        c->f_lds_split = bytes;
+    });
+}
+
 int gr_ctx_reset(gr_ctx* c) {
     return guarded([&] {
         if (!c) throw GrError(GR_E_ARG, "null context");
@@ -2102,29 +2141,49 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                 c->f_route_bytes[b] += net_rb[k];
             }
         }
-        // the DAGs of a wavefront step through their depths together: order
-        // each batch by decreasing depth, then size (results do not depend
-        // on the order of nets inside a batch)
+        // DAG classes by LDS share: the DAGs above f_lds_split bytes first,
+        // each class launched with shares sized by its own largest DAG, so
+        // the many small DAGs of a batch are not held to its largest one's
+        // share.  Inside a class the DAGs of a wavefront step through their
+        // depths together: decreasing depth, then size (results do not
+        // depend on the order of nets inside a batch).
         {
+            std::vector<char> big(nets);
+            for (int k = 0; k < nets; ++k)
+                big[k] = flt_lds_bytes(L, sizes[k].z, sizes[k].x, sizes[k].y, net_dnodes[k], sizes[k].w) >
+                         size_t(c->f_lds_split);
             std::vector<int> perm;
             std::vector<int4> dp, sp;
             std::vector<int> ip;
+            c->f_large.assign(nbatch, 0);
+            c->f_class_max.assign(size_t(nbatch) * kFltClasses, FltLdsShape{0, 0, 0, 0, 0});
             for (int b = 0; b < nbatch; ++b) {
                 const int k0 = batch_off[b], k1 = batch_off[b + 1];
                 perm.resize(k1 - k0);
                 for (int i = 0; i < k1 - k0; ++i) perm[i] = k0 + i;
                 std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) {
+                    if (big[x] != big[y]) return big[x] > big[y];
                     return sizes[x].w != sizes[y].w ? sizes[x].w > sizes[y].w : sizes[x].x > sizes[y].x;
                 });
                 dp.resize(k1 - k0), sp.resize(k1 - k0), ip.resize(k1 - k0);
-                for (int i = 0; i < k1 - k0; ++i) dp[i] = desc[perm[i]], sp[i] = sizes[perm[i]], ip[i] = ids[perm[i]];
+                for (int i = 0; i < k1 - k0; ++i) {
+                    const int k = perm[i];
+                    dp[i] = desc[k], sp[i] = sizes[k], ip[i] = ids[k];
+                    c->f_large[b] += big[k];
+                    FltLdsShape& m = c->f_class_max[size_t(b) * kFltClasses + (big[k] ? 0 : 1)];
+                    m.slots = std::max(m.slots, sizes[k].z);
+                    m.nodes = std::max(m.nodes, sizes[k].x);
+                    m.edges = std::max(m.edges, sizes[k].y);
+                    m.dnodes = std::max(m.dnodes, net_dnodes[k]);
+                    m.depths = std::max(m.depths, sizes[k].w);
+                }
                 for (int i = 0; i < k1 - k0; ++i) desc[k0 + i] = dp[i], sizes[k0 + i] = sp[i], ids[k0 + i] = ip[i];
             }
         }
         if (c->f_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->f_queue);
-            c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltQueues * kFltQueueStride);
+            c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltClasses * kFltQueues * kFltQueueStride);
             c->f_queue_cap = nbatch;
         }
         Stage dev{};
@@ -2173,7 +2232,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove, int mode, double* elapsed
         const RouteStore rs2{c->routes2, c->pin_off2, c->route2_per_pin};
         auto enqueue = [&] {
             int stamp = first_stamp;
-            const long long qwords = (long long)c->f_batches * kFltQueues * kFltQueueStride;
+            const long long qwords = (long long)c->f_batches * kFltClasses * kFltQueues * kFltQueueStride;
             if (mode == 2 && qwords)  // a kernel, not a memset: the loop may be captured
                 hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((qwords + 255) / 256)), dim3(256), 0, c->stream,
                                    c->f_queue, 0, qwords);
@@ -2190,7 +2249,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove, int mode, double* elapsed
                     if (threads)
                         hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
                 });
-                int* queue = mode == 2 ? c->f_queue + size_t(b) * kFltQueues * kFltQueueStride : nullptr;
+                int* queue = mode == 2 ? c->f_queue + size_t(b) * kFltClasses * kFltQueues * kFltQueueStride : nullptr;
                 timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
                     if (mode == 0) {
                         // plain: one DAG per wavefront, the batch tables in global memory

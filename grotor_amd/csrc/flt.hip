@@ -117,6 +117,11 @@ struct LdsTables {
     int* posof;                  // [node]: position in its depth
     float* vc;                   // [node * (L - 1) + j]: via cost of layer j at the node's cell
     __device__ __forceinline__ unsigned long long key(int row, int li) const { return packed[row * L + li - 1]; }
+    // the float cost of a slot key, from its upper word alone
+    __device__ __forceinline__ float cost(int row, int li) const {
+        const int i = reinterpret_cast<const int*>(packed)[2 * (row * L + li - 1) + 1];
+        return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF);
+    }
     __device__ __forceinline__ void lower(int row, int li, unsigned long long v) const { atomicMin(packed + row * L + li - 1, v); }
     __device__ __forceinline__ unsigned long long path(int row, int li) const { return best[row * L + li - 1]; }
     __device__ __forceinline__ void set_path(int row, int li, unsigned long long v) const { best[row * L + li - 1] = (unsigned)v; }
@@ -134,10 +139,11 @@ struct LdsTables {
 
 using LdsShape = FltLdsShape;
 
-// Lays out LdsTables over base (or only sizes them with base = nullptr);
-// returns the bytes, 16-byte aligned.
+// Lays out LdsTables t over base (or only sizes them with base = nullptr);
+// returns the bytes, 16-byte aligned.  (t is a reference: a null test on the
+// tables' address would keep them in private memory.)
 template <int L>
-__host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsTables<L>* t) {
+__host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsTables<L>& t) {
     size_t off = 0;
     auto take = [&](size_t bytes, size_t align) {
         off = (off + align - 1) & ~(align - 1);
@@ -158,21 +164,19 @@ __host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsT
     char* layer = take((size_t)s.nodes * 4, 4);
     char* posof = take((size_t)s.nodes * 4, 4);
     char* vc = take((size_t)s.nodes * (L > 1 ? L - 1 : 0) * 4, 4);
-    if (t) {
-        t->packed = reinterpret_cast<unsigned long long*>(packed);
-        t->best = reinterpret_cast<unsigned*>(best);
-        t->rootc = reinterpret_cast<double*>(rootc);
-        t->nrec = reinterpret_cast<int4*>(nrec);
-        t->erec = reinterpret_cast<int4*>(erec);
-        t->cvs = reinterpret_cast<double*>(cvs);
-        t->eorder = reinterpret_cast<int*>(eorder);
-        t->order = reinterpret_cast<int*>(order);
-        t->doff = reinterpret_cast<int*>(doff);
-        t->edoff = reinterpret_cast<int*>(edoff);
-        t->layer = reinterpret_cast<int*>(layer);
-        t->posof = reinterpret_cast<int*>(posof);
-        t->vc = reinterpret_cast<float*>(vc);
-    }
+    t.packed = reinterpret_cast<unsigned long long*>(packed);
+    t.best = reinterpret_cast<unsigned*>(best);
+    t.rootc = reinterpret_cast<double*>(rootc);
+    t.nrec = reinterpret_cast<int4*>(nrec);
+    t.erec = reinterpret_cast<int4*>(erec);
+    t.cvs = reinterpret_cast<double*>(cvs);
+    t.eorder = reinterpret_cast<int*>(eorder);
+    t.order = reinterpret_cast<int*>(order);
+    t.doff = reinterpret_cast<int*>(doff);
+    t.edoff = reinterpret_cast<int*>(edoff);
+    t.layer = reinterpret_cast<int*>(layer);
+    t.posof = reinterpret_cast<int*>(posof);
+    t.vc = reinterpret_cast<float*>(vc);
     return (off + 15) & ~size_t(15);
 }
 
@@ -491,32 +495,120 @@ __device__ __forceinline__ void solve_node_fast(const FltArgs& a, const Dag& d, 
     }
 }
 
-// trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
-// segments of the edge its chosen parent uses and its via stack, into w[].
+// A node's route words: its edge's segments e0..e5 (ne of them) and its via
+// segment v0, v1 (nv of them), in named registers.
+struct Words {
+    int e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0, v0 = 0, v1 = 0;
+    int ne = 0, nv = 0;
+    __device__ __forceinline__ int count() const { return ne + nv; }
+    // every word is a cell of the grid
+    __device__ __forceinline__ bool in_grid(int cells) const {
+        auto in = [&](int w) { return w >= 0 && w < cells; };
+        const bool edge2 = in(e0) && in(e1), edge6 = edge2 && in(e2) && in(e3) && in(e4) && in(e5);
+        return (ne == 0 || (ne == 2 ? edge2 : edge6)) && (nv == 0 || (in(v0) && in(v1)));
+    }
+    __device__ __forceinline__ void put(int* routes, int at) const {
+        int* p = routes + at;
+        if (ne) {
+            p[0] = e0;
+            p[1] = e1;
+        }
+        if (ne == 6) {
+            p[2] = e2;
+            p[3] = e3;
+            p[4] = e4;
+            p[5] = e5;
+        }
+        if (nv) {
+            p[ne] = v0;
+            p[ne + 1] = v1;
+        }
+    }
+};
+
+// The segments of edge e (record er) from node (cx, cy) to its parent for
+// the parent's best path `path` and the node's layer lv (trace_flt_node,
+// src/routing/flt_block.cuh:144-331): a straight wire on lv's routing layer,
+// or wire, via stack and wire through the edge's best layer-change position.
+// A zero-length edge has none.  A straight wire is the first two words of a
+// layer change on lv at the parent's end.
+__device__ __forceinline__ void edge_words(const FltArgs& a, const Dag& d, int e, const int4 er,
+                                           unsigned long long path, int lv, int cx, int cy, Words& wd) {
+    const DeviceGrid& g = a.g;
+    const int choice = (int)((path / kS) % 100);
+    const int px = er.z / g.Y % g.X, py = er.z % g.Y;
+    const bool vert = px == cx && cy != py, horz = py == cy && cx != px;
+    if (!vert && !horz) {
+        wd.ne = 0;
+        return;
+    }
+    const bool straight = choice == kStraight;
+    const int an = vert ? cy : cx, ap = vert ? py : px;  // coordinates along the edge
+    const int fl = straight ? lv - 1 : choice / 9, tl = straight ? lv - 1 : choice % 9;
+    const int pos = straight ? ap : d.best_change[change_slot(e, fl, tl)];
+    auto idx = [&](int ll, int along) { return vert ? (ll * g.X + cx) * g.Y + along : (ll * g.X + along) * g.Y + cy; };
+    wd.e0 = idx(fl, min(an, pos));
+    wd.e1 = idx(fl, max(an, pos));
+    wd.e2 = idx(min(fl, tl), pos);
+    wd.e3 = idx(max(fl, tl), pos);
+    wd.e4 = idx(tl, min(ap, pos));
+    wd.e5 = idx(tl, max(ap, pos));
+    wd.ne = straight ? 2 : 6;
+}
+
+// The node's via stack: from its own layer (pin nodes) or lv over the
+// layers its children's best paths enter at lv, as one segment.
 template <class Tb>
-__device__ int trace_node(const FltArgs& a, const Dag& d, const Tb& tb, int v, int* w) {
+__device__ __forceinline__ void via_words(const FltArgs& a, const Tb& tb, const int4 r, int lv, Words& wd) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
-    const int4 r = d.nodes[v];
     const int l = r.x / g.XY, cx = r.x / g.Y % g.X, cy = r.x % g.Y;
     const int row0 = r.w >> 3;
-    int n = 0;
+    const int kids = r.w & 7;
+    int minl = l + 1;
+    int maxl = (l + 1) == L + 1 ? 1 : minl;
+    minl = min(minl, lv);
+    maxl = max(maxl, lv);
+    for (int k = 0; k < kids; ++k) {
+        const int lc = (int)(tb.path(row0 + k, lv) % kS);
+        minl = min(minl, lc);
+        maxl = max(maxl, lc);
+    }
+    wd.v0 = ((minl - 1) * g.X + cx) * g.Y + cy;
+    wd.v1 = ((maxl - 1) * g.X + cx) * g.Y + cy;
+    wd.nv = minl < maxl ? 2 : 0;
+}
+
+// The root's layer (the first of its least-cost layers, 0 when none is
+// finite), as trace_flt_node picks it.
+template <class Tb>
+__device__ __forceinline__ int root_layer(const FltArgs& a, const Tb& tb, int v) {
+    int best = 0;
+    double bc = tb.root_cost(v, 0);
+    for (int layer = 1; layer < a.g.L + 1; ++layer) {
+        const double c = tb.root_cost(v, layer);
+        if (c < bc) {
+            bc = c;
+            best = layer;
+        }
+    }
+    return best;
+}
+
+// trace_flt_node (src/routing/flt_block.cuh:144-331): the node's layer, the
+// segments of the edge its chosen parent uses and its via stack, into wd.
+template <class Tb>
+__device__ __forceinline__ void trace_node(const FltArgs& a, const Dag& d, const Tb& tb, int v, Words& wd) {
+    const DeviceGrid& g = a.g;
+    const int4 r = d.nodes[v];
+    const int cx = r.x / g.Y % g.X, cy = r.x % g.Y;
     int lv;
     if (r.y == r.z) {
-        int best = 0;
-        double bc = tb.root_cost(v, 0);
-        for (int layer = 1; layer < L + 1; ++layer) {
-            const double c = tb.root_cost(v, layer);
-            if (c < bc) {
-                bc = c;
-                best = layer;
-            }
-        }
-        lv = best;
+        lv = root_layer(a, tb, v);
         tb.set_out(v, lv);
         if (!lv) {  // no finite cost at any layer: nothing to trace
             raise(a.error, kDevNoSolution);
-            return 0;
+            return;
         }
     } else {
         int par_layer = -1, par = -1, par_edge = -1;
@@ -535,67 +627,79 @@ __device__ int trace_node(const FltArgs& a, const Dag& d, const Tb& tb, int v, i
                 continue;
             }
             lv = choice == kStraight ? (int)(path % kS) : (int)(choice / 9 + 1);
-            const int px = er.z / g.Y % g.X, py = er.z % g.Y;
-            const int rl = lv - 1;
-            auto idx = [&](int ll, int xx, int yy) { return (ll * g.X + xx) * g.Y + yy; };
-            if (px == cx && cy != py) {
-                if (choice == kStraight) {
-                    w[n++] = idx(rl, px, min(cy, py));
-                    w[n++] = idx(rl, px, max(cy, py));
-                } else {
-                    const int fl = (int)choice / 9, tl = (int)choice % 9;
-                    const int pos = d.best_change[change_slot(e, fl, tl)];
-                    w[n++] = idx(fl, cx, min(cy, pos));
-                    w[n++] = idx(fl, cx, max(cy, pos));
-                    w[n++] = idx(min(fl, tl), cx, pos);
-                    w[n++] = idx(max(fl, tl), cx, pos);
-                    w[n++] = idx(tl, cx, min(py, pos));
-                    w[n++] = idx(tl, cx, max(py, pos));
-                }
-            } else if (py == cy && cx != px) {
-                if (choice == kStraight) {
-                    w[n++] = idx(rl, min(cx, px), py);
-                    w[n++] = idx(rl, max(cx, px), py);
-                } else {
-                    const int fl = (int)choice / 9, tl = (int)choice % 9;
-                    const int pos = d.best_change[change_slot(e, fl, tl)];
-                    w[n++] = idx(fl, min(cx, pos), cy);
-                    w[n++] = idx(fl, max(cx, pos), cy);
-                    w[n++] = idx(min(fl, tl), pos, cy);
-                    w[n++] = idx(max(fl, tl), pos, cy);
-                    w[n++] = idx(tl, min(px, pos), cy);
-                    w[n++] = idx(tl, max(px, pos), cy);
-                }
-            }
+            edge_words(a, d, e, er, path, lv, cx, cy, wd);
             break;
         }
         if (par_layer == -1) {
             tb.set_out(v, -1);
-            return 0;
+            wd.ne = 0;
+            return;
         }
         const int4 er = d.edges[par_edge];
         const unsigned long long path = tb.path((d.nodes[par].w >> 3) + er.y, par_layer);
         if (path / kS / 100 != (unsigned long long)v) {
             tb.set_out(v, -1);
-            return 0;
+            wd.ne = 0;
+            return;
         }
         tb.set_out(v, lv);
     }
-    const int kids = r.w & 7;
-    int minl = l + 1;
-    int maxl = (l + 1) == L + 1 ? 1 : minl;
-    minl = min(minl, lv);
-    maxl = max(maxl, lv);
-    for (int k = 0; k < kids; ++k) {
-        const int lc = (int)(tb.path(row0 + k, lv) % kS);
-        minl = min(minl, lc);
-        maxl = max(maxl, lc);
+    via_words(a, tb, r, lv, wd);
+}
+
+// The trace in two passes (LDS tables).  Pass one, depth by depth from the
+// root: each node's layer as trace_node sets it, and its chosen parent edge
+// into posof[] (free once the DP is done).  Pass two, every node at once:
+// trace_node's words from the layers and edges pass one left.  Only pass
+// one waits on the depth before it; pass two's global loads (layer-change
+// positions) are issued together.
+template <int L>
+__device__ __forceinline__ void trace_layer(const FltArgs& a, const Dag& d, const LdsTables<L>& tb, int v) {
+    const int4 r = d.nodes[v];
+    if (r.y == r.z) {
+        const int lv = root_layer(a, tb, v);
+        tb.set_out(v, lv);
+        if (!lv) raise(a.error, kDevNoSolution);  // no finite cost at any layer: nothing to trace
+        return;
     }
-    if (minl < maxl) {
-        w[n++] = ((minl - 1) * g.X + cx) * g.Y + cy;
-        w[n++] = ((maxl - 1) * g.X + cx) * g.Y + cy;
+    int par_edge = -1, lv = -1;
+    unsigned long long path = 0;
+    for (int e = r.y; e < r.z; ++e) {
+        const int4 er = d.edges[e];
+        const int pl = tb.out(er.x);
+        if (pl <= 0) continue;
+        par_edge = e;
+        path = tb.path((d.nodes[er.x].w >> 3) + er.y, pl);
+        if (path / kS / 100 != (unsigned long long)v) {
+            lv = -1;
+            continue;
+        }
+        const unsigned long long choice = (path / kS) % 100;
+        lv = choice == kStraight ? (int)(path % kS) : (int)(choice / 9 + 1);
+        break;
     }
-    return n;
+    // trace_node's second read of the last parent path is this `path`
+    if (par_edge == -1 || path / kS / 100 != (unsigned long long)v) {
+        tb.set_out(v, -1);
+        return;
+    }
+    tb.set_out(v, lv);
+    tb.posof[v] = par_edge;
+}
+
+template <int L>
+__device__ __forceinline__ void trace_words(const FltArgs& a, const Dag& d, const LdsTables<L>& tb, int v, Words& wd) {
+    const DeviceGrid& g = a.g;
+    const int lv = tb.out(v);
+    if (lv <= 0) return;
+    const int4 r = d.nodes[v];
+    if (r.y != r.z) {
+        const int e = tb.posof[v];
+        const int4 er = d.edges[e];
+        const unsigned long long path = tb.path((d.nodes[er.x].w >> 3) + er.y, tb.out(er.x));
+        edge_words(a, d, e, er, path, lv, r.x / g.Y % g.X, r.x % g.Y, wd);
+    }
+    via_words(a, tb, r, lv, wd);
 }
 
 // One DAG of the batch on W lanes (lane = this lane's index among them),
@@ -633,14 +737,14 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
         int chunks = (end - first + W - 1) / W;
         for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
         for (int ch = 0; ch < chunks; ++ch) {
-            int w[8];
+            Words wd;
             const int i = first + ch * W + lane;
-            int n = i < end ? trace_node(a, d, tb, order[i], w) : 0;
-            for (int j = 0; j < n; ++j)
-                if (w[j] < 0 || w[j] >= g_cells) {
-                    raise(a.error, kDevBadCoordinate);
-                    n = 0;
-                }
+            if (i < end) trace_node(a, d, tb, order[i], wd);
+            if (!wd.in_grid(g_cells)) {
+                raise(a.error, kDevBadCoordinate);
+                wd.ne = wd.nv = 0;
+            }
+            const int n = wd.count();
             // inclusive prefix of the word counts within this DAG's lanes
             int incl = n;
             for (int s = 1; s < W; s <<= 1) {
@@ -652,7 +756,7 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
                 if (at + n - 1 >= cap) {
                     raise(a.error, kDevRouteOverflow);
                 } else {
-                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
+                    wd.put(routes, at);
                 }
             }
             count += __shfl(incl, W - 1, W);
@@ -746,6 +850,127 @@ __device__ __forceinline__ void solve_node_wide(const FltArgs& a, const Dag& d, 
     tb.posof[v] = pos;
 #pragma unroll
     for (int li = 1; li <= L; ++li) tb.cvs[pos * L + li - 1] = cv[li - 1];
+}
+
+// Lanes per node of the lane-parallel solve: a power of two, at least L,
+// inside one 16-lane DPP row.
+template <int L>
+constexpr int kNodeLanes = L > 8 ? 16 : 8;
+
+// Lane g of each 16-lane row receives x of lane g - M of its row (DPP
+// row_shr); lanes with g < M keep their own x.
+template <int M>
+__device__ __forceinline__ int row_shr(int x) {
+    if constexpr (M == 0) return x;
+    else return __builtin_amdgcn_update_dpp(x, x, 0x110 + M, 0xF, 0xF, false);
+}
+template <int M>
+__device__ __forceinline__ double row_shr(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = row_shr<M>((int)(b & 0xFFFFFFFFLL)), hi = row_shr<M>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The first minimum over the lows of T[m] of lane g - m for m = M..0 (lows
+// ascending), a tie going to the lower low: lane g's layer cost and slot
+// positions.
+template <int M>
+__device__ __forceinline__ void merge_lows(const double* T, const unsigned* TR, int g, double& best, unsigned& at) {
+    if constexpr (M >= 0) {
+        const double t = row_shr<M>(T[M]);
+        const unsigned tr = (unsigned)row_shr<M>((int)TR[M]);
+        const bool take = g >= M && t < best;
+        best = take ? t : best;
+        at = take ? tr : at;
+        merge_lows<M - 1>(T, TR, g, best, at);
+    }
+}
+
+// solve_flt_node (src/routing/flt_block.cuh:6-142) for one node on G lanes
+// (g = this lane's index among them), for depths with few nodes.  Lane g
+// takes the via stacks with low end low = g + 1: it computes the costs
+// C(low, li') for li' = low..L (the via costs of layers low+1..li' summed
+// upward, then each slot's running first minimum over low..li' added, in the
+// order of the single-lane solves) and keeps T[m], the first minimum of
+// C(low, li') over li' >= low + m, a tie going to the lower li'.  Lane g then
+// reads, for its layer li = g + 1, T[li - low] of every lane with low <= li,
+// ascending, keeping the first strictly smaller one: the first minimum in
+// the order (low, then li' >= li) that the reference's passes leave at li.
+// Every cost is the same sum in the same order as in solve_node_fast and
+// solve_node_wide.
+template <int L, int K>
+__device__ __forceinline__ void solve_node_lanes(const FltArgs& a, const LdsTables<L>& tb, int v, const int4 r,
+                                                 int pos, int g) {
+    constexpr int G = kNodeLanes<L>;
+    const int kids = r.w & 7;
+    const int low_fixed = 1 + r.x / a.g.XY;
+    const int high_fixed = low_fixed == L + 1 ? 0 : low_fixed;
+    const int lf = min(L, low_fixed);
+    const int row0 = r.w >> 3;
+    const int low = g + 1;
+    // Straight-line code: lane-varying conditions are selects, and reads
+    // past a lane's layers or child slots read an in-range entry whose value
+    // is not used.
+    double T[L];
+    unsigned TR[L];  // slot positions of T, 4 bits per child slot
+    {
+        float M[K];
+        unsigned run = 0;
+        double via = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) M[k] = 0.f;
+#pragma unroll
+        for (int m = 0; m < L; ++m) {
+            const int li = low + m;
+            const bool in = li <= L;
+            const int lc = in ? li : L;
+            if (m) via += in ? (double)tb.vc[v * (L - 1) + lc - 2] : 0.0;
+            double cost = via;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const bool on = k < kids;
+                const float p = tb.cost(on ? row0 + k : row0, lc);
+                const bool take = on && in && (m == 0 || p < M[k]);
+                M[k] = take ? p : M[k];
+                run = take ? (run & ~(15u << (4 * k))) | ((unsigned)li << (4 * k)) : run;
+                cost = on ? cost + (double)M[k] : cost;
+            }
+            const bool ok = in && low <= lf && li >= high_fixed && cost < kInf;
+            T[m] = ok ? cost : kInf;
+            TR[m] = ok ? run : 0u;
+        }
+        double s = kInf;
+        unsigned sr = 0;
+#pragma unroll
+        for (int m = L - 1; m >= 0; --m) {
+            const bool take = T[m] < kInf && T[m] <= s;  // going down, a tie goes to the lower li'
+            s = take ? T[m] : s;
+            sr = take ? TR[m] : sr;
+            T[m] = s;
+            TR[m] = sr;
+        }
+    }
+    static_assert(G <= 16, "a node's lanes lie in one DPP row");
+    double best = kInf;
+    unsigned at = 0;
+    merge_lows<L - 1>(T, TR, g, best, at);  // lane g - m has low = li - m
+    if (g >= L) return;
+    const int li = g + 1;
+    if (best < kInf) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k >= kids) break;
+            const int j = (at >> (4 * k)) & 15;
+            const unsigned long long key = tb.key(row0 + k, j);
+            tb.set_path(row0 + k, li, (key & 0xFFFFFFFFULL) * kS + (unsigned long long)j);
+        }
+    }
+    if (r.y == r.z) {  // the root's trace reads its layer costs
+        if (!g) tb.set_root_cost(v, 0, kInf);
+        tb.set_root_cost(v, li, best);
+    }
+    if (!g) tb.posof[v] = pos;
+    tb.cvs[pos * L + li - 1] = best;
 }
 
 // One push item of a depth: edge e of a node at that depth towards target
@@ -878,13 +1103,32 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
         if (lane < items) first = load_item<L>(a, d, tb, e0, lane);
         if (here) {
             const int n0 = tb.doff[dep], end = tb.doff[dep + 1];
-            for (int i = n0 + lane; i < end; i += W) {
-                const int v = tb.order[i];
-                const int4 r = tb.nrec[v];
-                if ((r.w & 7) <= 2)
-                    solve_node_fast<L, 2, false>(a, d, tb, v, r, i - n0);
-                else
-                    solve_node_wide<L>(a, d, tb, v, r, i - n0);
+            // A node on kNodeLanes lanes costs about a quarter of a node on
+            // one lane: the lane-parallel solve takes depths whose nodes it
+            // covers in at most four times the single-lane passes.
+            constexpr int G = kNodeLanes<L>;
+            const int per = W / G, width = end - n0;
+            if (per && (width + per - 1) / per <= 4 * ((width + W - 1) / W)) {
+                for (int i = n0 + lane / G; i < end; i += per) {  // uniform over each node's G lanes
+                    const int v = tb.order[i];
+                    const int4 r = tb.nrec[v];
+                    const int kids = r.w & 7;
+                    if (kids <= 2)
+                        solve_node_lanes<L, 2>(a, tb, v, r, i - n0, lane % G);
+                    else if (kids <= 4)
+                        solve_node_lanes<L, 4>(a, tb, v, r, i - n0, lane % G);
+                    else
+                        solve_node_lanes<L, 6>(a, tb, v, r, i - n0, lane % G);
+                }
+            } else {
+                for (int i = n0 + lane; i < end; i += W) {
+                    const int v = tb.order[i];
+                    const int4 r = tb.nrec[v];
+                    if ((r.w & 7) <= 2)
+                        solve_node_fast<L, 2, false>(a, d, tb, v, r, i - n0);
+                    else
+                        solve_node_wide<L>(a, d, tb, v, r, i - n0);
+                }
             }
         }
         LdsTables<L>::barrier();
@@ -895,36 +1139,41 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
     int* routes = a.routes.routes + (live ? (size_t)a.routes.pin_acc_num[id] * a.routes.per_pin : 0);
     const long long cap = live ? (long long)(a.pin_acc_num[id + 1] - a.pin_acc_num[id]) * a.routes.per_pin : 0;
     const int g_cells = a.g.L * a.g.XY;
-    int count = 1;  // the root's trace sets the word count to 1
-    for (int dep = 0; dep < depths; ++dep) {
-        const int first = dep < dn.w ? tb.doff[dep] : 0, end = dep < dn.w ? tb.doff[dep + 1] : 0;
-        int chunks = (end - first + W - 1) / W;
-        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
-        for (int ch = 0; ch < chunks; ++ch) {
-            int w[8];
-            const int i = first + ch * W + lane;
-            int n = i < end ? trace_node(a, d, tb, tb.order[i], w) : 0;
-            for (int j = 0; j < n; ++j)
-                if (w[j] < 0 || w[j] >= g_cells) {
-                    raise(a.error, kDevBadCoordinate);
-                    n = 0;
-                }
-            int incl = n;
-            for (int s = 1; s < W; s <<= 1) {
-                const int o = __shfl_up(incl, s, W);
-                if (lane >= s) incl += o;
-            }
-            const int at = count + incl - n;
-            if (n) {
-                if (at + n - 1 >= cap) {
-                    raise(a.error, kDevRouteOverflow);
-                } else {
-                    for (int j = 0; j < n; ++j) routes[at + j] = w[j];
-                }
-            }
-            count += __shfl(incl, W - 1, W);
-        }
+    // trace pass one: the layers, depth by depth from the root
+    for (int dep = 0; dep < dn.w; ++dep) {
+        const int end = tb.doff[dep + 1];
+        for (int i = tb.doff[dep] + lane; i < end; i += W) trace_layer<L>(a, d, tb, tb.order[i]);
         LdsTables<L>::barrier();
+    }
+    // pass two: the words of every node in depth order, appended through a
+    // prefix sum over the DAG's lanes
+    const int n0 = dn.w ? tb.doff[0] : 0, total = dn.w ? tb.doff[dn.w] - n0 : 0;
+    int chunks = (total + W - 1) / W;
+    for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+    int count = 1;  // the root's trace sets the word count to 1
+    for (int ch = 0; ch < chunks; ++ch) {
+        Words wd;
+        const int q = ch * W + lane;
+        if (q < total) trace_words<L>(a, d, tb, tb.order[n0 + q], wd);
+        if (!wd.in_grid(g_cells)) {
+            raise(a.error, kDevBadCoordinate);
+            wd.ne = wd.nv = 0;
+        }
+        const int n = wd.count();
+        int incl = n;
+        for (int s = 1; s < W; s <<= 1) {
+            const int o = __shfl_up(incl, s, W);
+            if (lane >= s) incl += o;
+        }
+        const int at = count + incl - n;
+        if (n) {
+            if (at + n - 1 >= cap) {
+                raise(a.error, kDevRouteOverflow);
+            } else {
+                wd.put(routes, at);
+            }
+        }
+        count += __shfl(incl, W - 1, W);
     }
     if (live && lane == 0) routes[0] = count <= cap ? count : 1;
 }
@@ -1051,7 +1300,7 @@ __global__ __launch_bounds__(kWave) void flt_route_lds_kernel(FltArgs a) {
         const int4 dn = live ? a.sizes[k] : make_int4(0, 0, 0, 0);
         const Dag d = dag_of(a, ds);
         LdsTables<L> tb;
-        lds_layout<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_shape, &tb);
+        lds_layout<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_shape, tb);
         int depths = dn.w;
         for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
         route_dag_lds<L>(a, k, live, lane, W, d, tb, ds, dn, depths);
@@ -1085,7 +1334,10 @@ size_t flt_lds_bytes(int L, int slots, int nodes, int edges, int dnodes, int dep
     const LdsShape shape{slots, nodes, edges, dnodes, depths};
     switch (L) {
 #define GR_FLT_BYTES(n) \
-    case n: return lds_layout<n>(nullptr, shape, nullptr);
+    case n: {                                                    \
+        LdsTables<n> t;                                          \
+        return lds_layout<n>(nullptr, shape, t);                 \
+    }
         GR_FLT_BYTES(1)
         GR_FLT_BYTES(2)
         GR_FLT_BYTES(3)

@@ -182,6 +182,7 @@ __global__ void zshape_convert_kernel(ZshapeArgs a, const int* net_ids, RouteSto
 constexpr int kFltStride = 10;  // layer index 1..L of the FLT tables (L <= 9), 0 unused
 constexpr int kFltChangeWidth = 5;  // same-parity to-layers per (edge, from-layer)
 constexpr int kFltWideBatch = 512;  // FLT batches of at least this many DAGs form the second tuning class
+constexpr int kFltClasses = 2;  // FLT DAG classes by LDS share: one DP launch each, side by side
 // One DAG's share of LDS in flt_route_lds_kernel, sized by the batch's largest DAG.
 struct FltLdsShape {
     int slots, nodes, edges, dnodes, depths;  // child slots, nodes, edges, nodes of its widest depth, depths

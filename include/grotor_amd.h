@@ -258,6 +258,15 @@ int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave,
 int gr_lshape_tune_groups(gr_ctx *ctx, int nbatch, const int *batch_off, int *tiny_nodes, double *ms);
 int gr_ctx_set_tiny_tree(gr_ctx *ctx, int nodes);
 
+/* DAG classes of the FLT DP (gr_flt_run modes 1 and 2; no reference
+ * counterpart): per batch the DAGs whose LDS share exceeds `bytes` form one
+ * class and the rest another, each launched with shares sized by its own
+ * largest DAG (the large class on a second stream beside the other), so
+ * the small DAGs are not held to the batch's largest share.  0 puts every
+ * DAG in one class.  Default 20480; takes effect at the next
+ * gr_flt_upload.  Routes are the same for every split. */
+int gr_ctx_set_flt_split(gr_ctx *ctx, int bytes);
+
 /* Reset the routing state to that of a fresh context: demand, timestamps,
  * statistics and route words cleared, phase-two route storage released,
  * every track marked for refresh (the DAG-throughput benchmark's reset,
