@@ -1072,8 +1072,9 @@ __device__ __forceinline__ void finish_item(const FltArgs& a, const LdsTables<L>
 // into LDS, then the items lower their parents' slot keys; every other
 // access of the depth loop stays in LDS.
 template <int L>
-__device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live, int lane, int W, const Dag& dg,
+__device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live, int lane_in, int W, const Dag& dg,
                                               const LdsTables<L>& tb, const int4 ds, const int4 dn, int depths) {
+    const int lane = lane_in;
     // stage the DAG: child slots to the empty key, records, orders, via costs
     for (int i = lane; i < dn.z * L; i += W) tb.packed[i] = kEmpty;
     for (int v = lane; v < dn.x; v += W) {
@@ -1097,6 +1098,7 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
     d.nodes = tb.nrec;
     d.edges = tb.erec;  // {parent, slot, parent coordinate, child node}: the trace reads the first three
     for (int dep = depths - 1; dep >= 0; --dep) {
+        const int lane = opaque(lane_in);  // lane-derived addresses formed per depth
         const bool here = dep < dn.w;
         const int e0 = here ? tb.edoff[dep] : 0, items = here ? (tb.edoff[dep + 1] - e0) * L : 0;
         PushItem<L> first;

@@ -32,6 +32,14 @@ struct DeviceGrid {
 
 __host__ __device__ inline int layer_dir(const DeviceGrid& g, int l) { return (l & 1) ^ g.DIR; }
 
+// v through an opaque copy: values derived from it are formed where it is
+// used, not hoisted out of an enclosing loop where they would hold
+// registers for the whole kernel.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 __host__ __device__ inline int cell_of(const DeviceGrid& g, int l, int x, int y) {
     return l * g.XY + (layer_dir(g, l) ? x * g.Y + y : y * g.X + x);
 }

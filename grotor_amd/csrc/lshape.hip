@@ -146,6 +146,108 @@ __device__ __forceinline__ void gather_children(const Tables<L>& s, int i, int l
         if (hi >= lo) row[hi - lo] = acc[hi];
 }
 
+// Phase A on a node's L lanes (gl = layer c, wl = the lane's index in the
+// wavefront).  Lane gl takes row low = gl of the node's interval table and
+// walks its high ends downward, keeping the first minimum over high >= low
+// + m (a tie going to the lower high); at step m lane c reads that running
+// minimum from lane c - m (low = c - m, so the lows ascend as m falls) and
+// keeps the first strictly smaller one.  The result is the first minimum
+// over the intervals containing c in (low, high) row-major order, the
+// order the reference scans them in.
+template <int L>
+__device__ __forceinline__ void phase_a_lanes(const double* nd, int gl_in, int wl_in, double* gbest, int* gchoice) {
+    const int gl = opaque(gl_in), wl = opaque(wl_in);  // lane-derived addresses formed here
+    const int lo = gl;
+    double sm = kInf, best = kInf;
+    int sh = 0, chs = 0;
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) {
+        const int hi = lo + m;
+        const bool in = hi < L;  // lanes gl >= L: no row
+        const double v = nd[in ? tri<L>(lo, hi) : 0];
+        const bool take = in && v < kInf && v <= sm;  // going down, a tie goes to the lower high
+        sm = take ? v : sm;
+        sh = take ? hi : sh;
+        const int src = gl >= m ? wl - m : wl;
+        const double t = __shfl(sm, src, kWave);
+        const int th = __shfl(sh, src, kWave);
+        const bool pick = gl >= m && t < best;  // a tie goes to the lower low
+        best = pick ? t : best;
+        chs = pick ? (gl - m) * L + th : chs;
+        if (m % 2 == 0) asm volatile("" ::: "memory");  // bounds the loads in flight
+    }
+    if (gl < L) {
+        gbest[gl] = best;
+        gchoice[gl] = chs;
+    }
+}
+
+// Phase B (src/routing/lshape_warp.cuh:18-48) on a node's L lanes, lane gl
+// taking parent layer pl = gl.  A bent edge's candidates are the child
+// layers cl of the opposite parity; the corner via column between cl and pl
+// is summed from the lower layer up, starting from 0.0, exactly as the
+// reference folds it.  Lane a keeps those sums F(a, a + d) as d grows, over
+// both corner columns: the one its own layers' direction selects (lane
+// a + d reads it when a is that lane's candidate) and the other (lane a's
+// own candidates above it).  The reference keeps the first strictly smaller
+// candidate in ascending cl; taking the least (cost, cl) pair is the same
+// choice in any visiting order.
+template <int L>
+__device__ __forceinline__ void phase_b_lanes(const DeviceGrid& g, const Tables<L>& s, int i, int gl_in, int wl_in,
+                                              const double* gb, const int* gc) {
+    constexpr int LL = L * L;
+    const int gl = opaque(gl_in), wl = opaque(wl_in);
+    const int pl = gl;
+    const int vi = s.xy[i], vp = s.xy[s.par[i]];
+    const int x = vi >> 16, y = vi & 0xFFFF, px = vp >> 16, py = vp & 0xFFFF;
+    double best = kInf;
+    int bp = 0;
+    if (x == px || y == py) {
+        if (pl < L) {
+            const int d = layer_dir(g, pl);
+            if (!((d == 0 && y != py) || (d == 1 && x != px))) {
+                best = gb[pl] + s.w1[i * L + pl];
+                bp = gc[pl] * LL + pl * L + pl;
+            }
+        }
+    } else {
+        const int a = pl < L ? pl : L - 1;  // lanes gl >= L compute on layer L - 1, unused
+        // corner columns: (x, py) for a vertical first leg, (px, y) otherwise
+        const float* own = (layer_dir(g, a) ? s.va : s.vb) + i * L;
+        const float* other = (layer_dir(g, a ^ 1) ? s.va : s.vb) + i * L;
+        const double second = s.w2[i * L + a];
+        int bcl = L;
+        auto consider = [&](int cl, double fold) {
+            const double cost = gb[cl] + s.w1[i * L + cl];
+            const double cost2 = fold + second;
+            const double c = cost + cost2;
+            if (c < kInf && (c < best || (c == best && cl < bcl))) {
+                best = c;
+                bcl = cl;
+                bp = gc[cl] * LL + cl * L + pl;
+            }
+        };
+        double ua = 0.0, ub = 0.0;  // F(a, a + d) over the own and the other column
+#pragma unroll
+        for (int d = 1; d < L; ++d) {
+            if (d % 2) {
+                const int src = gl >= d ? wl - d : wl;
+                const double fold = __shfl(ua, src, kWave);  // F(pl - d, pl), lane pl - d's own column
+                if (pl < L && gl >= d) consider(pl - d, fold);
+                if (pl + d < L) consider(pl + d, ub);
+            }
+            const int j = a + d < L ? a + d : L - 1;
+            ua += (double)own[j];
+            ub += (double)other[j];
+            asm volatile("" ::: "memory");  // bounds the loads in flight
+        }
+    }
+    if (pl < L) {
+        s.diag[i * L + pl] = best;
+        s.dprev[i * L + pl] = bp;
+    }
+}
+
 template <int L>
 __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int first_slot) {
     constexpr int LL = L * L, T = kTri<L>;
@@ -235,7 +337,9 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
             }
             // phase A: the best interval containing each layer c, first
             // minimum in (low, high) row-major order
-            if (act) {
+            if (act && GL >= L) {
+                phase_a_lanes<L>(s.dist + size_t(i) * T, gl, int(threadIdx.x), s.gbest + grp * L, s.gchoice + grp * L);
+            } else if (act) {
                 const double* nd = s.dist + size_t(i) * T;
                 for (int c = gl; c < L; c += GL) {
                     double best = kInf;
@@ -258,7 +362,9 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
             }
             __syncthreads();
             // phase B (src/routing/lshape_warp.cuh:18-48)
-            if (act) {
+            if (act && GL >= L) {
+                phase_b_lanes<L>(g, s, i, gl, int(threadIdx.x), s.gbest + grp * L, s.gchoice + grp * L);
+            } else if (act) {
                 const int vi = s.xy[i], vp = s.xy[s.par[i]];
                 const int x = vi >> 16, y = vi & 0xFFFF, px = vp >> 16, py = vp & 0xFFFF;
                 const double* gb = s.gbest + grp * L;
