@@ -523,6 +523,34 @@ void parallel_items(int n, F&& f) {
     if (error) std::rethrow_exception(error);
 }
 
+// f(0) .. f(n - 1) for a few heavy items, one at a time per thread.
+template <class F>
+void parallel_tasks(int n, F&& f) {
+    const int threads = std::max(1, std::min<int>({16, n, int(std::thread::hardware_concurrency())}));
+    if (threads <= 1) {
+        for (int k = 0; k < n; ++k) f(k);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr error;
+    std::mutex m;
+    auto work = [&] {
+        try {
+            for (int k = next++; k < n && !failed.load(std::memory_order_relaxed); k = next++) f(k);
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(m);
+            if (!error) error = std::current_exception();
+            failed = true;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    if (error) std::rethrow_exception(error);
+}
+
 constexpr size_t kLdsLimit = 64 * 1024;
 
 void ensure_gscratch(gr_ctx* c, size_t bytes) {
@@ -2054,6 +2082,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // depth (stable, the reference's radix sort on (net, depth)), bytes
         std::vector<double> net_eb(nets), net_rb(nets);
         std::vector<int> net_dnodes(nets);
+        std::vector<char> big(nets);  // the DAG's LDS share exceeds f_lds_split
         parallel_items(nets, [&](int k) {
             const long long n0 = node_off[k], e0 = edge_off[k];
             const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
@@ -2103,6 +2132,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             }
             ek[depths] = ne_at;
             net_dnodes[k] = dn_max;
+            big[k] = flt_lds_bytes(L, slot, int(nn), int(ne), dn_max, depths) > size_t(c->f_lds_split);
             desc[k] = make_int4(int(n0), int(e0), int(sbase[k]), int(dbase[k]));
             sizes[k] = make_int4(int(nn), int(ne), slot, depths);
             ids[k] = net_ids[k];
@@ -2111,6 +2141,12 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             // (record, layer costs), per edge 8 (its record)
             net_rb[k] = nn * (16.0 + 8.0 * L) + 8.0 * ne;
         });
+        // the records cross to the device while the host orders the batches
+        // below; the per-net descriptors, last in the layout, follow them
+        Stage dev{};
+        c->f_ws.carve([&](auto& cv) { layout(cv, dev); });
+        const size_t bulk = size_t(reinterpret_cast<char*>(hs.desc) - c->f_stage);
+        if (bulk) upload(dev.nrec, c->f_stage, bulk, c->stream);
 
         // per batch: stage-global bases, the table sizes and the algorithmic
         // bytes per launch (the layer-change search, 12 * L * (len + 1) per
@@ -2147,17 +2183,13 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // share.  Inside a class the DAGs of a wavefront step through their
         // depths together: decreasing depth, then size (results do not
         // depend on the order of nets inside a batch).
-        {
-            std::vector<char> big(nets);
-            for (int k = 0; k < nets; ++k)
-                big[k] = flt_lds_bytes(L, sizes[k].z, sizes[k].x, sizes[k].y, net_dnodes[k], sizes[k].w) >
-                         size_t(c->f_lds_split);
+        c->f_large.assign(nbatch, 0);
+        c->f_class_max.assign(size_t(nbatch) * kFltClasses, FltLdsShape{0, 0, 0, 0, 0});
+        parallel_tasks(nbatch, [&](int b) {
             std::vector<int> perm;
             std::vector<int4> dp, sp;
             std::vector<int> ip;
-            c->f_large.assign(nbatch, 0);
-            c->f_class_max.assign(size_t(nbatch) * kFltClasses, FltLdsShape{0, 0, 0, 0, 0});
-            for (int b = 0; b < nbatch; ++b) {
+            {
                 const int k0 = batch_off[b], k1 = batch_off[b + 1];
                 perm.resize(k1 - k0);
                 for (int i = 0; i < k1 - k0; ++i) perm[i] = k0 + i;
@@ -2179,15 +2211,13 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                 }
                 for (int i = 0; i < k1 - k0; ++i) desc[k0 + i] = dp[i], sizes[k0 + i] = sp[i], ids[k0 + i] = ip[i];
             }
-        }
+        });
         if (c->f_queue_cap < nbatch) {
             c->drop_graph();
             c->mem.release(c->f_queue);
             c->f_queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * kFltClasses * kFltQueues * kFltQueueStride);
             c->f_queue_cap = nbatch;
         }
-        Stage dev{};
-        c->f_ws.carve([&](auto& cv) { layout(cv, dev); });
         FltArgs& f = c->f;
         c->f_tables.carve([&](auto& cv) {
             f.costs = cv.template take<double>(size_t(max_nodes) * kFltStride);
@@ -2197,7 +2227,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             f.cost_edges = cv.template take<double>(size_t(max_edges) * 9 * kFltChangeWidth);
             f.best_change = cv.template take<int>(size_t(max_edges) * 9 * kFltChangeWidth);
         });
-        if (stage_bytes) upload(dev.nrec, c->f_stage, stage_bytes, c->stream);
+        if (stage_bytes > bulk)
+            upload(reinterpret_cast<char*>(dev.nrec) + bulk, c->f_stage + bulk, stage_bytes - bulk, c->stream);
         f.g = g;
         f.node_rec = dev.nrec;
         f.edge_rec = dev.erec;
