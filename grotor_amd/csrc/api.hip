@@ -2275,11 +2275,7 @@ int gr_flt_run(gr_ctx* c, int first_stamp, int remove, int mode, double* elapsed
                 launch_refresh(c, 0);
                 if (!count) continue;
                 const FltArgs a = flt_batch_args(c, b);
-                const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
-                timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] {
-                    if (threads)
-                        hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
-                });
+                timed(c, GR_K_FLT_EDGE, c->f_edge_bytes[b], [&] { launch_flt_edges(a, c->stream); });
                 int* queue = mode == 2 ? c->f_queue + size_t(b) * kFltClasses * kFltQueues * kFltQueueStride : nullptr;
                 timed(c, GR_K_FLT, c->f_route_bytes[b], [&] {
                     if (mode == 0) {
@@ -2358,9 +2354,7 @@ int gr_flt_tune(gr_ctx* c, int* per_wave, double* ms) {
             const int begin = c->f_batch_off[b], count = c->f_batch_off[b + 1] - begin;
             if (!count) continue;
             const FltArgs a = flt_batch_args(c, b);
-            const long long threads = (long long)a.edges * c->g.L * kFltChangeWidth;
-            if (threads)
-                hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((threads + 255) / 256)), dim3(256), 0, c->stream, a);
+            launch_flt_edges(a, c->stream);
             const int cls = count >= kFltWideBatch;
             seen[cls] = true;
             for (int ci = 0; ci < 4; ++ci) {

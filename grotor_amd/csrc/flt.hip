@@ -1192,49 +1192,126 @@ __device__ __forceinline__ Dag dag_of(const FltArgs& a, const int4 ds) {
 
 }  // namespace
 
-// precompute_cost_edge (src/routing/flt_cost.cuh:50-130); one thread per
-// (edge, from-layer, to-layer of the same parity) of the batch.
-__global__ void flt_edge_kernel(FltArgs a) {
+// precompute_cost_edge (src/routing/flt_cost.cuh:50-130).  A workgroup
+// takes kEdgeGroup(L) edges of the batch: it first gathers, at every sample
+// position of each edge, the via costs of all layers and the prefix sums of
+// the layers that run along the edge into LDS (each value once, instead of
+// once per layer pair), then one thread per (edge, from-layer, same-parity
+// to-layer) scans the positions from LDS, summing exactly what the
+// reference sums in the same order.
+constexpr int kEdgeThreads = 256;
+constexpr int kEdgePositions = 40;  // sample positions of an edge: at most 38 (length 39)
+__host__ __device__ constexpr int edge_group(int L) { return kEdgeThreads / (L * kFltChangeWidth); }
+
+__global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     const DeviceGrid& g = a.g;
     const int L = g.L;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)a.edges * L * kFltChangeWidth) return;
-    const int e = (int)(t / (L * kFltChangeWidth));
-    const int r = (int)(t % (L * kFltChangeWidth));
+    const int E = edge_group(L);
+    struct EdgeInfo {
+        int fromx, fromy, tox, toy, start, dd, count, vertical;
+    };
+    __shared__ EdgeInfo info[kEdgeThreads / kFltChangeWidth];
+    extern __shared__ __align__(16) char smem[];
+    // per edge of the group: vc[k * (L - 1) + l] (float), ps[k * L + l] (double), k = position - 1
+    float* vcs = reinterpret_cast<float*>(smem);
+    double* pss = reinterpret_cast<double*>(smem + size_t(E) * kEdgePositions * (L - 1) * 4 + 8);
+    const int e0 = blockIdx.x * E;
+    if (threadIdx.x < E) {
+        EdgeInfo& in = info[threadIdx.x];
+        in.count = 0;
+        const int e = e0 + threadIdx.x;
+        if (e < a.edges) {
+            const int4 er = a.edge_rec[a.edge_base + e];
+            in.tox = er.z / g.Y % g.X, in.toy = er.z % g.Y;
+            in.fromx = er.w / g.Y % g.X, in.fromy = er.w % g.Y;
+            const int distance = abs(in.tox - in.fromx) + abs(in.toy - in.fromy);
+            in.vertical = in.fromx == in.tox;
+            in.start = in.vertical ? min(in.fromy, in.toy) : min(in.fromx, in.tox);
+            in.dd = distance ? distance / min(distance, 20) : 1;
+            // positions start + dd * k < start + distance, k >= 1
+            in.count = distance ? (distance - 1) / in.dd : 0;
+        }
+    }
+    __syncthreads();
+    // gather: every (edge, position, value) of the group, the positions of
+    // an edge being its cells at start + dd * k
+    const int per_pos = 2 * L - 1;  // L - 1 via costs, then L prefix sums
+    for (int q = threadIdx.x; q < E * kEdgePositions * per_pos; q += kEdgeThreads) {
+        const int j = q / (kEdgePositions * per_pos), rem = q % (kEdgePositions * per_pos);
+        const int k = rem / per_pos, v = rem % per_pos;
+        const EdgeInfo& in = info[j];
+        if (k >= in.count || k >= kEdgePositions) continue;
+        const int sp = in.start + in.dd * (k + 1);
+        const int sx = in.vertical ? in.fromx : sp, sy = in.vertical ? sp : in.fromy;
+        if (v < L - 1) {
+            vcs[(j * kEdgePositions + k) * (L - 1) + v] = a.vcost[cell_of(g, v, sx, sy)];
+        } else {
+            const int l = v - (L - 1);
+            // only the layers along the edge are read
+            if (layer_dir(g, l) == (in.vertical ? 1 : 0)) pss[(j * kEdgePositions + k) * L + l] = a.presum[cell_of(g, l, sx, sy)];
+        }
+    }
+    __syncthreads();
+    const int j = threadIdx.x / (L * kFltChangeWidth), r = threadIdx.x % (L * kFltChangeWidth);
+    const int e = e0 + j;
+    if (j >= E || e >= a.edges) return;
     const int f = r / kFltChangeWidth, to = (f & 1) + 2 * (r % kFltChangeWidth);
     // to == f is the straight edge, which the DP prices itself
     if (to >= L || to == f) return;
-    const int4 er = a.edge_rec[a.edge_base + e];
-    const int tox = er.z / g.Y % g.X, toy = er.z % g.Y;
-    const int fromx = er.w / g.Y % g.X, fromy = er.w % g.Y;
+    const EdgeInfo& in = info[j];
     // a from-layer across the edge: the reference fills its row with
     // infinity; the DP never reads it (its direction test comes first)
-    if ((layer_dir(g, f) == 0 && fromy != toy) || (layer_dir(g, f) == 1 && fromx != tox)) return;
-    const int distance = abs(tox - fromx) + abs(toy - fromy);
-    if (!distance) return;
-    const int dd = distance / min(distance, 20);
-    const bool vertical = fromx == tox;
-    const int start = vertical ? min(fromy, toy) : min(fromx, tox);
-    const int end = vertical ? max(fromy, toy) : max(fromx, tox);
+    if ((layer_dir(g, f) == 0 && in.fromy != in.toy) || (layer_dir(g, f) == 1 && in.fromx != in.tox)) return;
+    if (in.fromx == in.tox && in.fromy == in.toy) return;  // zero length
     double best = kInf;
     int pos = -1;
-    for (int s = start + dd; s < end; s += dd) {
-        const int sx = vertical ? fromx : s, sy = vertical ? s : fromy;
-        double cost = 0.0;
-        for (int l = min(f, to); l < max(f, to); ++l) cost += (double)a.vcost[cell_of(g, l, sx, sy)];
-        if (vertical)
-            cost += wire(a, f, fromx, fromx, min(fromy, sy), max(fromy, sy)) +
-                    wire(a, to, tox, tox, min(toy, sy), max(toy, sy));
-        else
-            cost += wire(a, f, min(fromx, sx), max(fromx, sx), fromy, fromy) +
-                    wire(a, to, min(tox, sx), max(tox, sx), toy, toy);
-        if (cost < best) {
-            best = cost;
-            pos = s;
+    const int lo = min(f, to), hi = max(f, to);
+    if (in.count <= kEdgePositions) {
+        // the wires from each end to the position: prefix sums at the far
+        // corner minus the near corner, as wire() forms them
+        const double pf = a.presum[cell_of(g, f, in.fromx, in.fromy)], pt = a.presum[cell_of(g, to, in.tox, in.toy)];
+        const int fa = in.vertical ? in.fromy : in.fromx, ta = in.vertical ? in.toy : in.tox;
+        const float* vc = vcs + size_t(j) * kEdgePositions * (L - 1);
+        const double* ps = pss + size_t(j) * kEdgePositions * L;
+        for (int k = 0; k < in.count; ++k) {
+            const int sp = in.start + in.dd * (k + 1);
+            double cost = 0.0;
+            for (int l = lo; l < hi; ++l) cost += (double)vc[k * (L - 1) + l];
+            const double sf = ps[k * L + f], st = ps[k * L + to];
+            const double wf = fa < sp ? sf - pf : pf - sf;
+            const double wt = ta < sp ? st - pt : pt - st;
+            cost += wf + wt;
+            if (cost < best) {
+                best = cost;
+                pos = sp;
+            }
+        }
+    } else {  // longer than any edge the sampling rule gives: the direct search
+        for (int sp = in.start + in.dd; sp < in.start + in.dd * (in.count + 1); sp += in.dd) {
+            const int sx = in.vertical ? in.fromx : sp, sy = in.vertical ? sp : in.fromy;
+            double cost = 0.0;
+            for (int l = lo; l < hi; ++l) cost += (double)a.vcost[cell_of(g, l, sx, sy)];
+            if (in.vertical)
+                cost += wire(a, f, in.fromx, in.fromx, min(in.fromy, sy), max(in.fromy, sy)) +
+                        wire(a, to, in.tox, in.tox, min(in.toy, sy), max(in.toy, sy));
+            else
+                cost += wire(a, f, min(in.fromx, sx), max(in.fromx, sx), in.fromy, in.fromy) +
+                        wire(a, to, min(in.tox, sx), max(in.tox, sx), in.toy, in.toy);
+            if (cost < best) {
+                best = cost;
+                pos = sp;
+            }
         }
     }
     a.cost_edges[change_slot(e, f, to)] = best;
     a.best_change[change_slot(e, f, to)] = pos;
+}
+
+void launch_flt_edges(const FltArgs& a, hipStream_t stream) {
+    const int L = a.g.L, E = edge_group(L);
+    if (!a.edges || L < 2) return;
+    const size_t lds = size_t(E) * kEdgePositions * ((L - 1) * 4 + L * 8) + 8;
+    hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((a.edges + E - 1) / E)), dim3(kEdgeThreads), lds, stream, a);
 }
 
 namespace {

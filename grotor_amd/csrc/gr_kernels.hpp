@@ -223,7 +223,8 @@ struct FltArgs {
 };
 constexpr int kFltQueues = 32;       // persistent FLT DP: work-queue counters per batch
 constexpr int kFltQueueStride = 32;  // ints between counters (one 128-byte line)
-__global__ void flt_edge_kernel(FltArgs a);
+// The layer-change search of a batch's edges (flt_edge_kernel).
+void launch_flt_edges(const FltArgs& a, hipStream_t stream);
 // LDS bytes of one DAG's share of flt_route_lds_kernel (child slots, nodes,
 // edges, nodes of its widest depth, depths; L layers)
 size_t flt_lds_bytes(int L, int slots, int nodes, int edges, int dnodes, int depths);
