@@ -332,6 +332,9 @@ def bench_pipeline(args, ga, design, topo):
     ctx.lshape_upload(tsched["net_ids"], tsched["node_off"], tsched["nodes"], tsched["parents"])
     l_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.lshape_tune(tsched["batch_off"])
     g_tune = (4, [-1.0] * 4) if args.no_tune else ctx.lshape_tune_groups(tsched["batch_off"])
+    t_tune = (0, [-1.0] * 4) if args.no_tune else ctx.lshape_tune_tiny(tsched["batch_off"])
+    # the bound again at the group's chosen nets per wavefront
+    g_tune = g_tune if args.no_tune else ctx.lshape_tune_groups(tsched["batch_off"])
     ctx.reset()
     ctx.lshape_run(tsched["batch_off"], first_stamp=1, mode=0)
     flags = ctx.ripup(nsub)
@@ -344,7 +347,8 @@ def bench_pipeline(args, ga, design, topo):
     ctx.flt_upload(design.flt_plan(fcolor, fnb))
     f_tune = ([1, 1], [[-1.0] * 4] * 2) if args.no_tune else ctx.flt_tune()
     print(f"[bench] autotuners: L-shape nets per wavefront {l_tune[0]}, persistent tiny-tree group <= {g_tune[0]} "
-          f"nodes {[round(v, 3) for v in g_tune[1]]}, FLT DAGs per wavefront {f_tune[0]}; "
+          f"nodes {[round(v, 3) for v in g_tune[1]]} at {t_tune[0]} per wavefront {[round(v, 3) for v in t_tune[1]]}, "
+          f"FLT DAGs per wavefront {f_tune[0]}; "
           f"{fs['nets']} rerouted nets, {fs['nodes']} DAG nodes, {fs['edges']} DAG edges",
           file=sys.stderr, flush=True)
 
@@ -482,7 +486,9 @@ def bench_pipeline(args, ga, design, topo):
                                                   "dp_ms_1_2_4_8": [[round(v, 3) for v in row] for row in f_tune[1]]},
                                           "lshape_persistent_groups": {
                                               "tiny_tree_bound": g_tune[0],
-                                              "dp_ms_bound_1_2_4_8": [round(v, 3) for v in g_tune[1]]}},
+                                              "dp_ms_bound_1_2_4_8": [round(v, 3) for v in g_tune[1]],
+                                              "tiny_nets_per_wavefront": t_tune[0],
+                                              "dp_ms_tiny_per_wave_1_2_4_8": [round(v, 3) for v in t_tune[1]]}},
                    "soft_dp": soft, "parallelism": "single GPU",
                    "kernel_breakdown": {"steps": 1, "form": "persistent", "classes": kernels}},
         "roofline": roofline,

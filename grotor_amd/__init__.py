@@ -532,6 +532,19 @@ class Context:
     def set_tiny_tree(self, nodes):
         _call("gr_ctx_set_tiny_tree", self._h, ctypes.c_int(nodes))
 
+    def lshape_tune_tiny(self, batch_off):
+        """Autotune the persistent form's tiny-tree group's nets per
+        wavefront: (chosen, ms for 1, 2, 4, 8)."""
+        batch_off = _i32(batch_off)
+        chosen = ctypes.c_int()
+        ms = np.zeros(4, np.float64)
+        _call("gr_lshape_tune_tiny", self._h, ctypes.c_int(len(batch_off) - 1), _ptr(batch_off, ctypes.c_int),
+              ctypes.byref(chosen), _ptr(ms, ctypes.c_double))
+        return chosen.value, ms.tolist()
+
+    def set_tiny_per_wave(self, per_wave):
+        _call("gr_ctx_set_tiny_per_wave", self._h, ctypes.c_int(per_wave))
+
     def set_flt_split(self, nbytes):
         """LDS-share bound (bytes) between the FLT DP's two DAG classes, for
         the next flt_upload (0: one class)."""

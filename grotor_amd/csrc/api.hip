@@ -227,6 +227,7 @@ struct gr_ctx {
     float *cxsum = nullptr, *cysum = nullptr;
     int per_wave[kTreeClasses] = {1, 1};  // L-shape DP nets per wavefront by tree size class (gr_lshape_tune)
     int tiny_bound = 4;  // persistent form: trees of at most this many nodes form its first group (gr_lshape_tune_groups)
+    int tiny_per_wave = 0;  // nets per wavefront of that group, 0: the small class's (gr_lshape_tune_tiny)
     int batch_mask_words = 2;  // prefix-mask words of the last batching call
     int color_waves_per_cu = 32;
     // kernel profiling (HIP events on the context stream)
@@ -695,9 +696,14 @@ void launch_lshape_plain(gr_ctx* c, const BatchLaunch& b) {
 // Slots [begin, begin + count) of class k (trees of at most nmax nodes)
 // drained by the persistent DP kernel from its queue counters (zero
 // beforehand), on `stream`; trees too large for LDS take the launch form.
-void launch_lshape_queue_range(gr_ctx* c, int begin, int count, int nmax, int k, int* queue, hipStream_t stream) {
+void launch_lshape_queue_range(gr_ctx* c, int begin, int count, int nmax, int k, int* queue, hipStream_t stream,
+                               int per_wave = 0) {
     if (!count) return;
-    const int p = per_wave_for(c, k, nmax);
+    int p = per_wave_for(c, k, nmax);
+    if (p > 0 && per_wave > 0) {  // an override, halved until the workgroup's tables fit in LDS
+        const size_t bytes = (lshape_scratch_bytes(c->g.L, nmax) + 7) / 8 * 8;
+        for (p = per_wave; p > 1 && bytes * p > kLdsLimit;) p /= 2;
+    }
     if (p <= 0) {  // tables beyond LDS: the global-table launch over the range
         const BatchLaunch one{begin, {count, 0}, {nmax, 1}};
         launch_lshape_class(c, one, 0, 0, stream);
@@ -755,7 +761,8 @@ void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     // profiled runs time each launch alone: every group on the main stream
     if (live < 2 || c->profiling) {
         for (int j = 0; j < kPersistentGroups; ++j)
-            launch_lshape_queue_range(c, grp[j].begin, grp[j].count, grp[j].nmax, grp[j].k, queue + j * qs, c->stream);
+            launch_lshape_queue_range(c, grp[j].begin, grp[j].count, grp[j].nmax, grp[j].k, queue + j * qs, c->stream,
+                                      j == 0 ? c->tiny_per_wave : 0);
         return;
     }
     hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
@@ -763,7 +770,7 @@ void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     hip_check(hipStreamWaitEvent(c->side2, c->fork_ev, 0), "hipStreamWaitEvent");
     launch_lshape_queue_range(c, grp[2].begin, grp[2].count, grp[2].nmax, 1, queue + 2 * qs, c->side);
     launch_lshape_queue_range(c, grp[1].begin, grp[1].count, grp[1].nmax, 0, queue + qs, c->side2);
-    launch_lshape_queue_range(c, grp[0].begin, grp[0].count, grp[0].nmax, 0, queue, c->stream);
+    launch_lshape_queue_range(c, grp[0].begin, grp[0].count, grp[0].nmax, 0, queue, c->stream, c->tiny_per_wave);
     hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
     hip_check(hipEventRecord(c->join2_ev, c->side2), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->stream, c->join_ev, 0), "hipStreamWaitEvent");
@@ -1444,7 +1451,8 @@ int gr_lshape_run(gr_ctx* c, int nbatch, const int* batch_off, int first_stamp, 
         // mode 2 issue the same launches directly
         const bool graph = mode == 2 && !c->profiling;
         if (graph) {
-            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode, c->tiny_bound};
+            std::vector<int> key{c->stage_gen, first_stamp, nbatch, c->per_wave[0], c->per_wave[1], mode, c->tiny_bound,
+                                 c->tiny_per_wave};
             key.insert(key.end(), batch_off, batch_off + nbatch + 1);
             // the graph reads the descriptors when it runs
             c->slots_h = slots;
@@ -1573,59 +1581,85 @@ int gr_lshape_tune(gr_ctx* c, int nbatch, const int* batch_off, int* per_wave, d
     });
 }
 
+// Times the persistent DP of the uploaded stage's batches (mode 2's launches,
+// the groups on their streams, at the stage's current costs; the routes it
+// writes are rewritten by the stage) for each candidate setting applied by
+// `set`: the best of two runs per candidate, into ms[i]; keeps the fastest
+// (applied again before returning) and returns its index.
+extern "C++" {
+template <class Set>
+int tune_persistent(gr_ctx* c, int nbatch, const int* batch_off, int ncand, Set&& set, double* ms) {
+    if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
+    require_stage(c, 0, batch_off[nbatch]);
+    if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
+    for (int b = 0; b < nbatch; ++b)
+        if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
+    const long long qwords = (long long)kPersistentGroups * kDpQueues * kDpQueueStride;
+    int* queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * qwords);
+    const bool profiling = c->profiling;
+    c->profiling = false;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "hipEventCreate");
+    hip_check(hipEventCreate(&e1), "hipEventCreate");
+    double best = 1e30;
+    int choice = 0;
+    std::vector<int> slots(c->stage_count);
+    launch_refresh(c, 0);
+    for (int ci = 0; ci < ncand; ++ci) {
+        set(ci);
+        std::vector<BatchLaunch> plan(nbatch);
+        for (int b = 0; b < nbatch; ++b) plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
+        reserve_dp_scratch(c, plan);
+        c->slots_h = slots;
+        upload_desc(c, slots, 0, c->stage_count);
+        double t = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((nbatch * qwords + 255) / 256)), dim3(256), 0, c->stream,
+                               queue, 0, nbatch * qwords);
+            hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
+            for (int b = 0; b < nbatch; ++b) launch_lshape_persistent(c, plan[b], queue + size_t(b) * qwords);
+            hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
+            hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+            float f = 0;
+            hip_check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+            t = std::min(t, double(f));
+        }
+        if (ms) ms[ci] = t;
+        if (t < best) best = t, choice = ci;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->mem.release(queue);
+    set(choice);
+    c->profiling = profiling;
+    c->drop_graph();
+    check_device_error(c, "persistent-form tuning");
+    return choice;
+}
+}  // extern "C++"
+
 int gr_lshape_tune_groups(gr_ctx* c, int nbatch, const int* batch_off, int* tiny_nodes, double* ms) {
     return guarded([&] {
-        if (!c || nbatch < 0 || !batch_off) throw GrError(GR_E_ARG, "invalid arguments");
-        require_stage(c, 0, batch_off[nbatch]);
-        if (batch_off[0] != 0) throw GrError(GR_E_ARG, "batch offsets must start at 0");
-        for (int b = 0; b < nbatch; ++b)
-            if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets must be non-decreasing");
-        const long long qwords = (long long)kPersistentGroups * kDpQueues * kDpQueueStride;
-        int* queue = c->mem.alloc<int>(size_t(std::max(nbatch, 1)) * qwords);
-        const bool profiling = c->profiling;
-        c->profiling = false;
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "hipEventCreate");
-        hip_check(hipEventCreate(&e1), "hipEventCreate");
         const int candidates[4] = {1, 2, 4, 8};  // 1: no group of its own (every routed tree has 2 nodes or more)
-        const int keep = c->tiny_bound;
-        double best = 1e30;
-        int choice = keep;
-        std::vector<int> slots(c->stage_count);
-        launch_refresh(c, 0);
-        for (int ci = 0; ci < 4; ++ci) {
-            c->tiny_bound = candidates[ci];
-            std::vector<BatchLaunch> plan(nbatch);
-            for (int b = 0; b < nbatch; ++b) plan[b] = plan_batch(c, batch_off[b], batch_off[b + 1] - batch_off[b], slots);
-            reserve_dp_scratch(c, plan);
-            c->slots_h = slots;
-            upload_desc(c, slots, 0, c->stage_count);
-            // every batch's persistent DP at the stage's current costs, the
-            // groups on their streams; the routes it writes are rewritten by
-            // the stage
-            double t = 1e30;
-            for (int rep = 0; rep < 2; ++rep) {
-                hipLaunchKernelGGL(fill_int_kernel, dim3(unsigned((nbatch * qwords + 255) / 256)), dim3(256), 0,
-                                   c->stream, queue, 0, nbatch * qwords);
-                hip_check(hipEventRecord(e0, c->stream), "hipEventRecord");
-                for (int b = 0; b < nbatch; ++b) launch_lshape_persistent(c, plan[b], queue + size_t(b) * qwords);
-                hip_check(hipEventRecord(e1, c->stream), "hipEventRecord");
-                hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
-                float f = 0;
-                hip_check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
-                t = std::min(t, double(f));
-            }
-            if (ms) ms[ci] = t;
-            if (t < best) best = t, choice = candidates[ci];
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        c->mem.release(queue);
-        c->tiny_bound = choice;
-        c->profiling = profiling;
-        c->drop_graph();
-        if (tiny_nodes) *tiny_nodes = choice;
-        check_device_error(c, "gr_lshape_tune_groups");
+        const int ci = tune_persistent(c, nbatch, batch_off, 4, [&](int i) { c->tiny_bound = candidates[i]; }, ms);
+        if (tiny_nodes) *tiny_nodes = candidates[ci];
+    });
+}
+
+int gr_lshape_tune_tiny(gr_ctx* c, int nbatch, const int* batch_off, int* per_wave, double* ms) {
+    return guarded([&] {
+        const int candidates[4] = {1, 2, 4, 8};
+        const int ci = tune_persistent(c, nbatch, batch_off, 4, [&](int i) { c->tiny_per_wave = candidates[i]; }, ms);
+        if (per_wave) *per_wave = candidates[ci];
+    });
+}
+
+int gr_ctx_set_tiny_per_wave(gr_ctx* c, int per_wave) {
+    return guarded([&] {
+        if (!c) throw GrError(GR_E_ARG, "null context");
+        if (per_wave != 0 && per_wave != 1 && per_wave != 2 && per_wave != 4 && per_wave != 8)
+            throw GrError(GR_E_ARG, "nets per wavefront must be 0 (the small class's), 1, 2, 4 or 8");
+        c->tiny_per_wave = per_wave;
     });
 }
 

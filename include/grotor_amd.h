@@ -258,6 +258,17 @@ int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave,
 int gr_lshape_tune_groups(gr_ctx *ctx, int nbatch, const int *batch_off, int *tiny_nodes, double *ms);
 int gr_ctx_set_tiny_tree(gr_ctx *ctx, int nodes);
 
+/* Nets per wavefront of the persistent form's tiny-tree group (no reference
+ * counterpart): times the persistent DP of the uploaded batches with 1, 2,
+ * 4 and 8 of its trees per wavefront (each then owns 64, 32, 16 or 8 lanes)
+ * at the current tiny-tree bound and keeps the fastest; *per_wave receives
+ * it, ms[4] (may be NULL) the times.  Routes are the same for every choice;
+ * route words are overwritten, so call it before the stage run.
+ * gr_ctx_set_tiny_per_wave sets it by hand (0, the default, takes the
+ * small class's value from gr_lshape_tune). */
+int gr_lshape_tune_tiny(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave, double *ms);
+int gr_ctx_set_tiny_per_wave(gr_ctx *ctx, int per_wave);
+
 /* DAG classes of the FLT DP (gr_flt_run modes 1 and 2; no reference
  * counterpart): per batch the DAGs whose LDS share exceeds `bytes` form one
  * class and the rest another, each launched with shares sized by its own

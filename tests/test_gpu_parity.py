@@ -186,13 +186,14 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     ctx.close()
 
 
-@pytest.mark.parametrize("per_wave", [(1, 1), (4, 1)])
-def test_persistent_mode_matches_oracle(ga, ora, per_wave):
+@pytest.mark.parametrize("per_wave,tiny", [((1, 1), (4, 0)), ((4, 1), (4, 0)), ((1, 1), (8, 4)), ((2, 1), (2, 8))])
+def test_persistent_mode_matches_oracle(ga, ora, per_wave, tiny):
     """Mode 2 (one persistent DP launch per batch draining a work queue,
     the loop captured in a HIP graph): replays and a switch back and forth
-    with mode 1 equal the oracle."""
+    with mode 1 equal the oracle, for several tiny-tree groups (bound, nets
+    per wavefront)."""
     _, t = topology(6, 5)
-    print(f"persistent mode config=6 seed=5 per_wave={per_wave}", flush=True)
+    print(f"persistent mode config=6 seed=5 per_wave={per_wave} tiny={tiny}", flush=True)
     geom = (t["pt_off"], t["pts"], t["hs_off"], t["hsegs"], t["vs_off"], t["vsegs"])
     color, nb = ora.batches(t["X"], t["Y"], *geom)
     sched = ga.execution_schedule(t, color, nb)
@@ -201,6 +202,8 @@ def test_persistent_mode_matches_oracle(ga, ora, per_wave):
     ctx.set_routes(t["pin_acc_num"])
     ctx.lshape_upload(sched["net_ids"], sched["node_off"], sched["nodes"], sched["parents"])
     ctx.set_nets_per_wave(per_wave)
+    ctx.set_tiny_tree(tiny[0])
+    ctx.set_tiny_per_wave(tiny[1])
     for rep, mode in enumerate((2, 2, 1, 0, 2)):
         ctx.reset()
         ctx.lshape_run(sched["batch_off"], first_stamp=1, mode=mode)
