@@ -979,15 +979,20 @@ __device__ __forceinline__ void solve_node_lanes(const FltArgs& a, const LdsTabl
 // solve_flt_node, :81-141, split by (edge, target) across lanes).  Its
 // global loads (prefix sums, layer-change costs) depend only on the edge, so
 // they are issued before the depth's nodes are solved.
+// An edge has H = ceil(L / 2) items: item mi of an edge that changes its
+// coordinate targets layer par + 2 mi (the only layers of the usable
+// parity); of a zero-length edge, layers mi + 1 and mi + 1 + H.
 template <int L>
 struct PushItem {
     static constexpr int H = (L + 1) / 2;
     int t = 0;           // target layer, 0: no item
+    int t2 = 0;          // zero-length edge: second target layer, 0: none
     int v = 0, row = 0;  // child node, parent slot row
     int par = 1;         // first from-layer of the usable parity
     bool zero = false;   // zero-length edge: straight on every layer
     bool far = false;    // long enough to change layers (> 4)
     double add[H];       // per from-layer par + 2i: wire cost (its layer == t) or layer-change cost
+    double add2 = 0.0;   // zero-length edge: the wire cost on layer t2
 };
 
 template <int L>
@@ -996,7 +1001,7 @@ __device__ __forceinline__ PushItem<L> load_item(const FltArgs& a, const Dag& d,
     constexpr int H = PushItem<L>::H;
     const DeviceGrid& g = a.g;
     PushItem<L> it;
-    const int e = tb.eorder[e0 + q / L], mi = q % L;
+    const int e = tb.eorder[e0 + q / H], mi = q % H;
     const int4 er = tb.erec[e];  // {parent, slot, parent coordinate, child node}
     const int v = er.w;
     const int cc = tb.nrec[v].x;
@@ -1009,6 +1014,10 @@ __device__ __forceinline__ PushItem<L> load_item(const FltArgs& a, const Dag& d,
     if (it.zero) {
         it.t = mi + 1;
         it.add[0] = wire(a, mi, minx, maxx, miny, maxy);
+        if (mi + 1 + H <= L) {
+            it.t2 = mi + 1 + H;
+            it.add2 = wire(a, mi + H, minx, maxx, miny, maxy);
+        }
         return it;
     }
     // a layer runs along y when ((layer & 1) ^ DIR) == 0 (:101-102)
@@ -1033,11 +1042,17 @@ __device__ __forceinline__ void finish_item(const FltArgs& a, const LdsTables<L>
     const double* cv = tb.cvs + tb.posof[it.v] * L - 1;  // cv[layer], layer 1..L
     if (it.zero) {
         const double base = cv[it.t];
-        if (!(base < kInf)) {  // the reference asserts here
+        if (!(base < kInf))  // the reference asserts here
             raise(a.error, kDevNoSolution);
-            return;
+        else
+            tb.lower(it.row, it.t, pack((float)(base + it.add[0]), it.v * 100 + kStraight));
+        if (it.t2) {
+            const double base2 = cv[it.t2];
+            if (!(base2 < kInf))
+                raise(a.error, kDevNoSolution);
+            else
+                tb.lower(it.row, it.t2, pack((float)(base2 + it.add2), it.v * 100 + kStraight));
         }
-        tb.lower(it.row, it.t, pack((float)(base + it.add[0]), it.v * 100 + kStraight));
         return;
     }
     unsigned long long best = kEmpty;
@@ -1100,7 +1115,7 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
     for (int dep = depths - 1; dep >= 0; --dep) {
         const int lane = opaque(lane_in);  // lane-derived addresses formed per depth
         const bool here = dep < dn.w;
-        const int e0 = here ? tb.edoff[dep] : 0, items = here ? (tb.edoff[dep + 1] - e0) * L : 0;
+        const int e0 = here ? tb.edoff[dep] : 0, items = here ? (tb.edoff[dep + 1] - e0) * PushItem<L>::H : 0;
         PushItem<L> first;
         if (lane < items) first = load_item<L>(a, d, tb, e0, lane);
         if (here) {
@@ -1353,7 +1368,8 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
 // kFltQueues striped counters, zero at launch: workgroup w dequeues from
 // counter w % kFltQueues, which hands out groups q, q + kFltQueues, ...
 template <int L>
-__global__ __launch_bounds__(kWave) void flt_route_lds_kernel(FltArgs a) {
+// two wavefronts per SIMD (256 registers) hold without spills
+__global__ __launch_bounds__(kWave, 2) void flt_route_lds_kernel(FltArgs a) {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_group;
     const int P = a.per_wave, W = kWave / P;
