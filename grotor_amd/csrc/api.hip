@@ -2026,6 +2026,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         if (!c->routes2) throw GrError(GR_E_STATE, "no phase-two route storage");
         const DeviceGrid& g = c->g;
         if (g.L > 9) throw GrError(GR_E_ARG, "FLT supports at most 9 routing layers");
+        if (g.X > 65535 || g.Y > 65535) throw GrError(GR_E_ARG, "FLT packs node positions in 16 bits per axis");
         if (batch_off[0] != 0 || node_off[0] != 0 || edge_off[0] != 0) throw GrError(GR_E_ARG, "offsets must start at 0");
         for (int b = 0; b < nbatch; ++b)
             if (batch_off[b + 1] < batch_off[b]) throw GrError(GR_E_ARG, "batch offsets out of order");

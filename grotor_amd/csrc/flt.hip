@@ -115,6 +115,7 @@ struct LdsTables {
     double* cvs;                 // [position in depth * L + li - 1]: layer costs of a depth's solved nodes
     int* layer;                  // [node]
     int* posof;                  // [node]: position in its depth
+    unsigned* xy;                // [node]: x << 16 | y, decoded once at staging
     float* vc;                   // [node * (L - 1) + j]: via cost of layer j at the node's cell
     __device__ __forceinline__ unsigned long long key(int row, int li) const { return packed[row * L + li - 1]; }
     // the float cost of a slot key, from its upper word alone
@@ -163,6 +164,7 @@ __host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsT
     char* edoff = take((size_t)(s.depths + 1) * 4, 4);
     char* layer = take((size_t)s.nodes * 4, 4);
     char* posof = take((size_t)s.nodes * 4, 4);
+    char* xy = take((size_t)s.nodes * 4, 4);
     char* vc = take((size_t)s.nodes * (L > 1 ? L - 1 : 0) * 4, 4);
     t.packed = reinterpret_cast<unsigned long long*>(packed);
     t.best = reinterpret_cast<unsigned*>(best);
@@ -176,6 +178,7 @@ __host__ __device__ inline size_t lds_layout(char* base, const LdsShape& s, LdsT
     t.edoff = reinterpret_cast<int*>(edoff);
     t.layer = reinterpret_cast<int*>(layer);
     t.posof = reinterpret_cast<int*>(posof);
+    t.xy = reinterpret_cast<unsigned*>(xy);
     t.vc = reinterpret_cast<float*>(vc);
     return (off + 15) & ~size_t(15);
 }
@@ -1004,8 +1007,8 @@ __device__ __forceinline__ PushItem<L> load_item(const FltArgs& a, const Dag& d,
     const int e = tb.eorder[e0 + q / H], mi = q % H;
     const int4 er = tb.erec[e];  // {parent, slot, parent coordinate, child node}
     const int v = er.w;
-    const int cc = tb.nrec[v].x;
-    const int x = cc / g.Y % g.X, y = cc % g.Y, px = er.z / g.Y % g.X, py = er.z % g.Y;
+    const unsigned cxy = tb.xy[v], pxy = tb.xy[er.x];  // the parent's cell is er.z
+    const int x = int(cxy >> 16), y = int(cxy & 0xFFFFu), px = int(pxy >> 16), py = int(pxy & 0xFFFFu);
     const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
     it.v = v;
     it.row = (tb.nrec[er.x].w >> 3) + er.y;
@@ -1097,6 +1100,7 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
         tb.nrec[v] = r;
         tb.order[v] = a.order[ds.x + v];
         const int x = r.x / a.g.Y % a.g.X, y = r.x % a.g.Y;
+        tb.xy[v] = (unsigned)x << 16 | (unsigned)y;
         for (int j = 0; j + 1 < L; ++j) tb.vc[v * (L - 1) + j] = a.vcost[cell_of(a.g, j, x, y)];
         for (int e = r.y; e < r.z; ++e) {
             const int4 er = dg.edges[e];
