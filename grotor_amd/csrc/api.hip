@@ -2142,7 +2142,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                     const int pcx = pc / g.Y % g.X, pcy = pc % g.Y, ccx = cc / g.Y % g.X, ccy = cc % g.Y;
                     if (pcx != ccx && pcy != ccy) throw GrError(GR_E_ARG, "FLT edge not straight");
                     erec[e0 + e] = make_int4(par, s2, pc, cc);
-                    eb += 12.0 * L * (std::abs(pcx - ccx) + std::abs(pcy - ccy) + 1);
+                    const int len = std::abs(pcx - ccx) + std::abs(pcy - ccy);
+                    if (len > 4) eb += 12.0 * L * (len + 1);  // shorter edges' change tables are never read
                 }
             }
             const int depths = net_depths[k];
@@ -2185,8 +2186,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
 
         // per batch: stage-global bases, the table sizes and the algorithmic
         // bytes per launch (the layer-change search, 12 * L * (len + 1) per
-        // edge for the presum and via cost gathers along it; the per-edge
-        // change tables are scratch and not counted)
+        // edge longer than 4 for the presum and via cost gathers along it;
+        // the per-edge change tables are scratch and not counted)
         long long max_nodes = 1, max_edges = 1, max_slots = 1;
         c->f_base.assign(nbatch, make_int4(0, 0, 0, 0));
         c->f_batch_max.assign(nbatch, FltLdsShape{0, 0, 0, 0, 0});

@@ -1227,7 +1227,7 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     const int L = g.L;
     const int E = edge_group(L);
     struct EdgeInfo {
-        int fromx, fromy, tox, toy, start, dd, count, vertical;
+        int fromx, fromy, tox, toy, start, dd, count, vertical, near;
     };
     __shared__ EdgeInfo info[kEdgeThreads / kFltChangeWidth];
     extern __shared__ __align__(16) char smem[];
@@ -1238,6 +1238,7 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     if (threadIdx.x < E) {
         EdgeInfo& in = info[threadIdx.x];
         in.count = 0;
+        in.near = 1;
         const int e = e0 + threadIdx.x;
         if (e < a.edges) {
             const int4 er = a.edge_rec[a.edge_base + e];
@@ -1249,6 +1250,10 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
             in.dd = distance ? distance / min(distance, 20) : 1;
             // positions start + dd * k < start + distance, k >= 1
             in.count = distance ? (distance - 1) / in.dd : 0;
+            // the DP and the trace read the change tables of edges longer
+            // than 4 only (solve_flt_node's distance test): shorter edges
+            // are skipped
+            in.near = distance <= 4;
         }
     }
     __syncthreads();
@@ -1259,7 +1264,7 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
         const int j = q / (kEdgePositions * per_pos), rem = q % (kEdgePositions * per_pos);
         const int k = rem / per_pos, v = rem % per_pos;
         const EdgeInfo& in = info[j];
-        if (k >= in.count || k >= kEdgePositions) continue;
+        if (in.near || k >= in.count || k >= kEdgePositions) continue;
         const int sp = in.start + in.dd * (k + 1);
         const int sx = in.vertical ? in.fromx : sp, sy = in.vertical ? sp : in.fromy;
         if (v < L - 1) {
@@ -1281,7 +1286,7 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     // a from-layer across the edge: the reference fills its row with
     // infinity; the DP never reads it (its direction test comes first)
     if ((layer_dir(g, f) == 0 && in.fromy != in.toy) || (layer_dir(g, f) == 1 && in.fromx != in.tox)) return;
-    if (in.fromx == in.tox && in.fromy == in.toy) return;  // zero length
+    if (in.near) return;  // zero length or not read
     double best = kInf;
     int pos = -1;
     const int lo = min(f, to), hi = max(f, to);
