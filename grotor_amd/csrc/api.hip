@@ -213,6 +213,7 @@ struct gr_ctx {
     // per batch: DAGs whose LDS share exceeds f_lds_split bytes first (f_large
     // of them), then the rest, each class with its own largest shape
     std::vector<int> f_large;
+    std::vector<int> f_far_off;  // per batch: offsets of its edges longer than 4 in the stage's far-edge list
     std::vector<FltLdsShape> f_class_max;  // [batch * kFltClasses + class], class 0 the large DAGs
     int f_lds_split = 20 * 1024;
     int* f_queue = nullptr;         // persistent FLT DP: kFltClasses x kFltQueues counters per batch
@@ -828,6 +829,8 @@ FltArgs flt_batch_args(gr_ctx* c, int b) {
     a.queue = nullptr;
     a.lds_dag_bytes = 0;
     a.lds_shape = c->f_batch_max[b];
+    a.far_edges = c->f.far_edges + c->f_far_off[b];
+    a.far_count = c->f_far_off[b + 1] - c->f_far_off[b];
     return a;
 }
 
@@ -2081,7 +2084,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         const size_t ndoff = size_t(dbase[nets]);
         struct Stage {
             int4 *nrec, *erec, *desc, *sizes;
-            int *order, *doff, *eorder, *edoff, *ids;
+            int *order, *doff, *eorder, *edoff, *ids, *far;
         };
         auto layout = [&](auto& cv, Stage& st) {
             st.nrec = cv.template take<int4>(size_t(total_nodes));
@@ -2093,6 +2096,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             st.desc = cv.template take<int4>(nets);
             st.sizes = cv.template take<int4>(nets);
             st.ids = cv.template take<int>(nets);
+            st.far = cv.template take<int>(size_t(total_edges));  // used up to the far-edge count
         };
         Carver dry{nullptr};
         {
@@ -2118,6 +2122,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         std::vector<double> net_eb(nets), net_rb(nets);
         std::vector<int> net_dnodes(nets);
         std::vector<char> big(nets);  // the DAG's LDS share exceeds f_lds_split
+        std::vector<int> net_far(nets);  // edges longer than 4
         parallel_items(nets, [&](int k) {
             const long long n0 = node_off[k], e0 = edge_off[k];
             const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
@@ -2143,7 +2148,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                     if (pcx != ccx && pcy != ccy) throw GrError(GR_E_ARG, "FLT edge not straight");
                     erec[e0 + e] = make_int4(par, s2, pc, cc);
                     const int len = std::abs(pcx - ccx) + std::abs(pcy - ccy);
-                    if (len > 4) eb += 12.0 * L * (len + 1);  // shorter edges' change tables are never read
+                    if (len > 4) eb += 12.0 * L * (len + 1), ++net_far[k];  // shorter edges' change tables are never read
                 }
             }
             const int depths = net_depths[k];
@@ -2219,6 +2224,26 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         // share.  Inside a class the DAGs of a wavefront step through their
         // depths together: decreasing depth, then size (results do not
         // depend on the order of nets inside a batch).
+        // the far-edge list: per batch its edges longer than 4, in edge order
+        c->f_far_off.assign(nbatch + 1, 0);
+        for (int b = 0; b < nbatch; ++b) {
+            long long nf = 0;
+            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k) nf += net_far[k];
+            c->f_far_off[b + 1] = c->f_far_off[b] + int(nf);
+        }
+        parallel_tasks(nbatch, [&](int b) {
+            int* out = hs.far + c->f_far_off[b];
+            for (int k = batch_off[b]; k < batch_off[b + 1]; ++k) {
+                if (!net_far[k]) continue;
+                const long long n0 = node_off[k], e0 = edge_off[k];
+                for (long long e = 0; e < edge_off[k + 1] - e0; ++e) {
+                    const int4 er = erec[e0 + e];  // {parent, slot, parent cell, child cell}
+                    const int pcx = er.z / g.Y % g.X, pcy = er.z % g.Y, ccx = er.w / g.Y % g.X, ccy = er.w % g.Y;
+                    if (std::abs(pcx - ccx) + std::abs(pcy - ccy) > 4) *out++ = int(e0 + e);
+                }
+                (void)n0;
+            }
+        });
         c->f_large.assign(nbatch, 0);
         c->f_class_max.assign(size_t(nbatch) * kFltClasses, FltLdsShape{0, 0, 0, 0, 0});
         parallel_tasks(nbatch, [&](int b) {
@@ -2263,8 +2288,9 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             f.cost_edges = cv.template take<double>(size_t(max_edges) * 9 * kFltChangeWidth);
             f.best_change = cv.template take<int>(size_t(max_edges) * 9 * kFltChangeWidth);
         });
-        if (stage_bytes > bulk)
-            upload(reinterpret_cast<char*>(dev.nrec) + bulk, c->f_stage + bulk, stage_bytes - bulk, c->stream);
+        // the descriptors and the used part of the far-edge list
+        const size_t tail = size_t(reinterpret_cast<char*>(hs.far + c->f_far_off[nbatch]) - c->f_stage);
+        if (tail > bulk) upload(reinterpret_cast<char*>(dev.nrec) + bulk, c->f_stage + bulk, tail - bulk, c->stream);
         f.g = g;
         f.node_rec = dev.nrec;
         f.edge_rec = dev.erec;
@@ -2275,6 +2301,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         f.desc = dev.desc;
         f.sizes = dev.sizes;
         f.net_ids = dev.ids;
+        f.far_edges = dev.far;
         f.routes = RouteStore{c->routes2, c->pin_off2, c->route2_per_pin};
         f.pin_acc_num = c->pin_acc_num;
         f.error = c->error;

@@ -1211,8 +1211,10 @@ __device__ __forceinline__ Dag dag_of(const FltArgs& a, const int4 ds) {
 
 }  // namespace
 
-// precompute_cost_edge (src/routing/flt_cost.cuh:50-130).  A workgroup
-// takes kEdgeGroup(L) edges of the batch: it first gathers, at every sample
+// precompute_cost_edge (src/routing/flt_cost.cuh:50-130) for the batch's
+// edges longer than 4 (the DP and the trace read no other change tables; a
+// list built at upload).  A workgroup takes edge_group(L) of them: it first
+// gathers, at every sample
 // position of each edge, the via costs of all layers and the prefix sums of
 // the layers that run along the edge into LDS (each value once, instead of
 // once per layer pair), then one thread per (edge, from-layer, same-parity
@@ -1234,14 +1236,13 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     // per edge of the group: vc[k * (L - 1) + l] (float), ps[k * L + l] (double), k = position - 1
     float* vcs = reinterpret_cast<float*>(smem);
     double* pss = reinterpret_cast<double*>(smem + size_t(E) * kEdgePositions * (L - 1) * 4 + 8);
-    const int e0 = blockIdx.x * E;
+    const int e0 = blockIdx.x * E;  // position in the batch's far-edge list
     if (threadIdx.x < E) {
         EdgeInfo& in = info[threadIdx.x];
         in.count = 0;
         in.near = 1;
-        const int e = e0 + threadIdx.x;
-        if (e < a.edges) {
-            const int4 er = a.edge_rec[a.edge_base + e];
+        if (e0 + int(threadIdx.x) < a.far_count) {
+            const int4 er = a.edge_rec[a.far_edges[e0 + threadIdx.x]];
             in.tox = er.z / g.Y % g.X, in.toy = er.z % g.Y;
             in.fromx = er.w / g.Y % g.X, in.fromy = er.w % g.Y;
             const int distance = abs(in.tox - in.fromx) + abs(in.toy - in.fromy);
@@ -1277,8 +1278,8 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
     }
     __syncthreads();
     const int j = threadIdx.x / (L * kFltChangeWidth), r = threadIdx.x % (L * kFltChangeWidth);
-    const int e = e0 + j;
-    if (j >= E || e >= a.edges) return;
+    if (j >= E || e0 + j >= a.far_count) return;
+    const int e = a.far_edges[e0 + j] - a.edge_base;  // the batch's edge index
     const int f = r / kFltChangeWidth, to = (f & 1) + 2 * (r % kFltChangeWidth);
     // to == f is the straight edge, which the DP prices itself
     if (to >= L || to == f) return;
@@ -1333,9 +1334,9 @@ __global__ __launch_bounds__(kEdgeThreads) void flt_edge_kernel(FltArgs a) {
 
 void launch_flt_edges(const FltArgs& a, hipStream_t stream) {
     const int L = a.g.L, E = edge_group(L);
-    if (!a.edges || L < 2) return;
+    if (!a.far_count || L < 2) return;
     const size_t lds = size_t(E) * kEdgePositions * ((L - 1) * 4 + L * 8) + 8;
-    hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((a.edges + E - 1) / E)), dim3(kEdgeThreads), lds, stream, a);
+    hipLaunchKernelGGL(flt_edge_kernel, dim3(unsigned((a.far_count + E - 1) / E)), dim3(kEdgeThreads), lds, stream, a);
 }
 
 namespace {

@@ -220,6 +220,10 @@ struct FltArgs {
     int lds_dag_bytes;
     FltLdsShape lds_shape;
     int* queue;  // persistent form: kFltQueues work-queue counters (nullptr: one workgroup per DAG group)
+    // the batch's edges longer than 4 (stage-global indices): the only ones
+    // whose layer-change tables the DP reads
+    const int* far_edges;
+    int far_count;
 };
 constexpr int kFltQueues = 32;       // persistent FLT DP: work-queue counters per batch
 constexpr int kFltQueueStride = 32;  // ints between counters (one 128-byte line)
