@@ -2123,6 +2123,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
         std::vector<int> net_dnodes(nets);
         std::vector<char> big(nets);  // the DAG's LDS share exceeds f_lds_split
         std::vector<int> net_far(nets);  // edges longer than 4
+        std::vector<char> edge_far(static_cast<size_t>(total_edges));
         parallel_items(nets, [&](int k) {
             const long long n0 = node_off[k], e0 = edge_off[k];
             const long long nn = node_off[k + 1] - n0, ne = edge_off[k + 1] - e0;
@@ -2148,6 +2149,7 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
                     if (pcx != ccx && pcy != ccy) throw GrError(GR_E_ARG, "FLT edge not straight");
                     erec[e0 + e] = make_int4(par, s2, pc, cc);
                     const int len = std::abs(pcx - ccx) + std::abs(pcy - ccy);
+                    edge_far[e0 + e] = len > 4;
                     if (len > 4) eb += 12.0 * L * (len + 1), ++net_far[k];  // shorter edges' change tables are never read
                 }
             }
@@ -2235,13 +2237,8 @@ int gr_flt_upload(gr_ctx* c, int nbatch, const int* batch_off, const int* net_id
             int* out = hs.far + c->f_far_off[b];
             for (int k = batch_off[b]; k < batch_off[b + 1]; ++k) {
                 if (!net_far[k]) continue;
-                const long long n0 = node_off[k], e0 = edge_off[k];
-                for (long long e = 0; e < edge_off[k + 1] - e0; ++e) {
-                    const int4 er = erec[e0 + e];  // {parent, slot, parent cell, child cell}
-                    const int pcx = er.z / g.Y % g.X, pcy = er.z % g.Y, ccx = er.w / g.Y % g.X, ccy = er.w % g.Y;
-                    if (std::abs(pcx - ccx) + std::abs(pcy - ccy) > 4) *out++ = int(e0 + e);
-                }
-                (void)n0;
+                for (long long e = edge_off[k]; e < edge_off[k + 1]; ++e)
+                    if (edge_far[e]) *out++ = int(e);
             }
         });
         c->f_large.assign(nbatch, 0);
