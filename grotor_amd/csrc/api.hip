@@ -420,6 +420,8 @@ CommitArgs commit_args(gr_ctx* c, int begin, int count, int stamp, int K) {
     return a;
 }
 
+constexpr int kCommitPackedNets = 4096;  // batches of at least this many nets commit four nets per wavefront
+
 // Commit (K = 1) or removal (K = -1) of stage slots [begin, begin + count):
 // routes from `rs` (phase-one storage when null), or, with `current`, each
 // net's current route (phase-two words for nets marked live).
@@ -430,11 +432,21 @@ void launch_commit(gr_ctx* c, int begin, int count, int stamp, int K, const int*
     if (net_ids) a.net_ids = net_ids;
     if (rs) a.rs = *rs;
     if (current) a.live2 = c->live2;
-    const int blocks = std::min((count + 3) / 4, 4096);
-    timed(c, GR_K_COMMIT_WIRES, 0.0,
-          [&] { hipLaunchKernelGGL(commit_wires_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
-    timed(c, GR_K_COMMIT_VIAS, 0.0,
-          [&] { hipLaunchKernelGGL(commit_vias_kernel, dim3(blocks), dim3(256), 0, c->stream, a); });
+    // many nets (a large batch): four per wavefront, so the batch's nets are
+    // resident at once; otherwise a wavefront per net
+    if (count >= kCommitPackedNets) {
+        const int blocks = std::min((count + 15) / 16, 4096);
+        timed(c, GR_K_COMMIT_WIRES, 0.0,
+              [&] { hipLaunchKernelGGL(commit_wires_kernel<16>, dim3(blocks), dim3(256), 0, c->stream, a); });
+        timed(c, GR_K_COMMIT_VIAS, 0.0,
+              [&] { hipLaunchKernelGGL(commit_vias_kernel<16>, dim3(blocks), dim3(256), 0, c->stream, a); });
+    } else {
+        const int blocks = std::min((count + 3) / 4, 4096);
+        timed(c, GR_K_COMMIT_WIRES, 0.0,
+              [&] { hipLaunchKernelGGL(commit_wires_kernel<64>, dim3(blocks), dim3(256), 0, c->stream, a); });
+        timed(c, GR_K_COMMIT_VIAS, 0.0,
+              [&] { hipLaunchKernelGGL(commit_vias_kernel<64>, dim3(blocks), dim3(256), 0, c->stream, a); });
+    }
     hip_check(hipGetLastError(), "commit kernels");
 }
 

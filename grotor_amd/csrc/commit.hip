@@ -31,30 +31,40 @@ __device__ __forceinline__ void add_demand(const CommitArgs& a, int cell, float 
     a.track_dirty[track_of_cell(a.g, cell)] = 1u;
 }
 
-// Wires: the unit edges of all of a net's segments are spread over the 64
-// lanes of its wavefront (segment geometry per lane, a wave prefix sum of
-// edge counts, then edge t goes to lane t mod 64), so a net costs
-// ceil(edges / 64) steps however its edges split into segments.
+// Wires: the unit edges of all of a net's segments are spread over the W
+// lanes of its group (W = 64, a wavefront, or 16: four nets per wavefront;
+// segment geometry per lane, a group prefix sum of edge counts, then edge t
+// goes to lane t mod W), so a net costs ceil(edges / W) steps however its
+// edges split into segments.
+template <int W>
 __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
-    __shared__ int s_off[4][kWave], s_cell[4][kWave], s_step[4][kWave], s_lo[4][kWave];
-    __shared__ unsigned char s_flags[4][kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wib = threadIdx.x / kWave;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int waves = gridDim.x * blockDim.x / kWave;
+    constexpr int G = 256 / W;  // groups per workgroup
+    __shared__ int s_off[G][W], s_cell[G][W], s_step[G][W], s_lo[G][W];
+    __shared__ unsigned char s_flags[G][W];
+    const int lane = threadIdx.x & (W - 1);
+    const int wib = threadIdx.x / W;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;  // this lane's group
+    const int waves = gridDim.x * blockDim.x / W;
     const DeviceGrid& g = a.g;
     int* off = s_off[wib];
     int* cell0 = s_cell[wib];
     int* step = s_step[wib];
     int* lo_of = s_lo[wib];
     unsigned char* flags = s_flags[wib];  // bit 0: along y, bit 1: one track
-    for (int s = wave; s < a.count; s += waves) {
-        const int slot = a.slots ? a.slots[a.begin + s] : a.begin + s;
-        const int* r = current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]);
-        const int nseg = (r[0] - 1) / 2;
+    // every group of a wavefront runs the same number of net steps (the
+    // group operations below need all lanes of the wavefront)
+    const int wave64 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int steps = (a.count - (wave64 * (kWave / W)) + waves - 1) / waves;
+    for (int it = 0, s = wave; it < steps; ++it, s += waves) {
+        const bool live = s < a.count;
+        const int slot = live ? (a.slots ? a.slots[a.begin + s] : a.begin + s) : 0;
+        const int* r = live ? current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]) : nullptr;
+        const int nseg = live ? (r[0] - 1) / 2 : 0;
         double length = 0.0;
         long long edges = 0;
-        for (int base = 0; base < nseg; base += kWave) {
+        int rounds = (nseg + W - 1) / W;  // the same in every group of the wavefront
+        for (int o = W; o < kWave; o <<= 1) rounds = max(rounds, __shfl_xor(rounds, o, kWave));
+        for (int base = 0; base < rounds * W; base += W) {
             int cnt = 0;
             if (base + lane < nseg) {
                 const int ra = r[1 + 2 * (base + lane)], rb = r[2 + 2 * (base + lane)];
@@ -77,17 +87,17 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
                 }
             }
             int incl = cnt;
-            for (int d = 1; d < kWave; d <<= 1) {
-                const int v = __shfl_up(incl, d, kWave);
+            for (int d = 1; d < W; d <<= 1) {
+                const int v = __shfl_up(incl, d, W);
                 if (lane >= d) incl += v;
             }
-            const int total = __shfl(incl, kWave - 1, kWave);
+            const int total = __shfl(incl, W - 1, W);
             off[lane] = incl - cnt;
             edges += cnt;
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            for (int t = lane; t < total; t += kWave) {
-                int lo = 0, hi = kWave - 1;  // last lane whose edges start at or before t
+            for (int t = lane; t < total; t += W) {
+                int lo = 0, hi = W - 1;  // last lane whose edges start at or before t
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (off[mid] <= t) lo = mid;
@@ -103,9 +113,9 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
             }
             __builtin_amdgcn_wave_barrier();
         }
-        for (int o = kWave / 2; o; o >>= 1) {
-            length += __shfl_xor(length, o, kWave);
-            edges += __shfl_xor(edges, o, kWave);
+        for (int o = W / 2; o; o >>= 1) {
+            length += __shfl_xor(length, o, W);
+            edges += __shfl_xor(edges, o, W);
         }
         Stats* st = a.stats + wave % kStatStripes;
         if (lane == 0 && length != 0.0) atomicAdd(&st->total_wirelength, length * a.K);
@@ -113,19 +123,24 @@ __global__ __launch_bounds__(256) void commit_wires_kernel(CommitArgs a) {
     }
 }
 
-// Vias: one lane per route segment; the timestamps of all its layer steps
-// are loaded before any is tested, so a stack costs one memory round trip.
+// Vias: one lane per route segment (W lanes per net, as for the wires); the
+// timestamps of all its layer steps are loaded before any is tested, so a
+// stack costs one memory round trip.
+template <int W>
 __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int waves = gridDim.x * blockDim.x / kWave;
+    const int lane = threadIdx.x & (W - 1);
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
+    const int waves = gridDim.x * blockDim.x / W;
     const DeviceGrid& g = a.g;
-    for (int s = wave; s < a.count; s += waves) {
-        const int slot = a.slots ? a.slots[a.begin + s] : a.begin + s;
-        const int* r = current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]);
-        const int end = r[0];
+    const int wave64 = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int steps = (a.count - (wave64 * (kWave / W)) + waves - 1) / waves;
+    for (int it = 0, s = wave; it < steps; ++it, s += waves) {
+        const bool live = s < a.count;
+        const int slot = live ? (a.slots ? a.slots[a.begin + s] : a.begin + s) : 0;
+        const int* r = live ? current_route(a.rs, a.rs2, a.live2, a.net_ids[slot]) : nullptr;
+        const int end = live ? r[0] : 0;
         long long vias = 0;
-        for (int i = 1 + 2 * lane; i < end; i += 2 * kWave) {
+        for (int i = 1 + 2 * lane; i < end; i += 2 * W) {
             const int ra = r[i], rb = r[i + 1];
             const int la = ra / g.XY, lb = rb / g.XY;
             const int low = min(la, lb), high = max(la, lb);
@@ -152,10 +167,15 @@ __global__ __launch_bounds__(256) void commit_vias_kernel(CommitArgs a) {
                 }
             }
         }
-        for (int off = kWave / 2; off; off >>= 1) vias += __shfl_xor(vias, off, kWave);
+        for (int off = W / 2; off; off >>= 1) vias += __shfl_xor(vias, off, W);
         if (lane == 0 && vias) atomicAdd(&a.stats[wave % kStatStripes].total_vias, (unsigned long long)(vias * a.K));
     }
 }
+
+template __global__ void commit_wires_kernel<16>(CommitArgs);
+template __global__ void commit_wires_kernel<64>(CommitArgs);
+template __global__ void commit_vias_kernel<16>(CommitArgs);
+template __global__ void commit_vias_kernel<64>(CommitArgs);
 
 // Exclusive prefix of per-edge overflow flags along every track; the last
 // cell of a track has no edge (src/routing/congestion.cuh:8-21, 42-72 of

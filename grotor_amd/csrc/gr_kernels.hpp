@@ -60,7 +60,10 @@ struct CommitArgs {
     RouteStore rs2;              // phase-two words, read for nets marked in live2
     const unsigned char* live2;  // nullptr: every net reads rs
 };
+// W lanes per net: 64 (a wavefront) or 16 (four nets per wavefront)
+template <int W>
 __global__ void commit_wires_kernel(CommitArgs a);
+template <int W>
 __global__ void commit_vias_kernel(CommitArgs a);
 
 struct LshapeArgs {
