@@ -760,7 +760,10 @@ constexpr int kPersistentGroups = 3;  // queues per batch in the persistent form
 // trees of at most tiny_bound nodes (most nets), the rest of the small
 // class, the large trees.  Each group's tables are sized for its own
 // largest tree, so the many tiny trees keep a small LDS footprint and a
-// high occupancy; the groups run on three streams, side by side.
+// high occupancy; the groups run on three streams, side by side.  The tiny
+// group takes its own nets per wavefront (tiny_per_wave), the other two one
+// net per wavefront whatever the launch form was tuned to: their trees
+// are solved fastest with a whole wavefront each.
 void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     const size_t qs = size_t(kDpQueues) * kDpQueueStride;
     const int mid = b.count[0] - b.tiny;
@@ -775,14 +778,14 @@ void launch_lshape_persistent(gr_ctx* c, const BatchLaunch& b, int* queue) {
     if (live < 2 || c->profiling) {
         for (int j = 0; j < kPersistentGroups; ++j)
             launch_lshape_queue_range(c, grp[j].begin, grp[j].count, grp[j].nmax, grp[j].k, queue + j * qs, c->stream,
-                                      j == 0 ? c->tiny_per_wave : 0);
+                                      j == 0 ? c->tiny_per_wave : 1);
         return;
     }
     hip_check(hipEventRecord(c->fork_ev, c->stream), "hipEventRecord");
     hip_check(hipStreamWaitEvent(c->side, c->fork_ev, 0), "hipStreamWaitEvent");
     hip_check(hipStreamWaitEvent(c->side2, c->fork_ev, 0), "hipStreamWaitEvent");
-    launch_lshape_queue_range(c, grp[2].begin, grp[2].count, grp[2].nmax, 1, queue + 2 * qs, c->side);
-    launch_lshape_queue_range(c, grp[1].begin, grp[1].count, grp[1].nmax, 0, queue + qs, c->side2);
+    launch_lshape_queue_range(c, grp[2].begin, grp[2].count, grp[2].nmax, 1, queue + 2 * qs, c->side, 1);
+    launch_lshape_queue_range(c, grp[1].begin, grp[1].count, grp[1].nmax, 0, queue + qs, c->side2, 1);
     launch_lshape_queue_range(c, grp[0].begin, grp[0].count, grp[0].nmax, 0, queue, c->stream, c->tiny_per_wave);
     hip_check(hipEventRecord(c->join_ev, c->side), "hipEventRecord");
     hip_check(hipEventRecord(c->join2_ev, c->side2), "hipEventRecord");
@@ -815,7 +818,9 @@ void upload_desc(gr_ctx* c, const std::vector<int>& slots, int first, int last) 
     bool changed = false;
     for (int j = first; j < last; ++j) {
         const int s = slots[j], id = c->net_ids_h[s];
-        const int4 d{c->node_off_h[s], c->node_off_h[s + 1] - c->node_off_h[s], c->route_per_pin * c->pin_acc_num_h[id],
+        // nodes (at most kMaxTreeNodes) | the root's height << 16
+        const int4 d{c->node_off_h[s], (c->node_off_h[s + 1] - c->node_off_h[s]) | c->height_h[s] << 16,
+                     c->route_per_pin * c->pin_acc_num_h[id],
                      c->route_per_pin * (c->pin_acc_num_h[id + 1] - c->pin_acc_num_h[id])};
         int4& h = c->desc_h[j];
         if (h.x != d.x || h.y != d.y || h.z != d.z || h.w != d.w) changed = true;

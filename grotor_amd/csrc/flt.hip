@@ -737,8 +737,7 @@ __device__ __forceinline__ void route_dag(const FltArgs& a, int k, bool live, in
     int count = 1;  // the root's trace sets the word count to 1
     for (int dep = 0; dep < depths; ++dep) {
         const int first = dep < dn.w ? doff[dep] : 0, end = dep < dn.w ? doff[dep + 1] : 0;
-        int chunks = (end - first + W - 1) / W;
-        for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+        const int chunks = wave_max((end - first + W - 1) / W);
         for (int ch = 0; ch < chunks; ++ch) {
             Words wd;
             const int i = first + ch * W + lane;
@@ -1169,8 +1168,7 @@ __device__ __forceinline__ void route_dag_lds(const FltArgs& a, int k, bool live
     // pass two: the words of every node in depth order, appended through a
     // prefix sum over the DAG's lanes
     const int n0 = dn.w ? tb.doff[0] : 0, total = dn.w ? tb.doff[dn.w] - n0 : 0;
-    int chunks = (total + W - 1) / W;
-    for (int off = kWave / 2; off; off >>= 1) chunks = max(chunks, __shfl_xor(chunks, off, kWave));
+    const int chunks = wave_max((total + W - 1) / W);
     int count = 1;  // the root's trace sets the word count to 1
     for (int ch = 0; ch < chunks; ++ch) {
         Words wd;
@@ -1365,8 +1363,7 @@ __global__ __launch_bounds__(256) void flt_route_kernel(FltArgs a) {
     tb.layer = a.layer_output + (ds.x - a.node_base);
     tb.vcost = a.vcost;
     tb.g = a.g;
-    int depths = dn.w;  // the wavefront's deepest DAG
-    for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
+    const int depths = wave_max(dn.w);  // the wavefront's deepest DAG
     route_dag<L>(a, k, live, lane, W, d, tb, dn, a.order + ds.x, a.depth_off + ds.w, depths);
 }
 
@@ -1406,8 +1403,7 @@ __global__ __launch_bounds__(kWave, 2) void flt_route_lds_kernel(FltArgs a) {
         const Dag d = dag_of(a, ds);
         LdsTables<L> tb;
         lds_layout<L>(smem + (size_t)sub * a.lds_dag_bytes, a.lds_shape, tb);
-        int depths = dn.w;
-        for (int off = kWave / 2; off; off >>= 1) depths = max(depths, __shfl_xor(depths, off, kWave));
+        const int depths = wave_max(dn.w);
         route_dag_lds<L>(a, k, live, lane, W, d, tb, ds, dn, depths);
         __syncthreads();  // the tables are reused by the next group
     }

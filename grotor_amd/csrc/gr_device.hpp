@@ -40,6 +40,19 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
+// Maximum over the wavefront's 64 lanes, the same in every lane; every lane
+// must be active.  Each row of 16 lanes reduces through DPP row shifts by 1,
+// 2, 4 and 8 (a lane whose source lies outside its row keeps its own
+// value), so lanes 15, 31, 47 and 63 hold the row maxima, read as scalars.
+__device__ __forceinline__ int wave_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
+    return max(max(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+               max(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
+
 __host__ __device__ inline int cell_of(const DeviceGrid& g, int l, int x, int y) {
     return l * g.XY + (layer_dir(g, l) ? x * g.Y + y : y * g.X + x);
 }

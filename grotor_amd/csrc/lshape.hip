@@ -115,8 +115,34 @@ __device__ __forceinline__ double wire_cost(const DeviceGrid& g, const double* p
     return presum[cell_of(g, l, xmax, ymax)] - presum[cell_of(g, l, xmin, ymin)];
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-    for (int off = kWave / 2; off; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+// Inclusive prefix sum over each group of W consecutive lanes (W a power of
+// two up to 64) and the group's total; every lane must be active.  Inside a
+// row of 16 lanes the sum doubles its reach through DPP row shifts (a source
+// outside the row reads 0, one outside the group is not added); the row
+// totals in lanes 15, 31, 47 and 63 carry across the rows of a group.
+__device__ __forceinline__ int group_scan(int v, int W, int& total) {
+    const int l = threadIdx.x & (kWave - 1), gi = l & (W - 1), row = l >> 4;
+    int t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);
+    v += gi >= 1 ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);
+    v += gi >= 2 ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);
+    v += gi >= 4 ? t : 0;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);
+    v += gi >= 8 ? t : 0;
+    const int r0 = __builtin_amdgcn_readlane(v, 15), r1 = __builtin_amdgcn_readlane(v, 31);
+    const int r2 = __builtin_amdgcn_readlane(v, 47), r3 = __builtin_amdgcn_readlane(v, 63);
+    if (W == 64) {
+        v += (row > 0 ? r0 : 0) + (row > 1 ? r1 : 0) + (row > 2 ? r2 : 0);
+        total = r0 + r1 + r2 + r3;
+    } else if (W == 32) {
+        v += row == 1 ? r0 : row == 3 ? r2 : 0;
+        total = row < 2 ? r0 + r1 : r2 + r3;
+    } else if (W == 16) {
+        total = row == 0 ? r0 : row == 1 ? r1 : row == 2 ? r2 : r3;
+    } else {
+        total = __shfl(v, l | (W - 1), kWave);
+    }
     return v;
 }
 
@@ -248,8 +274,15 @@ __device__ __forceinline__ void phase_b_lanes(const DeviceGrid& g, const Tables<
     }
 }
 
+// rows (node, layer) per lane and pass of the cost gathers
+constexpr int kGatherRows = 2;
+
+// With a queue counter, lane 0 dequeues the workgroup's next chunk before
+// the traceback and parks it in *stash once the route words are placed, so
+// the atomic's round trip overlaps the last phases of this net's solve.
 template <int L>
-__device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int first_slot) {
+__device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int first_slot, int* counter = nullptr,
+                          int* stash = nullptr) {
     constexpr int LL = L * L, T = kTri<L>;
     const int P = a.per_wave, W = kWave / P;
     const int sub = threadIdx.x / W, lane = threadIdx.x % W;  // lane within the net's lanes
@@ -261,20 +294,19 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
     const DeviceGrid& g = a.g;
     const int X = g.X, Y = g.Y;
     const int4 desc = live ? a.desc[a.begin + slot] : make_int4(0, 0, 0, 0);
-    const int n0 = desc.x, n = desc.y;
-    // height of the root = number of levels below it
-    const int H = live ? a.lvl[n0 + n - 1] >> 16 : 0;
+    // nodes | height of the root (the number of levels below it) << 16
+    const int n0 = desc.x, n = desc.y & 0xFFFF, H = desc.y >> 16;
     const int hmax = wave_max(H);
     for (int i = lane; i < n; i += W) {
         const int v = a.nodes[n0 + i];
+        const int q = a.lvl[n0 + i], qp = a.lvl[n0 + i - (i > 0)], h = q >> 16;
         s.nd[i] = v;
         s.par[i] = a.parents[n0 + i];
         s.xy[i] = (v / Y % X) << 16 | (v % Y);  // decoded once: no divisions below
         s.lay[i] = v / g.XY;
         s.link[i] = a.link[n0 + i];
-        const int q = a.lvl[n0 + i], h = q >> 16;
         s.order[i] = q & 0xFFFF;
-        if (i == 0 || (a.lvl[n0 + i - 1] >> 16) != h) s.lstart[h] = i;
+        if (i == 0 || (qp >> 16) != h) s.lstart[h] = i;
     }
     if (live && lane == 0) s.lstart[H + 1] = n;
     __syncthreads();
@@ -282,29 +314,52 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
     // bottom-up sweep below touches only LDS: the via stack at each node
     // (src/routing/lshape_warp.cuh:137-145), the edge's wire costs per layer
     // and the via columns at both corners of a bent edge (:18-48).
-    for (int r = lane; r < n * L; r += W) {
-        const int i = r / L, c = r % L;
-        const int x = s.xy[i] >> 16, y = s.xy[i] & 0xFFFF;
-        s.vs[r] = c < L - 1 ? a.vcost[cell_of(g, c, x, y)] : 0.f;
-        if (!i) continue;
-        const int q = s.xy[s.par[i]];
-        const int px = q >> 16, py = q & 0xFFFF;
-        const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
-        const int d = layer_dir(g, c);
-        double w1 = 0.0, w2 = 0.0;
-        float va = 0.f, vb = 0.f;
-        if (x == px || y == py) {
-            if (!((d == 0 && y != py) || (d == 1 && x != px))) w1 = wire_cost(g, a.presum, c, minx, maxx, miny, maxy);
-        } else {
-            w1 = d ? wire_cost(g, a.presum, c, x, x, miny, maxy) : wire_cost(g, a.presum, c, minx, maxx, y, y);
-            w2 = d ? wire_cost(g, a.presum, c, px, px, miny, maxy) : wire_cost(g, a.presum, c, minx, maxx, py, py);
-            va = a.vcost[cell_of(g, c, x, py)];
-            vb = a.vcost[cell_of(g, c, px, y)];
+    // A lane takes kGatherRows rows per pass with no branch between their
+    // loads (rows past the end read the root's cells), so all of them are in
+    // flight at once; the root row keeps only its via stack.
+    constexpr int U = kGatherRows;
+    for (int r0 = lane; r0 < n * L; r0 += U * W) {
+        double pe0[U], pe1[U], pf0[U], pf1[U];
+        float vsv[U], vav[U], vbv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int r = r0 + u * W < n * L ? r0 + u * W : 0;
+            const int i = r / L, c = r % L;
+            const int vq = s.xy[i], pq = s.xy[i ? s.par[i] : 0];
+            const int x = vq >> 16, y = vq & 0xFFFF, px = pq >> 16, py = pq & 0xFFFF;
+            const int minx = min(x, px), maxx = max(x, px), miny = min(y, py), maxy = max(y, py);
+            const bool straight = x == px || y == py, d = layer_dir(g, c);
+            // wire costs as prefix-sum differences P[e1] - P[e0] (the first
+            // leg, or the whole straight edge) and P[f1] - P[f0] (second leg)
+            const int ex0 = straight ? minx : d ? x : minx, ex1 = straight ? maxx : d ? x : maxx;
+            const int ey0 = straight ? miny : d ? miny : y, ey1 = straight ? maxy : d ? maxy : y;
+            const int fx0 = straight ? ex0 : d ? px : minx, fx1 = straight ? ex1 : d ? px : maxx;
+            const int fy0 = straight ? ey0 : d ? miny : py, fy1 = straight ? ey1 : d ? maxy : py;
+            pe0[u] = a.presum[cell_of(g, c, ex0, ey0)];
+            pe1[u] = a.presum[cell_of(g, c, ex1, ey1)];
+            pf0[u] = a.presum[cell_of(g, c, fx0, fy0)];
+            pf1[u] = a.presum[cell_of(g, c, fx1, fy1)];
+            vsv[u] = a.vcost[cell_of(g, c, x, y)];
+            vav[u] = a.vcost[cell_of(g, c, x, straight ? y : py)];  // corner (x, py)
+            vbv[u] = a.vcost[cell_of(g, c, straight ? x : px, y)];  // corner (px, y)
         }
-        s.w1[r] = w1;
-        s.w2[r] = w2;
-        s.va[r] = va;
-        s.vb[r] = vb;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int r = r0 + u * W;
+            if (r >= n * L) continue;
+            const int i = r / L, c = r % L;
+            s.vs[r] = c < L - 1 ? vsv[u] : 0.f;
+            if (!i) continue;
+            const int vq = s.xy[i], pq = s.xy[s.par[i]];
+            const int x = vq >> 16, y = vq & 0xFFFF, px = pq >> 16, py = pq & 0xFFFF;
+            const bool straight = x == px || y == py;
+            const int d = layer_dir(g, c);
+            const bool along = !((d == 0 && y != py) || (d == 1 && x != px));
+            s.w1[r] = straight && !along ? 0.0 : pe1[u] - pe0[u];
+            s.w2[r] = straight ? 0.0 : pf1[u] - pf0[u];
+            s.va[r] = straight ? 0.f : vav[u];
+            s.vb[r] = straight ? 0.f : vbv[u];
+        }
     }
     __syncthreads();
     for (int r = lane; r < n * L; r += W) {
@@ -450,6 +505,8 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
         if (!(cv < kInf)) atomicExch(a.error, (int)kDevNoSolution);
     }
     __syncthreads();
+    int next = 0;
+    if (counter && threadIdx.x == 0) next = atomicAdd(counter, 1);
     // traceback (src/routing/lshape_warp.cuh:200-213), top level first: a
     // node's interval follows from its parent's
     for (int lv = hmax - 1; lv >= 0; --lv) {
@@ -487,18 +544,16 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
                 w += st ? 2 : 6;
             }
         }
-        int incl = w;
-        for (int d = 1; d < W; d <<= 1) {
-            const int u = __shfl_up(incl, d, W);
-            if (lane >= d) incl += u;
-        }
+        int total;
+        const int incl = group_scan(w, W, total);
         if (live && i < n) s.woff[i] = carry + incl - w;
-        carry += __shfl(incl, W - 1, W);
+        carry += total;
     }
     if (live && lane == 0) {
         s.woff[n] = carry;
         if (carry > desc.w) atomicExch(a.error, (int)kDevRouteOverflow);
     }
+    if (counter && threadIdx.x == 0) *stash = next;
     __syncthreads();
     const int base_word = desc.z, cap = desc.w;
     if (!live || s.woff[n] > cap) return;
@@ -561,18 +616,21 @@ __global__ __launch_bounds__(kWave) void lshape_dp_global_kernel(LshapeArgs a) {
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_queue_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
-    __shared__ int s_first;
+    __shared__ int s_next;
     const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
     const int q = blockIdx.x % kDpQueues;
     int* counter = a.queue + q * kDpQueueStride;
-    while (true) {
-        if (threadIdx.x == 0) s_first = (atomicAdd(counter, 1) * kDpQueues + q) * a.per_wave;
-        __syncthreads();
-        const int first = s_first;
-        __syncthreads();  // every lane has read s_first before the next dequeue
-        if (first >= a.count) break;
-        solve_net<L>(a, smem, words, first);
-        __syncthreads();  // the tables are reused by the next nets
+    // Lane 0 dequeues; the workgroup is one wavefront, which reads the chunk
+    // from lane 0.  Each solve dequeues the next chunk near its end (late
+    // enough that chunks still go to the workgroups that free up first).
+    int got = 0;
+    if (threadIdx.x == 0) got = atomicAdd(counter, 1);
+    int first = (__builtin_amdgcn_readfirstlane(got) * kDpQueues + q) * a.per_wave;
+    while (first < a.count) {
+        solve_net<L>(a, smem, words, first, counter, &s_next);
+        __syncthreads();  // the tables and s_next are reused by the next nets
+        first = (__builtin_amdgcn_readfirstlane(s_next) * kDpQueues + q) * a.per_wave;
+        __syncthreads();  // every lane has read s_next before the next solve writes it
     }
 }
 

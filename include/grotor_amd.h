@@ -249,7 +249,9 @@ int gr_lshape_tune(gr_ctx *ctx, int nbatch, const int *batch_off, int *per_wave,
 /* Group shapes of the persistent form (gr_lshape_run mode 2; no reference
  * counterpart): per batch the trees of at most `tiny` nodes are drained
  * from a work queue of their own with DP tables sized for them, beside the
- * rest of the small class and the large trees.  Times the persistent DP of
+ * rest of the small class and the large trees (those two groups take one
+ * net per wavefront, whatever gr_ctx_set_nets_per_wave or gr_lshape_tune
+ * chose for the launch form).  Times the persistent DP of
  * the uploaded batches for tiny = 1 (no group of its own), 2, 4 and 8 and
  * keeps the fastest; *tiny_nodes receives it, ms[4] (may be NULL) the
  * times.  Routes are the same for every choice; route words are
