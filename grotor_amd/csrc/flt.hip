@@ -859,20 +859,6 @@ __device__ __forceinline__ void solve_node_wide(const FltArgs& a, const Dag& d, 
 template <int L>
 constexpr int kNodeLanes = L > 8 ? 16 : 8;
 
-// Lane g of each 16-lane row receives x of lane g - M of its row (DPP
-// row_shr); lanes with g < M keep their own x.
-template <int M>
-__device__ __forceinline__ int row_shr(int x) {
-    if constexpr (M == 0) return x;
-    else return __builtin_amdgcn_update_dpp(x, x, 0x110 + M, 0xF, 0xF, false);
-}
-template <int M>
-__device__ __forceinline__ double row_shr(double x) {
-    const long long b = __double_as_longlong(x);
-    const int lo = row_shr<M>((int)(b & 0xFFFFFFFFLL)), hi = row_shr<M>((int)(b >> 32));
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // The first minimum over the lows of T[m] of lane g - m for m = M..0 (lows
 // ascending), a tie going to the lower low: lane g's layer cost and slot
 // positions.

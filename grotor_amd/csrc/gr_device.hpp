@@ -40,6 +40,20 @@ __device__ __forceinline__ int opaque(int v) {
     return v;
 }
 
+// Lane g of each 16-lane row receives x of lane g - M of its row (DPP
+// row_shr); lanes with g < M keep their own x.
+template <int M>
+__device__ __forceinline__ int row_shr(int x) {
+    if constexpr (M == 0) return x;
+    else return __builtin_amdgcn_update_dpp(x, x, 0x110 + M, 0xF, 0xF, false);
+}
+template <int M>
+__device__ __forceinline__ double row_shr(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = row_shr<M>((int)(b & 0xFFFFFFFFLL)), hi = row_shr<M>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Maximum over the wavefront's 64 lanes, the same in every lane; every lane
 // must be active.  Each row of 16 lanes reduces through DPP row shifts by 1,
 // 2, 4 and 8 (a lane whose source lies outside its row keeps its own

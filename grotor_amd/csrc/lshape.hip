@@ -274,6 +274,53 @@ __device__ __forceinline__ void phase_b_lanes(const DeviceGrid& g, const Tables<
     }
 }
 
+// The root's first minimum in row-major order on nets of fewer than 16
+// lanes: per row the first minimum over high (lanes), then lane 0 takes the
+// first minimum over the rows.
+template <int L>
+__device__ __forceinline__ void root_first_min(const LshapeArgs& a, const Tables<L>& s, int lane, int W, bool live) {
+    for (int lo = live ? lane : L; lo < L; lo += W) {
+        double m = s.dist[tri<L>(lo, lo)];
+        int mh = lo;
+#pragma unroll
+        for (int hi = 1; hi < L; ++hi) {
+            if (hi <= lo) continue;
+            const double v = s.dist[tri<L>(lo, hi)];
+            if (v < m) {
+                m = v;
+                mh = hi;
+            }
+        }
+        s.gbest[lo] = m;
+        s.gchoice[lo] = mh;
+    }
+    __syncthreads();
+    if (lane == 0 && live) {
+        int ch = s.gchoice[0];
+        double cv = s.gbest[0];
+        for (int lo = 1; lo < L; ++lo)
+            if (s.gbest[lo] < cv) {
+                cv = s.gbest[lo];
+                ch = lo * L + s.gchoice[lo];
+            }
+        s.rng[0] = ch;
+        s.path[0] = 0;
+        if (!(cv < kInf)) atomicExch(a.error, (int)kDevNoSolution);
+    }
+    __syncthreads();
+}
+
+// One DPP step of the root's first minimum: lane g takes the value of lane
+// g - M of its row (a lower row of the root's table) when it is not larger.
+template <int M>
+__device__ __forceinline__ void root_merge(double& m, int& ch) {
+    const double om = row_shr<M>(m);
+    const int oc = row_shr<M>(ch);
+    const bool take = om <= m;
+    m = take ? om : m;
+    ch = take ? oc : ch;
+}
+
 // rows (node, layer) per lane and pass of the cost gathers
 constexpr int kGatherRows = 2;
 
@@ -365,16 +412,20 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
     for (int r = lane; r < n * L; r += W) {
         const int i = r / L, lo = r % L;
         const int fixed = s.lay[i];
+        const bool pin = fixed < L;  // a pin's intervals must contain its layer
         double* row = s.dist + size_t(i) * T + tri<L>(lo, lo);  // row[h - lo] = interval (lo, h)
+        const float* vs = s.vs + i * L;
         double v = 0.0;
-        row[0] = 0.0;
-        for (int hi = lo + 1; hi < L; ++hi) {
-            v = v + (double)s.vs[i * L + hi - 1];
-            row[hi - lo] = v;
+        row[0] = pin && lo != fixed ? kInf : 0.0;
+        // every high end with selects, so the via loads issue together
+#pragma unroll
+        for (int hi = 1; hi < L; ++hi) {
+            const float f = vs[hi - 1];
+            if (hi > lo) {
+                v = v + (double)f;
+                row[hi - lo] = pin && (lo > fixed || hi < fixed) ? kInf : v;
+            }
         }
-        if (fixed < L)
-            for (int hi = lo; hi < L; ++hi)
-                if (lo > fixed || hi < fixed) row[hi - lo] = kInf;
     }
     __syncthreads();
     for (int lv = 0; lv < hmax; ++lv) {
@@ -476,35 +527,36 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
     if (live && hmax > 0)
         for (int lo = lane; lo < L; lo += W) gather_children<L>(s, 0, lo);
     if (hmax > 0) __syncthreads();
-    for (int lo = live ? lane : L; lo < L; lo += W) {
+    if (W >= 16) {
+        // rows lo < L on lanes lo of the net's first 16-lane row, then the
+        // first minimum over the rows through DPP row shifts, a lower row
+        // winning ties; other lanes carry +infinity, which every table value
+        // undercuts (the first minimum lands in lane 15)
+        const int lo = opaque(lane < L ? lane : 0);
         double m = s.dist[tri<L>(lo, lo)];
         int mh = lo;
 #pragma unroll
         for (int hi = 1; hi < L; ++hi) {
-            if (hi <= lo) continue;
-            const double v = s.dist[tri<L>(lo, hi)];
-            if (v < m) {
-                m = v;
-                mh = hi;
-            }
+            const double v = s.dist[tri<L>(lo, hi > lo ? hi : lo)];
+            const bool take = hi > lo && v < m;
+            m = take ? v : m;
+            mh = take ? hi : mh;
         }
-        s.gbest[lo] = m;
-        s.gchoice[lo] = mh;
+        int ch = lo * L + mh;
+        if (!live || lane >= L) m = __builtin_huge_val();
+        root_merge<1>(m, ch);
+        root_merge<2>(m, ch);
+        root_merge<4>(m, ch);
+        root_merge<8>(m, ch);
+        if (live && lane == 15) {
+            s.rng[0] = ch;
+            s.path[0] = 0;
+            if (!(m < kInf)) atomicExch(a.error, (int)kDevNoSolution);
+        }
+        __syncthreads();
+    } else {
+        root_first_min<L>(a, s, lane, W, live);
     }
-    __syncthreads();
-    if (lane == 0 && live) {
-        int ch = s.gchoice[0];
-        double cv = s.gbest[0];
-        for (int lo = 1; lo < L; ++lo)
-            if (s.gbest[lo] < cv) {
-                cv = s.gbest[lo];
-                ch = lo * L + s.gchoice[lo];
-            }
-        s.rng[0] = ch;
-        s.path[0] = 0;
-        if (!(cv < kInf)) atomicExch(a.error, (int)kDevNoSolution);
-    }
-    __syncthreads();
     int next = 0;
     if (counter && threadIdx.x == 0) next = atomicAdd(counter, 1);
     // traceback (src/routing/lshape_warp.cuh:200-213), top level first: a
