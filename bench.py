@@ -102,7 +102,7 @@ def copy_calibration(mib):
 
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` (configN, or
-    configN-launches for the L-shape stage's modes 0 and 1) from the
+    configN-launches for the L-shape stage's mode 0) from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE table
     (profiles/hbm_traffic.txt), scaled by its plain-copy calibration; None
     when not measured."""
@@ -233,8 +233,8 @@ def main():
     achieved = prof["bytes"][dom] / (dom_ms / 1e3) / 1e9 if dom_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": measured_traffic(KIND_NAMES[dom], f"config{args.config}" if args.mode == 2
-                                            else f"config{args.config}-launches"),
+                "traffic": measured_traffic(KIND_NAMES[dom], {2: f"config{args.config}",
+                                                              0: f"config{args.config}-launches"}.get(args.mode, "")),
                 "events": "timed steps (this class only)" if inline_profile
                           else "one extra step after the timed steps, its launches issued directly",
                 "kernel": KIND_NAMES[dom], "avg_launch_us": round(dom_ms * 1e3 / max(dom_launches, 1), 3),
