@@ -186,7 +186,8 @@ def test_graph_mode_replays_match_oracle(ga, ora):
     ctx.close()
 
 
-@pytest.mark.parametrize("per_wave,tiny", [((1, 1), (4, 0)), ((4, 1), (4, 0)), ((1, 1), (8, 4)), ((2, 1), (2, 8))])
+@pytest.mark.parametrize("per_wave,tiny", [((1, 1), (4, 0)), ((4, 1), (4, 0)), ((1, 1), (8, 4)), ((2, 1), (2, 8)),
+                                           ((2, 1), (8, 2))])
 def test_persistent_mode_matches_oracle(ga, ora, per_wave, tiny):
     """Mode 2 (one persistent DP launch per batch draining a work queue,
     the loop captured in a HIP graph): replays and a switch back and forth
