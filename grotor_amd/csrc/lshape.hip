@@ -47,12 +47,12 @@ struct Tables {
     double* diag;   // nmax * L      best arrival on each parent layer
     double* w1;     // nmax * L      first-leg wire (child side; the whole edge when straight)
     double* w2;     // nmax * L      second-leg wire (parent side, bent edges)
-    double* gbest;  // kWave         per lane group: best interval containing each layer
+    double* gbest;  // lanes         per lane group: best interval containing each layer
     float* vs;      // nmax * L      via costs on the node's column
     float* va;      // nmax * L      via costs at corner (x, py)
     float* vb;      // nmax * L      via costs at corner (px, y)
     int* dprev;     // nmax * L
-    int* gchoice;   // kWave         per lane group: that interval, low * L + high
+    int* gchoice;   // lanes         per lane group: that interval, low * L + high
     int* nd;        // nmax
     int* par;       // nmax
     int* rng;       // nmax
@@ -75,26 +75,39 @@ __host__ __device__ constexpr int tri(int lo, int hi) {
     return lo * (2 * L - lo + 1) / 2 + (hi - lo);
 }
 
+// Entries of the per-group arrays (gbest, gchoice) of a net with 64 /
+// per_wave lanes: its lane groups use at most that many, and at least L
+// (one group of L layers).
 template <int L>
-__host__ __device__ constexpr size_t table_bytes(int nmax) {
-    return 8 * (size_t(nmax) * (kTri<L> + 3 * L) + kWave) + 4 * (size_t(nmax) * 3 * L) +
-           4 * (size_t(nmax) * L + kWave + 8 * size_t(nmax) + 2 * (size_t(nmax) + 1));
+__host__ __device__ constexpr int net_lanes(int per_wave) {
+    return kWave / per_wave > L ? kWave / per_wave : L;
+}
+
+// Bytes of one net's tables (lanes: net_lanes; the host sizes LDS fits and
+// global scratch with kWave, an upper bound).  The per-group arrays come
+// last, so the other arrays' offsets do not depend on the lane count.
+template <int L>
+__host__ __device__ constexpr size_t table_head_bytes(int nmax) {
+    return 8 * (size_t(nmax) * (kTri<L> + 3 * L)) + 4 * (size_t(nmax) * 3 * L) +
+           4 * (size_t(nmax) * L + 8 * size_t(nmax) + 2 * (size_t(nmax) + 1));
+}
+template <int L>
+__host__ __device__ constexpr size_t table_bytes(int nmax, int lanes = kWave) {
+    return (table_head_bytes<L>(nmax) + 7) / 8 * 8 + 12 * size_t(lanes);
 }
 
 template <int L>
-__device__ inline Tables<L> carve(double* base, int nmax) {
+__device__ inline Tables<L> carve(double* base, int nmax, int lanes) {
     Tables<L> s;
     s.dist = base;
     s.diag = s.dist + size_t(nmax) * kTri<L>;
     s.w1 = s.diag + size_t(nmax) * L;
     s.w2 = s.w1 + size_t(nmax) * L;
-    s.gbest = s.w2 + size_t(nmax) * L;
-    s.vs = reinterpret_cast<float*>(s.gbest + kWave);
+    s.vs = reinterpret_cast<float*>(s.w2 + size_t(nmax) * L);
     s.va = s.vs + size_t(nmax) * L;
     s.vb = s.va + size_t(nmax) * L;
     s.dprev = reinterpret_cast<int*>(s.vb + size_t(nmax) * L);
-    s.gchoice = s.dprev + size_t(nmax) * L;
-    s.nd = s.gchoice + kWave;
+    s.nd = s.dprev + size_t(nmax) * L;
     s.par = s.nd + nmax;
     s.rng = s.par + nmax;
     s.path = s.rng + nmax;
@@ -104,6 +117,8 @@ __device__ inline Tables<L> carve(double* base, int nmax) {
     s.link = s.order + nmax;
     s.woff = s.link + nmax;
     s.lstart = s.woff + nmax + 1;
+    s.gbest = base + (table_head_bytes<L>(nmax) + 7) / 8;
+    s.gchoice = reinterpret_cast<int*>(s.gbest + lanes);
     return s;
 }
 
@@ -337,7 +352,7 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
     const int grp = lane / GL, gl = lane % GL;                 // grp >= G: an idle lane
     const int slot = first_slot + sub;
     const bool live = slot < a.count;
-    const Tables<L> s = carve<L>(base + words * sub, a.nmax);
+    const Tables<L> s = carve<L>(base + words * sub, a.nmax, net_lanes<L>(P));
     const DeviceGrid& g = a.g;
     const int X = g.X, Y = g.Y;
     const int4 desc = live ? a.desc[a.begin + slot] : make_int4(0, 0, 0, 0);
@@ -650,12 +665,12 @@ __device__ void solve_net(const LshapeArgs& a, double* base, size_t words, int f
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_lds_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
-    solve_net<L>(a, smem, (table_bytes<L>(a.nmax) + 7) / 8, blockIdx.x * a.per_wave);
+    solve_net<L>(a, smem, (table_bytes<L>(a.nmax, net_lanes<L>(a.per_wave)) + 7) / 8, blockIdx.x * a.per_wave);
 }
 
 template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_global_kernel(LshapeArgs a) {
-    const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
+    const size_t words = (table_bytes<L>(a.nmax, net_lanes<L>(a.per_wave)) + 7) / 8;
     solve_net<L>(a, a.gscratch + words * a.per_wave * blockIdx.x, words, blockIdx.x * a.per_wave);
 }
 
@@ -669,7 +684,7 @@ template <int L>
 __global__ __launch_bounds__(kWave) void lshape_dp_queue_kernel(LshapeArgs a) {
     extern __shared__ double smem[];
     __shared__ int s_next;
-    const size_t words = (table_bytes<L>(a.nmax) + 7) / 8;
+    const size_t words = (table_bytes<L>(a.nmax, net_lanes<L>(a.per_wave)) + 7) / 8;
     const int q = blockIdx.x % kDpQueues;
     int* counter = a.queue + q * kDpQueueStride;
     // Lane 0 dequeues; the workgroup is one wavefront, which reads the chunk
@@ -688,7 +703,7 @@ __global__ __launch_bounds__(kWave) void lshape_dp_queue_kernel(LshapeArgs a) {
 
 template <int L>
 void launch_for(const LshapeArgs& a, int blocks, bool lds, hipStream_t st) {
-    const size_t lds_bytes = ((table_bytes<L>(a.nmax) + 7) / 8) * 8 * a.per_wave;
+    const size_t lds_bytes = ((table_bytes<L>(a.nmax, net_lanes<L>(a.per_wave)) + 7) / 8) * 8 * a.per_wave;
     if (a.queue) {
         int resident = 0;  // workgroups per compute unit; any value keeps the queue correct
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, lshape_dp_queue_kernel<L>, kWave, lds_bytes) !=
